@@ -1,0 +1,155 @@
+"""Headline benchmark: 2048^2 ASM propagations/s on one MI355X, FP32 vs FP64.
+
+Workload (the polychromatic set of the batched configuration): 64 fields of
+2048 x 2048, dx = 2 um, each a seeded Gaussian-Schell realisation (w0 = 300
+um, sigma_c = 50 um) generated on the device, each propagated by the angular
+spectrum method over z = 20 mm at its own wavelength (64 wavelengths evenly
+spaced over 400-700 nm). One step is one complete propagation of all 64
+fields (forward 2-D FFT -> transfer function -> inverse 2-D FFT), device
+resident, replayed from a captured hipGraph. `value` is the FP32 bulk path;
+the FP64 path runs the same batch beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tetsuhiro-wos_amd"))
+
+N = 2048
+DX = 2e-6
+Z = 20e-3
+UNITS = 64
+W0, SIGMA_C, SEED = 300e-6, 50e-6, 20240607
+HBM_PEAK_GBS = 8000.0
+METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
+
+
+def rel_l2(a, b):
+    den = np.sum(np.abs(b) ** 2)
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / den))
+
+
+def run_gpu(wos, precision, steps, warmup):
+    wls = np.linspace(400e-9, 700e-9, UNITS)
+    zs = np.full(UNITS, Z)
+    b = wos.Batch(N, DX, wls, zs, wos.ctx_for(550e-9), "asm", precision)
+    b.fill_schell(W0, SIGMA_C, SEED)
+    inp0 = b.download(0)
+    b.capture()
+    b.run(1)
+    b.sync()
+    out = {"unit0_in": inp0, "unit0": b.download(0), "unit_last": b.download(UNITS - 1),
+           "wl0": wls[0]}
+    b.run(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.sync()
+    t1 = time.perf_counter()
+    out["ms_per_step"] = (t1 - t0) * 1e3 / steps
+    ms_ev, kms = b.time(2, max(steps, 10), kernels=True)
+    out["ms_per_step_events"] = ms_ev
+    out["kernel_ms"] = kms
+    b.close()
+    return out
+
+
+def cpu_baseline(inp, wl, budget_s=12.0):
+    """Restated reference CPU path (the FP64 oracle: radix-2 FFT over all host
+    threads, serial TF loop and Power sums) on one 2048^2 unit of the same batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    ctx = O.ctx_for(wl)
+    t = time.perf_counter()
+    out, _, _, _ = O.propagate(inp, None, DX, Z, "asm", ctx)
+    first = time.perf_counter() - t
+    count, t0 = 0, time.perf_counter()
+    while True:
+        O.propagate(inp, None, DX, Z, "asm", ctx)
+        count += 1
+        if time.perf_counter() - t0 >= budget_s or count >= 50:
+            break
+    el = time.perf_counter() - t0
+    return out, {"value": count / el, "unit": "props/s", "cores": threads, "kind": "port",
+                 "sample": f"{count} FP64 ASM propagations of one 2048x2048 complex128 unit "
+                           f"(unit 0 of the batch) after 1 warm-up ({first:.2f} s), "
+                           f"{threads} threads, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(every unit is independent; see the sharding note in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import wos_hip as wos
+    wos.init()
+    r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup)
+    r64 = run_gpu(wos, wos.FP64, args.steps, args.warmup)
+
+    props32 = UNITS / (r32["ms_per_step"] * 1e-3)
+    props64 = UNITS / (r64["ms_per_step"] * 1e-3)
+    err32 = max(rel_l2(r32["unit0"], r64["unit0"]), rel_l2(r32["unit_last"], r64["unit_last"]))
+
+    # dominant kernel of the FP32 pipeline: one read + one write of the batch per launch
+    names = ["row_fft_fwd", "col_fft_tf_ifft", "row_fft_inv"]
+    k = r32["kernel_ms"]
+    dom = int(np.argmax(k))
+    bytes_launch = 2.0 * UNITS * N * N * 8
+    achieved = bytes_launch / (k[dom] * 1e-3) / 1e9
+    # whole step: algorithmic bytes 8*M^2*S per propagation
+    step_gbs = props32 * 8.0 * N * N * 8 / 1e9
+
+    line = {
+        "metric": METRIC,
+        "value": round(props32, 1),
+        "unit": "props/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 complex; fp64 twiddles, kz*z phase and sums)",
+        "data": "synthetic: seeded Gaussian-Schell fields generated on the device",
+        "config": {"workload": "64 x 2048^2 ASM propagations per step (64 wavelengths 400-700 nm, "
+                               "dx=2um, z=20mm), device resident, hipGraph replay",
+                   "grid": N, "units_per_step": UNITS, "method": "asm"},
+        "fp64": {"value": round(props64, 1), "unit": "props/s", "ms_per_step": round(r64["ms_per_step"], 4),
+                 "kernel_ms": [round(x, 4) for x in r64["kernel_ms"]]},
+        "fp32_over_fp64": round(props32 / props64, 3),
+        "rel_l2_fp32_vs_fp64": err32,
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel_ms": [round(x, 4) for x in k],
+                     "step_algorithmic_GBs": round(step_gbs, 1),
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu:
+        ref, cb = cpu_baseline(r64["unit0_in"], r64["wl0"])
+        line["cpu_baseline"] = cb
+        line["rel_l2_fp64_vs_oracle"] = rel_l2(r64["unit0"], ref)
+        line["rel_l2_fp32_vs_oracle"] = rel_l2(r32["unit0"], ref)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * wos_hip.h — C ABI of libwos_hip, the MI355X (gfx950) propagation engine
+ * behind Tetsuhiro-WOS's fft2D/fft1DAny dispatch and its Propagate /
+ * PropagateSplitStep / PropagatePolychromatic / Propagate3D operators.
+ *
+ * Conventions (all entry points):
+ *  - Plain C types only; complex arrays are interleaved (re, im), row-major
+ *    idx = y*N + x: complex128 = double[2], complex64 = float[2]. A Go
+ *    []complex128 is passed as (*C.double)(unsafe.Pointer(&a[0])).
+ *  - Host pointers are read and written synchronously; nothing is retained
+ *    after return (cgo pointer rules). Device-resident batches use the opaque
+ *    wos_batch handle.
+ *  - Every call returns WOS_OK (0) or a WOS_ERR_* code; wos_hip_last_error()
+ *    returns the calling thread's last message. No C++ exception crosses the
+ *    ABI. The library never falls back to a CPU path: the Go caller keeps its
+ *    own pure-Go fallback (optics/fft.go:136-141, optics/fft.go:210-215).
+ *  - Forward transforms are unnormalised with exp(-2 pi i .); inverse
+ *    transforms are scaled by 1/n (1-D) or 1/n^2 (2-D), as fft1DAny / fft2D.
+ *  - Calls are thread-safe: one device mutex serialises work on the
+ *    library's stream, as gpuExecMu does (optics/gpu.go:97-100).
+ */
+#ifndef WOS_HIP_H
+#define WOS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define WOS_OK 0
+#define WOS_ERR_NO_DEVICE 1   /* no HIP device / library not initialised */
+#define WOS_ERR_INVALID 2     /* bad argument (size, method, pointer) */
+#define WOS_ERR_HIP 3         /* a HIP runtime call failed */
+#define WOS_ERR_UNSUPPORTED 4 /* size/method outside what the engine builds */
+#define WOS_ERR_NOMEM 5       /* device allocation failed */
+
+/* ---- precision of the bulk FFT passes ---- */
+#define WOS_FP64 0 /* complex128 everywhere (parity path) */
+#define WOS_FP32 1 /* complex64 bulk passes; FP64 twiddles, chirps, kz*z, sums */
+
+/* ---- propagation methods (optics/propagate.go:40-50) ---- */
+#define WOS_METHOD_ASM 0
+#define WOS_METHOD_ASM_PAD 1
+#define WOS_METHOD_ASM_SHIFT 2
+#define WOS_METHOD_ASM_SHIFT_PAD 3
+#define WOS_METHOD_FRESNEL_TF 4
+#define WOS_METHOD_FRESNEL_IR 5
+
+/* ---- warning codes (optics/propagate.go:203-264, 515-518, 586-589) ---- */
+#define WOS_WARN_EVANESCENT_FILTERED 0
+#define WOS_WARN_BACKWARD_REGULARIZED 1
+#define WOS_WARN_BACKWARD_EVANESCENT 2
+#define WOS_WARN_ASM_ALIAS_WRAP 3
+#define WOS_WARN_FRESNEL_TF_ALIAS 4
+#define WOS_WARN_FRESNEL_IR_ALIAS 5
+#define WOS_WARN_SLOTS 8
+
+/* The Context fields the path reads (optics/field.go:237-254). */
+typedef struct wos_ctx {
+    double wavelength;           /* Wavelength, m */
+    int32_t evanescent_zero;     /* Evanescent == "zero" */
+    int32_t backward_regularize; /* BackwardRegularize */
+    double evanescent_limit;     /* EvanescentLimit, nepers (0 = off) */
+    double tikhonov_alpha;       /* TikhonovAlpha (<= 0 -> 1e-3) */
+} wos_ctx;
+
+/* Warnings.Add accumulation (optics/field.go:322-338): count[code] is the
+ * number of Add calls, value[code] the last non-zero value. The Go shim
+ * replays them into ctx.Warnings with the reference's messages. */
+typedef struct wos_warnings {
+    int32_t count[WOS_WARN_SLOTS];
+    double value[WOS_WARN_SLOTS];
+} wos_warnings;
+
+/* Diagnostics of one Propagate call. */
+typedef struct wos_prop_stats {
+    double p_in, p_out;   /* Field.Power before / after (FP64, compensated) */
+    double fx_c, fy_c;    /* spectral centroid (asm_shift*, else 0) */
+    int32_t method_used;  /* method actually run (asm_shift* fall back) */
+    int32_t reserved;
+} wos_prop_stats;
+
+/* ---- device / library ---- */
+/* replaces wos_device_count (optics/gpu.go:28-33) */
+int wos_hip_device_count(void);
+/* replaces wos_device_name (optics/gpu.go:35-41); writes a NUL-terminated name */
+int wos_hip_device_name(char* buf, int len);
+/* creates the stream and twiddle cache on device 0; idempotent (gpuInit, optics/gpu.go:106-114) */
+int wos_hip_init(void);
+/* thread-local message of the last failing call on this thread */
+const char* wos_hip_last_error(void);
+/* library build string */
+const char* wos_hip_version(void);
+/* frees every cached plan/buffer (device memory returns to the runtime) */
+int wos_hip_release_cache(void);
+
+/* ---- drop-in transforms: gpuFFT2D / gpuFFT1D (optics/gpu.go:275-304) ---- */
+/* In-place 2-D transform of an n*n complex128 host array (fft2D, optics/fft.go:209-216). */
+int wos_hip_fft2d_z(double* a, int n, int inverse);
+/* In-place 1-D transform of a length-n complex128 host array (fft1DAny, optics/fft.go:135-142). */
+int wos_hip_fft1d_z(double* a, int n, int inverse);
+/* complex64 variants (no reference counterpart: the FP32 precision mode). */
+int wos_hip_fft2d_c(float* a, int n, int inverse);
+int wos_hip_fft1d_c(float* a, int n, int inverse);
+
+/* ---- operators ---- */
+/* Propagate (optics/propagate.go:65-102) for asm, asm_pad, asm_shift,
+ * asm_shift_pad, fresnel_tf, fresnel_ir. ex/ey are n*n complex128 host arrays
+ * updated in place; ey == NULL for a scalar field (Polarized == false).
+ * precision selects the bulk-pass type (results are returned in complex128
+ * either way). w and st may be NULL. z == 0 is a no-op. */
+int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int method,
+                      const wos_ctx* ctx, int precision, wos_warnings* w, wos_prop_stats* st);
+
+/* Propagate1D (optics/propagate.go:718-738): 1-D ASM of a length-n complex128 array. */
+int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const wos_ctx* ctx,
+                        int precision, wos_warnings* w);
+
+/* PropagateSplitStep (optics/medium.go:22-45). The Go IndexFunc closure
+ * cannot run on the device, so the caller samples it on the grid in the
+ * reference's order: n_planes == 1 for a z-invariant medium n(x, y), or
+ * n_planes == steps with plane s sampled at zc = z*s/steps + dz/2
+ * (optics/medium.go:35,55). index is n_planes*n*n complex128. */
+int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int steps,
+                       const double* index, int n_planes, const wos_ctx* ctx, int precision,
+                       wos_warnings* w);
+
+/* PropagatePolychromatic (optics/polychromatic.go:18-43): out (n*n float64)
+ * = sum_s weight_s * |U_s|^2; the input field is not modified. */
+int wos_hip_polychromatic(const double* ex, const double* ey, int n, double dx, double z,
+                          const double* wavelengths, const double* weights, int n_samples,
+                          int method, const wos_ctx* ctx, int precision, double* out,
+                          wos_warnings* w);
+
+/* Propagate3D (optics/volume.go:8-18): out_ex/out_ey hold nz planes of n*n
+ * complex128 (out_ey may be NULL when ey is NULL). */
+int wos_hip_volume(const double* ex, const double* ey, int n, double dx, const double* zs, int nz,
+                   int method, const wos_ctx* ctx, int precision, double* out_ex, double* out_ey,
+                   wos_warnings* w);
+
+/* ---- device-resident batches (no reference counterpart: batching is new) ----
+ * A batch holds n_units independent fields of n x n on the device in
+ * `precision`, each propagated by `method` with its own wavelength and
+ * distance: Jones components, polychromatic samples, volume planes or
+ * realisations. One wos_hip_batch_run is one propagation of every unit. */
+typedef struct wos_batch_s* wos_batch;
+
+int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int precision, int n_units,
+                         const double* wavelengths, const double* zs, const wos_ctx* ctx);
+int wos_hip_batch_destroy(wos_batch b);
+/* copy one unit from / to a complex128 host array of n*n */
+int wos_hip_batch_upload(wos_batch b, int unit, const double* ex);
+int wos_hip_batch_download(wos_batch b, int unit, double* ex);
+/* fill every unit with a Gaussian-Schell realisation generated on the device
+ * (optics/coherence.go:22-80 with a counter-based normal generator; unit u
+ * is realisation u of `seed`) */
+int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t seed);
+/* capture one run into a hipGraph; later runs replay it */
+int wos_hip_batch_capture(wos_batch b);
+/* enqueue `iters` runs on the batch's stream (asynchronous) */
+int wos_hip_batch_run(wos_batch b, int iters);
+int wos_hip_batch_sync(wos_batch b);
+/* device pointer of unit 0 (units are contiguous, n*n elements apart) */
+void* wos_hip_batch_device_ptr(wos_batch b);
+/* HIP-event timing on the batch's stream: ms_run = mean time of one run over
+ * `iters` runs after `warmup`; kernel_ms[0..2] = mean duration of the row
+ * forward, column (forward -> TF -> inverse) and row inverse launches, each
+ * timed back to back over `iters` launches. kernel_ms may be NULL. */
+int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WOS_HIP_H */

@@ -1,0 +1,324 @@
+"""ctypes view of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker, never as the product.
+Arrays are numpy complex128 (the memory layout of a Go []complex128),
+row-major, index y*N + x.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+ASM, ASM_PAD, ASM_SHIFT, ASM_SHIFT_PAD, FRESNEL_TF, FRESNEL_IR = range(6)
+METHODS = {"asm": ASM, "asm_pad": ASM_PAD, "asm_shift": ASM_SHIFT,
+           "asm_shift_pad": ASM_SHIFT_PAD, "fresnel_tf": FRESNEL_TF,
+           "fresnel_ir": FRESNEL_IR}
+WARN_CODES = ["evanescent_filtered", "backward_regularized", "backward_evanescent",
+              "asm_alias_wrap", "fresnel_tf_alias", "fresnel_ir_alias"]
+
+
+class Ctx(C.Structure):
+    """Context fields of the path (optics/field.go:237-254)."""
+    _fields_ = [("wavelength", C.c_double),
+                ("evanescent_zero", C.c_int32),
+                ("backward_regularize", C.c_int32),
+                ("evanescent_limit", C.c_double),
+                ("tikhonov_alpha", C.c_double)]
+
+
+class Warnings(C.Structure):
+    _fields_ = [("count", C.c_int32 * 8), ("value", C.c_double * 8)]
+
+    def as_dict(self):
+        return {code: (self.count[i], self.value[i])
+                for i, code in enumerate(WARN_CODES) if self.count[i]}
+
+
+class Stats(C.Structure):
+    _fields_ = [("p_in", C.c_double), ("p_out", C.c_double),
+                ("fx_c", C.c_double), ("fy_c", C.c_double),
+                ("method_used", C.c_int32), ("pad_", C.c_int32)]
+
+
+class PlaneStats(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("power", "peak", "centroid_x", "centroid_y", "rms_x", "rms_y",
+                 "strehl", "intensity_min", "intensity_max")]
+
+
+def ctx_for(wl, evanescent="decay", limit=0.0, backward_regularize=False, alpha=0.0):
+    """ctxFor (optics/propagate_test.go:8-10) plus the other policy fields."""
+    return Ctx(wl, 1 if evanescent == "zero" else 0, 1 if backward_regularize else 0,
+               limit, alpha)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle not built: {_LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        D, I = C.c_double, C.c_int
+        L.or_fft1d.argtypes = [P, I, I]
+        L.or_fft1d_radix2.argtypes = [P, I, I]
+        L.or_fft1d_bluestein.argtypes = [P, I, I]
+        L.or_fft2d.argtypes = [P, I, I]
+        L.or_set_threads.argtypes = [I]
+        L.or_get_threads.restype = I
+        L.or_freq.argtypes = [I, I, D]
+        L.or_freq.restype = D
+        L.or_power.argtypes = [P, P, I, D]
+        L.or_power.restype = D
+        L.or_hypot.argtypes = [D, D]
+        L.or_hypot.restype = D
+        L.or_propagate.argtypes = [P, P, I, D, D, I, C.POINTER(Ctx), C.POINTER(Warnings),
+                                   C.POINTER(Stats)]
+        L.or_propagate.restype = I
+        L.or_spectral_radius.argtypes = [P, P, I, D, D]
+        L.or_spectral_radius.restype = D
+        L.or_spectral_centroid.argtypes = [P, P, I, D, C.POINTER(D), C.POINTER(D)]
+        L.or_propagate1d.argtypes = [P, I, D, D, D, C.POINTER(Ctx), C.POINTER(Warnings)]
+        L.or_propagate1d.restype = I
+        L.or_split_step.argtypes = [P, P, I, D, D, I, P, I, C.POINTER(Ctx), C.POINTER(Warnings)]
+        L.or_split_step.restype = I
+        L.or_polychromatic.argtypes = [P, P, I, D, D, P, P, I, I, C.POINTER(Ctx), P,
+                                       C.POINTER(Warnings)]
+        L.or_polychromatic.restype = I
+        L.or_volume.argtypes = [P, P, I, D, P, I, I, C.POINTER(Ctx), P, P, C.POINTER(Warnings)]
+        L.or_volume.restype = I
+        L.or_gaussian.argtypes = [P, I, D, D]
+        L.or_apply_lens.argtypes = [P, P, I, D, D, D]
+        L.or_apply_circle_aperture.argtypes = [P, P, I, D, D]
+        L.or_hash_noise01.argtypes = [C.c_int64, C.c_int64, C.c_int64]
+        L.or_hash_noise01.restype = D
+        L.or_random_phase.argtypes = [P, I, C.c_int64]
+        L.or_normal.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_normal.restype = D
+        L.or_schell_sqrt_spectrum.argtypes = [P, I, D, D]
+        L.or_schell_realization.argtypes = [P, P, I, D, D, C.c_uint64, C.c_uint64]
+        L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
+        L.or_mask_loss_grad.argtypes = [P, P, I, D, D, C.POINTER(Ctx), P]
+        L.or_mask_loss_grad.restype = D
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c128(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def set_threads(p):
+    lib().or_set_threads(int(p))
+
+
+def fft1d(a, inverse=False):
+    a = _c128(a).copy()
+    lib().or_fft1d(_p(a), a.size, int(inverse))
+    return a
+
+
+def fft1d_radix2(a, inverse=False):
+    a = _c128(a).copy()
+    lib().or_fft1d_radix2(_p(a), a.size, int(inverse))
+    return a
+
+
+def fft1d_bluestein(a, inverse=False):
+    a = _c128(a).copy()
+    lib().or_fft1d_bluestein(_p(a), a.size, int(inverse))
+    return a
+
+
+def fft2d(a, inverse=False):
+    a = _c128(a).copy()
+    n = a.shape[0]
+    assert a.shape == (n, n)
+    lib().or_fft2d(_p(a), n, int(inverse))
+    return a
+
+
+def power(ex, ey, dx):
+    n = ex.shape[0]
+    return lib().or_power(_p(_c128(ex)), _p(None if ey is None else _c128(ey)), n, dx)
+
+
+def propagate(ex, ey, dx, z, method, ctx):
+    """Propagate (optics/propagate.go:65) on copies; returns (ex, ey, warnings, stats)."""
+    m = METHODS[method] if isinstance(method, str) else method
+    ex = _c128(ex).copy()
+    ey = None if ey is None else _c128(ey).copy()
+    w, st = Warnings(), Stats()
+    rc = lib().or_propagate(_p(ex), _p(ey), ex.shape[0], dx, z, m, C.byref(ctx),
+                            C.byref(w), C.byref(st))
+    if rc != 0:
+        raise ValueError(f"or_propagate rc={rc}")
+    return ex, ey, w, st
+
+
+def propagate1d(a, dx, z, wl, ctx):
+    a = _c128(a).copy()
+    w = Warnings()
+    lib().or_propagate1d(_p(a), a.size, dx, z, wl, C.byref(ctx), C.byref(w))
+    return a, w
+
+
+def split_step(ex, ey, dx, z, steps, index, ctx):
+    """index: (n,n) complex (z-invariant) or (steps,n,n) per-slice planes."""
+    ex = _c128(ex).copy()
+    ey = None if ey is None else _c128(ey).copy()
+    index = _c128(index)
+    planes = 1 if index.ndim == 2 else index.shape[0]
+    w = Warnings()
+    rc = lib().or_split_step(_p(ex), _p(ey), ex.shape[0], dx, z, steps, _p(index), planes,
+                             C.byref(ctx), C.byref(w))
+    if rc != 0:
+        raise ValueError(f"or_split_step rc={rc}")
+    return ex, ey, w
+
+
+def polychromatic(ex, ey, dx, z, wavelengths, weights, method, ctx):
+    m = METHODS[method] if isinstance(method, str) else method
+    ex = _c128(ex)
+    ey = None if ey is None else _c128(ey)
+    n = ex.shape[0]
+    wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+    wt = np.ascontiguousarray(weights, dtype=np.float64)
+    out = np.zeros((n, n), dtype=np.float64)
+    w = Warnings()
+    rc = lib().or_polychromatic(_p(ex), _p(ey), n, dx, z, _p(wl), _p(wt), wl.size, m,
+                                C.byref(ctx), _p(out), C.byref(w))
+    if rc != 0:
+        raise ValueError(f"or_polychromatic rc={rc}")
+    return out, w
+
+
+def volume(ex, ey, dx, zs, method, ctx):
+    m = METHODS[method] if isinstance(method, str) else method
+    ex = _c128(ex)
+    ey = None if ey is None else _c128(ey)
+    n = ex.shape[0]
+    zs = np.ascontiguousarray(zs, dtype=np.float64)
+    ox = np.zeros((zs.size, n, n), dtype=np.complex128)
+    oy = None if ey is None else np.zeros((zs.size, n, n), dtype=np.complex128)
+    w = Warnings()
+    lib().or_volume(_p(ex), _p(ey), n, dx, _p(zs), zs.size, m, C.byref(ctx), _p(ox), _p(oy),
+                    C.byref(w))
+    return ox, oy, w
+
+
+def spectral_radius(ex, ey, dx, frac=0.9):
+    n = ex.shape[0]
+    return lib().or_spectral_radius(_p(_c128(ex)), _p(None if ey is None else _c128(ey)),
+                                    n, dx, frac)
+
+
+def spectral_centroid(ex, ey, dx):
+    fx, fy = C.c_double(), C.c_double()
+    lib().or_spectral_centroid(_p(_c128(ex)), _p(None if ey is None else _c128(ey)),
+                               ex.shape[0], dx, C.byref(fx), C.byref(fy))
+    return fx.value, fy.value
+
+
+def gaussian(n, dx, w0):
+    a = np.zeros((n, n), dtype=np.complex128)
+    lib().or_gaussian(_p(a), n, dx, w0)
+    return a
+
+
+def apply_lens(ex, ey, dx, wl, f):
+    lib().or_apply_lens(_p(ex), _p(ey), ex.shape[0], dx, wl, f)
+
+
+def apply_circle_aperture(ex, ey, dx, radius):
+    lib().or_apply_circle_aperture(_p(ex), _p(ey), ex.shape[0], dx, radius)
+
+
+def hash_noise01(x, y, seed):
+    return lib().or_hash_noise01(x, y, seed)
+
+
+def random_phase(n, seed):
+    a = np.zeros((n, n), dtype=np.complex128)
+    lib().or_random_phase(_p(a), n, seed)
+    return a
+
+
+def schell_sqrt_spectrum(n, dx, sigma_c):
+    s = np.zeros((n, n), dtype=np.complex128)
+    lib().or_schell_sqrt_spectrum(_p(s), n, dx, sigma_c)
+    return s
+
+
+def schell_realization(sqrt_s, dx, w0, seed, r):
+    n = sqrt_s.shape[0]
+    a = np.zeros((n, n), dtype=np.complex128)
+    lib().or_schell_realization(_p(a), _p(_c128(sqrt_s)), n, dx, w0, seed, r)
+    return a
+
+
+def compute_stats(ex, ey, dx, wl, ap=0.0, dist=0.0):
+    s = PlaneStats()
+    lib().or_compute_stats(_p(_c128(ex)), _p(None if ey is None else _c128(ey)), ex.shape[0],
+                           dx, wl, ap, dist, C.byref(s))
+    return s
+
+
+def mask_loss_grad(theta, target, dx, z, ctx, want_grad=True):
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    n = theta.shape[0]
+    g = np.zeros_like(theta) if want_grad else None
+    L = lib().or_mask_loss_grad(_p(theta), _p(target), n, dx, z, C.byref(ctx), _p(g))
+    return L, g
+
+
+# ---- deterministic generators restated from the reference's tests ----
+
+class SimpleRNG:
+    """simpleRNG (optics/fft_test.go:62-68)."""
+
+    def __init__(self, seed):
+        self.seed = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.seed = (self.seed * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return float(self.seed >> 11) / 9007199254740992.0 - 0.5
+
+    def complex_array(self, count):
+        out = np.empty(count, dtype=np.complex128)
+        for i in range(count):
+            re = self.next()
+            im = self.next()
+            out[i] = complex(re, im)
+        return out
+
+
+def ref_field(n, seed):
+    """refField (optics/gpu_test.go:12-22), vectorised."""
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    re = np.sin(seed * (i + 1) * 0.7) + 0.5 * np.cos(seed * (j + 1))
+    im = np.cos(seed * (i - j) * 0.3) - 0.25 * np.sin(seed * (i * j + 1))
+    return (re + 1j * im).astype(np.complex128)
+
+
+def rel_l2(a, b):
+    """relCmpL2 (optics/complex_field_test.go:15-26)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.sum(np.abs(b) ** 2)
+    if den <= 0:
+        return 0.0
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / den))

@@ -1,0 +1,941 @@
+/*
+ * wos_oracle.c — FP64 CPU restatement of the Tetsuhiro-WOS propagation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libwos_hip. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Each function follows the reference Go source cited beside it (paths are
+ * relative to the reference repository). Arithmetic keeps the Go expression
+ * order; the file is compiled with -O2 -ffp-contract=off and without
+ * -ffast-math, which matches the reference's amd64 build (GOAMD64=v1 emits no
+ * FMA). Go evaluates constant sub-expressions such as 2*math.Pi exactly before
+ * rounding; 2.0*M_PI gives the same double because scaling by 2 is exact.
+ */
+#define _GNU_SOURCE
+#include "wos_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+#include <omp.h>
+
+/* ------------------------------------------------------------------ */
+/* complex helpers: Go complex128 semantics (no FMA, same op order)     */
+/* ------------------------------------------------------------------ */
+static inline or_cplx cmul(or_cplx a, or_cplx b) {
+    /* Go ssagen: real = ar*br - ai*bi ; imag = ar*bi + ai*br */
+    or_cplx r;
+    r.re = a.re * b.re - a.im * b.im;
+    r.im = a.re * b.im + a.im * b.re;
+    return r;
+}
+static inline or_cplx cadd(or_cplx a, or_cplx b) { or_cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline or_cplx csub(or_cplx a, or_cplx b) { or_cplx r = {a.re - b.re, a.im - b.im}; return r; }
+static inline or_cplx cre(double x) { or_cplx r = {x, 0.0}; return r; }
+
+/* cexpI (optics/field.go:231-234): math.Sincos */
+static inline or_cplx cexpI(double phi) {
+    double s, c;
+    sincos(phi, &s, &c);
+    or_cplx r = {c, s};
+    return r;
+}
+
+/* Go math.Hypot (both the portable and the amd64 assembly version compute
+ * max * sqrt(1 + (min/max)^2)); the special Inf/NaN cases are kept. */
+double or_hypot(double p, double q) {
+    p = fabs(p);
+    q = fabs(q);
+    if (isinf(p) || isinf(q)) return INFINITY;
+    if (isnan(p) || isnan(q)) return NAN;
+    if (p < q) { double t = p; p = q; q = t; }
+    if (p == 0) return 0;
+    q = q / p;
+    return p * sqrt(1 + q * q);
+}
+
+/* ------------------------------------------------------------------ */
+/* parFor (optics/fft.go:91-122)                                        */
+/* ------------------------------------------------------------------ */
+static int g_threads = 0;  /* 0: all hardware threads */
+
+void or_set_threads(int p) { g_threads = p; }
+int or_get_threads(void) { return g_threads > 0 ? g_threads : omp_get_num_procs(); }
+
+/* ------------------------------------------------------------------ */
+/* FFT plan cache (optics/fft.go:14-51)                                 */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int n, bits;
+    int* rev;
+    or_cplx* tw;  /* tw[k] = exp(-2 pi i k/n), k = 0..n/2 */
+} fft_plan;
+
+static fft_plan* g_plans[40];
+static pthread_mutex_t g_plan_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static fft_plan* plan_fft(int n) {
+    int bits = 0;
+    for (int m = n; m > 1; m >>= 1) bits++;
+    pthread_mutex_lock(&g_plan_mu);
+    fft_plan* p = g_plans[bits];
+    if (!p) {
+        p = (fft_plan*)calloc(1, sizeof(fft_plan));
+        p->n = n;
+        p->bits = bits;
+        p->rev = (int*)malloc(sizeof(int) * (size_t)n);
+        for (int i = 0; i < n; i++) {
+            int r = 0, v = i;
+            for (int b = 0; b < bits; b++) { r = (r << 1) | (v & 1); v >>= 1; }
+            p->rev[i] = r;
+        }
+        int half = n / 2;
+        p->tw = (or_cplx*)malloc(sizeof(or_cplx) * (size_t)(half + 1));
+        for (int k = 0; k <= half; k++) {
+            double ang = (-2.0 * M_PI) * (double)k / (double)n;
+            p->tw[k].re = cos(ang);
+            p->tw[k].im = sin(ang);
+        }
+        g_plans[bits] = p;
+    }
+    pthread_mutex_unlock(&g_plan_mu);
+    return p;
+}
+
+/* fft1D (optics/fft.go:55-87): in-place radix-2 DIT; inverse conjugates the
+ * twiddles and multiplies by complex(1/n, 0). */
+static void fft1d_plan(or_cplx* a, const fft_plan* plan, int inverse) {
+    int n = plan->n;
+    for (int i = 0; i < n; i++) {
+        int j = plan->rev[i];
+        if (i < j) { or_cplx t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (int size = 2; size <= n; size <<= 1) {
+        int half = size / 2;
+        int step = n / size;
+        for (int i = 0; i < n; i += size) {
+            for (int j = 0; j < half; j++) {
+                or_cplx w = plan->tw[j * step];
+                if (inverse) w.im = -w.im;
+                or_cplx u = a[i + j];
+                or_cplx v = cmul(a[i + j + half], w);
+                a[i + j] = cadd(u, v);
+                a[i + j + half] = csub(u, v);
+            }
+        }
+    }
+    if (inverse) {
+        or_cplx inv = cre(1.0 / (double)n);
+        for (int i = 0; i < n; i++) a[i] = cmul(a[i], inv);
+    }
+}
+
+void or_fft1d_radix2(or_cplx* a, int n, int inverse) {
+    if (n <= 1) return;
+    fft1d_plan(a, plan_fft(n), inverse);
+}
+
+/* fft1DBluestein (optics/fft.go:159-204) */
+void or_fft1d_bluestein(or_cplx* a, int n, int inverse) {
+    if (n <= 1) return;
+    int m = 1;
+    while (m < 2 * n - 1) m <<= 1;
+    double sign = inverse ? 1.0 : -1.0;
+    or_cplx* chirp = (or_cplx*)malloc(sizeof(or_cplx) * (size_t)n);
+    for (int k = 0; k < n; k++) {
+        /* sign * math.Pi * float64(k*k) / float64(n) */
+        chirp[k] = cexpI(sign * M_PI * (double)((int64_t)k * k) / (double)n);
+    }
+    or_cplx* a0 = (or_cplx*)calloc((size_t)m, sizeof(or_cplx));
+    or_cplx* b0 = (or_cplx*)calloc((size_t)m, sizeof(or_cplx));
+    for (int k = 0; k < n; k++) a0[k] = cmul(a[k], chirp[k]);
+    b0[0] = cre(1.0);
+    for (int k = 1; k < n; k++) {
+        or_cplx ck = {chirp[k].re, -chirp[k].im};
+        b0[k] = ck;
+        b0[m - k] = ck;
+    }
+    fft_plan* plan = plan_fft(m);
+    fft1d_plan(a0, plan, 0);
+    fft1d_plan(b0, plan, 0);
+    for (int k = 0; k < m; k++) a0[k] = cmul(a0[k], b0[k]);
+    fft1d_plan(a0, plan, 1);
+    for (int k = 0; k < n; k++) a[k] = cmul(a0[k], chirp[k]);
+    if (inverse) {
+        or_cplx inv = cre(1.0 / (double)n);
+        for (int k = 0; k < n; k++) a[k] = cmul(a[k], inv);
+    }
+    free(chirp);
+    free(a0);
+    free(b0);
+}
+
+/* fft1DAnyCPU (optics/fft.go:146-153) */
+void or_fft1d(or_cplx* a, int n, int inverse) {
+    if (n <= 0) return;
+    if ((n & (n - 1)) == 0) {
+        or_fft1d_radix2(a, n, inverse);
+        return;
+    }
+    or_fft1d_bluestein(a, n, inverse);
+}
+
+/* fft2DCPU (optics/fft.go:219-242): rows, then strided columns through a
+ * per-worker scratch buffer; parFor runs serially for n < 64. */
+void or_fft2d(or_cplx* a, int n, int inverse) {
+    int P = or_get_threads();
+    if (P > n) P = n;
+    int par = (P > 1 && n >= 64);
+    int chunk = par ? (n + P - 1) / P : n;
+    if (par) omp_set_num_threads(P);
+#pragma omp parallel for schedule(static, chunk) if (par)
+    for (int r = 0; r < n; r++) or_fft1d(a + (size_t)r * n, n, inverse);
+#pragma omp parallel if (par)
+    {
+        or_cplx* col = (or_cplx*)malloc(sizeof(or_cplx) * (size_t)n);
+#pragma omp for schedule(static, chunk)
+        for (int c = 0; c < n; c++) {
+            for (int r = 0; r < n; r++) col[r] = a[(size_t)r * n + c];
+            or_fft1d(col, n, inverse);
+            for (int r = 0; r < n; r++) a[(size_t)r * n + c] = col[r];
+        }
+        free(col);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Field helpers (optics/field.go)                                      */
+/* ------------------------------------------------------------------ */
+/* freq (optics/field.go:297-303) */
+double or_freq(int i, int n, double dx) {
+    if (i <= n / 2) return (double)i / ((double)n * dx);
+    return (double)(i - n) / ((double)n * dx);
+}
+
+/* X/Y (optics/field.go:59-62) */
+static inline double coord(int i, int n, double dx) { return ((double)i - (double)n / 2) * dx; }
+
+static inline double intensity_at(const or_cplx* ex, const or_cplx* ey, size_t idx) {
+    double p = ex[idx].re * ex[idx].re + ex[idx].im * ex[idx].im;
+    if (ey) p += ey[idx].re * ey[idx].re + ey[idx].im * ey[idx].im;
+    return p;
+}
+
+/* Power (optics/field.go:77-83): serial sum, then * DX * DX */
+double or_power(const or_cplx* ex, const or_cplx* ey, int n, double dx) {
+    double p = 0;
+    size_t nn = (size_t)n * n;
+    for (size_t i = 0; i < nn; i++) p += intensity_at(ex, ey, i);
+    return p * dx * dx;
+}
+
+static void warn_add(or_warnings* w, int code, double value) {
+    if (!w) return;
+    w->count[code]++;
+    if (value != 0) w->value[code] = value;
+}
+
+/* ------------------------------------------------------------------ */
+/* Transfer functions (optics/propagate.go)                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double wl, z, k, az;
+    int zero_ev, reg, backward;
+    double alpha, limit;
+} asm_tf;
+
+/* evanescentPolicy (optics/propagate.go:186-199) */
+static void evanescent_policy(const or_ctx* ctx, double z, int* zero, int* reg, double* alpha,
+                              double* limit) {
+    *limit = ctx->evanescent_limit;
+    if (z < 0) {
+        if (ctx->backward_regularize) {
+            *alpha = ctx->tikhonov_alpha;
+            if (*alpha <= 0) *alpha = 1e-3;
+            *zero = 0;
+            *reg = 1;
+            return;
+        }
+        *zero = 1;
+        *reg = 0;
+        *alpha = 0;
+        return;
+    }
+    *zero = ctx->evanescent_zero != 0;
+    *reg = 0;
+    *alpha = 0;
+}
+
+/* asmTF (optics/propagate.go:159-181) */
+static asm_tf make_asm_tf(double wl, double z, int zero_ev, int reg, double alpha, double limit) {
+    asm_tf t;
+    t.wl = wl;
+    t.z = z;
+    t.k = 2.0 * M_PI / wl;
+    t.az = fabs(z);
+    t.backward = z < 0;
+    t.zero_ev = zero_ev;
+    t.reg = reg;
+    t.alpha = alpha;
+    t.limit = limit;
+    return t;
+}
+
+static inline or_cplx asm_tf_eval(const asm_tf* t, double fx, double fy) {
+    double rho = or_hypot(t->wl * fx, t->wl * fy);
+    if (rho <= 1) return cexpI(t->k * t->z * sqrt((1 - rho) * (1 + rho)));
+    double decay = t->k * t->az * sqrt((rho - 1) * (rho + 1));
+    if (t->zero_ev || (t->limit > 0 && decay > t->limit)) return cre(0);
+    if (t->backward) {
+        if (t->reg) {
+            double a = exp(decay);
+            return cre(a / (1 + t->alpha * t->alpha * a * a));
+        }
+        return cre(exp(decay));
+    }
+    return cre(exp(-decay));
+}
+
+/* asmTF1D (optics/propagate.go:679-701) */
+static inline or_cplx asm_tf1d_eval(const asm_tf* t, double fx) {
+    double rho = fabs(t->wl * fx);
+    if (rho <= 1) return cexpI(t->k * t->z * sqrt((1 - rho) * (1 + rho)));
+    double decay = t->k * t->az * sqrt((rho - 1) * (rho + 1));
+    if (t->zero_ev || (t->limit > 0 && decay > t->limit)) return cre(0);
+    if (t->backward) {
+        if (t->reg) {
+            double a = exp(decay);
+            return cre(a / (1 + t->alpha * t->alpha * a * a));
+        }
+        return cre(exp(decay));
+    }
+    return cre(exp(-decay));
+}
+
+/* asmWarn (optics/propagate.go:203-218) */
+static void asm_warn(or_warnings* w, double z, double p_in, double p_out, int zero_ev, int reg) {
+    if (p_in > 0 && p_out < p_in * (1 - 1e-8) && z > 0) warn_add(w, OR_WARN_EVANESCENT_FILTERED, (p_in - p_out) / p_in);
+    if (z < 0) {
+        if (reg) warn_add(w, OR_WARN_BACKWARD_REGULARIZED, 0);
+        else if (zero_ev) warn_add(w, OR_WARN_BACKWARD_EVANESCENT, 0);
+    }
+}
+
+/* spectralRadius (optics/propagate.go:270-320) */
+double or_spectral_radius(const or_cplx* ex, const or_cplx* ey, int n, double dx, double frac) {
+    if (frac <= 0 || frac >= 1) frac = 0.99;
+    const int bins = 96;
+    double fnyq = 1 / (2 * dx);
+    if (n == 0 || dx <= 0) return 0;
+    double hist[97];
+    memset(hist, 0, sizeof(hist));
+    double tot = 0;
+    size_t nn = (size_t)n * n;
+    or_cplx* a = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    for (int comp = 0; comp < 2; comp++) {
+        const or_cplx* c = comp == 0 ? ex : ey;
+        if (!c) continue;
+        memcpy(a, c, sizeof(or_cplx) * nn);
+        or_fft2d(a, n, 0);
+        for (int j = 0; j < n; j++) {
+            double fy = or_freq(j, n, dx);
+            for (int i = 0; i < n; i++) {
+                or_cplx v = a[(size_t)j * n + i];
+                double wgt = v.re * v.re + v.im * v.im;
+                if (wgt == 0) continue;
+                int b = (int)(or_hypot(or_freq(i, n, dx), fy) / fnyq * bins);
+                if (b > bins) b = bins;
+                hist[b] += wgt;
+                tot += wgt;
+            }
+        }
+    }
+    free(a);
+    if (tot <= 0) return 0;
+    double run = 0.0;
+    for (int b = 0; b <= bins; b++) {
+        run += hist[b];
+        if (run >= frac * tot) return (double)b / bins * fnyq;
+    }
+    return fnyq;
+}
+
+/* asmWrapWarn (optics/propagate.go:247-264) */
+static void asm_wrap_warn(const or_cplx* ex, const or_cplx* ey, int n, double dx, double z,
+                          const or_ctx* ctx, or_warnings* w, double carry) {
+    if (!w || z == 0 || carry <= 0) return;
+    if (fabs(z) * ctx->wavelength / (2 * dx) <= carry) return;
+    double fsig = or_spectral_radius(ex, ey, n, dx, 0.90);
+    if (fsig <= 0) return;
+    double walk = fabs(z) * ctx->wavelength * fsig;
+    if (walk > carry) warn_add(w, OR_WARN_ASM_ALIAS_WRAP, walk / carry);
+}
+
+/* ctx.transfer (optics/propagate.go:132-148) specialised per kernel kind */
+static void transfer_asm(or_cplx* a, int n, double dx, const asm_tf* t) {
+    or_fft2d(a, n, 0);
+    for (int j = 0; j < n; j++) {
+        double fy = or_freq(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            size_t idx = (size_t)j * n + i;
+            a[idx] = cmul(a[idx], asm_tf_eval(t, or_freq(i, n, dx), fy));
+        }
+    }
+    or_fft2d(a, n, 1);
+}
+
+/* propASM (optics/propagate.go:325-332) */
+static void prop_asm(or_cplx* ex, or_cplx* ey, int n, double dx, double z, const or_ctx* ctx,
+                     or_warnings* w, or_stats* st) {
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+    asm_wrap_warn(ex, ey, n, dx, z, ctx, w, (double)n * dx);
+    double p_in = or_power(ex, ey, n, dx);
+    asm_tf t = make_asm_tf(ctx->wavelength, z, zero, reg, alpha, limit);
+    transfer_asm(ex, n, dx, &t);
+    if (ey) transfer_asm(ey, n, dx, &t);
+    double p_out = or_power(ex, ey, n, dx);
+    asm_warn(w, z, p_in, p_out, zero, reg);
+    if (st) { st->p_in = p_in; st->p_out = p_out; }
+}
+
+/* propASMPadCore (optics/propagate.go:346-387) */
+static void prop_asm_pad_core(or_cplx* ex, or_cplx* ey, int n, double dx, double z,
+                              const or_ctx* ctx, or_warnings* w, or_stats* st, double fx_off,
+                              double fy_off) {
+    int m = 2 * n;
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+    asm_tf t = make_asm_tf(ctx->wavelength, z, zero, reg, alpha, limit);
+    asm_wrap_warn(ex, ey, n, dx, z, ctx, w, (double)n * dx);
+    double p_in = or_power(ex, ey, n, dx);
+    size_t mm = (size_t)m * m;
+    or_cplx* buf = (or_cplx*)malloc(sizeof(or_cplx) * mm);
+    int off = n / 2;
+    for (int comp = 0; comp < 2; comp++) {
+        or_cplx* a = comp == 0 ? ex : ey;
+        if (!a) continue;
+        memset(buf, 0, sizeof(or_cplx) * mm);
+        for (int j = 0; j < n; j++)
+            memcpy(buf + (size_t)(j + off) * m + off, a + (size_t)j * n, sizeof(or_cplx) * (size_t)n);
+        or_fft2d(buf, m, 0);
+        for (int j = 0; j < m; j++) {
+            double fy = or_freq(j, m, dx) + fy_off;
+            for (int i = 0; i < m; i++) {
+                size_t idx = (size_t)j * m + i;
+                buf[idx] = cmul(buf[idx], asm_tf_eval(&t, or_freq(i, m, dx) + fx_off, fy));
+            }
+        }
+        or_fft2d(buf, m, 1);
+        for (int j = 0; j < n; j++)
+            memcpy(a + (size_t)j * n, buf + (size_t)(j + off) * m + off, sizeof(or_cplx) * (size_t)n);
+    }
+    free(buf);
+    double p_out = or_power(ex, ey, n, dx);
+    asm_warn(w, z, p_in, p_out, zero, reg);
+    if (st) { st->p_in = p_in; st->p_out = p_out; }
+}
+
+/* spectralCentroid (optics/propagate.go:391-416) */
+void or_spectral_centroid(const or_cplx* ex, const or_cplx* ey, int n, double dx, double* fxo,
+                          double* fyo) {
+    double sx = 0, sy = 0, sw = 0;
+    size_t nn = (size_t)n * n;
+    or_cplx* a = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    for (int comp = 0; comp < 2; comp++) {
+        const or_cplx* c = comp == 0 ? ex : ey;
+        if (!c) continue;
+        memcpy(a, c, sizeof(or_cplx) * nn);
+        or_fft2d(a, n, 0);
+        for (int j = 0; j < n; j++) {
+            double fy = or_freq(j, n, dx);
+            for (int i = 0; i < n; i++) {
+                double fx = or_freq(i, n, dx);
+                or_cplx v = a[(size_t)j * n + i];
+                double wgt = v.re * v.re + v.im * v.im;
+                sw += wgt;
+                sx += wgt * fx;
+                sy += wgt * fy;
+            }
+        }
+    }
+    free(a);
+    if (sw <= 0) { *fxo = 0; *fyo = 0; return; }
+    *fxo = sx / sw;
+    *fyo = sy / sw;
+}
+
+/* the carrier ramp closure of propASMShift/propASMShiftPad
+ * (optics/propagate.go:437-444, 487-494) */
+static void shift_ramp(or_cplx* a, int n, double dx, double fxc, double fyc, double sgn) {
+    for (int j = 0; j < n; j++) {
+        double y = coord(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            size_t idx = (size_t)j * n + i;
+            a[idx] = cmul(a[idx], cexpI(sgn * 2 * M_PI * (fxc * coord(i, n, dx) + fyc * y)));
+        }
+    }
+}
+
+/* propASMShift (optics/propagate.go:426-471) */
+static void prop_asm_shift(or_cplx* ex, or_cplx* ey, int n, double dx, double z, const or_ctx* ctx,
+                           or_warnings* w, or_stats* st) {
+    double fxc, fyc;
+    or_spectral_centroid(ex, ey, n, dx, &fxc, &fyc);
+    if (st) { st->fx_c = fxc; st->fy_c = fyc; }
+    double fbin = 1 / ((double)n * dx);
+    if (fabs(fxc) < 0.5 * fbin && fabs(fyc) < 0.5 * fbin) {
+        if (st) st->method_used = OR_ASM;
+        prop_asm(ex, ey, n, dx, z, ctx, w, st);
+        return;
+    }
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+    asm_tf t = make_asm_tf(ctx->wavelength, z, zero, reg, alpha, limit);
+    asm_wrap_warn(ex, ey, n, dx, z, ctx, w, (double)n * dx);
+    double p_in = or_power(ex, ey, n, dx);
+    shift_ramp(ex, n, dx, fxc, fyc, -1);
+    if (ey) shift_ramp(ey, n, dx, fxc, fyc, -1);
+    for (int comp = 0; comp < 2; comp++) {
+        or_cplx* a = comp == 0 ? ex : ey;
+        if (!a) continue;
+        or_fft2d(a, n, 0);
+        for (int j = 0; j < n; j++) {
+            double fy = or_freq(j, n, dx) + fyc;
+            for (int i = 0; i < n; i++) {
+                size_t idx = (size_t)j * n + i;
+                a[idx] = cmul(a[idx], asm_tf_eval(&t, or_freq(i, n, dx) + fxc, fy));
+            }
+        }
+        or_fft2d(a, n, 1);
+    }
+    shift_ramp(ex, n, dx, fxc, fyc, +1);
+    if (ey) shift_ramp(ey, n, dx, fxc, fyc, +1);
+    double p_out = or_power(ex, ey, n, dx);
+    asm_warn(w, z, p_in, p_out, zero, reg);
+    if (st) { st->p_in = p_in; st->p_out = p_out; st->method_used = OR_ASM_SHIFT; }
+}
+
+/* propASMShiftPad (optics/propagate.go:479-504) */
+static void prop_asm_shift_pad(or_cplx* ex, or_cplx* ey, int n, double dx, double z,
+                               const or_ctx* ctx, or_warnings* w, or_stats* st) {
+    double fxc, fyc;
+    or_spectral_centroid(ex, ey, n, dx, &fxc, &fyc);
+    if (st) { st->fx_c = fxc; st->fy_c = fyc; }
+    double fbin = 1 / ((double)n * dx);
+    if (fabs(fxc) < 0.5 * fbin && fabs(fyc) < 0.5 * fbin) {
+        if (st) st->method_used = OR_ASM_PAD;
+        prop_asm_pad_core(ex, ey, n, dx, z, ctx, w, st, 0, 0);
+        return;
+    }
+    shift_ramp(ex, n, dx, fxc, fyc, -1);
+    if (ey) shift_ramp(ey, n, dx, fxc, fyc, -1);
+    prop_asm_pad_core(ex, ey, n, dx, z, ctx, w, st, fxc, fyc);
+    shift_ramp(ex, n, dx, fxc, fyc, +1);
+    if (ey) shift_ramp(ey, n, dx, fxc, fyc, +1);
+    if (st) st->method_used = OR_ASM_SHIFT_PAD;
+}
+
+/* propFresnelTF (optics/propagate.go:507-519) */
+static void prop_fresnel_tf(or_cplx* ex, or_cplx* ey, int n, double dx, double z,
+                            const or_ctx* ctx, or_warnings* w) {
+    double wl = ctx->wavelength;
+    double k = 2.0 * M_PI / wl;
+    for (int comp = 0; comp < 2; comp++) {
+        or_cplx* a = comp == 0 ? ex : ey;
+        if (!a) continue;
+        or_fft2d(a, n, 0);
+        for (int j = 0; j < n; j++) {
+            double fy = or_freq(j, n, dx);
+            for (int i = 0; i < n; i++) {
+                double fx = or_freq(i, n, dx);
+                or_cplx h = cmul(cexpI(k * z), cexpI(-M_PI * wl * z * (fx * fx + fy * fy)));
+                size_t idx = (size_t)j * n + i;
+                a[idx] = cmul(a[idx], h);
+            }
+        }
+        or_fft2d(a, n, 1);
+    }
+    if (fabs(z) > (double)n * dx * dx / wl) warn_add(w, OR_WARN_FRESNEL_TF_ALIAS, 0);
+}
+
+/* propFresnelIR (optics/propagate.go:533-590) */
+static void prop_fresnel_ir(or_cplx* ex, or_cplx* ey, int n, double dx, double z,
+                            const or_ctx* ctx, or_warnings* w) {
+    double wl = ctx->wavelength;
+    double k = 2.0 * M_PI / wl;
+    int m = 2 * n;
+    size_t mm = (size_t)m * m;
+    or_cplx* h = (or_cplx*)malloc(sizeof(or_cplx) * mm);
+    for (int j = 0; j < m; j++) {
+        double dy = (double)j;
+        if (j > m / 2) dy -= (double)m;
+        double y = dy * dx;
+        for (int i = 0; i < m; i++) {
+            double dxi = (double)i;
+            if (i > m / 2) dxi -= (double)m;
+            double x = dxi * dx;
+            h[(size_t)j * m + i] = cexpI(M_PI * (x * x + y * y) / (wl * z));
+        }
+    }
+    or_fft2d(h, m, 0);
+    or_cplx c0 = {0, -1 / (wl * z)};
+    or_cplx pre = cmul(cmul(c0, cexpI(k * z)), cre(dx * dx));
+    or_cplx* buf = (or_cplx*)malloc(sizeof(or_cplx) * mm);
+    int off = n / 2;
+    for (int comp = 0; comp < 2; comp++) {
+        or_cplx* a = comp == 0 ? ex : ey;
+        if (!a) continue;
+        memset(buf, 0, sizeof(or_cplx) * mm);
+        for (int j = 0; j < n; j++)
+            memcpy(buf + (size_t)(j + off) * m + off, a + (size_t)j * n, sizeof(or_cplx) * (size_t)n);
+        or_fft2d(buf, m, 0);
+        for (size_t i = 0; i < mm; i++) buf[i] = cmul(buf[i], h[i]);
+        or_fft2d(buf, m, 1);
+        for (size_t i = 0; i < mm; i++) buf[i] = cmul(buf[i], pre);
+        for (int j = 0; j < n; j++)
+            memcpy(a + (size_t)j * n, buf + (size_t)(j + off) * m + off, sizeof(or_cplx) * (size_t)n);
+    }
+    free(buf);
+    free(h);
+    if (fabs(z) < (double)n * dx * dx / wl) warn_add(w, OR_WARN_FRESNEL_IR_ALIAS, 0);
+}
+
+/* Propagate (optics/propagate.go:65-102) for the methods on this path. */
+int or_propagate(or_cplx* ex, or_cplx* ey, int n, double dx, double z, int method,
+                 const or_ctx* ctx, or_warnings* w, or_stats* st) {
+    if (st) { memset(st, 0, sizeof(*st)); st->method_used = method; }
+    if (z == 0) return 0;
+    if (!ctx) return 2;
+    switch (method) {
+    case OR_ASM: prop_asm(ex, ey, n, dx, z, ctx, w, st); break;
+    case OR_ASM_PAD: prop_asm_pad_core(ex, ey, n, dx, z, ctx, w, st, 0, 0); break;
+    case OR_ASM_SHIFT: prop_asm_shift(ex, ey, n, dx, z, ctx, w, st); break;
+    case OR_ASM_SHIFT_PAD: prop_asm_shift_pad(ex, ey, n, dx, z, ctx, w, st); break;
+    case OR_FRESNEL_TF: prop_fresnel_tf(ex, ey, n, dx, z, ctx, w); break;
+    case OR_FRESNEL_IR: prop_fresnel_ir(ex, ey, n, dx, z, ctx, w); break;
+    default: return 2;
+    }
+    return 0;
+}
+
+/* Propagate1D (optics/propagate.go:718-738) */
+int or_propagate1d(or_cplx* a, int n, double dx, double z, double wl, const or_ctx* ctx,
+                   or_warnings* w) {
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+    asm_tf t = make_asm_tf(wl, z, zero, reg, alpha, limit);
+    double p_in = 0;
+    for (int i = 0; i < n; i++) p_in += a[i].re * a[i].re + a[i].im * a[i].im;
+    or_fft1d(a, n, 0);
+    for (int i = 0; i < n; i++) a[i] = cmul(a[i], asm_tf1d_eval(&t, or_freq(i, n, dx)));
+    or_fft1d(a, n, 1);
+    double p_out = 0;
+    for (int i = 0; i < n; i++) p_out += a[i].re * a[i].re + a[i].im * a[i].im;
+    if (p_in > 0 && p_out < p_in * (1 - 1e-8) && z > 0) warn_add(w, OR_WARN_EVANESCENT_FILTERED, (p_in - p_out) / p_in);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Split-step BPM (optics/medium.go:22-65)                              */
+/* ------------------------------------------------------------------ */
+static void apply_medium_phase(or_cplx* ex, or_cplx* ey, int n, const or_cplx* index, double k,
+                               double dz) {
+    size_t nn = (size_t)n * n;
+    for (size_t idx = 0; idx < nn; idx++) {
+        or_cplx nv = index[idx];
+        double ph = k * (nv.re - 1) * dz;
+        or_cplx t = cmul(cre(exp(-k * nv.im * dz)), cexpI(ph));
+        ex[idx] = cmul(ex[idx], t);
+        if (ey) ey[idx] = cmul(ey[idx], t);
+    }
+}
+
+int or_split_step(or_cplx* ex, or_cplx* ey, int n, double dx, double z, int steps,
+                  const or_cplx* index, int n_planes, const or_ctx* ctx, or_warnings* w) {
+    if (z == 0) return 0;
+    if (steps <= 0) steps = 1;
+    if (n_planes != 1 && n_planes != steps) return 2;
+    double dz = z / (double)steps;
+    double k = 2.0 * M_PI / ctx->wavelength;
+    size_t nn = (size_t)n * n;
+    for (int s = 0; s < steps; s++) {
+        or_propagate(ex, ey, n, dx, dz / 2, OR_ASM, ctx, w, NULL);
+        apply_medium_phase(ex, ey, n, index + (n_planes == 1 ? 0 : (size_t)s * nn), k, dz);
+        or_propagate(ex, ey, n, dx, dz / 2, OR_ASM, ctx, w, NULL);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Polychromatic and volume (optics/polychromatic.go:18-43,             */
+/* optics/volume.go:8-18)                                               */
+/* ------------------------------------------------------------------ */
+int or_polychromatic(const or_cplx* ex, const or_cplx* ey, int n, double dx, double z,
+                     const double* wavelengths, const double* weights, int nw, int method,
+                     const or_ctx* ctx, double* out, or_warnings* w) {
+    if (nw <= 0) return 2;
+    size_t nn = (size_t)n * n;
+    memset(out, 0, sizeof(double) * nn);
+    or_cplx* gx = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    or_cplx* gy = ey ? (or_cplx*)malloc(sizeof(or_cplx) * nn) : NULL;
+    for (int s = 0; s < nw; s++) {
+        if (wavelengths[s] <= 0) { free(gx); free(gy); return 2; }
+        memcpy(gx, ex, sizeof(or_cplx) * nn);
+        if (gy) memcpy(gy, ey, sizeof(or_cplx) * nn);
+        or_ctx c = *ctx;
+        c.wavelength = wavelengths[s];
+        or_propagate(gx, gy, n, dx, z, method, &c, w, NULL);
+        for (size_t i = 0; i < nn; i++) out[i] += weights[s] * intensity_at(gx, gy, i);
+    }
+    free(gx);
+    free(gy);
+    return 0;
+}
+
+int or_volume(const or_cplx* ex, const or_cplx* ey, int n, double dx, const double* zs, int nz,
+              int method, const or_ctx* ctx, or_cplx* out_ex, or_cplx* out_ey, or_warnings* w) {
+    size_t nn = (size_t)n * n;
+    for (int i = 0; i < nz; i++) {
+        or_cplx* gx = out_ex + (size_t)i * nn;
+        or_cplx* gy = (ey && out_ey) ? out_ey + (size_t)i * nn : NULL;
+        memcpy(gx, ex, sizeof(or_cplx) * nn);
+        if (gy) memcpy(gy, ey, sizeof(or_cplx) * nn);
+        or_propagate(gx, gy, n, dx, zs[i], method, ctx, w, NULL);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Input constructors                                                   */
+/* ------------------------------------------------------------------ */
+/* BuildSource "gaussian" before NormalizePower (optics/source.go:113-125) */
+void or_gaussian(or_cplx* ex, int n, double dx, double w0) {
+    for (int j = 0; j < n; j++) {
+        double dy = coord(j, n, dx) - 0;
+        for (int i = 0; i < n; i++) {
+            double ddx = coord(i, n, dx) - 0;
+            double r2 = ddx * ddx + dy * dy;
+            ex[(size_t)j * n + i] = cre(exp(-r2 / (w0 * w0)));
+        }
+    }
+}
+
+/* lensEl.Apply, no aperture, centred (optics/elements.go:203-220) */
+void or_apply_lens(or_cplx* ex, or_cplx* ey, int n, double dx, double wl, double f) {
+    double k = 2.0 * M_PI / wl;
+    double c = -k / (2 * f);
+    for (int j = 0; j < n; j++) {
+        double dy = coord(j, n, dx) - 0;
+        for (int i = 0; i < n; i++) {
+            double ddx = coord(i, n, dx) - 0;
+            double r2 = ddx * ddx + dy * dy;
+            or_cplx t = cexpI(c * r2);
+            size_t idx = (size_t)j * n + i;
+            ex[idx] = cmul(ex[idx], t);
+            if (ey) ey[idx] = cmul(ey[idx], t);
+        }
+    }
+}
+
+/* apertureEl.Apply with a hard-edged circle (optics/elements.go:288-310,
+ * optics/shape.go:162-170, step at optics/elements.go:143-148) */
+void or_apply_circle_aperture(or_cplx* ex, or_cplx* ey, int n, double dx, double radius) {
+    for (int j = 0; j < n; j++) {
+        double dy = coord(j, n, dx) - 0;
+        for (int i = 0; i < n; i++) {
+            double ddx = coord(i, n, dx) - 0;
+            double d = radius - or_hypot(ddx, dy);
+            or_cplx t = cre(d >= 0 ? 1.0 : 0.0);
+            size_t idx = (size_t)j * n + i;
+            ex[idx] = cmul(ex[idx], t);
+            if (ey) ey[idx] = cmul(ey[idx], t);
+        }
+    }
+}
+
+/* hashNoise01 (optics/elements.go:710-721) */
+double or_hash_noise01(int64_t x, int64_t y, int64_t seed) {
+    uint64_t h = (uint64_t)seed * 0x9E3779B97F4A7C15ULL ^ (uint64_t)x * 0xC2B2AE3D27D4EB4FULL ^
+                 (uint64_t)y * 0x165667B19E3779F9ULL;
+    h ^= h >> 30;
+    h *= 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 27;
+    h *= 0x94D049BB133111EBULL;
+    h ^= h >> 31;
+    return (double)(h >> 11) / (double)(1ULL << 53);
+}
+
+/* One i.i.d. uniform phase per pixel from the diffuser hash, lattice cell
+ * (i - n/2, j - n/2): ex = exp(i * 2pi * u). */
+void or_random_phase(or_cplx* ex, int n, int64_t seed) {
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            double u = or_hash_noise01((int64_t)i - n / 2, (int64_t)j - n / 2, seed);
+            ex[(size_t)j * n + i] = cexpI(2 * M_PI * u);
+        }
+}
+
+/* Counter-based standard normal: two SplitMix64 draws, Box-Muller. */
+static inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+double or_normal(uint64_t seed, uint64_t counter) {
+    uint64_t h1 = splitmix64(seed ^ splitmix64(2 * counter));
+    uint64_t h2 = splitmix64(seed ^ splitmix64(2 * counter + 1));
+    double u1 = ((double)(h1 >> 11) + 0.5) / 9007199254740992.0;
+    double u2 = (double)(h2 >> 11) / 9007199254740992.0;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+/* GenerateSchellRealizations, spectrum part (optics/coherence.go:156-179) */
+void or_schell_sqrt_spectrum(or_cplx* s, int n, double dx, double sigma_c) {
+    for (int j = 0; j < n; j++) {
+        double dy = (j <= n / 2) ? (double)j * dx : (double)(j - n) * dx;
+        for (int i = 0; i < n; i++) {
+            double d = (i <= n / 2) ? (double)i * dx : (double)(i - n) * dx;
+            double mu = exp(-(d * d + dy * dy) / (2 * sigma_c * sigma_c));
+            s[(size_t)j * n + i] = cre(mu);
+        }
+    }
+    or_fft2d(s, n, 0);
+    size_t nn = (size_t)n * n;
+    for (size_t i = 0; i < nn; i++) {
+        double v = s[i].re;
+        if (v < 0) v = 0;
+        s[i] = cre(sqrt(v));
+    }
+}
+
+/* GenerateSchellRealizations, one realisation (optics/coherence.go:181-205);
+ * noise counter for pixel i of realisation r: (r*n*n + i)*2 (+1 for imag). */
+void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx, double w0,
+                           uint64_t seed, uint64_t realization) {
+    size_t nn = (size_t)n * n;
+    const double inv_sqrt2 = 1 / M_SQRT2;
+    for (size_t i = 0; i < nn; i++) {
+        uint64_t c = (realization * (uint64_t)nn + i) * 2;
+        or_cplx v = {or_normal(seed, c) * inv_sqrt2, or_normal(seed, c + 1) * inv_sqrt2};
+        ex[i] = cmul(v, sqrt_s[i]);
+    }
+    or_fft2d(ex, n, 1);
+    for (int j = 0; j < n; j++) {
+        double y = ((double)j - (double)n / 2) * dx;
+        for (int i = 0; i < n; i++) {
+            double x = ((double)i - (double)n / 2) * dx;
+            double I = exp(-2 * (x * x + y * y) / (w0 * w0));
+            size_t idx = (size_t)j * n + i;
+            ex[idx] = cmul(ex[idx], cre(sqrt(I)));
+        }
+    }
+}
+
+/* ComputeStats (optics/metrics.go:31-91), without the phase range. */
+void or_compute_stats(const or_cplx* ex, const or_cplx* ey, int n, double dx, double wl,
+                      double strehl_aperture, double strehl_distance, or_plane_stats* s) {
+    memset(s, 0, sizeof(*s));
+    double p = 0, px = 0, py = 0;
+    s->intensity_min = INFINITY;
+    double peak = -1.0;
+    for (int j = 0; j < n; j++) {
+        double y = coord(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            size_t idx = (size_t)j * n + i;
+            double wv = intensity_at(ex, ey, idx);
+            p += wv;
+            px += wv * coord(i, n, dx);
+            py += wv * y;
+            if (wv < s->intensity_min) s->intensity_min = wv;
+            if (wv > s->intensity_max) s->intensity_max = wv;
+            if (wv > peak) peak = wv;
+        }
+    }
+    s->power = p * dx * dx;
+    s->peak = peak;
+    if (p <= 0) return;
+    s->centroid_x = px / p;
+    s->centroid_y = py / p;
+    double vx = 0, vy = 0;
+    for (int j = 0; j < n; j++) {
+        double y = coord(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            double wv = intensity_at(ex, ey, (size_t)j * n + i);
+            double ddx = coord(i, n, dx) - s->centroid_x;
+            double ddy = y - s->centroid_y;
+            vx += wv * ddx * ddx;
+            vy += wv * ddy * ddy;
+        }
+    }
+    s->rms_x = sqrt(vx / p);
+    s->rms_y = sqrt(vy / p);
+    if (strehl_aperture > 0 && strehl_distance > 0) {
+        double ideal = s->power * M_PI * strehl_aperture * strehl_aperture /
+                       (wl * wl * strehl_distance * strehl_distance);
+        if (ideal > 0) s->strehl = s->peak / ideal;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Phase-mask loss and adjoint gradient (no reference counterpart).     */
+/* Forward: U = A exp(i theta), A = F^-1 H F (propASM's transfer);      */
+/* I = |U|^2, S = sum I, L = sum (I/S - T)^2.                            */
+/* Backward: w = dL/dI = (g - sum g*I/S)/S with g = 2(I/S - T);         */
+/* G = A^H (w U) = F^-1 conj(H) F (w U); dL/dtheta = -2 Im(u0 conj(G)). */
+/* ------------------------------------------------------------------ */
+double or_mask_loss_grad(const double* theta, const double* target, int n, double dx, double z,
+                         const or_ctx* ctx, double* grad) {
+    size_t nn = (size_t)n * n;
+    or_cplx* u = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    for (size_t i = 0; i < nn; i++) u[i] = cexpI(theta[i]);
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+    asm_tf t = make_asm_tf(ctx->wavelength, z, zero, reg, alpha, limit);
+    transfer_asm(u, n, dx, &t);
+    double S = 0;
+    for (size_t i = 0; i < nn; i++) S += u[i].re * u[i].re + u[i].im * u[i].im;
+    double L = 0, gsum = 0;
+    for (size_t i = 0; i < nn; i++) {
+        double I = u[i].re * u[i].re + u[i].im * u[i].im;
+        double d = I / S - target[i];
+        L += d * d;
+        gsum += 2 * d * (I / S);
+    }
+    if (grad) {
+        for (size_t i = 0; i < nn; i++) {
+            double I = u[i].re * u[i].re + u[i].im * u[i].im;
+            double g = 2 * (I / S - target[i]);
+            double wq = (g - gsum) / S;
+            u[i].re *= wq;
+            u[i].im *= wq;
+        }
+        /* A^H = F^-1 conj(H) F */
+        or_fft2d(u, n, 0);
+        for (int j = 0; j < n; j++) {
+            double fy = or_freq(j, n, dx);
+            for (int i = 0; i < n; i++) {
+                size_t idx = (size_t)j * n + i;
+                or_cplx h = asm_tf_eval(&t, or_freq(i, n, dx), fy);
+                h.im = -h.im;
+                u[idx] = cmul(u[idx], h);
+            }
+        }
+        or_fft2d(u, n, 1);
+        for (size_t i = 0; i < nn; i++) {
+            or_cplx u0 = cexpI(theta[i]);
+            /* Im(u0 * conj(G)) = u0.im*G.re - u0.re*G.im */
+            grad[i] = -2 * (u0.im * u[i].re - u0.re * u[i].im);
+        }
+    }
+    free(u);
+    return L;
+}

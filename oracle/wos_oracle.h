@@ -1,0 +1,138 @@
+/*
+ * wos_oracle.h — FP64 CPU restatement of the Tetsuhiro-WOS propagation path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker the HIP library is compared
+ * against. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it; the product library never links or calls it.
+ *
+ * Every function restates the reference's Go code (file:line cited at each
+ * definition in wos_oracle.c) in the same left-to-right IEEE operation order,
+ * compiled with -ffp-contract=off so no multiply-add is fused: the kz*z phase
+ * reaches ~5e5 rad, where one ulp of the argument is ~6e-11 rad.
+ *
+ * Complex arrays are interleaved complex128 (re, im), row-major idx = y*N + x,
+ * exactly the memory layout of a Go []complex128.
+ */
+#ifndef WOS_ORACLE_H
+#define WOS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } or_cplx;
+
+/* Same field order and meaning as wos_ctx in include/wos_hip.h and as the Go
+ * Context (optics/field.go:237-254) fields the path reads. */
+typedef struct {
+    double wavelength;
+    int32_t evanescent_zero;      /* Evanescent == "zero" */
+    int32_t backward_regularize;  /* BackwardRegularize */
+    double evanescent_limit;      /* EvanescentLimit (nepers, 0 = off) */
+    double tikhonov_alpha;        /* TikhonovAlpha (<= 0 -> 1e-3) */
+} or_ctx;
+
+/* Warning codes, same order as WOS_WARN_* in include/wos_hip.h. */
+enum {
+    OR_WARN_EVANESCENT_FILTERED = 0,
+    OR_WARN_BACKWARD_REGULARIZED = 1,
+    OR_WARN_BACKWARD_EVANESCENT = 2,
+    OR_WARN_ASM_ALIAS_WRAP = 3,
+    OR_WARN_FRESNEL_TF_ALIAS = 4,
+    OR_WARN_FRESNEL_IR_ALIAS = 5,
+    OR_WARN_COUNT = 8
+};
+
+/* Warnings.Add semantics (optics/field.go:322-338): count accumulates, value
+ * is replaced when the new value is non-zero. */
+typedef struct {
+    int32_t count[OR_WARN_COUNT];
+    double value[OR_WARN_COUNT];
+} or_warnings;
+
+/* Method codes, same as WOS_METHOD_* in include/wos_hip.h. */
+enum {
+    OR_ASM = 0, OR_ASM_PAD = 1, OR_ASM_SHIFT = 2, OR_ASM_SHIFT_PAD = 3,
+    OR_FRESNEL_TF = 4, OR_FRESNEL_IR = 5
+};
+
+/* Per-call diagnostics of one Propagate (last call for loops). */
+typedef struct {
+    double p_in, p_out;     /* Field.Power before/after (0 if not computed) */
+    double fx_c, fy_c;      /* spectral centroid (shift methods) */
+    int32_t method_used;    /* after the asm_shift -> asm fallback */
+    int32_t pad_;
+} or_stats;
+
+void or_set_threads(int p);            /* GOMAXPROCS stand-in for parFor */
+int or_get_threads(void);
+
+/* optics/fft.go */
+void or_fft1d(or_cplx* a, int n, int inverse);          /* fft1DAnyCPU */
+void or_fft1d_radix2(or_cplx* a, int n, int inverse);   /* fft1D (n pow2) */
+void or_fft1d_bluestein(or_cplx* a, int n, int inverse);/* fft1DBluestein */
+void or_fft2d(or_cplx* a, int n, int inverse);          /* fft2DCPU */
+
+/* optics/field.go */
+double or_freq(int i, int n, double dx);
+double or_power(const or_cplx* ex, const or_cplx* ey, int n, double dx);
+double or_hypot(double p, double q);                    /* Go math.Hypot formula */
+
+/* optics/propagate.go: Propagate for the FFT methods. ey == NULL: scalar. */
+int or_propagate(or_cplx* ex, or_cplx* ey, int n, double dx, double z, int method,
+                 const or_ctx* ctx, or_warnings* w, or_stats* st);
+double or_spectral_radius(const or_cplx* ex, const or_cplx* ey, int n, double dx, double frac);
+void or_spectral_centroid(const or_cplx* ex, const or_cplx* ey, int n, double dx,
+                          double* fx, double* fy);
+int or_propagate1d(or_cplx* a, int n, double dx, double z, double wl,
+                   const or_ctx* ctx, or_warnings* w);
+
+/* optics/medium.go: PropagateSplitStep with the index sampled on the grid.
+ * index holds n_planes planes of n*n complex n(x,y); n_planes == 1 is a
+ * z-invariant medium, n_planes == steps gives plane s to slice s (sampled at
+ * zc = z*s/steps + dz/2 by the caller, as applyMediumPhase does). */
+int or_split_step(or_cplx* ex, or_cplx* ey, int n, double dx, double z, int steps,
+                  const or_cplx* index, int n_planes, const or_ctx* ctx, or_warnings* w);
+
+/* optics/polychromatic.go, optics/volume.go */
+int or_polychromatic(const or_cplx* ex, const or_cplx* ey, int n, double dx, double z,
+                     const double* wavelengths, const double* weights, int nw,
+                     int method, const or_ctx* ctx, double* out, or_warnings* w);
+int or_volume(const or_cplx* ex, const or_cplx* ey, int n, double dx,
+              const double* zs, int nz, int method, const or_ctx* ctx,
+              or_cplx* out_ex, or_cplx* out_ey, or_warnings* w);
+
+/* Input constructors (optics/source.go, optics/elements.go, optics/shape.go). */
+void or_gaussian(or_cplx* ex, int n, double dx, double w0);       /* unit amplitude */
+void or_apply_lens(or_cplx* ex, or_cplx* ey, int n, double dx, double wl, double f);
+void or_apply_circle_aperture(or_cplx* ex, or_cplx* ey, int n, double dx, double radius);
+double or_hash_noise01(int64_t x, int64_t y, int64_t seed);
+void or_random_phase(or_cplx* ex, int n, int64_t seed);           /* exp(i 2pi u(i,j)) */
+
+/* optics/coherence.go with a counter-based normal generator in place of
+ * Go's math/rand stream (that stream is not reproducible without Go). */
+double or_normal(uint64_t seed, uint64_t counter);
+void or_schell_sqrt_spectrum(or_cplx* sqrt_s, int n, double dx, double sigma_c);
+void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx,
+                           double w0, uint64_t seed, uint64_t realization);
+
+/* optics/metrics.go:31-91 (power, peak, centroid, rms, strehl). */
+typedef struct {
+    double power, peak, centroid_x, centroid_y, rms_x, rms_y, strehl;
+    double intensity_min, intensity_max;
+} or_plane_stats;
+void or_compute_stats(const or_cplx* ex, const or_cplx* ey, int n, double dx, double wl,
+                      double strehl_aperture, double strehl_distance, or_plane_stats* s);
+
+/* Phase-mask design (no reference counterpart): forward ASM of exp(i theta),
+ * loss L = sum (I/sum I - T)^2 and its gradient dL/dtheta by the adjoint
+ * (conjugate transfer function). theta and target are n*n; grad may be NULL. */
+double or_mask_loss_grad(const double* theta, const double* target, int n, double dx,
+                         double z, const or_ctx* ctx, double* grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

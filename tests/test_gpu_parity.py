@@ -1,0 +1,185 @@
+"""GPU parity: libwos_hip (through its C ABI) against the FP64 oracle.
+
+Tolerances: FP64 paths match the oracle to rel-L2 <= 1e-12 (the bar of the
+FP64 GPU path); the drop-in FFT max error is also held to the reference's own
+1e-9 GPU-vs-CPU bound (optics/gpu_test.go:62,153) tightened to 1e-12. FP32
+paths are held to rel-L2 <= 1e-5 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def rel_l2(a, b):
+    den = np.sum(np.abs(b) ** 2)
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / den)) if den > 0 else 0.0
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b))))
+
+
+def test_device_is_gfx950(wos):
+    name = wos.device_name()
+    print("device:", name, flush=True)
+    assert "gfx950" in name
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048])
+def test_fft2d_pow2_matches_oracle(wos, oracle, n):
+    """TestGPUFFT2DMatchesCPU (optics/gpu_test.go:44-75) on refField inputs."""
+    src = oracle.ref_field(n, 1.3)
+    cpu = oracle.fft2d(src)
+    gpu = wos.fft2d(src)
+    assert rel_max(gpu, cpu) <= FP64_TOL
+    assert rel_l2(gpu, cpu) <= FP64_TOL
+    back = wos.fft2d(gpu, inverse=True)
+    assert np.max(np.abs(back - src)) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 1024, 4096, 8192])
+def test_fft1d_pow2_matches_oracle(wos, oracle, n):
+    """TestGPUFFT1DMatchesCPU (optics/gpu_test.go:77-108)."""
+    i = np.arange(n)
+    src = np.sin(i * 0.01) + 1j * np.cos(i * 0.017)
+    cpu = oracle.fft1d(src)
+    gpu = wos.fft1d(src)
+    assert rel_max(gpu, cpu) <= FP64_TOL
+    back = wos.fft1d(gpu, inverse=True)
+    assert np.max(np.abs(back - src)) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 512, 2048])
+def test_fft2d_fp32_error_vs_fp64_oracle(wos, oracle, n):
+    src = oracle.ref_field(n, 0.7)
+    cpu = oracle.fft2d(src)
+    gpu = wos.fft2d(src, precision=wos.FP32)
+    err = rel_l2(gpu.astype(np.complex128), cpu)
+    print(f"n={n} FP32 fft2d rel-L2 {err:.3e}", flush=True)
+    assert err <= FP32_TOL
+
+
+def config1_input(oracle):
+    """512^2, dx=2um, 632.8nm Gaussian w0=200um through a thin lens f=50mm."""
+    n, dx, wl = 512, 2e-6, 632.8e-9
+    ex = oracle.gaussian(n, dx, 200e-6)
+    oracle.apply_lens(ex, None, dx, wl, 0.05)
+    return ex, n, dx, wl
+
+
+def test_config1_asm_fp64_parity(wos, oracle):
+    ex, n, dx, wl = config1_input(oracle)
+    ctx_o = oracle.ctx_for(wl)
+    ctx_g = wos.ctx_for(wl)
+    ref, _, wo, so = oracle.propagate(ex, None, dx, 0.05, "asm", ctx_o)
+    out, _, wg, sg = wos.propagate(ex, None, dx, 0.05, "asm", ctx_g)
+    err = rel_l2(out, ref)
+    print(f"config1 asm FP64 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    assert abs(sg.p_in - so.p_in) <= 1e-12 * so.p_in
+    assert abs(sg.p_out - so.p_out) <= 1e-12 * so.p_out
+    assert wg.as_dict().keys() == wo.as_dict().keys()
+
+
+def test_config1_asm_fp32_error(wos, oracle):
+    ex, n, dx, wl = config1_input(oracle)
+    ref, _, _, _ = oracle.propagate(ex, None, dx, 0.05, "asm", oracle.ctx_for(wl))
+    out, _, _, _ = wos.propagate(ex, None, dx, 0.05, "asm", wos.ctx_for(wl), precision=wos.FP32)
+    err = rel_l2(out, ref)
+    print(f"config1 asm FP32 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP32_TOL
+
+
+def gaussian_tilted(oracle, n, width, wl, waist, theta):
+    """buildGaussianTilted without NormalizePower (optics/asm_accuracy_test.go:10-16)."""
+    dx = width / n
+    ex = oracle.gaussian(n, dx, waist)
+    k = 2 * np.pi / wl
+    x = (np.arange(n) - n / 2) * dx
+    ex = ex * np.exp(1j * k * np.sin(theta) * x)[None, :]
+    return ex, dx
+
+
+@pytest.mark.parametrize("method", ["asm", "asm_pad", "asm_shift", "asm_shift_pad", "fresnel_tf", "fresnel_ir"])
+@pytest.mark.parametrize("jones", [False, True])
+def test_methods_fp64_parity(wos, oracle, method, jones):
+    n, wl = 256, 632.8e-9
+    ex, dx = gaussian_tilted(oracle, n, 0.004, wl, 4e-4, 0.004)
+    ey = 0.5 * ex[::-1, :].copy() if jones else None
+    z = 0.05
+    ref_x, ref_y, wo, so = oracle.propagate(ex, ey, dx, z, method, oracle.ctx_for(wl))
+    out_x, out_y, wg, sg = wos.propagate(ex, ey, dx, z, method, wos.ctx_for(wl))
+    err = rel_l2(out_x, ref_x)
+    if jones:
+        err = max(err, rel_l2(out_y, ref_y))
+    print(f"{method} jones={jones} FP64 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    assert sg.method_used == so.method_used
+    assert wg.as_dict().keys() == wo.as_dict().keys()
+    for code, (cnt, val) in wo.as_dict().items():
+        assert wg.as_dict()[code][0] == cnt
+        assert abs(wg.as_dict()[code][1] - val) <= 1e-9 * max(abs(val), 1e-300)
+
+
+@pytest.mark.parametrize("z", [-0.02, 0.02])
+@pytest.mark.parametrize("reg", [False, True])
+def test_evanescent_policies_parity(wos, oracle, z, reg):
+    """Evanescent content (optics/asm_improve_test.go:169-253): forward decay,
+    backward zeroing or Tikhonov."""
+    n, dx, wl = 128, 0.25e-6, 1e-6
+    rng = np.random.default_rng(3)
+    ex = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    zz = z * 1e-4
+    co = oracle.ctx_for(wl, backward_regularize=reg, alpha=1e-3)
+    cg = wos.ctx_for(wl, backward_regularize=reg, alpha=1e-3)
+    ref, _, wo, _ = oracle.propagate(ex, None, dx, zz, "asm", co)
+    out, _, wg, _ = wos.propagate(ex, None, dx, zz, "asm", cg)
+    assert rel_l2(out, ref) <= FP64_TOL
+    assert wg.as_dict().keys() == wo.as_dict().keys()
+
+
+def test_split_step_fp64_parity(wos, oracle):
+    """PropagateSplitStep on a graded-index fibre (config-3 medium, smaller grid)."""
+    n, dx, wl = 128, 0.5e-6, 1.55e-6
+    x = (np.arange(n) - n / 2) * dx
+    X, Y = np.meshgrid(x, x)
+    r = np.hypot(X, Y)
+    nre = np.where(r < 100e-6, 1.45 - 0.01 * (r / 100e-6) ** 2, 1.44)
+    index = nre + 1j * 1e-5
+    ex = oracle.gaussian(n, dx, 10e-6)
+    ref, _, wo = oracle.split_step(ex, None, dx, 200e-6, 40, index, oracle.ctx_for(wl))
+    out, _, wg = wos.split_step(ex, None, dx, 200e-6, 40, index, wos.ctx_for(wl))
+    err = rel_l2(out, ref)
+    print(f"split-step FP64 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    assert wg.as_dict().keys() == wo.as_dict().keys()
+
+
+def test_polychromatic_and_volume_fp64_parity(wos, oracle):
+    n, wl = 128, 632.8e-9
+    ex, dx = gaussian_tilted(oracle, n, 0.004, wl, 4e-4, 0.0)
+    wls = [450e-9, 532e-9, 632.8e-9]
+    wts = [0.5, 1.0, 0.75]
+    ref, _ = oracle.polychromatic(ex, None, dx, 0.05, wls, wts, "asm", oracle.ctx_for(wl))
+    out, _ = wos.polychromatic(ex, None, dx, 0.05, wls, wts, "asm", wos.ctx_for(wl))
+    assert rel_l2(out, ref) <= FP64_TOL
+    zs = [0.0, 0.01, 0.05, -0.02]
+    rv, _, _ = oracle.volume(ex, None, dx, zs, "asm", oracle.ctx_for(wl))
+    gv, _, _ = wos.volume(ex, None, dx, zs, "asm", wos.ctx_for(wl))
+    for i in range(len(zs)):
+        assert rel_l2(gv[i], rv[i]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_propagate1d_fp64_parity(wos, oracle, n):
+    dx, wl = 1e-5, 632.8e-9
+    x = (np.arange(n) - n / 2) * dx
+    a = np.exp(-(x / 3e-4) ** 2) * np.exp(1j * 0.3 * x / dx)
+    ref, wo = oracle.propagate1d(a, dx, 0.3, wl, oracle.ctx_for(wl))
+    out, wg = wos.propagate1d(a, dx, 0.3, wl, wos.ctx_for(wl))
+    assert rel_l2(out, ref) <= FP64_TOL
+    assert wg.as_dict().keys() == wo.as_dict().keys()

@@ -1,0 +1,306 @@
+// wos_fft.hpp — device building blocks of the libwos_hip FFT engine (gfx950).
+//
+// Stockham autosort FFT over a power-of-two length N held by a block of
+// threads. Each thread owns V = 16 complex values (4 butterflies of radix 4,
+// 2 of radix 8, or 1 of radix 16); stages exchange data through LDS, the
+// first stage reads global memory directly and the last stage writes global
+// memory directly, both fully coalesced. Twiddles come from a table
+// tw[m] = exp(-2 pi i m / N) generated on the device in FP64.
+//
+// Stockham stage (radix RAD, sub-transform length Ns, butterfly j):
+//   k = j mod Ns; v[r] = x[j + r N/RAD] * w^(r k), w = exp(dir 2 pi i/(Ns RAD));
+//   v = DFT_RAD(v);  y[(j - k) RAD + k + r Ns] = v[r].
+// After the last stage (Ns RAD = N) the output is in natural order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wos {
+
+template <typename R>
+struct __align__(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename R>
+__device__ __forceinline__ cpx<R> cmulc(cpx<R> a, cpx<R> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.x * s, a.y * s}; }
+
+// multiply by DIR*i (DIR = -1 forward, +1 inverse)
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
+  if constexpr (DIR < 0) return {a.y, -a.x};
+  else return {-a.y, a.x};
+}
+
+// exp(DIR * 2 pi i m / 16) constants
+template <typename R>
+struct W16 {
+  static constexpr double C1 = 0.92387953251128675613;  // cos(pi/8)
+  static constexpr double S1 = 0.38268343236508977173;  // sin(pi/8)
+  static constexpr double H = 0.70710678118654752440;   // sqrt(1/2)
+};
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft2(cpx<R>* v) {
+  cpx<R> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft4(cpx<R>& v0, cpx<R>& v1, cpx<R>& v2, cpx<R>& v3) {
+  cpx<R> t0 = v0 + v2, t1 = v0 - v2, t2 = v1 + v3, t3 = mul_i<DIR>(v1 - v3);
+  v0 = t0 + t2;
+  v2 = t0 - t2;
+  v1 = t1 + t3;
+  v3 = t1 - t3;
+}
+
+// multiply by exp(DIR * 2 pi i m / 8), m = 1, 3
+template <int DIR, int M, typename R>
+__device__ __forceinline__ cpx<R> w8(cpx<R> a) {
+  const R c = (R)W16<R>::H;
+  if constexpr (M == 1) {
+    if constexpr (DIR < 0) return {c * (a.x + a.y), c * (a.y - a.x)};
+    else return {c * (a.x - a.y), c * (a.x + a.y)};
+  } else {  // M == 3
+    if constexpr (DIR < 0) return {c * (a.y - a.x), -c * (a.x + a.y)};
+    else return {-c * (a.x + a.y), c * (a.x - a.y)};
+  }
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft8(cpx<R>* v) {
+  cpx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cpx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  o1 = w8<DIR, 1>(o1);
+  o2 = mul_i<DIR>(o2);
+  o3 = w8<DIR, 3>(o3);
+  v[0] = e0 + o0;
+  v[4] = e0 - o0;
+  v[1] = e1 + o1;
+  v[5] = e1 - o1;
+  v[2] = e2 + o2;
+  v[6] = e2 - o2;
+  v[3] = e3 + o3;
+  v[7] = e3 - o3;
+}
+
+// multiply by exp(DIR * 2 pi i m / 16) for a compile-time m in 0..15
+template <int DIR, int M, typename R>
+__device__ __forceinline__ cpx<R> w16(cpx<R> a) {
+  constexpr int m = M & 15;
+  if constexpr (m == 0) return a;
+  else if constexpr (m == 4) return mul_i<DIR>(a);
+  else if constexpr (m == 8) return {-a.x, -a.y};
+  else if constexpr (m == 12) return mul_i<-DIR>(a);
+  else if constexpr (m == 2) return w8<DIR, 1>(a);
+  else if constexpr (m == 6) return w8<DIR, 3>(a);
+  else {
+    // generic: cos/sin of 2 pi m / 16
+    constexpr double C1 = W16<R>::C1, S1 = W16<R>::S1;
+    constexpr double cs[16] = {1, C1, 0, S1, 0, -S1, 0, -C1, -1, -C1, 0, -S1, 0, S1, 0, C1};
+    constexpr double sn[16] = {0, S1, 0, C1, 1, C1, 0, S1, 0, -S1, 0, -C1, -1, -C1, 0, -S1};
+    const R c = (R)cs[m];
+    const R s = (R)(DIR < 0 ? -sn[m] : sn[m]);
+    return {a.x * c - a.y * s, a.x * s + a.y * c};
+  }
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft16(cpx<R>* v) {
+  // n = 4 n1 + n2: radix-4 over n1 for each n2
+  dft4<DIR>(v[0], v[4], v[8], v[12]);
+  dft4<DIR>(v[1], v[5], v[9], v[13]);
+  dft4<DIR>(v[2], v[6], v[10], v[14]);
+  dft4<DIR>(v[3], v[7], v[11], v[15]);
+  // slot 4 k1 + n2 holds X[k1][n2]; twiddle by W16^(n2 k1)
+  v[5] = w16<DIR, 1>(v[5]);
+  v[6] = w16<DIR, 2>(v[6]);
+  v[7] = w16<DIR, 3>(v[7]);
+  v[9] = w16<DIR, 2>(v[9]);
+  v[10] = w16<DIR, 4>(v[10]);
+  v[11] = w16<DIR, 6>(v[11]);
+  v[13] = w16<DIR, 3>(v[13]);
+  v[14] = w16<DIR, 6>(v[14]);
+  v[15] = w16<DIR, 9>(v[15]);
+  // radix-4 over n2 for each k1: result y[k1 + 4 k2] lands in slot 4 k1 + k2
+  dft4<DIR>(v[0], v[1], v[2], v[3]);
+  dft4<DIR>(v[4], v[5], v[6], v[7]);
+  dft4<DIR>(v[8], v[9], v[10], v[11]);
+  dft4<DIR>(v[12], v[13], v[14], v[15]);
+  // transpose 4x4 so slot k holds y[k]
+  cpx<R> t;
+  t = v[1]; v[1] = v[4]; v[4] = t;
+  t = v[2]; v[2] = v[8]; v[8] = t;
+  t = v[3]; v[3] = v[12]; v[12] = t;
+  t = v[6]; v[6] = v[9]; v[9] = t;
+  t = v[7]; v[7] = v[13]; v[13] = t;
+  t = v[11]; v[11] = v[14]; v[14] = t;
+}
+
+template <int RAD, int DIR, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v) {
+  if constexpr (RAD == 2) dft2<DIR>(v);
+  else if constexpr (RAD == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (RAD == 8) dft8<DIR>(v);
+  else if constexpr (RAD == 16) dft16<DIR>(v);
+}
+
+// ---------------------------------------------------------------------------
+// Radix plan for a power-of-two length N: values per thread V, stage radices.
+// Forward: [16, 16, ..., rem]; inverse: [rem, 16, ..., 16], so the last
+// forward stage and the first inverse stage share the same butterfly map and
+// a fused forward->multiply->inverse keeps the data in registers.
+// ---------------------------------------------------------------------------
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <int N>
+struct Plan {
+  static constexpr int LOG = ilog2(N);
+  static constexpr int V = N >= 16 ? 16 : N;            // values per thread
+  static constexpr int T = N / V;                        // threads per transform
+  static constexpr int N16 = LOG / 4;                    // radix-16 stages
+  static constexpr int REM = 1 << (LOG % 4);             // leftover radix (1: none)
+  static constexpr int S = N16 + (REM > 1 ? 1 : 0);      // number of stages
+  // radix of stage s, forward order
+  static constexpr int fwd_rad(int s) {
+    return N < 16 ? N : (s < N16 ? 16 : REM);
+  }
+  // radix of stage s, inverse order
+  static constexpr int inv_rad(int s) {
+    return N < 16 ? N : (REM > 1 ? (s == 0 ? REM : 16) : 16);
+  }
+  // sub-transform length before stage s
+  static constexpr int ns_fwd(int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= fwd_rad(i);
+    return ns;
+  }
+  static constexpr int ns_inv(int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= inv_rad(i);
+    return ns;
+  }
+};
+
+// LDS padding: one slot every 16 to break power-of-two bank strides.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+constexpr int lds_len(int n) { return n + n / 16 + 1; }
+
+// Twiddle for stage (RAD, Ns) of an N-point transform: w^(r k) with
+// w = exp(DIR 2 pi i /(Ns RAD)) = tw[r k N/(Ns RAD)] (conjugated for DIR>0).
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> twiddle(const cpx<R>* __restrict__ tw, int m) {
+  cpx<R> w = tw[m];
+  if constexpr (DIR > 0) w.y = -w.y;
+  return w;
+}
+
+// Apply twiddles and the radix-RAD DFT to the butterflies a thread owns.
+// v[b*RAD + r] holds x[j_b + r N/RAD], j_b = t + b*T.
+template <typename R, int N, int DIR, int RAD, int NS>
+__device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __restrict__ tw) {
+  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int NB = V / RAD;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if constexpr (NS > 1) {
+      const int j = t + b * T;
+      const int k = j & (NS - 1);
+      constexpr int STRIDE = N / (NS * RAD);
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) v[b * RAD + r] = cmul(v[b * RAD + r], twiddle<DIR>(tw, r * k * STRIDE));
+    }
+    dft<RAD, DIR>(v + b * RAD);
+  }
+}
+
+// Write a stage's outputs to LDS (Stockham positions), element stride ES
+// between consecutive positions of one transform (ES = W for column tiles).
+template <typename R, int N, int RAD, int NS, int ES>
+__device__ __forceinline__ void stage_store_lds(const cpx<R>* v, int t, cpx<R>* lds) {
+  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int NB = V / RAD;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+    const int k = j & (NS - 1);
+    const int base = (j - k) * RAD + k;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) lds[lpad(base + r * NS) * ES] = v[b * RAD + r];
+  }
+}
+
+// Read the next stage's inputs from LDS: v[b*RAD + r] = x[j_b + r N/RAD].
+template <typename R, int N, int RAD, int ES>
+__device__ __forceinline__ void stage_load_lds(cpx<R>* v, int t, const cpx<R>* lds) {
+  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int NB = V / RAD;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[lpad(j + r * (N / RAD)) * ES];
+  }
+}
+
+// Position (within the transform) held by slot (b, r) at the start of the
+// first stage of radix RAD0, and at the end of the last stage of radix RADL.
+template <int N, int RAD>
+__device__ __forceinline__ int slot_pos(int t, int b, int r) {
+  return t + b * Plan<N>::T + r * (N / RAD);
+}
+
+// Run stages [S0, S1) of the forward (DIR<0) or inverse (DIR>0) plan with
+// LDS exchanges between them. On entry v holds the inputs of stage S0 in its
+// butterfly order; on exit v holds the outputs of stage S1-1 in the order
+// "slot (b, r) = position t + b T + r N/RAD_last" (valid for the last stage).
+template <typename R, int N, int DIR, int S, int SEND, int ES>
+__device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
+  if constexpr (S < SEND) {
+    constexpr int RAD = DIR < 0 ? Plan<N>::fwd_rad(S) : Plan<N>::inv_rad(S);
+    constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
+    stage_compute<R, N, DIR, RAD, NS>(v, t, tw);
+    if constexpr (S + 1 < SEND) {
+      constexpr int RADN = DIR < 0 ? Plan<N>::fwd_rad(S + 1) : Plan<N>::inv_rad(S + 1);
+      __syncthreads();  // previous readers of lds are done
+      stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
+      __syncthreads();
+      stage_load_lds<R, N, RADN, ES>(v, t, lds);
+      run_stages<R, N, DIR, S + 1, SEND, ES>(v, t, tw, lds);
+    }
+  }
+}
+
+// First-stage radix (input map) and last-stage radix (output map).
+template <int N, int DIR>
+constexpr int first_rad() { return DIR < 0 ? Plan<N>::fwd_rad(0) : Plan<N>::inv_rad(0); }
+template <int N, int DIR>
+constexpr int last_rad() {
+  return DIR < 0 ? Plan<N>::fwd_rad(Plan<N>::S - 1) : Plan<N>::inv_rad(Plan<N>::S - 1);
+}
+
+// Full N-point transform of the values in v (input map of first_rad, output
+// map of last_rad).
+template <typename R, int N, int DIR, int ES>
+__device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
+  run_stages<R, N, DIR, 0, Plan<N>::S, ES>(v, t, tw, lds);
+}
+
+}  // namespace wos

@@ -1,0 +1,1599 @@
+// wos_hip.hip — C ABI of libwos_hip and the host orchestration behind it.
+//
+// The ABI mirrors what a `-tags hip` cgo file binds in place of the cuFFT
+// backend (optics/gpu.go) and of the FFT-based operators of optics/
+// (Propagate, PropagateSplitStep, PropagatePolychromatic, Propagate3D,
+// Propagate1D). Device work runs on one non-blocking stream of device 0 under
+// a recursive mutex (the gpuExecMu contract, optics/gpu.go:97-100).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wos_hip.h"
+#include "wos_engine.hpp"
+
+namespace wos {
+
+int launch_rows_f64(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_rows_f32(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_cols_f64(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
+int launch_cols_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
+int default_col_w_f64(int N, int cols);
+int default_col_w_f32(int N, int cols);
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int max_pow2_len() { return 8192; }
+
+static size_t csize(int prec) { return prec == WOS_FP64 ? 16 : 8; }
+
+int DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap) return 0;
+  if (p) {
+    (void)hipStreamSynchronize(device().stream);  // no pending work may touch the old block
+    (void)hipFree(p);
+  }
+  p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    p = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return WOS_ERR_NOMEM;
+  }
+  cap = bytes;
+  return 0;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+int Device::ensure_host_pin(size_t bytes) {
+  if (bytes <= host_pin_cap) return 0;
+  if (host_pin) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipHostFree(host_pin);
+  }
+  host_pin = nullptr;
+  host_pin_cap = 0;
+  hipError_t e = hipHostMalloc(&host_pin, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    host_pin = nullptr;
+    set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    return WOS_ERR_NOMEM;
+  }
+  host_pin_cap = bytes;
+  return 0;
+}
+
+int Device::upload_params(void* dst, const void* src, size_t bytes) {
+  int rc;
+  if (pin_ev) WOS_HIP_TRY(hipEventSynchronize(pin_ev));
+  if ((rc = ensure_host_pin(bytes))) return rc;
+  memcpy(host_pin, src, bytes);
+  WOS_HIP_TRY(hipMemcpyAsync(dst, host_pin, bytes, hipMemcpyHostToDevice, stream));
+  if (!pin_ev) WOS_HIP_TRY(hipEventCreateWithFlags(&pin_ev, hipEventDisableTiming));
+  WOS_HIP_TRY(hipEventRecord(pin_ev, stream));
+  return 0;
+}
+
+Device& device() {
+  static Device d;
+  return d;
+}
+
+int ensure_init() {
+  Device& d = device();
+  std::lock_guard<std::recursive_mutex> lk(d.mu);
+  if (d.ready) return 0;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    set_error("no HIP device");
+    return WOS_ERR_NO_DEVICE;
+  }
+  WOS_HIP_TRY(hipSetDevice(0));
+  hipDeviceProp_t prop;
+  WOS_HIP_TRY(hipGetDeviceProperties(&prop, 0));
+  snprintf(d.name, sizeof(d.name), "%s (%s)", prop.name, prop.gcnArchName);
+  WOS_HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+  d.ready = true;
+  return 0;
+}
+
+int get_twiddles(int n, int prec, const void** out) {
+  Device& d = device();
+  long long key = (long long)n * 2 + prec;
+  auto it = d.twiddles.find(key);
+  if (it != d.twiddles.end()) {
+    *out = it->second;
+    return 0;
+  }
+  void* p = nullptr;
+  WOS_HIP_TRY(hipMalloc(&p, (size_t)n * csize(prec)));
+  int bs = 256, gs = (n + bs - 1) / bs;
+  if (prec == WOS_FP64) hipLaunchKernelGGL(k_twiddle<double>, dim3(gs), dim3(bs), 0, d.stream, (cpx<double>*)p, n);
+  else hipLaunchKernelGGL(k_twiddle<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, n);
+  WOS_HIP_TRY(hipGetLastError());
+  d.twiddles[key] = p;
+  *out = p;
+  return 0;
+}
+
+int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s) {
+  return prec == WOS_FP64 ? launch_rows_f64(N, dir, a, s) : launch_rows_f32(N, dir, a, s);
+}
+int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
+  if (W <= 0) W = default_col_w(prec, N, a.cols);
+  return prec == WOS_FP64 ? launch_cols_f64(N, mode, dir, W, a, s) : launch_cols_f32(N, mode, dir, W, a, s);
+}
+int default_col_w(int prec, int N, int cols) {
+  return prec == WOS_FP64 ? default_col_w_f64(N, cols) : default_col_w_f32(N, cols);
+}
+
+// rows handled by one row-pass workgroup (mirrors rows_per_block<N>())
+static int rows_per_block_host(int N) {
+  int V = N >= 16 ? 16 : N;
+  int T = N / V;
+  return T >= 256 ? 1 : 256 / T;
+}
+
+// ---------------------------------------------------------------------------
+// Auxiliary kernels (pointwise, reductions).
+// ---------------------------------------------------------------------------
+__global__ void k_z2c(const cpx<double>* __restrict__ in, cpx<float>* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = {(float)in[i].x, (float)in[i].y};
+}
+__global__ void k_c2z(const cpx<float>* __restrict__ in, cpx<double>* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = {(double)in[i].x, (double)in[i].y};
+}
+
+// Radial spectral histogram of spectralRadius (optics/propagate.go:282-304):
+// bin = int(hypot(fx, fy)/fnyq*96) clamped to 96, weights |F|^2; one
+// 97-bin partial per workgroup.
+template <typename R>
+__global__ void k_spec_hist(const cpx<R>* __restrict__ spec, int n, size_t total, double den, double fnyq,
+                            double* __restrict__ partial) {
+  __shared__ double h[97];
+  for (int i = threadIdx.x; i < 97; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const size_t nn = (size_t)n * n;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    cpx<R> v = spec[idx];
+    double w = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    if (w == 0) continue;
+    size_t rem = idx % nn;
+    int j = (int)(rem / n), i = (int)(rem % n);
+    double fr;
+    {
+#pragma clang fp contract(off)
+      fr = go_hypot(freq_of(i, n, den), freq_of(j, n, den)) / fnyq * 96;
+    }
+    int b = (int)fr;
+    if (b > 96) b = 96;
+    atomicAdd(&h[b], w);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 97; i += blockDim.x) partial[(size_t)blockIdx.x * 97 + i] = h[i];
+}
+
+// Spectral centroid sums of spectralCentroid (optics/propagate.go:391-416)
+// in the reference's serial order (component, row, column),
+// one lane accumulating while the other waves prefetch the next chunk into
+// LDS. The carrier feeds a phase ramp whose result moves by ~4e-12 rel-L2
+// per 1e-15 relative change of the centroid, so the reference's naive serial
+// rounding is reproduced rather than a more accurate tree sum.
+template <typename R>
+__global__ __launch_bounds__(256) void k_spec_centroid_serial(const cpx<R>* __restrict__ spec, int n, int ncomp,
+                                                              double den, double* __restrict__ out3) {
+  constexpr int CH = 2048;
+  __shared__ double fq[8192];
+  __shared__ cpx<R> buf[2][CH];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < n; i += 256) fq[i] = freq_of(i, n, den);
+  const size_t total = (size_t)ncomp * n * n;
+  const size_t nch = (total + CH - 1) / CH;
+  for (int i = tid; i < CH && (size_t)i < total; i += 256) buf[0][i] = spec[i];
+  __syncthreads();
+  double sw = 0, sx = 0, sy = 0;
+  int ci = 0, cj = 0;  // column / row of the next element
+  for (size_t c = 0; c < nch; ++c) {
+    const size_t base = c * CH;
+    const int len = (int)((total - base) < (size_t)CH ? (total - base) : (size_t)CH);
+    if (wave != 0) {
+      const size_t nb = base + CH;
+      if (nb < total) {
+        const int nlen = (int)((total - nb) < (size_t)CH ? (total - nb) : (size_t)CH);
+        for (int i = tid - 64; i < nlen; i += 192) buf[(c + 1) & 1][i] = spec[nb + i];
+      }
+    } else if (lane == 0) {
+      const cpx<R>* b = buf[c & 1];
+      for (int e = 0; e < len; ++e) {
+#pragma clang fp contract(off)
+        const double re = (double)b[e].x, im = (double)b[e].y;
+        const double w = re * re + im * im;
+        sw += w;
+        sx += w * fq[ci];
+        sy += w * fq[cj];
+        if (++ci == n) {
+          ci = 0;
+          if (++cj == n) cj = 0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out3[0] = sw;
+    out3[1] = sx;
+    out3[2] = sy;
+  }
+}
+
+// Carrier ramp of propASMShift (optics/propagate.go:437-444):
+// a *= cexpI(sgn*2*pi*(fxc*X(i) + fyc*y)), phase formed in FP64 in the
+// reference's order.
+template <typename R>
+__global__ void k_ramp(cpx<R>* __restrict__ a, int n, int units, double dx, double fxc, double fyc, double sgn) {
+  const size_t nn = (size_t)n * n, total = nn * units;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    size_t rem = idx % nn;
+    int j = (int)(rem / n), i = (int)(rem % n);
+    double ph;
+    {
+#pragma clang fp contract(off)
+      double y = ((double)j - (double)n / 2) * dx;
+      double x = ((double)i - (double)n / 2) * dx;
+      ph = sgn * 2 * M_PI * (fxc * x + fyc * y);
+    }
+    double s, c;
+    sincos(ph, &s, &c);
+    cpx<double> v = {(double)a[idx].x, (double)a[idx].y};
+    cpx<double> t = {c, s};
+    cpx<double> r;
+    {
+#pragma clang fp contract(off)
+      r = cmul(v, t);
+    }
+    a[idx] = {(R)r.x, (R)r.y};
+  }
+}
+
+// BPM screen table of applyMediumPhase (optics/medium.go:50-65):
+// t = complex(exp(-k*Im n*dz), 0) * cexpI(k*(Re n - 1)*dz), built in FP64.
+__global__ void k_screen_table(const cpx<double>* __restrict__ index, cpx<double>* __restrict__ t, size_t nn,
+                               double k, double dz) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+#pragma clang fp contract(off)
+    cpx<double> nv = index[i];
+    double ph = k * (nv.x - 1) * dz;
+    double e = exp(-k * nv.y * dz);
+    double s, c;
+    sincos(ph, &s, &c);
+    cpx<double> ee = {e, 0.0};
+    cpx<double> cc = {c, s};
+    t[i] = cmul(ee, cc);
+  }
+}
+
+// a[u] *= t (t shared by all units); FP64 multiply, stored in R.
+template <typename R>
+__global__ void k_mul_table(cpx<R>* __restrict__ a, const cpx<double>* __restrict__ t, size_t nn, int units) {
+  const size_t total = nn * units;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    cpx<double> v = {(double)a[idx].x, (double)a[idx].y};
+    cpx<double> r;
+    {
+#pragma clang fp contract(off)
+      r = cmul(v, t[idx % nn]);
+    }
+    a[idx] = {(R)r.x, (R)r.y};
+  }
+}
+
+// out[i] += w * Intensity(i) over the components of one field
+// (optics/polychromatic.go:36-38, optics/field.go:65-74).
+template <typename R>
+__global__ void k_accum_intensity(const cpx<R>* __restrict__ a, int ncomp, size_t nn, double w, double* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+#pragma clang fp contract(off)
+    double p = 0;
+    for (int c = 0; c < ncomp; ++c) {
+      cpx<R> v = a[(size_t)c * nn + i];
+      double re = v.x, im = v.y;
+      if (c == 0) p = re * re + im * im;
+      else p += re * re + im * im;
+    }
+    out[i] += w * p;
+  }
+}
+
+// Fresnel impulse response of propFresnelIR (optics/propagate.go:543-558) on
+// the 2N grid in wrap-around layout: cexpI(pi*(x^2 + y^2)/(wl*z)).
+template <typename R>
+__global__ void k_fresnel_ir_kernel(cpx<R>* __restrict__ h, int m, double dx, double wl, double z) {
+  const size_t total = (size_t)m * m;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    int j = (int)(idx / m), i = (int)(idx % m);
+    double ph;
+    {
+#pragma clang fp contract(off)
+      double dy = (double)j;
+      if (j > m / 2) dy -= (double)m;
+      double y = dy * dx;
+      double dxi = (double)i;
+      if (i > m / 2) dxi -= (double)m;
+      double x = dxi * dx;
+      ph = M_PI * (x * x + y * y) / (wl * z);
+    }
+    double s, c;
+    sincos(ph, &s, &c);
+    h[idx] = {(R)c, (R)s};
+  }
+}
+
+// asmTF1D (optics/propagate.go:679-701) applied to a 1-D spectrum.
+template <typename R>
+__global__ void k_tf1d(cpx<R>* __restrict__ a, int n, double dx, TFUnit u) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    cpx<double> h;
+    {
+#pragma clang fp contract(off)
+      double fx = i <= n / 2 ? (double)i / ((double)n * dx) : (double)(i - n) / ((double)n * dx);
+      double rho = fabs(u.wl * fx);
+      if (rho <= 1) {
+        double s, c;
+        sincos(u.k * u.z * sqrt((1 - rho) * (1 + rho)), &s, &c);
+        h = {c, s};
+      } else {
+        double decay = u.k * u.az * sqrt((rho - 1) * (rho + 1));
+        if (u.zero_ev || (u.limit > 0 && decay > u.limit)) h = {0.0, 0.0};
+        else if (u.backward) {
+          if (u.reg) {
+            double e = exp(decay);
+            h = {e / (1 + u.alpha * u.alpha * e * e), 0.0};
+          } else h = {exp(decay), 0.0};
+        } else h = {exp(-decay), 0.0};
+      }
+    }
+    cpx<double> v = {(double)a[i].x, (double)a[i].y};
+    cpx<double> r;
+    {
+#pragma clang fp contract(off)
+      r = cmul(v, h);
+    }
+    a[i] = {(R)r.x, (R)r.y};
+  }
+}
+
+// Counter-based standard normal (SplitMix64 pair, Box-Muller), the same
+// generator the oracle uses in place of Go's math/rand stream.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double normal_at(uint64_t seed, uint64_t counter) {
+  uint64_t h1 = splitmix64(seed ^ splitmix64(2 * counter));
+  uint64_t h2 = splitmix64(seed ^ splitmix64(2 * counter + 1));
+  double u1 = ((double)(h1 >> 11) + 0.5) / 9007199254740992.0;
+  double u2 = (double)(h2 >> 11) / 9007199254740992.0;
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+// Degree of coherence mu on the wrapped difference grid
+// (optics/coherence.go:156-172), FP64.
+__global__ void k_schell_mu(cpx<double>* __restrict__ s, int n, double dx, double sigma_c) {
+  const size_t nn = (size_t)n * n;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+#pragma clang fp contract(off)
+    int j = (int)(idx / n), i = (int)(idx % n);
+    double dy = (j <= n / 2) ? (double)j * dx : (double)(j - n) * dx;
+    double d = (i <= n / 2) ? (double)i * dx : (double)(i - n) * dx;
+    s[idx] = {exp(-(d * d + dy * dy) / (2 * sigma_c * sigma_c)), 0.0};
+  }
+}
+__global__ void k_schell_sqrt(cpx<double>* __restrict__ s, size_t nn) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+    double v = s[i].x;
+    if (v < 0) v = 0;
+    s[i] = {sqrt(v), 0.0};
+  }
+}
+// complex Gaussian noise / sqrt(2) times sqrt(S) (optics/coherence.go:195-200)
+__global__ void k_schell_noise(cpx<double>* __restrict__ out, const cpx<double>* __restrict__ sq, size_t nn,
+                               uint64_t seed, uint64_t realization) {
+  const double inv_sqrt2 = 0.70710678118654752440;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+#pragma clang fp contract(off)
+    uint64_t c = (realization * (uint64_t)nn + i) * 2;
+    cpx<double> v = {normal_at(seed, c) * inv_sqrt2, normal_at(seed, c + 1) * inv_sqrt2};
+    out[i] = cmul(v, sq[i]);
+  }
+}
+// times sqrt(I), I = exp(-2 r^2 / w0^2) (optics/coherence.go:182-189, 202-204), stored in R
+template <typename R>
+__global__ void k_schell_envelope(const cpx<double>* __restrict__ in, cpx<R>* __restrict__ out, int n, double dx,
+                                  double w0) {
+  const size_t nn = (size_t)n * n;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+#pragma clang fp contract(off)
+    int j = (int)(idx / n), i = (int)(idx % n);
+    double y = ((double)j - (double)n / 2) * dx;
+    double x = ((double)i - (double)n / 2) * dx;
+    double a = sqrt(exp(-2 * (x * x + y * y) / (w0 * w0)));
+    cpx<double> v = cmul(in[idx], cpx<double>{a, 0.0});
+    out[idx] = {(R)v.x, (R)v.y};
+  }
+}
+
+static unsigned grid_for(size_t total, int bs = 256) {
+  size_t g = (total + bs - 1) / bs;
+  if (g > 65536) g = 65536;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers.
+// ---------------------------------------------------------------------------
+struct Neumaier {
+  double s = 0, c = 0;
+  void add(double x) {
+    double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  double value() const { return s + c; }
+};
+
+static void warn_add(wos_warnings* w, int code, double value) {
+  if (!w) return;
+  w->count[code]++;
+  if (value != 0) w->value[code] = value;
+}
+
+// evanescentPolicy (optics/propagate.go:186-199)
+static void evanescent_policy(const wos_ctx& ctx, double z, int* zero, int* reg, double* alpha, double* limit) {
+  *limit = ctx.evanescent_limit;
+  if (z < 0) {
+    if (ctx.backward_regularize) {
+      *alpha = ctx.tikhonov_alpha;
+      if (*alpha <= 0) *alpha = 1e-3;
+      *zero = 0;
+      *reg = 1;
+      return;
+    }
+    *zero = 1;
+    *reg = 0;
+    *alpha = 0;
+    return;
+  }
+  *zero = ctx.evanescent_zero != 0;
+  *reg = 0;
+  *alpha = 0;
+}
+
+static TFUnit make_asm_tf(double wl, double z, const wos_ctx& ctx, int M, double dx, double fxo, double fyo) {
+  TFUnit u;
+  memset(&u, 0, sizeof(u));
+  int zero, reg;
+  double alpha, limit;
+  evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+  u.wl = wl;
+  u.z = z;
+  u.k = 2.0 * M_PI / wl;
+  u.az = std::fabs(z);
+  u.alpha = alpha;
+  u.limit = limit;
+  u.fx_off = fxo;
+  u.fy_off = fyo;
+  u.den = (double)M * dx;
+  u.M = M;
+  u.kind = TF_ASM;
+  u.zero_ev = zero;
+  u.reg = reg;
+  u.backward = z < 0;
+  return u;
+}
+
+// asmWarn (optics/propagate.go:203-218)
+static void asm_warn(wos_warnings* w, double z, double p_in, double p_out, int zero_ev, int reg) {
+  if (p_in > 0 && p_out < p_in * (1 - 1e-8) && z > 0) warn_add(w, WOS_WARN_EVANESCENT_FILTERED, (p_in - p_out) / p_in);
+  if (z < 0) {
+    if (reg) warn_add(w, WOS_WARN_BACKWARD_REGULARIZED, 0);
+    else if (zero_ev) warn_add(w, WOS_WARN_BACKWARD_EVANESCENT, 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident building blocks.
+// ---------------------------------------------------------------------------
+// In-place 2-D transform of `units` n x n fields (power-of-two n).
+static int fft2d_dev(int prec, void* d, int n, int units, int inverse) {
+  Device& dv = device();
+  if (!pow2(n) || n > max_pow2_len()) {
+    set_error("2-D FFT length not supported: " + std::to_string(n));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const void* tw;
+  int rc = get_twiddles(n, prec, &tw);
+  if (rc) return rc;
+  const int dir = inverse ? 1 : -1;
+  const double sc = inverse ? 1.0 / (double)n : 1.0;
+  RowArgs ra;
+  memset(&ra, 0, sizeof(ra));
+  ra.in = d;
+  ra.out = d;
+  ra.in_unit = ra.out_unit = (long long)n * n;
+  ra.in_ld = ra.out_ld = n;
+  ra.rows = n;
+  ra.units = units;
+  ra.in_col0 = 0;
+  ra.in_cols = n;
+  ra.out_col0 = 0;
+  ra.out_cols = n;
+  ra.scale_re = sc;
+  ra.tw = tw;
+  if ((rc = launch_rows(prec, n, dir, ra, dv.stream))) return rc;
+  ColArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  ca.data = d;
+  ca.unit = (long long)n * n;
+  ca.ld = n;
+  ca.cols = n;
+  ca.units = units;
+  ca.in_row0 = 0;
+  ca.rows = n;
+  ca.out_row0 = 0;
+  ca.out_rows = n;
+  ca.scale = sc;
+  ca.tw = tw;
+  return launch_cols(prec, n, 0, dir, 0, ca, dv.stream);
+}
+
+struct TransferOut {
+  std::vector<double> p_in, p_out;  // per field group (sum of |.|^2, no dx^2)
+};
+
+// Enqueue one FFT -> TF -> IFFT step on `units` fields of n x n on stream s:
+// power-of-two grid side M = n, or M = 2n zero-padded with the centre crop.
+// d_tf: device array of per-unit TFUnit; inter: padded intermediate
+// (units * n * M elements) or d itself; post multiplies the output;
+// pw_in / pw_out receive per-workgroup power partials (nullable).
+static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bool pad, const TFUnit* d_tf,
+                            double post_re, double post_im, double* pw_in, double* pw_out, hipStream_t s,
+                            int col_w = 0) {
+  const int M = pad ? 2 * n : n;
+  const int off = pad ? n / 2 : 0;
+  if (!pow2(M) || M > max_pow2_len()) {
+    set_error("transform length not supported: " + std::to_string(M));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const void* tw;
+  int rc = get_twiddles(M, prec, &tw);
+  if (rc) return rc;
+  const long long iunit = pad ? (long long)n * M : (long long)n * n;
+  RowArgs ra;
+  memset(&ra, 0, sizeof(ra));
+  ra.in = d;
+  ra.out = inter;
+  ra.in_unit = (long long)n * n;
+  ra.out_unit = iunit;
+  ra.in_ld = n;
+  ra.out_ld = M;
+  ra.rows = n;
+  ra.units = units;
+  ra.in_col0 = off;
+  ra.in_cols = n;
+  ra.out_col0 = 0;
+  ra.out_cols = M;
+  ra.scale_re = 1.0;
+  ra.pw_in = pw_in;
+  ra.tw = tw;
+  if ((rc = launch_rows(prec, M, -1, ra, s))) return rc;
+
+  ColArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  ca.data = inter;
+  ca.unit = iunit;
+  ca.ld = M;
+  ca.cols = M;
+  ca.units = units;
+  ca.in_row0 = off;
+  ca.rows = n;
+  ca.out_row0 = off;
+  ca.out_rows = n;
+  ca.scale = 1.0;
+  ca.tf = d_tf;
+  ca.tw = tw;
+  if ((rc = launch_cols(prec, M, 1, 0, col_w, ca, s))) return rc;
+
+  RowArgs rb;
+  memset(&rb, 0, sizeof(rb));
+  rb.in = inter;
+  rb.out = d;
+  rb.in_unit = iunit;
+  rb.out_unit = (long long)n * n;
+  rb.in_ld = M;
+  rb.out_ld = n;
+  rb.rows = n;
+  rb.units = units;
+  rb.in_col0 = 0;
+  rb.in_cols = M;
+  rb.out_col0 = off;
+  rb.out_cols = n;
+  const double inv = 1.0 / ((double)M * (double)M);
+  rb.scale_re = post_re * inv;
+  rb.scale_im = post_im * inv;
+  rb.pw_out = pw_out;
+  rb.tw = tw;
+  return launch_rows(prec, M, 1, rb, s);
+}
+
+// Synchronous wrapper on the device stream with per-field power sums.
+static int transfer_dev(int prec, void* d, int n, int units, bool pad, const std::vector<TFUnit>& tf, int group,
+                        double post_re, double post_im, TransferOut* pw) {
+  Device& dv = device();
+  const int M = pad ? 2 * n : n;
+  int rc;
+  void* inter = d;
+  if (pad) {
+    if ((rc = dv.scratch.ensure((size_t)units * n * M * csize(prec)))) return rc;
+    inter = dv.scratch.p;
+  }
+  if ((rc = dv.params.ensure(sizeof(TFUnit) * units))) return rc;
+  if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * units))) return rc;
+  const int rpb = rows_per_block_host(M);
+  const int bpu = (n + rpb - 1) / rpb;
+  const int nblocks = bpu * units;
+  if (pw) {
+    if ((rc = dv.partial_in.ensure(sizeof(double) * nblocks))) return rc;
+    if ((rc = dv.partial_out.ensure(sizeof(double) * nblocks))) return rc;
+  }
+  if ((rc = enqueue_transfer(prec, d, inter, n, units, pad, (const TFUnit*)dv.params.p, post_re, post_im,
+                             pw ? (double*)dv.partial_in.p : nullptr, pw ? (double*)dv.partial_out.p : nullptr,
+                             dv.stream)))
+    return rc;
+  if (pw) {
+    std::vector<double> hin(nblocks), hout(nblocks);
+    WOS_HIP_TRY(hipMemcpyAsync(hin.data(), dv.partial_in.p, sizeof(double) * nblocks, hipMemcpyDeviceToHost, dv.stream));
+    WOS_HIP_TRY(hipMemcpyAsync(hout.data(), dv.partial_out.p, sizeof(double) * nblocks, hipMemcpyDeviceToHost, dv.stream));
+    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+    const int groups = units / group;
+    pw->p_in.assign(groups, 0);
+    pw->p_out.assign(groups, 0);
+    for (int g = 0; g < groups; ++g) {
+      Neumaier si, so;
+      for (int b = g * group * bpu; b < (g + 1) * group * bpu; ++b) {
+        si.add(hin[b]);
+        so.add(hout[b]);
+      }
+      pw->p_in[g] = si.value();
+      pw->p_out[g] = so.value();
+    }
+  }
+  return 0;
+}
+
+// spectralRadius (optics/propagate.go:270-320) of one field (ncomp
+// components at d), computed from a device FFT of a copy.
+static int spectral_radius_dev(int prec, const void* d, int n, int ncomp, double dx, double frac, double* out) {
+  Device& dv = device();
+  if (frac <= 0 || frac >= 1) frac = 0.99;
+  const double fnyq = 1 / (2 * dx);
+  *out = 0;
+  if (n == 0 || dx <= 0) return 0;
+  int rc;
+  const size_t bytes = (size_t)ncomp * n * n * csize(prec);
+  if ((rc = dv.aux.ensure(bytes))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(dv.aux.p, d, bytes, hipMemcpyDeviceToDevice, dv.stream));
+  if ((rc = fft2d_dev(prec, dv.aux.p, n, ncomp, 0))) return rc;
+  const size_t total = (size_t)ncomp * n * n;
+  const unsigned gs = std::min<unsigned>(grid_for(total), 1024);
+  if ((rc = dv.hist.ensure(sizeof(double) * 97 * gs))) return rc;
+  const double den = (double)n * dx;
+  if (prec == WOS_FP64)
+    hipLaunchKernelGGL(k_spec_hist<double>, dim3(gs), dim3(256), 0, dv.stream, (const cpx<double>*)dv.aux.p, n, total, den, fnyq, (double*)dv.hist.p);
+  else
+    hipLaunchKernelGGL(k_spec_hist<float>, dim3(gs), dim3(256), 0, dv.stream, (const cpx<float>*)dv.aux.p, n, total, den, fnyq, (double*)dv.hist.p);
+  WOS_HIP_TRY(hipGetLastError());
+  std::vector<double> h((size_t)97 * gs);
+  WOS_HIP_TRY(hipMemcpyAsync(h.data(), dv.hist.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  double hist[97];
+  double tot = 0;
+  for (int b = 0; b < 97; ++b) {
+    Neumaier s;
+    for (unsigned g = 0; g < gs; ++g) s.add(h[(size_t)g * 97 + b]);
+    hist[b] = s.value();
+    tot += hist[b];
+  }
+  if (tot <= 0) return 0;
+  double run = 0.0;
+  for (int b = 0; b <= 96; ++b) {
+    run += hist[b];
+    if (run >= frac * tot) {
+      *out = (double)b / 96 * fnyq;
+      return 0;
+    }
+  }
+  *out = fnyq;
+  return 0;
+}
+
+// asmWrapWarn (optics/propagate.go:247-264)
+static int asm_wrap_warn_dev(int prec, const void* d, int n, int ncomp, double dx, double z, double wl,
+                             wos_warnings* w, double carry) {
+  if (!w || z == 0 || carry <= 0) return 0;
+  if (std::fabs(z) * wl / (2 * dx) <= carry) return 0;
+  double fsig = 0;
+  int rc = spectral_radius_dev(prec, d, n, ncomp, dx, 0.90, &fsig);
+  if (rc) return rc;
+  if (fsig <= 0) return 0;
+  double walk = std::fabs(z) * wl * fsig;
+  if (walk > carry) warn_add(w, WOS_WARN_ASM_ALIAS_WRAP, walk / carry);
+  return 0;
+}
+
+// spectralCentroid (optics/propagate.go:391-416), summed in the
+// reference's serial order.
+static int spectral_centroid_dev(int prec, const void* d, int n, int ncomp, double dx, double* fx, double* fy) {
+  Device& dv = device();
+  int rc;
+  if (n > 8192) {
+    set_error("spectral centroid: grid too large");
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const size_t bytes = (size_t)ncomp * n * n * csize(prec);
+  if ((rc = dv.aux.ensure(bytes))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(dv.aux.p, d, bytes, hipMemcpyDeviceToDevice, dv.stream));
+  if ((rc = fft2d_dev(prec, dv.aux.p, n, ncomp, 0))) return rc;
+  if ((rc = dv.hist.ensure(sizeof(double) * 3))) return rc;
+  const double den = (double)n * dx;
+  if (prec == WOS_FP64)
+    hipLaunchKernelGGL(k_spec_centroid_serial<double>, dim3(1), dim3(256), 0, dv.stream, (const cpx<double>*)dv.aux.p, n,
+                       ncomp, den, (double*)dv.hist.p);
+  else
+    hipLaunchKernelGGL(k_spec_centroid_serial<float>, dim3(1), dim3(256), 0, dv.stream, (const cpx<float>*)dv.aux.p, n,
+                       ncomp, den, (double*)dv.hist.p);
+  WOS_HIP_TRY(hipGetLastError());
+  double h[3];
+  WOS_HIP_TRY(hipMemcpyAsync(h, dv.hist.p, sizeof(h), hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  if (h[0] <= 0) {
+    *fx = 0;
+    *fy = 0;
+    return 0;
+  }
+  *fx = h[1] / h[0];
+  *fy = h[2] / h[0];
+  return 0;
+}
+
+static int ramp_dev(int prec, void* d, int n, int units, double dx, double fxc, double fyc, double sgn) {
+  Device& dv = device();
+  const size_t total = (size_t)units * n * n;
+  if (prec == WOS_FP64)
+    hipLaunchKernelGGL(k_ramp<double>, dim3(grid_for(total)), dim3(256), 0, dv.stream, (cpx<double>*)d, n, units, dx, fxc, fyc, sgn);
+  else
+    hipLaunchKernelGGL(k_ramp<float>, dim3(grid_for(total)), dim3(256), 0, dv.stream, (cpx<float>*)d, n, units, dx, fxc, fyc, sgn);
+  WOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Propagate (optics/propagate.go:65-102) of one field (ncomp components,
+// device resident) by one FFT method.
+static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double z, int method, const wos_ctx& ctx,
+                         wos_warnings* w, wos_prop_stats* st) {
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->method_used = method;
+  }
+  if (z == 0) return 0;
+  const double wl = ctx.wavelength;
+  int rc;
+  int zero, reg;
+  double alpha, limit;
+  evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+  const double carry = (double)n * dx;
+  switch (method) {
+    case WOS_METHOD_ASM:
+    case WOS_METHOD_ASM_PAD: {
+      const bool pad = method == WOS_METHOD_ASM_PAD;
+      const int M = pad ? 2 * n : n;
+      if ((rc = asm_wrap_warn_dev(prec, d, n, ncomp, dx, z, wl, w, carry))) return rc;
+      std::vector<TFUnit> tf(ncomp, make_asm_tf(wl, z, ctx, M, dx, 0, 0));
+      TransferOut pw;
+      if ((rc = transfer_dev(prec, d, n, ncomp, pad, tf, ncomp, 1.0, 0.0, &pw))) return rc;
+      const double p_in = pw.p_in[0] * dx * dx, p_out = pw.p_out[0] * dx * dx;
+      asm_warn(w, z, p_in, p_out, zero, reg);
+      if (st) {
+        st->p_in = p_in;
+        st->p_out = p_out;
+      }
+      return 0;
+    }
+    case WOS_METHOD_ASM_SHIFT:
+    case WOS_METHOD_ASM_SHIFT_PAD: {
+      const bool pad = method == WOS_METHOD_ASM_SHIFT_PAD;
+      double fxc, fyc;
+      if ((rc = spectral_centroid_dev(prec, d, n, ncomp, dx, &fxc, &fyc))) return rc;
+      if (st) {
+        st->fx_c = fxc;
+        st->fy_c = fyc;
+      }
+      const double fbin = 1 / ((double)n * dx);
+      if (std::fabs(fxc) < 0.5 * fbin && std::fabs(fyc) < 0.5 * fbin) {
+        const int fallback = pad ? WOS_METHOD_ASM_PAD : WOS_METHOD_ASM;
+        wos_prop_stats inner;
+        rc = propagate_dev(prec, d, n, ncomp, dx, z, fallback, ctx, w, &inner);
+        if (st) {
+          st->p_in = inner.p_in;
+          st->p_out = inner.p_out;
+          st->method_used = fallback;
+        }
+        return rc;
+      }
+      const int M = pad ? 2 * n : n;
+      // propASMShift checks wrap on the unshifted field; propASMShiftPad on the shifted one
+      if (!pad && (rc = asm_wrap_warn_dev(prec, d, n, ncomp, dx, z, wl, w, carry))) return rc;
+      if ((rc = ramp_dev(prec, d, n, ncomp, dx, fxc, fyc, -1))) return rc;
+      if (pad && (rc = asm_wrap_warn_dev(prec, d, n, ncomp, dx, z, wl, w, carry))) return rc;
+      std::vector<TFUnit> tf(ncomp, make_asm_tf(wl, z, ctx, M, dx, fxc, fyc));
+      TransferOut pw;
+      if ((rc = transfer_dev(prec, d, n, ncomp, pad, tf, ncomp, 1.0, 0.0, &pw))) return rc;
+      if ((rc = ramp_dev(prec, d, n, ncomp, dx, fxc, fyc, +1))) return rc;
+      const double p_in = pw.p_in[0] * dx * dx, p_out = pw.p_out[0] * dx * dx;
+      asm_warn(w, z, p_in, p_out, zero, reg);
+      if (st) {
+        st->p_in = p_in;
+        st->p_out = p_out;
+      }
+      return 0;
+    }
+    case WOS_METHOD_FRESNEL_TF: {
+      const double k = 2.0 * M_PI / wl;
+      TFUnit u = make_asm_tf(wl, z, ctx, n, dx, 0, 0);
+      u.kind = TF_FRESNEL;
+      u.fr_c = -M_PI * wl * z;
+      double s, c;
+      sincos(k * z, &s, &c);
+      u.pre_re = c;
+      u.pre_im = s;
+      std::vector<TFUnit> tf(ncomp, u);
+      if ((rc = transfer_dev(prec, d, n, ncomp, false, tf, ncomp, 1.0, 0.0, nullptr))) return rc;
+      if (std::fabs(z) > (double)n * dx * dx / wl) warn_add(w, WOS_WARN_FRESNEL_TF_ALIAS, 0);
+      return 0;
+    }
+    case WOS_METHOD_FRESNEL_IR: {
+      Device& dv = device();
+      const double k = 2.0 * M_PI / wl;
+      const int m = 2 * n;
+      if (!pow2(m) || m > max_pow2_len()) {
+        set_error("fresnel_ir transform length not supported");
+        return WOS_ERR_UNSUPPORTED;
+      }
+      // impulse-response spectrum (optics/propagate.go:543-559) in its own buffer
+      const size_t hb = (size_t)m * m * csize(prec);
+      DevBuf& ir = dv.ir;
+      if ((rc = ir.ensure(hb))) return rc;
+      const size_t total = (size_t)m * m;
+      if (prec == WOS_FP64)
+        hipLaunchKernelGGL(k_fresnel_ir_kernel<double>, dim3(grid_for(total)), dim3(256), 0, dv.stream, (cpx<double>*)ir.p, m, dx, wl, z);
+      else
+        hipLaunchKernelGGL(k_fresnel_ir_kernel<float>, dim3(grid_for(total)), dim3(256), 0, dv.stream, (cpx<float>*)ir.p, m, dx, wl, z);
+      WOS_HIP_TRY(hipGetLastError());
+      if ((rc = fft2d_dev(prec, ir.p, m, 1, 0))) return rc;
+      TFUnit u = make_asm_tf(wl, z, ctx, m, dx, 0, 0);
+      u.kind = TF_TABLE;
+      u.table = ir.p;
+      // pre = complex(0, -1/(wl*z)) * cexpI(k*z) * complex(dx*dx, 0)
+      double s, c;
+      sincos(k * z, &s, &c);
+      const double b = -1 / (wl * z);
+      double p1r = 0 * c - b * s, p1i = 0 * s + b * c;
+      double d2 = dx * dx;
+      double pr = p1r * d2 - p1i * 0, pi = p1r * 0 + p1i * d2;
+      std::vector<TFUnit> tf(ncomp, u);
+      if ((rc = transfer_dev(prec, d, n, ncomp, true, tf, ncomp, pr, pi, nullptr))) return rc;
+      if (std::fabs(z) < (double)n * dx * dx / wl) warn_add(w, WOS_WARN_FRESNEL_IR_ALIAS, 0);
+      return 0;
+    }
+    default:
+      set_error("unknown propagation method");
+      return WOS_ERR_INVALID;
+  }
+}
+
+// Upload host complex128 arrays into the field buffer in `prec`.
+static int upload_fields(int prec, const double* const* src, int ncomp, size_t nn, void** dptr) {
+  Device& dv = device();
+  int rc;
+  if ((rc = dv.field.ensure((size_t)ncomp * nn * csize(prec)))) return rc;
+  if (prec == WOS_FP64) {
+    for (int c = 0; c < ncomp; ++c)
+      WOS_HIP_TRY(hipMemcpyAsync((char*)dv.field.p + (size_t)c * nn * 16, src[c], nn * 16, hipMemcpyHostToDevice, dv.stream));
+  } else {
+    if ((rc = dv.stage.ensure((size_t)ncomp * nn * 16))) return rc;
+    for (int c = 0; c < ncomp; ++c)
+      WOS_HIP_TRY(hipMemcpyAsync((char*)dv.stage.p + (size_t)c * nn * 16, src[c], nn * 16, hipMemcpyHostToDevice, dv.stream));
+    hipLaunchKernelGGL(k_z2c, dim3(grid_for(ncomp * nn)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.stage.p,
+                       (cpx<float>*)dv.field.p, ncomp * nn);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  *dptr = dv.field.p;
+  return 0;
+}
+
+static int download_fields(int prec, const void* d, double* const* dst, int ncomp, size_t nn) {
+  Device& dv = device();
+  int rc;
+  if (prec == WOS_FP64) {
+    for (int c = 0; c < ncomp; ++c)
+      WOS_HIP_TRY(hipMemcpyAsync(dst[c], (const char*)d + (size_t)c * nn * 16, nn * 16, hipMemcpyDeviceToHost, dv.stream));
+  } else {
+    if ((rc = dv.stage.ensure((size_t)ncomp * nn * 16))) return rc;
+    hipLaunchKernelGGL(k_c2z, dim3(grid_for(ncomp * nn)), dim3(256), 0, dv.stream, (const cpx<float>*)d,
+                       (cpx<double>*)dv.stage.p, ncomp * nn);
+    WOS_HIP_TRY(hipGetLastError());
+    for (int c = 0; c < ncomp; ++c)
+      WOS_HIP_TRY(hipMemcpyAsync(dst[c], (const char*)dv.stage.p + (size_t)c * nn * 16, nn * 16, hipMemcpyDeviceToHost, dv.stream));
+  }
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return 0;
+}
+
+static bool valid_prec(int p) { return p == WOS_FP64 || p == WOS_FP32; }
+
+}  // namespace wos
+
+using namespace wos;
+
+#define WOS_ENTER()                                       \
+  int rc_ = ensure_init();                                \
+  if (rc_) return rc_;                                    \
+  std::lock_guard<std::recursive_mutex> lk_(device().mu); \
+  (void)hipSetDevice(device().id)
+
+extern "C" {
+
+int wos_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int wos_hip_device_name(char* buf, int len) {
+  WOS_ENTER();
+  if (!buf || len <= 0) {
+    set_error("wos_hip_device_name: empty buffer");
+    return WOS_ERR_INVALID;
+  }
+  snprintf(buf, (size_t)len, "%s", device().name);
+  return WOS_OK;
+}
+
+int wos_hip_init(void) { return ensure_init(); }
+
+const char* wos_hip_last_error(void) { return g_err.c_str(); }
+
+const char* wos_hip_version(void) { return "libwos_hip 0.1 (gfx950, Stockham radix-16/8/4/2, LDS-staged)"; }
+
+int wos_hip_release_cache(void) {
+  WOS_ENTER();
+  Device& d = device();
+  (void)hipStreamSynchronize(d.stream);
+  for (auto& kv : d.twiddles) (void)hipFree(kv.second);
+  d.twiddles.clear();
+  for (auto& kv : d.chirps) (void)hipFree(kv.second);
+  d.chirps.clear();
+  for (DevBuf* b : {&d.field, &d.stage, &d.work, &d.aux, &d.scratch, &d.ir, &d.screens, &d.params, &d.partial_in,
+                    &d.partial_out, &d.hist, &d.out})
+    b->release();
+  return WOS_OK;
+}
+
+static int fft2d_host(int prec, void* a, int n, int inverse) {
+  if (!a || n <= 0) {
+    set_error("fft2d: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  const size_t bytes = nn * csize(prec);
+  int rc;
+  if ((rc = dv.field.ensure(bytes))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(dv.field.p, a, bytes, hipMemcpyHostToDevice, dv.stream));
+  if ((rc = fft2d_dev(prec, dv.field.p, n, 1, inverse))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(a, dv.field.p, bytes, hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
+static int fft1d_host(int prec, void* a, int n, int inverse) {
+  if (!a || n <= 0) {
+    set_error("fft1d: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (n == 1) return WOS_OK;
+  Device& dv = device();
+  if (!pow2(n) || n > max_pow2_len()) {
+    set_error("1-D FFT length not supported: " + std::to_string(n));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const size_t bytes = (size_t)n * csize(prec);
+  int rc;
+  if ((rc = dv.field.ensure(bytes))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(dv.field.p, a, bytes, hipMemcpyHostToDevice, dv.stream));
+  const void* tw;
+  if ((rc = get_twiddles(n, prec, &tw))) return rc;
+  RowArgs ra;
+  memset(&ra, 0, sizeof(ra));
+  ra.in = dv.field.p;
+  ra.out = dv.field.p;
+  ra.in_unit = ra.out_unit = n;
+  ra.in_ld = ra.out_ld = n;
+  ra.rows = 1;
+  ra.units = 1;
+  ra.in_cols = n;
+  ra.out_cols = n;
+  ra.scale_re = inverse ? 1.0 / (double)n : 1.0;
+  ra.tw = tw;
+  if ((rc = launch_rows(prec, n, inverse ? 1 : -1, ra, dv.stream))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(a, dv.field.p, bytes, hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
+int wos_hip_fft2d_z(double* a, int n, int inverse) {
+  WOS_ENTER();
+  return fft2d_host(WOS_FP64, a, n, inverse);
+}
+int wos_hip_fft2d_c(float* a, int n, int inverse) {
+  WOS_ENTER();
+  return fft2d_host(WOS_FP32, a, n, inverse);
+}
+int wos_hip_fft1d_z(double* a, int n, int inverse) {
+  WOS_ENTER();
+  return fft1d_host(WOS_FP64, a, n, inverse);
+}
+int wos_hip_fft1d_c(float* a, int n, int inverse) {
+  WOS_ENTER();
+  return fft1d_host(WOS_FP32, a, n, inverse);
+}
+
+int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int method, const wos_ctx* ctx,
+                      int precision, wos_warnings* w, wos_prop_stats* st) {
+  WOS_ENTER();
+  if (!ex || n <= 0 || !ctx || !valid_prec(precision) || !(dx > 0)) {
+    set_error("wos_hip_propagate: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->method_used = method;
+  }
+  if (z == 0) return WOS_OK;
+  const int ncomp = ey ? 2 : 1;
+  const size_t nn = (size_t)n * n;
+  const double* src[2] = {ex, ey};
+  double* dst[2] = {ex, ey};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
+  if ((rc = propagate_dev(precision, d, n, ncomp, dx, z, method, *ctx, w, st))) return rc;
+  return download_fields(precision, d, dst, ncomp, nn);
+}
+
+
+// Propagate1D (optics/propagate.go:718-738)
+int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const wos_ctx* ctx, int precision,
+                        wos_warnings* w) {
+  WOS_ENTER();
+  if (!a || n <= 0 || !ctx || !valid_prec(precision)) {
+    set_error("wos_hip_propagate1d: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (!pow2(n) || n > max_pow2_len()) {
+    set_error("1-D propagation length not supported: " + std::to_string(n));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  Device& dv = device();
+  const double* src[1] = {a};
+  double* dst[1] = {a};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, 1, (size_t)n, &d))) return rc;
+  const void* tw;
+  if ((rc = get_twiddles(n, precision, &tw))) return rc;
+  if ((rc = dv.partial_in.ensure(sizeof(double)))) return rc;
+  if ((rc = dv.partial_out.ensure(sizeof(double)))) return rc;
+  RowArgs ra;
+  memset(&ra, 0, sizeof(ra));
+  ra.in = d;
+  ra.out = d;
+  ra.in_unit = ra.out_unit = n;
+  ra.in_ld = ra.out_ld = n;
+  ra.rows = 1;
+  ra.units = 1;
+  ra.in_cols = n;
+  ra.out_cols = n;
+  ra.scale_re = 1.0;
+  ra.pw_in = (double*)dv.partial_in.p;
+  ra.tw = tw;
+  if ((rc = launch_rows(precision, n, -1, ra, dv.stream))) return rc;
+  TFUnit u = make_asm_tf(wl, z, *ctx, n, dx, 0, 0);
+  if (precision == WOS_FP64)
+    hipLaunchKernelGGL(k_tf1d<double>, dim3(grid_for(n)), dim3(256), 0, dv.stream, (cpx<double>*)d, n, dx, u);
+  else
+    hipLaunchKernelGGL(k_tf1d<float>, dim3(grid_for(n)), dim3(256), 0, dv.stream, (cpx<float>*)d, n, dx, u);
+  WOS_HIP_TRY(hipGetLastError());
+  RowArgs rb = ra;
+  rb.pw_in = nullptr;
+  rb.pw_out = (double*)dv.partial_out.p;
+  rb.scale_re = 1.0 / (double)n;
+  if ((rc = launch_rows(precision, n, 1, rb, dv.stream))) return rc;
+  double p_in = 0, p_out = 0;
+  WOS_HIP_TRY(hipMemcpyAsync(&p_in, dv.partial_in.p, sizeof(double), hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipMemcpyAsync(&p_out, dv.partial_out.p, sizeof(double), hipMemcpyDeviceToHost, dv.stream));
+  if ((rc = download_fields(precision, d, dst, 1, (size_t)n))) return rc;
+  if (p_in > 0 && p_out < p_in * (1 - 1e-8) && z > 0) warn_add(w, WOS_WARN_EVANESCENT_FILTERED, (p_in - p_out) / p_in);
+  return WOS_OK;
+}
+
+// PropagateSplitStep (optics/medium.go:22-45): Strang slices
+// [ASM dz/2, medium phase dz, ASM dz/2], each ASM half step a full Propagate
+// with its own diagnostics, as the reference executes them.
+int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int steps, const double* index,
+                       int n_planes, const wos_ctx* ctx, int precision, wos_warnings* w) {
+  WOS_ENTER();
+  if (!ex || n <= 0 || !ctx || !index || !valid_prec(precision) || !(dx > 0)) {
+    set_error("wos_hip_split_step: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (z == 0) return WOS_OK;
+  if (steps <= 0) steps = 1;
+  if (n_planes != 1 && n_planes != steps) {
+    set_error("wos_hip_split_step: n_planes must be 1 or steps");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const int ncomp = ey ? 2 : 1;
+  const size_t nn = (size_t)n * n;
+  const double* src[2] = {ex, ey};
+  double* dst[2] = {ex, ey};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
+  const double dz = z / (double)steps;
+  const double k = 2.0 * M_PI / ctx->wavelength;
+  // screen tables t(x, y) per plane, FP64
+  if ((rc = dv.screens.ensure((size_t)n_planes * nn * 16))) return rc;
+  if ((rc = dv.work.ensure(nn * 16))) return rc;
+  for (int p = 0; p < n_planes; ++p) {
+    WOS_HIP_TRY(hipMemcpyAsync(dv.work.p, index + (size_t)p * nn * 2, nn * 16, hipMemcpyHostToDevice, dv.stream));
+    hipLaunchKernelGGL(k_screen_table, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.work.p,
+                       (cpx<double>*)dv.screens.p + (size_t)p * nn, nn, k, dz);
+    WOS_HIP_TRY(hipGetLastError());
+    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  }
+  for (int s = 0; s < steps; ++s) {
+    if ((rc = propagate_dev(precision, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, *ctx, w, nullptr))) return rc;
+    const cpx<double>* t = (const cpx<double>*)dv.screens.p + (n_planes == 1 ? 0 : (size_t)s * nn);
+    if (precision == WOS_FP64)
+      hipLaunchKernelGGL(k_mul_table<double>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<double>*)d, t, nn, ncomp);
+    else
+      hipLaunchKernelGGL(k_mul_table<float>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<float>*)d, t, nn, ncomp);
+    WOS_HIP_TRY(hipGetLastError());
+    if ((rc = propagate_dev(precision, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, *ctx, w, nullptr))) return rc;
+  }
+  return download_fields(precision, d, dst, ncomp, nn);
+}
+
+// PropagatePolychromatic (optics/polychromatic.go:18-43)
+int wos_hip_polychromatic(const double* ex, const double* ey, int n, double dx, double z, const double* wavelengths,
+                          const double* weights, int n_samples, int method, const wos_ctx* ctx, int precision,
+                          double* out, wos_warnings* w) {
+  WOS_ENTER();
+  if (!ex || n <= 0 || !ctx || !out || !wavelengths || !weights || !valid_prec(precision) || !(dx > 0)) {
+    set_error("wos_hip_polychromatic: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (n_samples <= 0) {
+    set_error("polychromatic: no wavelength samples");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const int ncomp = ey ? 2 : 1;
+  const size_t nn = (size_t)n * n;
+  const double* src[2] = {ex, ey};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
+  if ((rc = dv.out.ensure(nn * sizeof(double)))) return rc;
+  WOS_HIP_TRY(hipMemsetAsync(dv.out.p, 0, nn * sizeof(double), dv.stream));
+  const size_t fb = (size_t)ncomp * nn * csize(precision);
+  if ((rc = dv.work.ensure(fb))) return rc;
+  for (int s = 0; s < n_samples; ++s) {
+    if (wavelengths[s] <= 0) {
+      set_error("polychromatic: wavelength must be > 0");
+      return WOS_ERR_INVALID;
+    }
+    WOS_HIP_TRY(hipMemcpyAsync(dv.work.p, d, fb, hipMemcpyDeviceToDevice, dv.stream));
+    wos_ctx c = *ctx;
+    c.wavelength = wavelengths[s];
+    if ((rc = propagate_dev(precision, dv.work.p, n, ncomp, dx, z, method, c, w, nullptr))) return rc;
+    if (precision == WOS_FP64)
+      hipLaunchKernelGGL(k_accum_intensity<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<double>*)dv.work.p, ncomp, nn, weights[s], (double*)dv.out.p);
+    else
+      hipLaunchKernelGGL(k_accum_intensity<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<float>*)dv.work.p, ncomp, nn, weights[s], (double*)dv.out.p);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  WOS_HIP_TRY(hipMemcpyAsync(out, dv.out.p, nn * sizeof(double), hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
+// Propagate3D (optics/volume.go:8-18)
+int wos_hip_volume(const double* ex, const double* ey, int n, double dx, const double* zs, int nz, int method,
+                   const wos_ctx* ctx, int precision, double* out_ex, double* out_ey, wos_warnings* w) {
+  WOS_ENTER();
+  if (!ex || n <= 0 || !ctx || !out_ex || (nz > 0 && !zs) || !valid_prec(precision) || !(dx > 0) ||
+      (ey && !out_ey)) {
+    set_error("wos_hip_volume: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const int ncomp = ey ? 2 : 1;
+  const size_t nn = (size_t)n * n;
+  const double* src[2] = {ex, ey};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
+  const size_t fb = (size_t)ncomp * nn * csize(precision);
+  if ((rc = dv.work.ensure(fb))) return rc;
+  for (int i = 0; i < nz; ++i) {
+    WOS_HIP_TRY(hipMemcpyAsync(dv.work.p, d, fb, hipMemcpyDeviceToDevice, dv.stream));
+    if ((rc = propagate_dev(precision, dv.work.p, n, ncomp, dx, zs[i], method, *ctx, w, nullptr))) return rc;
+    double* dst[2] = {out_ex + (size_t)i * nn * 2, ey ? out_ey + (size_t)i * nn * 2 : nullptr};
+    if ((rc = download_fields(precision, dv.work.p, dst, ncomp, nn))) return rc;
+  }
+  return WOS_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Device-resident batches.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct wos_batch_s {
+  int n = 0, M = 0, method = 0, prec = 0, units = 0;
+  bool pad = false;
+  double dx = 0;
+  hipStream_t stream = nullptr;
+  void* fields = nullptr;   // units * n * n
+  void* inter = nullptr;    // padded intermediate (pad) or == fields
+  TFUnit* d_tf = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int col_w = 0;
+};
+
+static int batch_enqueue(wos_batch_s* b) {
+  return enqueue_transfer(b->prec, b->fields, b->inter, b->n, b->units, b->pad, b->d_tf, 1.0, 0.0, nullptr, nullptr,
+                          b->stream, b->col_w);
+}
+
+extern "C" {
+
+int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int precision, int n_units,
+                         const double* wavelengths, const double* zs, const wos_ctx* ctx) {
+  WOS_ENTER();
+  if (!out || n <= 0 || n_units <= 0 || !wavelengths || !zs || !ctx || !valid_prec(precision) || !(dx > 0)) {
+    set_error("wos_hip_batch_create: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (method != WOS_METHOD_ASM && method != WOS_METHOD_ASM_PAD) {
+    set_error("wos_hip_batch_create: batches run asm or asm_pad");
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const bool pad = method == WOS_METHOD_ASM_PAD;
+  const int M = pad ? 2 * n : n;
+  if (!pow2(M) || M > max_pow2_len()) {
+    set_error("wos_hip_batch_create: transform length not supported");
+    return WOS_ERR_UNSUPPORTED;
+  }
+  wos_batch_s* b = new wos_batch_s();
+  b->n = n;
+  b->M = M;
+  b->method = method;
+  b->prec = precision;
+  b->units = n_units;
+  b->pad = pad;
+  b->dx = dx;
+  const size_t nn = (size_t)n * n;
+  auto fail = [&](int code) {
+    if (b->fields) (void)hipFree(b->fields);
+    if (b->inter && b->inter != b->fields) (void)hipFree(b->inter);
+    if (b->d_tf) (void)hipFree(b->d_tf);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return code;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return fail(WOS_ERR_HIP);
+  }
+  if ((e = hipMalloc(&b->fields, nn * n_units * csize(precision))) != hipSuccess) {
+    set_error(std::string("hipMalloc(batch fields): ") + hipGetErrorString(e));
+    return fail(WOS_ERR_NOMEM);
+  }
+  (void)hipMemset(b->fields, 0, nn * n_units * csize(precision));
+  b->inter = b->fields;
+  if (pad && (e = hipMalloc(&b->inter, (size_t)n * M * n_units * csize(precision))) != hipSuccess) {
+    b->inter = nullptr;
+    set_error(std::string("hipMalloc(batch intermediate): ") + hipGetErrorString(e));
+    return fail(WOS_ERR_NOMEM);
+  }
+  std::vector<TFUnit> tf(n_units);
+  for (int u = 0; u < n_units; ++u) tf[u] = make_asm_tf(wavelengths[u], zs[u], *ctx, M, dx, 0, 0);
+  if ((e = hipMalloc((void**)&b->d_tf, sizeof(TFUnit) * n_units)) != hipSuccess) {
+    set_error(std::string("hipMalloc(batch params): ") + hipGetErrorString(e));
+    return fail(WOS_ERR_NOMEM);
+  }
+  if ((e = hipMemcpy(b->d_tf, tf.data(), sizeof(TFUnit) * n_units, hipMemcpyHostToDevice)) != hipSuccess) {
+    set_error(std::string("hipMemcpy(batch params): ") + hipGetErrorString(e));
+    return fail(WOS_ERR_HIP);
+  }
+  const void* tw;
+  int rc = get_twiddles(M, precision, &tw);
+  if (rc) return fail(rc);
+  if ((e = hipStreamSynchronize(device().stream)) != hipSuccess) {
+    set_error(std::string("twiddles: ") + hipGetErrorString(e));
+    return fail(WOS_ERR_HIP);
+  }
+  if (const char* w = getenv("WOS_COL_W")) b->col_w = atoi(w);
+  *out = b;
+  return WOS_OK;
+}
+
+int wos_hip_batch_destroy(wos_batch b) {
+  WOS_ENTER();
+  if (!b) return WOS_OK;
+  (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->inter && b->inter != b->fields) (void)hipFree(b->inter);
+  if (b->fields) (void)hipFree(b->fields);
+  if (b->d_tf) (void)hipFree(b->d_tf);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return WOS_OK;
+}
+
+void* wos_hip_batch_device_ptr(wos_batch b) { return b ? b->fields : nullptr; }
+
+int wos_hip_batch_upload(wos_batch b, int unit, const double* ex) {
+  WOS_ENTER();
+  if (!b || !ex || unit < 0 || unit >= b->units) {
+    set_error("wos_hip_batch_upload: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const size_t nn = (size_t)b->n * b->n;
+  int rc;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->prec == WOS_FP64) {
+    WOS_HIP_TRY(hipMemcpy((char*)b->fields + unit * nn * 16, ex, nn * 16, hipMemcpyHostToDevice));
+    return WOS_OK;
+  }
+  if ((rc = dv.stage.ensure(nn * 16))) return rc;
+  WOS_HIP_TRY(hipMemcpyAsync(dv.stage.p, ex, nn * 16, hipMemcpyHostToDevice, dv.stream));
+  hipLaunchKernelGGL(k_z2c, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.stage.p,
+                     (cpx<float>*)b->fields + unit * nn, nn);
+  WOS_HIP_TRY(hipGetLastError());
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
+int wos_hip_batch_download(wos_batch b, int unit, double* ex) {
+  WOS_ENTER();
+  if (!b || !ex || unit < 0 || unit >= b->units) {
+    set_error("wos_hip_batch_download: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const size_t nn = (size_t)b->n * b->n;
+  int rc;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->prec == WOS_FP64) {
+    WOS_HIP_TRY(hipMemcpy(ex, (char*)b->fields + unit * nn * 16, nn * 16, hipMemcpyDeviceToHost));
+    return WOS_OK;
+  }
+  if ((rc = dv.stage.ensure(nn * 16))) return rc;
+  hipLaunchKernelGGL(k_c2z, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (const cpx<float>*)b->fields + unit * nn,
+                     (cpx<double>*)dv.stage.p, nn);
+  WOS_HIP_TRY(hipGetLastError());
+  WOS_HIP_TRY(hipMemcpyAsync(ex, dv.stage.p, nn * 16, hipMemcpyDeviceToHost, dv.stream));
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
+int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t seed) {
+  WOS_ENTER();
+  if (!b || !(w0 > 0) || !(sigma_c > 0)) {
+    set_error("wos_hip_batch_fill_schell: width and coherence must be > 0");
+    return WOS_ERR_INVALID;
+  }
+  Device& dv = device();
+  const int n = b->n;
+  const size_t nn = (size_t)n * n;
+  int rc;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if ((rc = dv.aux.ensure(nn * 16))) return rc;    // sqrt(S)
+  if ((rc = dv.work.ensure(nn * 16))) return rc;   // one realisation
+  hipLaunchKernelGGL(k_schell_mu, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, n, b->dx, sigma_c);
+  WOS_HIP_TRY(hipGetLastError());
+  if ((rc = fft2d_dev(WOS_FP64, dv.aux.p, n, 1, 0))) return rc;
+  hipLaunchKernelGGL(k_schell_sqrt, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, nn);
+  WOS_HIP_TRY(hipGetLastError());
+  for (int u = 0; u < b->units; ++u) {
+    hipLaunchKernelGGL(k_schell_noise, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.work.p,
+                       (const cpx<double>*)dv.aux.p, nn, seed, (uint64_t)u);
+    WOS_HIP_TRY(hipGetLastError());
+    if ((rc = fft2d_dev(WOS_FP64, dv.work.p, n, 1, 1))) return rc;
+    if (b->prec == WOS_FP64)
+      hipLaunchKernelGGL(k_schell_envelope<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<double>*)dv.work.p, (cpx<double>*)b->fields + u * nn, n, b->dx, w0);
+    else
+      hipLaunchKernelGGL(k_schell_envelope<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<double>*)dv.work.p, (cpx<float>*)b->fields + u * nn, n, b->dx, w0);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
+int wos_hip_batch_capture(wos_batch b) {
+  WOS_ENTER();
+  if (!b) return WOS_ERR_INVALID;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  WOS_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return WOS_ERR_HIP;
+  }
+  b->graph = g;
+  WOS_HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return WOS_OK;
+}
+
+int wos_hip_batch_run(wos_batch b, int iters) {
+  WOS_ENTER();
+  if (!b || iters < 0) return WOS_ERR_INVALID;
+  for (int i = 0; i < iters; ++i) {
+    if (b->exec) {
+      WOS_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    } else {
+      int rc = batch_enqueue(b);
+      if (rc) return rc;
+    }
+  }
+  return WOS_OK;
+}
+
+int wos_hip_batch_sync(wos_batch b) {
+  if (!b) return WOS_ERR_INVALID;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  return WOS_OK;
+}
+
+int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, double* kernel_ms) {
+  WOS_ENTER();
+  if (!b || iters <= 0 || !ms_run) return WOS_ERR_INVALID;
+  int rc;
+  hipEvent_t e0, e1;
+  WOS_HIP_TRY(hipEventCreate(&e0));
+  WOS_HIP_TRY(hipEventCreate(&e1));
+  if ((rc = wos_hip_batch_run(b, warmup))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e0, b->stream));
+  if ((rc = wos_hip_batch_run(b, iters))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e1, b->stream));
+  WOS_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ms_run = ms / iters;
+  if (kernel_ms) {
+    // each kernel of the pipeline back to back on the batch stream
+    const int n = b->n, M = b->M, off = b->pad ? n / 2 : 0;
+    const void* tw;
+    if ((rc = get_twiddles(M, b->prec, &tw))) return rc;
+    const long long iunit = b->pad ? (long long)n * M : (long long)n * n;
+    for (int kk = 0; kk < 3; ++kk) {
+      RowArgs ra;
+      memset(&ra, 0, sizeof(ra));
+      ColArgs ca;
+      memset(&ca, 0, sizeof(ca));
+      if (kk == 0 || kk == 2) {
+        const bool fwd = kk == 0;
+        ra.in = fwd ? b->fields : b->inter;
+        ra.out = fwd ? b->inter : b->fields;
+        ra.in_unit = fwd ? (long long)n * n : iunit;
+        ra.out_unit = fwd ? iunit : (long long)n * n;
+        ra.in_ld = fwd ? n : M;
+        ra.out_ld = fwd ? M : n;
+        ra.rows = n;
+        ra.units = b->units;
+        ra.in_col0 = fwd ? off : 0;
+        ra.in_cols = fwd ? n : M;
+        ra.out_col0 = fwd ? 0 : off;
+        ra.out_cols = fwd ? M : n;
+        ra.scale_re = fwd ? 1.0 : 1.0 / ((double)M * M);
+        ra.tw = tw;
+      } else {
+        ca.data = b->inter;
+        ca.unit = iunit;
+        ca.ld = M;
+        ca.cols = M;
+        ca.units = b->units;
+        ca.in_row0 = off;
+        ca.rows = n;
+        ca.out_row0 = off;
+        ca.out_rows = n;
+        ca.scale = 1.0;
+        ca.tf = b->d_tf;
+        ca.tw = tw;
+      }
+      auto launch = [&]() {
+        return kk == 1 ? launch_cols(b->prec, M, 1, 0, b->col_w, ca, b->stream)
+                       : launch_rows(b->prec, M, kk == 0 ? -1 : 1, ra, b->stream);
+      };
+      if ((rc = launch())) return rc;
+      WOS_HIP_TRY(hipEventRecord(e0, b->stream));
+      for (int i = 0; i < iters; ++i)
+        if ((rc = launch())) return rc;
+      WOS_HIP_TRY(hipEventRecord(e1, b->stream));
+      WOS_HIP_TRY(hipEventSynchronize(e1));
+      WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      kernel_ms[kk] = ms / iters;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return WOS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+"""ctypes binding of libwos_hip's C ABI (include/wos_hip.h).
+
+This is the host-side mirror of the reference's optics operators for the
+propagation path: fft2d/fft1d stand for fft2D/fft1DAny (optics/fft.go:135,
+209), propagate for Propagate (optics/propagate.go:65), split_step for
+PropagateSplitStep (optics/medium.go:22), polychromatic for
+PropagatePolychromatic (optics/polychromatic.go:18), volume for Propagate3D
+(optics/volume.go:8) and propagate1d for Propagate1D
+(optics/propagate.go:718). Errors raise WosError; there is no CPU fallback:
+loading fails loudly when libwos_hip.so is absent.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwos_hip.so")
+
+FP64, FP32 = 0, 1
+METHODS = {"asm": 0, "asm_pad": 1, "asm_shift": 2, "asm_shift_pad": 3,
+           "fresnel_tf": 4, "fresnel_ir": 5}
+WARN_CODES = ["evanescent_filtered", "backward_regularized", "backward_evanescent",
+              "asm_alias_wrap", "fresnel_tf_alias", "fresnel_ir_alias"]
+EXPORTS = [
+    "wos_hip_device_count", "wos_hip_device_name", "wos_hip_init", "wos_hip_last_error",
+    "wos_hip_version", "wos_hip_release_cache", "wos_hip_fft2d_z", "wos_hip_fft1d_z",
+    "wos_hip_fft2d_c", "wos_hip_fft1d_c", "wos_hip_propagate", "wos_hip_propagate1d",
+    "wos_hip_split_step", "wos_hip_polychromatic", "wos_hip_volume",
+    "wos_hip_batch_create", "wos_hip_batch_destroy", "wos_hip_batch_upload",
+    "wos_hip_batch_download", "wos_hip_batch_fill_schell", "wos_hip_batch_capture",
+    "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
+]
+
+
+class WosError(RuntimeError):
+    pass
+
+
+class Ctx(C.Structure):
+    """wos_ctx: the Context fields of the path (optics/field.go:237-254)."""
+    _fields_ = [("wavelength", C.c_double),
+                ("evanescent_zero", C.c_int32),
+                ("backward_regularize", C.c_int32),
+                ("evanescent_limit", C.c_double),
+                ("tikhonov_alpha", C.c_double)]
+
+
+class Warnings(C.Structure):
+    _fields_ = [("count", C.c_int32 * 8), ("value", C.c_double * 8)]
+
+    def as_dict(self):
+        return {code: (self.count[i], self.value[i])
+                for i, code in enumerate(WARN_CODES) if self.count[i]}
+
+
+class PropStats(C.Structure):
+    _fields_ = [("p_in", C.c_double), ("p_out", C.c_double),
+                ("fx_c", C.c_double), ("fy_c", C.c_double),
+                ("method_used", C.c_int32), ("reserved", C.c_int32)]
+
+
+def ctx_for(wl, evanescent="decay", limit=0.0, backward_regularize=False, alpha=0.0):
+    return Ctx(wl, 1 if evanescent == "zero" else 0, 1 if backward_regularize else 0, limit, alpha)
+
+
+_lib = None
+
+
+def load(path=None):
+    """Load libwos_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise WosError(f"libwos_hip.so not built at {path}; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    P, D, I = C.c_void_p, C.c_double, C.c_int
+    L.wos_hip_device_count.restype = I
+    L.wos_hip_device_name.argtypes = [C.c_char_p, I]
+    L.wos_hip_init.restype = I
+    L.wos_hip_last_error.restype = C.c_char_p
+    L.wos_hip_version.restype = C.c_char_p
+    for nm in ("wos_hip_fft2d_z", "wos_hip_fft1d_z", "wos_hip_fft2d_c", "wos_hip_fft1d_c"):
+        getattr(L, nm).argtypes = [P, I, I]
+        getattr(L, nm).restype = I
+    L.wos_hip_propagate.argtypes = [P, P, I, D, D, I, C.POINTER(Ctx), I, C.POINTER(Warnings),
+                                    C.POINTER(PropStats)]
+    L.wos_hip_propagate1d.argtypes = [P, I, D, D, D, C.POINTER(Ctx), I, C.POINTER(Warnings)]
+    L.wos_hip_split_step.argtypes = [P, P, I, D, D, I, P, I, C.POINTER(Ctx), I, C.POINTER(Warnings)]
+    L.wos_hip_polychromatic.argtypes = [P, P, I, D, D, P, P, I, I, C.POINTER(Ctx), I, P,
+                                        C.POINTER(Warnings)]
+    L.wos_hip_volume.argtypes = [P, P, I, D, P, I, I, C.POINTER(Ctx), I, P, P, C.POINTER(Warnings)]
+    L.wos_hip_batch_create.argtypes = [C.POINTER(P), I, D, I, I, I, P, P, C.POINTER(Ctx)]
+    L.wos_hip_batch_destroy.argtypes = [P]
+    L.wos_hip_batch_upload.argtypes = [P, I, P]
+    L.wos_hip_batch_download.argtypes = [P, I, P]
+    L.wos_hip_batch_fill_schell.argtypes = [P, D, D, C.c_uint64]
+    L.wos_hip_batch_capture.argtypes = [P]
+    L.wos_hip_batch_run.argtypes = [P, I]
+    L.wos_hip_batch_sync.argtypes = [P]
+    L.wos_hip_batch_device_ptr.argtypes = [P]
+    L.wos_hip_batch_device_ptr.restype = P
+    L.wos_hip_batch_time.argtypes = [P, I, I, C.POINTER(D), P]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load().wos_hip_last_error().decode(errors="replace")
+        raise WosError(f"{what} failed (rc={rc}): {msg}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def device_count():
+    return load().wos_hip_device_count()
+
+
+def init():
+    _check(load().wos_hip_init(), "wos_hip_init")
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    _check(load().wos_hip_device_name(buf, 256), "wos_hip_device_name")
+    return buf.value.decode()
+
+
+def fft2d(a, inverse=False, precision=FP64):
+    """fft2D on a copy: forward unnormalised, inverse scaled by 1/n^2."""
+    if precision == FP64:
+        a = _c128(a).copy()
+        _check(load().wos_hip_fft2d_z(_p(a), a.shape[0], int(inverse)), "fft2d_z")
+    else:
+        a = np.ascontiguousarray(a, dtype=np.complex64).copy()
+        _check(load().wos_hip_fft2d_c(_p(a), a.shape[0], int(inverse)), "fft2d_c")
+    return a
+
+
+def fft1d(a, inverse=False, precision=FP64):
+    """fft1DAny on a copy: inverse scaled by 1/n."""
+    if precision == FP64:
+        a = _c128(a).copy()
+        _check(load().wos_hip_fft1d_z(_p(a), a.size, int(inverse)), "fft1d_z")
+    else:
+        a = np.ascontiguousarray(a, dtype=np.complex64).copy()
+        _check(load().wos_hip_fft1d_c(_p(a), a.size, int(inverse)), "fft1d_c")
+    return a
+
+
+def propagate(ex, ey, dx, z, method, ctx, precision=FP64):
+    """Propagate on copies; returns (ex, ey, warnings, stats)."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    ex = _c128(ex).copy()
+    ey = None if ey is None else _c128(ey).copy()
+    w, st = Warnings(), PropStats()
+    _check(load().wos_hip_propagate(_p(ex), _p(ey), ex.shape[0], dx, z, m, C.byref(ctx), precision,
+                                    C.byref(w), C.byref(st)), "propagate")
+    return ex, ey, w, st
+
+
+def propagate1d(a, dx, z, wl, ctx, precision=FP64):
+    a = _c128(a).copy()
+    w = Warnings()
+    _check(load().wos_hip_propagate1d(_p(a), a.size, dx, z, wl, C.byref(ctx), precision, C.byref(w)),
+           "propagate1d")
+    return a, w
+
+
+def split_step(ex, ey, dx, z, steps, index, ctx, precision=FP64):
+    ex = _c128(ex).copy()
+    ey = None if ey is None else _c128(ey).copy()
+    index = _c128(index)
+    planes = 1 if index.ndim == 2 else index.shape[0]
+    w = Warnings()
+    _check(load().wos_hip_split_step(_p(ex), _p(ey), ex.shape[0], dx, z, steps, _p(index), planes,
+                                     C.byref(ctx), precision, C.byref(w)), "split_step")
+    return ex, ey, w
+
+
+def polychromatic(ex, ey, dx, z, wavelengths, weights, method, ctx, precision=FP64):
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    ex = _c128(ex)
+    ey = None if ey is None else _c128(ey)
+    n = ex.shape[0]
+    wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+    wt = np.ascontiguousarray(weights, dtype=np.float64)
+    out = np.zeros((n, n), dtype=np.float64)
+    w = Warnings()
+    _check(load().wos_hip_polychromatic(_p(ex), _p(ey), n, dx, z, _p(wl), _p(wt), wl.size, m,
+                                        C.byref(ctx), precision, _p(out), C.byref(w)), "polychromatic")
+    return out, w
+
+
+def volume(ex, ey, dx, zs, method, ctx, precision=FP64):
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    ex = _c128(ex)
+    ey = None if ey is None else _c128(ey)
+    n = ex.shape[0]
+    zs = np.ascontiguousarray(zs, dtype=np.float64)
+    ox = np.zeros((zs.size, n, n), dtype=np.complex128)
+    oy = None if ey is None else np.zeros((zs.size, n, n), dtype=np.complex128)
+    w = Warnings()
+    _check(load().wos_hip_volume(_p(ex), _p(ey), n, dx, _p(zs), zs.size, m, C.byref(ctx), precision,
+                                 _p(ox), _p(oy), C.byref(w)), "volume")
+    return ox, oy, w
+
+
+class Batch:
+    """Device-resident batch of independent n x n propagations (wos_batch)."""
+
+    def __init__(self, n, dx, wavelengths, zs, ctx, method="asm", precision=FP64):
+        self.n = n
+        self.precision = precision
+        wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+        z = np.ascontiguousarray(zs, dtype=np.float64)
+        assert wl.size == z.size
+        self.units = wl.size
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        h = C.c_void_p()
+        _check(load().wos_hip_batch_create(C.byref(h), n, dx, m, precision, wl.size, _p(wl), _p(z),
+                                           C.byref(ctx)), "batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(load().wos_hip_batch_destroy(self.h), "batch_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, unit, ex):
+        ex = _c128(ex)
+        _check(load().wos_hip_batch_upload(self.h, unit, _p(ex)), "batch_upload")
+
+    def download(self, unit):
+        out = np.zeros((self.n, self.n), dtype=np.complex128)
+        _check(load().wos_hip_batch_download(self.h, unit, _p(out)), "batch_download")
+        return out
+
+    def fill_schell(self, w0, sigma_c, seed):
+        _check(load().wos_hip_batch_fill_schell(self.h, w0, sigma_c, seed), "batch_fill_schell")
+
+    def capture(self):
+        _check(load().wos_hip_batch_capture(self.h), "batch_capture")
+
+    def run(self, iters=1):
+        _check(load().wos_hip_batch_run(self.h, iters), "batch_run")
+
+    def sync(self):
+        _check(load().wos_hip_batch_sync(self.h), "batch_sync")
+
+    def time(self, warmup, iters, kernels=False):
+        ms = C.c_double()
+        k = np.zeros(3, dtype=np.float64)
+        _check(load().wos_hip_batch_time(self.h, warmup, iters, C.byref(ms), _p(k) if kernels else None),
+               "batch_time")
+        return ms.value, (k if kernels else None)
