@@ -1,0 +1,96 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point include/wos_hip.h declares, the binding's structures match the C
+layout, and the error path reports a missing device instead of computing."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wos_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wos_hip_\w+)\s*\(", text)))
+
+
+def test_header_declares_and_library_exports_everything():
+    import wos_hip
+    lib = wos_hip.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(wos_hip.EXPORTS) == names
+
+
+def test_library_symbols_are_c_linkage():
+    import wos_hip
+    out = subprocess.run(["nm", "-D", "--defined-only", wos_hip.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (wos_hip_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Compile a probe against the header with gcc and compare offsets."""
+    import wos_hip
+    src = tmp_path / "probe.c"
+    src.write_text(r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "wos_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(wos_ctx), offsetof(wos_ctx, evanescent_zero),
+         offsetof(wos_ctx, backward_regularize), offsetof(wos_ctx, evanescent_limit), offsetof(wos_ctx, tikhonov_alpha));
+  printf("%zu %zu\n", sizeof(wos_warnings), offsetof(wos_warnings, value));
+  printf("%zu %zu\n", sizeof(wos_prop_stats), offsetof(wos_prop_stats, method_used));
+  printf("%d %d %d %d %d %d\n", WOS_METHOD_ASM, WOS_METHOD_ASM_PAD, WOS_METHOD_ASM_SHIFT,
+         WOS_METHOD_ASM_SHIFT_PAD, WOS_METHOD_FRESNEL_TF, WOS_METHOD_FRESNEL_IR);
+  printf("%d %d\n", WOS_FP64, WOS_FP32);
+  return 0;
+}
+''')
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    ctx = [int(x) for x in lines[0].split()]
+    assert ctx == [C.sizeof(wos_hip.Ctx), wos_hip.Ctx.evanescent_zero.offset,
+                   wos_hip.Ctx.backward_regularize.offset, wos_hip.Ctx.evanescent_limit.offset,
+                   wos_hip.Ctx.tikhonov_alpha.offset]
+    assert [int(x) for x in lines[1].split()] == [C.sizeof(wos_hip.Warnings), wos_hip.Warnings.value.offset]
+    assert [int(x) for x in lines[2].split()] == [C.sizeof(wos_hip.PropStats), wos_hip.PropStats.method_used.offset]
+    assert [int(x) for x in lines[3].split()] == [wos_hip.METHODS[m] for m in
+                                                  ("asm", "asm_pad", "asm_shift", "asm_shift_pad",
+                                                   "fresnel_tf", "fresnel_ir")]
+    assert [int(x) for x in lines[4].split()] == [wos_hip.FP64, wos_hip.FP32]
+
+
+def test_oracle_structs_match_library_structs():
+    import oracle as O
+    import wos_hip
+    assert C.sizeof(O.Ctx) == C.sizeof(wos_hip.Ctx)
+    assert [f[0] for f in O.Ctx._fields_] == [f[0] for f in wos_hip.Ctx._fields_]
+    assert C.sizeof(O.Warnings) == C.sizeof(wos_hip.Warnings)
+    assert O.WARN_CODES == wos_hip.WARN_CODES
+    assert O.METHODS == wos_hip.METHODS
+
+
+def test_no_device_reports_error_without_computing():
+    """Without a visible GPU every compute entry point returns
+    WOS_ERR_NO_DEVICE (the Go shim then falls back to its pure-Go path);
+    nothing is computed on the host."""
+    import numpy as np
+    import wos_hip
+    lib = wos_hip.load()
+    if lib.wos_hip_device_count() > 0:
+        pytest.skip("a GPU is visible; the no-device path is not reachable")
+    a = np.zeros((8, 8), dtype=np.complex128)
+    rc = lib.wos_hip_fft2d_z(a.ctypes.data, 8, 0)
+    assert rc == 1
+    assert b"device" in lib.wos_hip_last_error()
+    with pytest.raises(wos_hip.WosError):
+        wos_hip.propagate(a, None, 1e-6, 1e-3, "asm", wos_hip.ctx_for(632.8e-9))

@@ -67,8 +67,10 @@ int ensure_init();
 int get_twiddles(int n, int prec, const void** out);
 
 // ---- launchers (return 0 or a WOS_ERR_* code) ----
-// Row pass on `units` units of rows x N.
-int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s);
+// Row pass on `units` units of rows x N; rows = rows per workgroup (0: default).
+int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows = 0);
+bool rows_valid(int prec, int N, int rows);
+bool col_w_valid(int prec, int N, int W);
 // Column pass; mode 0 plain (dir), 1 fused forward/TF/inverse. W = 0: default.
 int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int default_col_w(int prec, int N, int cols);

@@ -20,12 +20,16 @@
 
 namespace wos {
 
-int launch_rows_f64(int N, int dir, const RowArgs& a, hipStream_t s);
-int launch_rows_f32(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_rows_f64(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
+int launch_rows_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
+bool rows_valid_f64(int N, int rows);
+bool rows_valid_f32(int N, int rows);
 int launch_cols_f64(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int launch_cols_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int default_col_w_f64(int N, int cols);
 int default_col_w_f32(int N, int cols);
+bool col_w_valid_f64(int N, int W);
+bool col_w_valid_f32(int N, int W);
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
@@ -129,8 +133,14 @@ int get_twiddles(int n, int prec, const void** out) {
   return 0;
 }
 
-int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s) {
-  return prec == WOS_FP64 ? launch_rows_f64(N, dir, a, s) : launch_rows_f32(N, dir, a, s);
+int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows) {
+  return prec == WOS_FP64 ? launch_rows_f64(N, dir, rows, a, s) : launch_rows_f32(N, dir, rows, a, s);
+}
+bool rows_valid(int prec, int N, int rows) {
+  return prec == WOS_FP64 ? rows_valid_f64(N, rows) : rows_valid_f32(N, rows);
+}
+bool col_w_valid(int prec, int N, int W) {
+  return prec == WOS_FP64 ? col_w_valid_f64(N, W) : col_w_valid_f32(N, W);
 }
 int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
   if (W <= 0) W = default_col_w(prec, N, a.cols);
@@ -502,6 +512,8 @@ static TFUnit make_asm_tf(double wl, double z, const wos_ctx& ctx, int M, double
   u.fx_off = fxo;
   u.fy_off = fyo;
   u.den = (double)M * dx;
+  u.inv_den = 1.0 / u.den;
+  u.kz = u.k * z;
   u.M = M;
   u.kind = TF_ASM;
   u.zero_ev = zero;
@@ -534,8 +546,7 @@ static int fft2d_dev(int prec, void* d, int n, int units, int inverse) {
   if (rc) return rc;
   const int dir = inverse ? 1 : -1;
   const double sc = inverse ? 1.0 / (double)n : 1.0;
-  RowArgs ra;
-  memset(&ra, 0, sizeof(ra));
+  RowArgs ra = row_args_init();
   ra.in = d;
   ra.out = d;
   ra.in_unit = ra.out_unit = (long long)n * n;
@@ -569,32 +580,77 @@ struct TransferOut {
   std::vector<double> p_in, p_out;  // per field group (sum of |.|^2, no dx^2)
 };
 
-// Enqueue one FFT -> TF -> IFFT step on `units` fields of n x n on stream s:
-// power-of-two grid side M = n, or M = 2n zero-padded with the centre crop.
-// d_tf: device array of per-unit TFUnit; inter: padded intermediate
-// (units * n * M elements) or d itself; post multiplies the output;
-// pw_in / pw_out receive per-workgroup power partials (nullable).
-static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bool pad, const TFUnit* d_tf,
-                            double post_re, double post_im, double* pw_in, double* pw_out, hipStream_t s,
-                            int col_w = 0) {
+// Launch shape of one transfer step: tw = column-tile width of the
+// intermediate (also the column pass's W), rows = rows per row-pass
+// workgroup, xcd_remap = XCD-aware order of the column workgroups.
+struct Tuning {
+  int tw = 0;
+  int rows = 0;
+  int xcd_remap = 0;
+};
+
+// Fill unset fields: the tile width and rows per workgroup that make one
+// tile chunk of ROWS x TW elements a whole 128-byte line.
+static Tuning resolve_tuning(int prec, int M, Tuning t) {
+  const int S = (int)csize(prec);
+  if (t.tw <= 0) {
+    int cand[] = {prec == WOS_FP32 ? 4 : 2, 4, 8, 16, 2, 1};
+    for (int w : cand)
+      if (w <= M && (M % w) == 0 && col_w_valid(prec, M, w)) {
+        t.tw = w;
+        break;
+      }
+  }
+  if (t.rows <= 0) {
+    const int rd = rows_per_block_host(M);
+    int want = std::max(rd, 128 / (S * std::max(1, t.tw)));
+    while (want > rd && !rows_valid(prec, M, want)) want /= 2;
+    t.rows = rows_valid(prec, M, want) ? want : rd;
+  }
+  return t;
+}
+
+// The three launches of one transfer step (forward rows, fused columns,
+// inverse rows) on `units` fields of n x n: power-of-two grid side M = n, or
+// M = 2n zero-padded with the centre crop. inter is the column-tiled
+// intermediate of units * n * M elements and must not alias d.
+struct TransferLaunch {
+  int prec = 0, M = 0, mode = 1;
+  Tuning tu;
+  RowArgs fwd, inv;
+  ColArgs col;
+};
+
+static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int n, int units, bool pad,
+                          const TFUnit* d_tf, int tf_kind, double post_re, double post_im, double* pw_in,
+                          double* pw_out, Tuning tu) {
   const int M = pad ? 2 * n : n;
   const int off = pad ? n / 2 : 0;
   if (!pow2(M) || M > max_pow2_len()) {
     set_error("transform length not supported: " + std::to_string(M));
     return WOS_ERR_UNSUPPORTED;
   }
+  if (tf_kind != TF_ASM) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
+  tu = resolve_tuning(prec, M, tu);
   const void* tw;
   int rc = get_twiddles(M, prec, &tw);
   if (rc) return rc;
-  const long long iunit = pad ? (long long)n * M : (long long)n * n;
-  RowArgs ra;
-  memset(&ra, 0, sizeof(ra));
+  L.prec = prec;
+  L.M = M;
+  L.mode = tf_kind == TF_ASM ? 1 : 2;
+  L.tu = tu;
+  const long long iunit = (long long)n * M;
+  const int tstride = n * tu.tw;
+  const int lg = ilog2(tu.tw);
+  RowArgs& ra = L.fwd;
+  ra = row_args_init();
   ra.in = d;
   ra.out = inter;
   ra.in_unit = (long long)n * n;
   ra.out_unit = iunit;
   ra.in_ld = n;
-  ra.out_ld = M;
+  ra.out_tw = lg;
+  ra.out_tstride = tstride;
   ra.rows = n;
   ra.units = units;
   ra.in_col0 = off;
@@ -604,13 +660,12 @@ static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bo
   ra.scale_re = 1.0;
   ra.pw_in = pw_in;
   ra.tw = tw;
-  if ((rc = launch_rows(prec, M, -1, ra, s))) return rc;
-
-  ColArgs ca;
+  ColArgs& ca = L.col;
   memset(&ca, 0, sizeof(ca));
   ca.data = inter;
   ca.unit = iunit;
-  ca.ld = M;
+  ca.tiled = 1;
+  ca.tstride = tstride;
   ca.cols = M;
   ca.units = units;
   ca.in_row0 = off;
@@ -618,17 +673,17 @@ static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bo
   ca.out_row0 = off;
   ca.out_rows = n;
   ca.scale = 1.0;
+  ca.xcd_remap = tu.xcd_remap;
   ca.tf = d_tf;
   ca.tw = tw;
-  if ((rc = launch_cols(prec, M, 1, 0, col_w, ca, s))) return rc;
-
-  RowArgs rb;
-  memset(&rb, 0, sizeof(rb));
+  RowArgs& rb = L.inv;
+  rb = row_args_init();
   rb.in = inter;
   rb.out = d;
   rb.in_unit = iunit;
   rb.out_unit = (long long)n * n;
-  rb.in_ld = M;
+  rb.in_tw = lg;
+  rb.in_tstride = tstride;
   rb.out_ld = n;
   rb.rows = n;
   rb.units = units;
@@ -641,7 +696,23 @@ static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bo
   rb.scale_im = post_im * inv;
   rb.pw_out = pw_out;
   rb.tw = tw;
-  return launch_rows(prec, M, 1, rb, s);
+  return 0;
+}
+
+// kernel k of a transfer step: 0 forward rows, 1 columns, 2 inverse rows
+static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s) {
+  if (k == 0) return launch_rows(L.prec, L.M, -1, L.fwd, s, L.tu.rows);
+  if (k == 1) return launch_cols(L.prec, L.M, L.mode, 0, L.tu.tw, L.col, s);
+  return launch_rows(L.prec, L.M, 1, L.inv, s, L.tu.rows);
+}
+
+static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bool pad, const TFUnit* d_tf,
+                            int tf_kind, double post_re, double post_im, double* pw_in, double* pw_out,
+                            hipStream_t s, Tuning tu) {
+  TransferLaunch L;
+  int rc = build_transfer(L, prec, d, inter, n, units, pad, d_tf, tf_kind, post_re, post_im, pw_in, pw_out, tu);
+  for (int k = 0; k < 3 && !rc; ++k) rc = launch_transfer_kernel(L, k, s);
+  return rc;
 }
 
 // Synchronous wrapper on the device stream with per-field power sums.
@@ -650,23 +721,27 @@ static int transfer_dev(int prec, void* d, int n, int units, bool pad, const std
   Device& dv = device();
   const int M = pad ? 2 * n : n;
   int rc;
-  void* inter = d;
-  if (pad) {
-    if ((rc = dv.scratch.ensure((size_t)units * n * M * csize(prec)))) return rc;
-    inter = dv.scratch.p;
+  if (!pow2(M) || M > max_pow2_len()) {
+    set_error("transform length not supported: " + std::to_string(M));
+    return WOS_ERR_UNSUPPORTED;
   }
+  if ((rc = dv.scratch.ensure((size_t)units * n * M * csize(prec)))) return rc;
+  void* inter = dv.scratch.p;
   if ((rc = dv.params.ensure(sizeof(TFUnit) * units))) return rc;
   if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * units))) return rc;
-  const int rpb = rows_per_block_host(M);
+  Tuning tu;
+  if (tf[0].kind != TF_ASM) tu.tw = default_col_w(prec, M, M);
+  tu = resolve_tuning(prec, M, tu);
+  const int rpb = tu.rows;
   const int bpu = (n + rpb - 1) / rpb;
   const int nblocks = bpu * units;
   if (pw) {
     if ((rc = dv.partial_in.ensure(sizeof(double) * nblocks))) return rc;
     if ((rc = dv.partial_out.ensure(sizeof(double) * nblocks))) return rc;
   }
-  if ((rc = enqueue_transfer(prec, d, inter, n, units, pad, (const TFUnit*)dv.params.p, post_re, post_im,
+  if ((rc = enqueue_transfer(prec, d, inter, n, units, pad, (const TFUnit*)dv.params.p, tf[0].kind, post_re, post_im,
                              pw ? (double*)dv.partial_in.p : nullptr, pw ? (double*)dv.partial_out.p : nullptr,
-                             dv.stream)))
+                             dv.stream, tu)))
     return rc;
   if (pw) {
     std::vector<double> hin(nblocks), hout(nblocks);
@@ -1041,8 +1116,7 @@ static int fft1d_host(int prec, void* a, int n, int inverse) {
   WOS_HIP_TRY(hipMemcpyAsync(dv.field.p, a, bytes, hipMemcpyHostToDevice, dv.stream));
   const void* tw;
   if ((rc = get_twiddles(n, prec, &tw))) return rc;
-  RowArgs ra;
-  memset(&ra, 0, sizeof(ra));
+  RowArgs ra = row_args_init();
   ra.in = dv.field.p;
   ra.out = dv.field.p;
   ra.in_unit = ra.out_unit = n;
@@ -1122,8 +1196,7 @@ int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const 
   if ((rc = get_twiddles(n, precision, &tw))) return rc;
   if ((rc = dv.partial_in.ensure(sizeof(double)))) return rc;
   if ((rc = dv.partial_out.ensure(sizeof(double)))) return rc;
-  RowArgs ra;
-  memset(&ra, 0, sizeof(ra));
+  RowArgs ra = row_args_init();
   ra.in = d;
   ra.out = d;
   ra.in_unit = ra.out_unit = n;
@@ -1288,17 +1361,35 @@ struct wos_batch_s {
   bool pad = false;
   double dx = 0;
   hipStream_t stream = nullptr;
-  void* fields = nullptr;   // units * n * n
-  void* inter = nullptr;    // padded intermediate (pad) or == fields
+  void* fields = nullptr;   // units * n * n, row-major
+  void* inter = nullptr;    // units * n * M, column-tiled intermediate
   TFUnit* d_tf = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int col_w = 0;
+  Tuning tune;              // zero fields: defaults
 };
 
+static int batch_build(wos_batch_s* b, TransferLaunch& L) {
+  return build_transfer(L, b->prec, b->fields, b->inter, b->n, b->units, b->pad, b->d_tf, TF_ASM, 1.0, 0.0,
+                        nullptr, nullptr, b->tune);
+}
+
 static int batch_enqueue(wos_batch_s* b) {
-  return enqueue_transfer(b->prec, b->fields, b->inter, b->n, b->units, b->pad, b->d_tf, 1.0, 0.0, nullptr, nullptr,
-                          b->stream, b->col_w);
+  TransferLaunch L;
+  int rc = batch_build(b, L);
+  for (int k = 0; k < 3 && !rc; ++k) rc = launch_transfer_kernel(L, k, b->stream);
+  return rc;
+}
+
+static void batch_drop_graph(wos_batch_s* b) {
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
 }
 
 extern "C" {
@@ -1331,7 +1422,7 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
   const size_t nn = (size_t)n * n;
   auto fail = [&](int code) {
     if (b->fields) (void)hipFree(b->fields);
-    if (b->inter && b->inter != b->fields) (void)hipFree(b->inter);
+    if (b->inter) (void)hipFree(b->inter);
     if (b->d_tf) (void)hipFree(b->d_tf);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1343,12 +1434,12 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
     return fail(WOS_ERR_HIP);
   }
   if ((e = hipMalloc(&b->fields, nn * n_units * csize(precision))) != hipSuccess) {
+    b->fields = nullptr;
     set_error(std::string("hipMalloc(batch fields): ") + hipGetErrorString(e));
     return fail(WOS_ERR_NOMEM);
   }
   (void)hipMemset(b->fields, 0, nn * n_units * csize(precision));
-  b->inter = b->fields;
-  if (pad && (e = hipMalloc(&b->inter, (size_t)n * M * n_units * csize(precision))) != hipSuccess) {
+  if ((e = hipMalloc(&b->inter, (size_t)n * M * n_units * csize(precision))) != hipSuccess) {
     b->inter = nullptr;
     set_error(std::string("hipMalloc(batch intermediate): ") + hipGetErrorString(e));
     return fail(WOS_ERR_NOMEM);
@@ -1356,6 +1447,7 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
   std::vector<TFUnit> tf(n_units);
   for (int u = 0; u < n_units; ++u) tf[u] = make_asm_tf(wavelengths[u], zs[u], *ctx, M, dx, 0, 0);
   if ((e = hipMalloc((void**)&b->d_tf, sizeof(TFUnit) * n_units)) != hipSuccess) {
+    b->d_tf = nullptr;
     set_error(std::string("hipMalloc(batch params): ") + hipGetErrorString(e));
     return fail(WOS_ERR_NOMEM);
   }
@@ -1370,7 +1462,6 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
     set_error(std::string("twiddles: ") + hipGetErrorString(e));
     return fail(WOS_ERR_HIP);
   }
-  if (const char* w = getenv("WOS_COL_W")) b->col_w = atoi(w);
   *out = b;
   return WOS_OK;
 }
@@ -1381,7 +1472,7 @@ int wos_hip_batch_destroy(wos_batch b) {
   (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->inter && b->inter != b->fields) (void)hipFree(b->inter);
+  if (b->inter) (void)hipFree(b->inter);
   if (b->fields) (void)hipFree(b->fields);
   if (b->d_tf) (void)hipFree(b->d_tf);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1472,18 +1563,83 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
   return WOS_OK;
 }
 
+// One run of the batch timed with HIP events (mean over iters after a warm-up).
+static int batch_time_runs(wos_batch_s* b, int iters, hipEvent_t e0, hipEvent_t e1, double* ms_out) {
+  int rc;
+  if ((rc = batch_enqueue(b))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e0, b->stream));
+  for (int i = 0; i < iters; ++i)
+    if ((rc = batch_enqueue(b))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e1, b->stream));
+  WOS_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms / iters;
+  return 0;
+}
+
+int wos_hip_batch_autotune(wos_batch b, int iters, int* best_w, int* best_rows, double* best_ms) {
+  WOS_ENTER();
+  if (!b || iters <= 0) return WOS_ERR_INVALID;
+  int rc;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  hipEvent_t e0, e1;
+  WOS_HIP_TRY(hipEventCreate(&e0));
+  WOS_HIP_TRY(hipEventCreate(&e1));
+  const int S = (int)csize(b->prec);
+  double best = 1e300;
+  Tuning bt = resolve_tuning(b->prec, b->M, Tuning());
+  const Tuning saved = b->tune;
+  for (int W = 1; W <= 16; W *= 2) {
+    if ((b->M % W) != 0 || !col_w_valid(b->prec, b->M, W)) continue;
+    for (int rows = 1; rows <= 16; rows *= 2) {
+      if (!rows_valid(b->prec, b->M, rows)) continue;
+      if (rows * W * S < 64) continue;  // chunks under half a line: never competitive
+      Tuning t;
+      t.tw = W;
+      t.rows = rows;
+      b->tune = t;
+      double per;
+      if ((rc = batch_time_runs(b, iters, e0, e1, &per))) {
+        b->tune = saved;
+        return rc;
+      }
+      if (per < best) {
+        best = per;
+        bt = t;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tune = bt;
+  if (best_w) *best_w = bt.tw;
+  if (best_rows) *best_rows = bt.rows;
+  if (best_ms) *best_ms = best;
+  return WOS_OK;
+}
+
+int wos_hip_batch_set_tuning(wos_batch b, int col_w, int rows) {
+  WOS_ENTER();
+  if (!b) return WOS_ERR_INVALID;
+  if (col_w <= 0 || (b->M % col_w) != 0 || !col_w_valid(b->prec, b->M, col_w) ||
+      (rows > 0 && !rows_valid(b->prec, b->M, rows))) {
+    set_error("wos_hip_batch_set_tuning: tile width or rows per workgroup not built for this grid");
+    return WOS_ERR_INVALID;
+  }
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  b->tune.tw = col_w;
+  b->tune.rows = rows;
+  return WOS_OK;
+}
+
 int wos_hip_batch_capture(wos_batch b) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
-  if (b->exec) {
-    (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
-  }
-  if (b->graph) {
-    (void)hipGraphDestroy(b->graph);
-    b->graph = nullptr;
-  }
+  batch_drop_graph(b);
   WOS_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = batch_enqueue(b);
   hipGraph_t g = nullptr;
@@ -1538,57 +1694,17 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
   *ms_run = ms / iters;
   if (kernel_ms) {
     // each kernel of the pipeline back to back on the batch stream
-    const int n = b->n, M = b->M, off = b->pad ? n / 2 : 0;
-    const void* tw;
-    if ((rc = get_twiddles(M, b->prec, &tw))) return rc;
-    const long long iunit = b->pad ? (long long)n * M : (long long)n * n;
-    for (int kk = 0; kk < 3; ++kk) {
-      RowArgs ra;
-      memset(&ra, 0, sizeof(ra));
-      ColArgs ca;
-      memset(&ca, 0, sizeof(ca));
-      if (kk == 0 || kk == 2) {
-        const bool fwd = kk == 0;
-        ra.in = fwd ? b->fields : b->inter;
-        ra.out = fwd ? b->inter : b->fields;
-        ra.in_unit = fwd ? (long long)n * n : iunit;
-        ra.out_unit = fwd ? iunit : (long long)n * n;
-        ra.in_ld = fwd ? n : M;
-        ra.out_ld = fwd ? M : n;
-        ra.rows = n;
-        ra.units = b->units;
-        ra.in_col0 = fwd ? off : 0;
-        ra.in_cols = fwd ? n : M;
-        ra.out_col0 = fwd ? 0 : off;
-        ra.out_cols = fwd ? M : n;
-        ra.scale_re = fwd ? 1.0 : 1.0 / ((double)M * M);
-        ra.tw = tw;
-      } else {
-        ca.data = b->inter;
-        ca.unit = iunit;
-        ca.ld = M;
-        ca.cols = M;
-        ca.units = b->units;
-        ca.in_row0 = off;
-        ca.rows = n;
-        ca.out_row0 = off;
-        ca.out_rows = n;
-        ca.scale = 1.0;
-        ca.tf = b->d_tf;
-        ca.tw = tw;
-      }
-      auto launch = [&]() {
-        return kk == 1 ? launch_cols(b->prec, M, 1, 0, b->col_w, ca, b->stream)
-                       : launch_rows(b->prec, M, kk == 0 ? -1 : 1, ra, b->stream);
-      };
-      if ((rc = launch())) return rc;
+    TransferLaunch L;
+    if ((rc = batch_build(b, L))) return rc;
+    for (int k = 0; k < 3; ++k) {
+      if ((rc = launch_transfer_kernel(L, k, b->stream))) return rc;
       WOS_HIP_TRY(hipEventRecord(e0, b->stream));
       for (int i = 0; i < iters; ++i)
-        if ((rc = launch())) return rc;
+        if ((rc = launch_transfer_kernel(L, k, b->stream))) return rc;
       WOS_HIP_TRY(hipEventRecord(e1, b->stream));
       WOS_HIP_TRY(hipEventSynchronize(e1));
       WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-      kernel_ms[kk] = ms / iters;
+      kernel_ms[k] = ms / iters;
     }
   }
   (void)hipEventDestroy(e0);

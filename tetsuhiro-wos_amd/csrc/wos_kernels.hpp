@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "wos_fft.hpp"
 
@@ -27,6 +28,8 @@ struct TFUnit {
   double alpha, limit;     // Tikhonov alpha, evanescent truncation (nepers)
   double fx_off, fy_off;   // carrier offset (asm_shift*), else 0
   double den;              // (double)M * dx: freq(i) = i/den (i <= M/2) else (i-M)/den
+  double inv_den;          // 1/den (FP32 path only; the FP64 path divides exactly)
+  double kz;               // k*z (FP32 path)
   double fr_c;             // fresnel_tf: ((-pi*wl)*z)
   double pre_re, pre_im;   // fresnel_tf: cexpI(k*z)
   int M;                   // transform side
@@ -78,36 +81,118 @@ __device__ __forceinline__ cpx<double> asm_tf_fp64(const TFUnit& u, double fx, d
 }
 
 // FP32 bulk path: the phase is formed in FP64 and wrapped into [-pi, pi]
-// in FP64 before rounding to FP32 for sincosf.
+// in FP64 before rounding to FP32.
 __device__ __forceinline__ float wrap_phase_f32(double ph) {
   const double inv2pi = 0.15915494309189533577;
-  const double two_pi_hi = 6.28318530717958623200;   // fl(2 pi)
-  const double two_pi_lo = 2.44929359829470635445e-16; // 2 pi - fl(2 pi)
+  const double two_pi_hi = 6.28318530717958623200;     // fl(2 pi)
+  const double two_pi_lo = 2.44929359829470635445e-16;  // 2 pi - fl(2 pi)
   double nq = rint(ph * inv2pi);
   double r = fma(-nq, two_pi_hi, ph);
   r = fma(-nq, two_pi_lo, r);
   return (float)r;
 }
 
-__device__ __forceinline__ cpx<float> asm_tf_fp32(const TFUnit& u, double fx, double fy) {
-  double ax = u.wl * fx, ay = u.wl * fy;
-  double rho2 = ax * ax + ay * ay;
-  if (rho2 <= 1) {
-    double ph = u.k * u.z * sqrt(1 - rho2);
-    float s, c;
-    sincosf(wrap_phase_f32(ph), &s, &c);
-    return {c, s};
-  }
-  double decay = u.k * u.az * sqrt(rho2 - 1);
-  if (u.zero_ev || (u.limit > 0 && decay > u.limit)) return {0.f, 0.f};
+// sin/cos for the FP32 path of a phase given in quarter turns (phase =
+// q4 * pi/2): the integer quadrant and the remainder in [-1/2, 1/2] quarter
+// turns are split off in FP64, the remainder is rounded to FP32 and the
+// Cephes single-precision minimax polynomials (S. Moshier, Cephes Math
+// Library) evaluate sin/cos on [-pi/4, pi/4]. Branch-free; exact argument
+// reduction for |q4| < 2^31.
+__device__ __forceinline__ void sincos_quarter_f32(double q4, float* s, float* c) {
+  const double qn = rint(q4);
+  const float yf = (float)(q4 - qn) * 1.57079632679489661923f;
+  const int qi = (int)qn & 3;
+  const float z = yf * yf;
+  const float sp = fmaf(yf * z, fmaf(z, fmaf(z, -1.9515295891e-4f, 8.3321608736e-3f), -1.6666654611e-1f), yf);
+  const float cp = fmaf(z * z, fmaf(z, fmaf(z, 2.443315711809948e-5f, -1.388731625493765e-3f), 4.166664568298827e-2f),
+                        fmaf(-0.5f, z, 1.0f));
+  float ss = (qi & 1) ? cp : sp;
+  float cc = (qi & 1) ? sp : cp;
+  ss = (qi & 2) ? -ss : ss;
+  cc = ((qi + 1) & 2) ? -cc : cc;
+  *s = ss;
+  *c = cc;
+}
+
+// sin/cos of an FP64 phase in radians for the FP32 path.
+__device__ __forceinline__ void sincos_phase_f32(double ph, float* s, float* c) {
+  sincos_quarter_f32(ph * 0.63661977236758134308, s, c);  // 2/pi
+}
+
+// Evanescent branch of asmTF (optics/propagate.go:168-180), out of line.
+__device__ __noinline__ double asm_evanescent(const TFUnit& u, double decay) {
+  if (u.zero_ev || (u.limit > 0 && decay > u.limit)) return 0.0;
   if (u.backward) {
     if (u.reg) {
       double a = exp(decay);
-      return {(float)(a / (1 + u.alpha * u.alpha * a * a)), 0.f};
+      return a / (1 + u.alpha * u.alpha * a * a);
     }
-    return {(float)exp(decay), 0.f};
+    return exp(decay);
   }
-  return {(float)exp(-decay), 0.f};
+  return exp(-decay);
+}
+
+
+// sqrt for the FP32 path's phase: hardware reciprocal square root refined by
+// one Newton step (relative error ~1e-16, enough for a phase kz*s with
+// kz < 1e7 to stay far below the FP32 rounding of the field). 0 for x <= 0.
+__device__ __forceinline__ double sqrt_fast(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  const double g = x * r, h = 0.5 * r;
+  const double d = fma(-g, g, x);
+  const double s = fma(d, h, g);
+  return x > 0 ? s : 0.0;
+}
+
+// Per-column constants of the FP32-path asmTF.
+struct TF32Col {
+  double one_m_ax2;  // 1 - (wl*fx)^2
+  double cwl, coff;  // wl*fy = row*cwl + coff
+  double kq;         // k*z in quarter turns: (k*z)*(2/pi)
+};
+
+__device__ __forceinline__ TF32Col tf32_col(const TFUnit& u, int col) {
+  const double fx = (double)(col <= u.M / 2 ? col : col - u.M) * u.inv_den + u.fx_off;
+  const double ax = u.wl * fx;
+  TF32Col c;
+  c.one_m_ax2 = fma(-ax, ax, 1.0);
+  c.cwl = u.wl * u.inv_den;
+  c.coff = u.wl * u.fy_off;
+  c.kq = u.kz * 0.63661977236758134308;
+  return c;
+}
+
+// Propagating asmTF of the FP32 path at transform row iy: exp(i kz
+// sqrt(1 - rho^2)) with the phase in FP64; rows with rho > 1 return 1 and
+// raise *evan for the evanescent fix-up (not bit-exact with the reference,
+// which only the FP64 path requires).
+__device__ __forceinline__ cpx<float> asm_tf_fp32_prop(const TF32Col& c, int iy, int M, bool* evan) {
+  const double ay = fma((double)(iy <= M / 2 ? iy : iy - M), c.cwl, c.coff);
+  const double x = fma(-ay, ay, c.one_m_ax2);
+  *evan |= x < 0;
+  float sn, cs;
+  sincos_quarter_f32(c.kq * sqrt_fast(x), &sn, &cs);
+  return {cs, sn};
+}
+
+// Evanescent factor of the FP32 path at row iy (rho > 1), else 1.
+__device__ __noinline__ float asm_tf_fp32_evan(const TFUnit& u, const TF32Col& c, int iy) {
+  const double ay = fma((double)(iy <= u.M / 2 ? iy : iy - u.M), c.cwl, c.coff);
+  const double x = fma(-ay, ay, c.one_m_ax2);
+  if (x >= 0) return 1.f;
+  return (float)asm_evanescent(u, u.k * u.az * sqrt(-x));
+}
+
+__device__ __noinline__ cpx<float> asm_tf_fp32_generic(const TFUnit& u, double fx, double fy) {
+  double ax = u.wl * fx, ay = u.wl * fy;
+  double rho2 = ax * ax + ay * ay;
+  if (rho2 <= 1) {
+    float s, c;
+    sincos_phase_f32(u.k * u.z * sqrt(1 - rho2), &s, &c);
+    return {c, s};
+  }
+  const double decay = u.k * u.az * sqrt(rho2 - 1);
+  return {(float)asm_evanescent(u, decay), 0.f};
 }
 
 // Fresnel TF (optics/propagate.go:511-513): cexpI(k z) * cexpI(-pi wl z f^2)
@@ -121,16 +206,26 @@ __device__ __forceinline__ cpx<double> fresnel_tf_fp64(const TFUnit& u, double f
 __device__ __forceinline__ cpx<float> fresnel_tf_fp32(const TFUnit& u, double fx, double fy) {
   double ph = u.fr_c * (fx * fx + fy * fy);
   float s, c;
-  sincosf(wrap_phase_f32(ph), &s, &c);
+  sincos_phase_f32(ph, &s, &c);
   float pr = (float)u.pre_re, pi = (float)u.pre_im;
   return {pr * c - pi * s, pr * s + pi * c};
 }
 
+// Exact FP64 asmTF at transform position (ix, iy), out of line so the
+// sixteen call sites of a column thread share one copy of the code.
+__device__ __noinline__ cpx<double> asm_tf_fp64_at(const TFUnit& u, int ix, int iy) {
+#pragma clang fp contract(off)
+  double fx = freq_of(ix, u.M, u.den) + u.fx_off;
+  double fy = freq_of(iy, u.M, u.den) + u.fy_off;
+  return asm_tf_fp64(u, fx, fy);
+}
+
+// Any transfer-function kind (asm, conjugate asm, fresnel, table), out of line.
 template <typename R>
-__device__ __forceinline__ cpx<R> tf_eval(const TFUnit& u, int ix, int iy);
+__device__ cpx<R> tf_eval(const TFUnit& u, int ix, int iy);
 
 template <>
-__device__ __forceinline__ cpx<double> tf_eval<double>(const TFUnit& u, int ix, int iy) {
+__device__ __noinline__ cpx<double> tf_eval<double>(const TFUnit& u, int ix, int iy) {
 #pragma clang fp contract(off)
   double fx = freq_of(ix, u.M, u.den) + u.fx_off;
   double fy = freq_of(iy, u.M, u.den) + u.fy_off;
@@ -147,13 +242,13 @@ __device__ __forceinline__ cpx<double> tf_eval<double>(const TFUnit& u, int ix, 
 }
 
 template <>
-__device__ __forceinline__ cpx<float> tf_eval<float>(const TFUnit& u, int ix, int iy) {
+__device__ __noinline__ cpx<float> tf_eval<float>(const TFUnit& u, int ix, int iy) {
   double fx = freq_of(ix, u.M, u.den) + u.fx_off;
   double fy = freq_of(iy, u.M, u.den) + u.fy_off;
   switch (u.kind) {
-    case TF_ASM: return asm_tf_fp32(u, fx, fy);
+    case TF_ASM: return asm_tf_fp32_generic(u, fx, fy);
     case TF_CONJ_ASM: {
-      cpx<float> h = asm_tf_fp32(u, fx, fy);
+      cpx<float> h = asm_tf_fp32_generic(u, fx, fy);
       return {h.x, -h.y};
     }
     case TF_FRESNEL: return fresnel_tf_fp32(u, fx, fy);
@@ -196,12 +291,22 @@ __global__ void k_twiddle(cpx<R>* tw, int n) {
 
 // ---------------------------------------------------------------------------
 // Row pass.
+//
+// Intermediate layout. Between the forward and the inverse row pass the
+// spectrum is stored column-tiled: element (row r, column p) of a unit at
+//   (p / TW) * tstride + r * TW + p % TW      (TW a power of two),
+// so the column pass streams each tile of TW columns as one contiguous block
+// and the row passes move whole 128-byte lines (ROWS rows x TW columns of a
+// tile are contiguous). tw_log2 < 0 selects plain row-major (ld) addressing.
+// In-unit offsets are 32-bit (a unit holds at most 2^26 elements).
 // ---------------------------------------------------------------------------
 struct RowArgs {
   const void* in;
   void* out;
   long long in_unit, out_unit;  // elements between units
-  int in_ld, out_ld;            // elements between rows
+  int in_ld, out_ld;            // elements between rows (row-major)
+  int in_tw, out_tw;            // log2 of the column-tile width of a tiled layout (-1: row-major)
+  int in_tstride, out_tstride;  // elements between column tiles (tiled layout)
   int rows;                     // rows per unit processed
   int units;
   int in_col0, in_cols;         // position p reads in[p - in_col0] when in range, else 0
@@ -212,11 +317,27 @@ struct RowArgs {
   const void* tw;               // N-point twiddles
 };
 
+// RowArgs with row-major addressing on both sides and no embed window.
+inline RowArgs row_args_init() {
+  RowArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in_tw = a.out_tw = -1;
+  a.scale_re = 1.0;
+  return a;
+}
+
 template <int N>
 constexpr int rows_per_block() {
   return Plan<N>::T >= 256 ? 1 : 256 / Plan<N>::T;
 }
 
+__device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int tstride) {
+  if (lg < 0) return row * ld + q;
+  return (q >> lg) * tstride + (row << lg) + (q & ((1 << lg) - 1));
+}
+
+// Threads are interleaved over the ROWS rows of a workgroup (row = tid %
+// ROWS), so one wavefront covers 64/ROWS consecutive positions of ROWS rows.
 template <typename R, int N, int DIR, int ROWS>
 __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   constexpr int T = Plan<N>::T, V = Plan<N>::V;
@@ -225,61 +346,102 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   __shared__ cpx<R> lds[ROWS * lds_len(N)];
   __shared__ double red[(NT + 63) / 64];
   const int tid = threadIdx.x;
-  const int lr = tid / T, t = tid % T;
+  const int lr = tid % ROWS, t = tid / ROWS;
   const int bpu = (a.rows + ROWS - 1) / ROWS;
   const int unit = blockIdx.x / bpu;
   const int row = (blockIdx.x % bpu) * ROWS + lr;
   const bool active = row < a.rows && unit < a.units;
-  const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit + (size_t)row * a.in_ld;
-  cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit + (size_t)row * a.out_ld;
+  const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
+  cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
   const cpx<R>* tw = (const cpx<R>*)a.tw;
+  // uniform: every row of the grid is whole and no embed window applies
+  const bool full_in = (a.rows % ROWS) == 0 && a.in_col0 == 0 && a.in_cols == N;
+  const bool full_out = (a.rows % ROWS) == 0 && a.out_col0 == 0 && a.out_cols == N;
 
   cpx<R> v[V];
-  double pin = 0;
+  if (full_in) {
 #pragma unroll
-  for (int b = 0; b < V / R0; ++b)
+    for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-      const int q = slot_pos<N, R0>(t, b, r) - a.in_col0;
-      cpx<R> x = {R(0), R(0)};
-      if (active && q >= 0 && q < a.in_cols) x = in[q];
-      v[b * R0 + r] = x;
-      pin += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
-    }
-  fft_block<R, N, DIR, 1>(v, t, tw, lds + lr * lds_len(N));
-  double pout = 0;
-  const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
-  const bool real_scale = a.scale_im == 0;
+      for (int r = 0; r < R0; ++r)
+        v[b * R0 + r] = in[tiled_offset(slot_pos<N, R0>(t, b, r), row, a.in_ld, a.in_tw, a.in_tstride)];
+  } else {
 #pragma unroll
-  for (int b = 0; b < V / RL; ++b)
+    for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-    for (int r = 0; r < RL; ++r) {
-      const int q = slot_pos<N, RL>(t, b, r) - a.out_col0;
-      if (active && q >= 0 && q < a.out_cols) {
-        cpx<R> y = real_scale ? cscale(v[b * RL + r], sc.x) : cmul(v[b * RL + r], sc);
-        out[q] = y;
-        pout += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+      for (int r = 0; r < R0; ++r) {
+        const int q = slot_pos<N, R0>(t, b, r) - a.in_col0;
+        cpx<R> x = {R(0), R(0)};
+        if (active && q >= 0 && q < a.in_cols) x = in[tiled_offset(q, row, a.in_ld, a.in_tw, a.in_tstride)];
+        v[b * R0 + r] = x;
       }
+  }
+  if (a.pw_in) {  // uniform: FP64 power of the input (zeros outside the window add nothing)
+    double pin = 0;
+#pragma unroll
+    for (int i = 0; i < V; ++i) pin += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
+    block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
+  }
+  fft_block<R, N, DIR, 1>(v, t, tw, lds + lr * lds_len(N));
+  const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
+  if (a.scale_im == 0) {
+    if (a.scale_re != 1) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = cscale(v[i], sc.x);
     }
-  if (a.pw_in) block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
-  if (a.pw_out) block_sum_store<NT>(pout, red, a.pw_out, blockIdx.x);
+  } else {
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);
+  }
+  if (full_out) {
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r)
+        out[tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride)] = v[b * RL + r];
+  } else {
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int q = slot_pos<N, RL>(t, b, r) - a.out_col0;
+        if (active && q >= 0 && q < a.out_cols) out[tiled_offset(q, row, a.out_ld, a.out_tw, a.out_tstride)] = v[b * RL + r];
+      }
+  }
+  if (a.pw_out) {  // uniform: FP64 power of the stored outputs
+    double pout = 0;
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int q = slot_pos<N, RL>(t, b, r) - a.out_col0;
+        const cpx<R> y = v[b * RL + r];
+        if (active && q >= 0 && q < a.out_cols) pout += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+      }
+    block_sum_store<NT>(pout, red, a.pw_out, blockIdx.x);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Column pass over a tile of W adjacent columns. MODE 0: plain transform
-// (direction DIR, output scale); MODE 1: forward -> TF -> inverse (ASM step).
+// (direction DIR, output scale); MODE 1: forward -> asm TF -> inverse (the
+// ASM step); MODE 2: forward -> any TF kind -> inverse. With `tiled` the
+// tile is one contiguous block (row stride W, tiles tstride apart); otherwise
+// rows are ld apart in a row-major array.
 // ---------------------------------------------------------------------------
 struct ColArgs {
   void* data;          // in place
   long long unit;      // elements between units
-  int ld;              // elements between rows
+  int ld;              // elements between rows (row-major)
+  int tiled;           // column-tiled layout with tile width W
+  int tstride;         // elements between tiles (tiled)
   int cols;            // columns per unit (multiple of W)
   int units;
   int in_row0, rows;   // transform position p maps to data row p - in_row0 when in range (else 0)
   int out_row0, out_rows;  // position p is stored to row p - out_row0 when in range
   double scale;
   int xcd_remap;       // spread consecutive tiles over one XCD
-  const TFUnit* tf;    // per unit (MODE 1)
+  const TFUnit* tf;    // per unit (MODE 1, 2)
   const void* tw;
 };
 
@@ -299,48 +461,91 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
   const int bid = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int tpu = a.cols / W;
   const int unit = bid / tpu;
-  const int col = (bid % tpu) * W + c;
-  cpx<R>* base = (cpx<R>*)a.data + (size_t)unit * a.unit + col;
+  const int tile = bid % tpu;
+  const int col = tile * W + c;
+  cpx<R>* base = (cpx<R>*)a.data + (size_t)unit * a.unit + (a.tiled ? tile * a.tstride + c : col);
   const cpx<R>* tw = (const cpx<R>*)a.tw;
   const bool active = unit < a.units;
+  // uniform fast path: contiguous tile, every transform position present
+  const bool fast = a.tiled && a.in_row0 == 0 && a.rows == N && a.out_row0 == 0 && a.out_rows == N && active;
 
-  constexpr int DIR0 = MODE == 1 ? -1 : DIR;
+  constexpr int DIR0 = MODE >= 1 ? -1 : DIR;
   constexpr int R0 = first_rad<N, DIR0>();
   cpx<R> v[V];
+  if (fast) {
 #pragma unroll
-  for (int b = 0; b < V / R0; ++b)
+    for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-      const int q = slot_pos<N, R0>(t, b, r) - a.in_row0;
-      cpx<R> x = {R(0), R(0)};
-      if (active && q >= 0 && q < a.rows) x = base[(size_t)q * a.ld];
-      v[b * R0 + r] = x;
-    }
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = base[slot_pos<N, R0>(t, b, r) * W];
+  } else {
+    const int ld = a.tiled ? W : a.ld;
+#pragma unroll
+    for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        const int q = slot_pos<N, R0>(t, b, r) - a.in_row0;
+        cpx<R> x = {R(0), R(0)};
+        if (active && q >= 0 && q < a.rows) x = base[q * ld];
+        v[b * R0 + r] = x;
+      }
+  }
   fft_block<R, N, DIR0, W>(v, t, tw, lds + c);
-  if constexpr (MODE == 1) {
+  if constexpr (MODE >= 1) {
     constexpr int RL = last_rad<N, -1>();
     if (active) {
       const TFUnit u = a.tf[unit];
+      if constexpr (MODE == 1 && sizeof(R) == 4) {
+        const TF32Col cc = tf32_col(u, col);
+        bool evan = false;
 #pragma unroll
-      for (int b = 0; b < V / RL; ++b)
+        for (int b = 0; b < V / RL; ++b)
 #pragma unroll
-        for (int r = 0; r < RL; ++r) {
-          const int p = slot_pos<N, RL>(t, b, r);
-          v[b * RL + r] = cmul(v[b * RL + r], tf_eval<R>(u, col, p));
+          for (int r = 0; r < RL; ++r)
+            v[b * RL + r] = cmul(v[b * RL + r], asm_tf_fp32_prop(cc, slot_pos<N, RL>(t, b, r), u.M, &evan));
+        if (evan) {  // rare: rows past the evanescent cut-off
+#pragma unroll
+          for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+            for (int r = 0; r < RL; ++r)
+              v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan(u, cc, slot_pos<N, RL>(t, b, r)));
         }
+      } else {
+#pragma unroll
+        for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            const int p = slot_pos<N, RL>(t, b, r);
+            cpx<R> h;
+            if constexpr (MODE == 1) h = asm_tf_fp64_at(u, col, p);
+            else h = tf_eval<R>(u, col, p);
+            v[b * RL + r] = cmul(v[b * RL + r], h);
+          }
+      }
     }
     fft_block<R, N, +1, W>(v, t, tw, lds + c);
   }
-  constexpr int DIRL = MODE == 1 ? +1 : DIR;
+  constexpr int DIRL = MODE >= 1 ? +1 : DIR;
   constexpr int RL = last_rad<N, DIRL>();
   const R sc = (R)a.scale;
+  if (sc != R(1)) {
 #pragma unroll
-  for (int b = 0; b < V / RL; ++b)
+    for (int i = 0; i < V; ++i) v[i] = cscale(v[i], sc);
+  }
+  if (fast) {
 #pragma unroll
-    for (int r = 0; r < RL; ++r) {
-      const int q = slot_pos<N, RL>(t, b, r) - a.out_row0;
-      if (active && q >= 0 && q < a.out_rows) base[(size_t)q * a.ld] = cscale(v[b * RL + r], sc);
-    }
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) base[slot_pos<N, RL>(t, b, r) * W] = v[b * RL + r];
+  } else {
+    const int ld = a.tiled ? W : a.ld;
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int q = slot_pos<N, RL>(t, b, r) - a.out_row0;
+        if (active && q >= 0 && q < a.out_rows) base[q * ld] = v[b * RL + r];
+      }
+  }
 }
 
 }  // namespace wos

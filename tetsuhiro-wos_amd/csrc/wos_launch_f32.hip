@@ -2,9 +2,13 @@
 #include "wos_launch.inc"
 
 namespace wos {
-int launch_rows_f32(int N, int dir, const RowArgs& a, hipStream_t s) { return launch_rows_T<float>(N, dir, a, s); }
+int launch_rows_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s) {
+  return launch_rows_T<float>(N, dir, rows, a, s);
+}
+bool rows_valid_f32(int N, int rows) { return rows_valid_T<float>(N, rows); }
 int launch_cols_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
   return launch_cols_T<float>(N, mode, dir, W, a, s);
 }
 int default_col_w_f32(int N, int cols) { return default_col_w_T<float>(N, cols); }
+bool col_w_valid_f32(int N, int W) { return col_w_valid_T<float>(N, W); }
 }  // namespace wos

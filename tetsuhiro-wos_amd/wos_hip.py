@@ -30,6 +30,7 @@ EXPORTS = [
     "wos_hip_batch_create", "wos_hip_batch_destroy", "wos_hip_batch_upload",
     "wos_hip_batch_download", "wos_hip_batch_fill_schell", "wos_hip_batch_capture",
     "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
+    "wos_hip_batch_autotune", "wos_hip_batch_set_tuning",
 ]
 
 
@@ -103,6 +104,8 @@ def load(path=None):
     L.wos_hip_batch_device_ptr.argtypes = [P]
     L.wos_hip_batch_device_ptr.restype = P
     L.wos_hip_batch_time.argtypes = [P, I, I, C.POINTER(D), P]
+    L.wos_hip_batch_autotune.argtypes = [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D)]
+    L.wos_hip_batch_set_tuning.argtypes = [P, I, I]
     _lib = L
     return L
 
@@ -253,6 +256,16 @@ class Batch:
 
     def fill_schell(self, w0, sigma_c, seed):
         _check(load().wos_hip_batch_fill_schell(self.h, w0, sigma_c, seed), "batch_fill_schell")
+
+    def autotune(self, iters=5):
+        """Pick the column-tile width and row-pass rows; returns (W, rows, ms per run)."""
+        w, r, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(load().wos_hip_batch_autotune(self.h, iters, C.byref(w), C.byref(r), C.byref(ms)),
+               "batch_autotune")
+        return w.value, r.value, ms.value
+
+    def set_tuning(self, col_w, rows=0):
+        _check(load().wos_hip_batch_set_tuning(self.h, col_w, rows), "batch_set_tuning")
 
     def capture(self):
         _check(load().wos_hip_batch_capture(self.h), "batch_capture")
