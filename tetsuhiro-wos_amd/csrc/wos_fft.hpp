@@ -4,8 +4,10 @@
 // threads. Each thread owns V = 16 complex values (4 butterflies of radix 4,
 // 2 of radix 8, or 1 of radix 16); stages exchange data through LDS, the
 // first stage reads global memory directly and the last stage writes global
-// memory directly, both fully coalesced. Twiddles come from a table
-// tw[m] = exp(-2 pi i m / N) generated on the device in FP64.
+// memory directly, both fully coalesced. Twiddles come from per-stage tables
+// generated on the device in FP64: for stage (RAD, Ns) entry [k][r-1] holds
+// exp(-2 pi i r k / (Ns RAD)), so a butterfly reads its RAD-1 twiddles from
+// one base address.
 //
 // Stockham stage (radix RAD, sub-transform length Ns, butterfly j):
 //   k = j mod Ns; v[r] = x[j + r N/RAD] * w^(r k), w = exp(dir 2 pi i/(Ns RAD));
@@ -197,25 +199,30 @@ struct Plan {
     for (int i = 0; i < s; ++i) ns *= inv_rad(i);
     return ns;
   }
+  // twiddle-table offsets: forward stages 1..S-1, then inverse stages 1..S-1
+  static constexpr int tw_off_fwd(int s) {
+    int o = 0;
+    for (int i = 1; i < s; ++i) o += ns_fwd(i) * (fwd_rad(i) - 1);
+    return o;
+  }
+  static constexpr int tw_off_inv(int s) {
+    int o = tw_off_fwd(S);
+    for (int i = 1; i < s; ++i) o += ns_inv(i) * (inv_rad(i) - 1);
+    return o;
+  }
+  static constexpr int tw_total() { return tw_off_inv(S); }
 };
 
 // LDS padding: one slot every 16 to break power-of-two bank strides.
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 constexpr int lds_len(int n) { return n + n / 16 + 1; }
 
-// Twiddle for stage (RAD, Ns) of an N-point transform: w^(r k) with
-// w = exp(DIR 2 pi i /(Ns RAD)) = tw[r k N/(Ns RAD)] (conjugated for DIR>0).
-template <int DIR, typename R>
-__device__ __forceinline__ cpx<R> twiddle(const cpx<R>* __restrict__ tw, int m) {
-  cpx<R> w = tw[m];
-  if constexpr (DIR > 0) w.y = -w.y;
-  return w;
-}
-
 // Apply twiddles and the radix-RAD DFT to the butterflies a thread owns.
-// v[b*RAD + r] holds x[j_b + r N/RAD], j_b = t + b*T.
-template <typename R, int N, int DIR, int RAD, int NS>
-__device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __restrict__ tw) {
+// v[b*RAD + r] holds x[j_b + r N/RAD], j_b = t + b*T. The stage's twiddle
+// table starts at tws + OFF; entry [k][r-1] = exp(-2 pi i r k/(NS RAD)),
+// conjugated for the inverse direction.
+template <typename R, int N, int DIR, int RAD, int NS, int OFF>
+__device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __restrict__ tws) {
   constexpr int V = Plan<N>::V, T = Plan<N>::T;
   constexpr int NB = V / RAD;
 #pragma unroll
@@ -223,13 +230,21 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __
     if constexpr (NS > 1) {
       const int j = t + b * T;
       const int k = j & (NS - 1);
-      constexpr int STRIDE = N / (NS * RAD);
+      const cpx<R>* tk = tws + OFF + k * (RAD - 1);
 #pragma unroll
-      for (int r = 1; r < RAD; ++r) v[b * RAD + r] = cmul(v[b * RAD + r], twiddle<DIR>(tw, r * k * STRIDE));
+      for (int r = 1; r < RAD; ++r) {
+        const cpx<R> w = tk[r - 1];
+        v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], w) : cmulc(v[b * RAD + r], w);
+      }
     }
     dft<RAD, DIR>(v + b * RAD);
   }
 }
+
+// LDS slot of transform position base + c where c = r*X (X = NS for stores,
+// N/RAD for loads) is lpad(base) + c + (c >> 4): base's low four bits plus
+// c mod 16 never carry past 16 in a Stockham stage, so the padding splits
+// into a per-butterfly part and a compile-time part.
 
 // Write a stage's outputs to LDS (Stockham positions), element stride ES
 // between consecutive positions of one transform (ES = W for column tiles).
@@ -241,9 +256,9 @@ __device__ __forceinline__ void stage_store_lds(const cpx<R>* v, int t, cpx<R>* 
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
     const int k = j & (NS - 1);
-    const int base = (j - k) * RAD + k;
+    cpx<R>* p = lds + lpad((j - k) * RAD + k) * ES;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) lds[lpad(base + r * NS) * ES] = v[b * RAD + r];
+    for (int r = 0; r < RAD; ++r) p[(r * NS + ((r * NS) >> 4)) * ES] = v[b * RAD + r];
   }
 }
 
@@ -252,11 +267,13 @@ template <typename R, int N, int RAD, int ES>
 __device__ __forceinline__ void stage_load_lds(cpx<R>* v, int t, const cpx<R>* lds) {
   constexpr int V = Plan<N>::V, T = Plan<N>::T;
   constexpr int NB = V / RAD;
+  constexpr int X = N / RAD;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
+    const cpx<R>* p = lds + lpad(j) * ES;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[lpad(j + r * (N / RAD)) * ES];
+    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = p[(r * X + ((r * X) >> 4)) * ES];
   }
 }
 
@@ -276,7 +293,8 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __res
   if constexpr (S < SEND) {
     constexpr int RAD = DIR < 0 ? Plan<N>::fwd_rad(S) : Plan<N>::inv_rad(S);
     constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
-    stage_compute<R, N, DIR, RAD, NS>(v, t, tw);
+    constexpr int OFF = DIR < 0 ? Plan<N>::tw_off_fwd(S) : Plan<N>::tw_off_inv(S);
+    stage_compute<R, N, DIR, RAD, NS, OFF>(v, t, tw);
     if constexpr (S + 1 < SEND) {
       constexpr int RADN = DIR < 0 ? Plan<N>::fwd_rad(S + 1) : Plan<N>::inv_rad(S + 1);
       __syncthreads();  // previous readers of lds are done

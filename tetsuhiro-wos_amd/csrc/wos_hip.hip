@@ -114,6 +114,29 @@ int ensure_init() {
   return 0;
 }
 
+// Host mirror of Plan<N> (wos_fft.hpp): stage radices of the forward and
+// inverse plans.
+static void plan_stages(int n, std::vector<int>& fwd, std::vector<int>& inv) {
+  fwd.clear();
+  inv.clear();
+  if (n < 16) {
+    fwd.push_back(n);
+    inv.push_back(n);
+    return;
+  }
+  int lg = 0;
+  while ((1 << lg) < n) ++lg;
+  const int n16 = lg / 4, rem = 1 << (lg % 4);
+  for (int i = 0; i < n16; ++i) fwd.push_back(16);
+  if (rem > 1) {
+    fwd.push_back(rem);
+    inv.push_back(rem);
+  }
+  for (int i = 0; i < n16; ++i) inv.push_back(16);
+}
+
+// Stage twiddle tables of an n-point transform (forward stages 1..S-1, then
+// inverse stages 1..S-1, the layout Plan<N>::tw_off_* addresses), cached.
 int get_twiddles(int n, int prec, const void** out) {
   Device& d = device();
   long long key = (long long)n * 2 + prec;
@@ -122,12 +145,30 @@ int get_twiddles(int n, int prec, const void** out) {
     *out = it->second;
     return 0;
   }
+  std::vector<int> fwd, inv;
+  plan_stages(n, fwd, inv);
+  struct St { int off, ns, rad; };
+  std::vector<St> st;
+  int off = 0;
+  for (const std::vector<int>* plan : {&fwd, &inv}) {
+    int ns = (*plan)[0];
+    for (size_t i = 1; i < plan->size(); ++i) {
+      const int rad = (*plan)[i];
+      st.push_back({off, ns, rad});
+      off += ns * (rad - 1);
+      ns *= rad;
+    }
+  }
   void* p = nullptr;
-  WOS_HIP_TRY(hipMalloc(&p, (size_t)n * csize(prec)));
-  int bs = 256, gs = (n + bs - 1) / bs;
-  if (prec == WOS_FP64) hipLaunchKernelGGL(k_twiddle<double>, dim3(gs), dim3(bs), 0, d.stream, (cpx<double>*)p, n);
-  else hipLaunchKernelGGL(k_twiddle<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, n);
-  WOS_HIP_TRY(hipGetLastError());
+  WOS_HIP_TRY(hipMalloc(&p, (size_t)std::max(off, 1) * csize(prec)));
+  for (const St& q : st) {
+    const int cnt = q.ns * (q.rad - 1), bs = 256, gs = (cnt + bs - 1) / bs;
+    if (prec == WOS_FP64)
+      hipLaunchKernelGGL(k_stage_twiddles<double>, dim3(gs), dim3(bs), 0, d.stream, (cpx<double>*)p, q.off, q.ns, q.rad);
+    else
+      hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, q.off, q.ns, q.rad);
+    WOS_HIP_TRY(hipGetLastError());
+  }
   d.twiddles[key] = p;
   *out = p;
   return 0;

@@ -278,15 +278,18 @@ __device__ __forceinline__ void block_sum_store(double v, double* red, double* o
 }
 
 // ---------------------------------------------------------------------------
-// Twiddle table tw[m] = exp(-2 pi i m / n), m in [0, n), built in FP64.
+// Stage twiddle table of one Stockham stage (NS, RAD), built in FP64:
+// tw[off + k*(RAD-1) + r-1] = exp(-2 pi i r k / (NS RAD)), k < NS, 1 <= r < RAD.
+// The angle -2 r k/(NS RAD) (in units of pi) is exact: NS*RAD is a power of two.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_twiddle(cpx<R>* tw, int n) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
+__global__ void k_stage_twiddles(cpx<R>* tw, int off, int ns, int rad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ns * (rad - 1)) return;
+  const int k = e / (rad - 1), r = e % (rad - 1) + 1;
   double s, c;
-  sincospi(-2.0 * (double)m / (double)n, &s, &c);
-  tw[m] = {(R)c, (R)s};
+  sincospi(-2.0 * (double)(r * k) / (double)(ns * rad), &s, &c);
+  tw[off + e] = {(R)c, (R)s};
 }
 
 // ---------------------------------------------------------------------------
