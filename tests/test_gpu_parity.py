@@ -183,3 +183,71 @@ def test_propagate1d_fp64_parity(wos, oracle, n):
     out, wg = wos.propagate1d(a, dx, 0.3, wl, wos.ctx_for(wl))
     assert rel_l2(out, ref) <= FP64_TOL
     assert wg.as_dict().keys() == wo.as_dict().keys()
+
+
+# ---------------------------------------------------------------- non-power-of-two (Bluestein)
+
+@pytest.mark.parametrize("n", [12, 27, 100, 384, 1000])
+def test_fft2d_bluestein_matches_oracle(wos, oracle, n):
+    """TestGPUFFT2DMatchesCPU's non-power-of-two sizes (optics/gpu_test.go:54),
+    through the chirp-z path of optics/fft.go:159-204."""
+    src = oracle.ref_field(n, 1.3)
+    cpu = oracle.fft2d(src)
+    gpu = wos.fft2d(src)
+    err = rel_l2(gpu, cpu)
+    print(f"n={n} bluestein fft2d rel-L2 {err:.3e} max {rel_max(gpu, cpu):.3e}", flush=True)
+    assert err <= FP64_TOL and rel_max(gpu, cpu) <= FP64_TOL
+    back = wos.fft2d(gpu, inverse=True)
+    assert np.max(np.abs(back - src)) <= 1e-11
+
+
+@pytest.mark.parametrize("n", [100, 1000, 2049])
+def test_fft1d_bluestein_matches_oracle(wos, oracle, n):
+    i = np.arange(n)
+    src = np.sin(i * 0.01) + 1j * np.cos(i * 0.017)
+    cpu = oracle.fft1d(src)
+    gpu = wos.fft1d(src)
+    assert rel_max(gpu, cpu) <= FP64_TOL
+    assert np.max(np.abs(wos.fft1d(gpu, inverse=True) - src)) <= 1e-11
+
+
+@pytest.mark.parametrize("method", ["asm", "asm_pad", "asm_shift", "asm_shift_pad", "fresnel_tf", "fresnel_ir"])
+def test_methods_non_pow2_grid_fp64_parity(wos, oracle, method):
+    """ASM family on an even non-power-of-two grid (TestASMNonPowerOfTwoGrid,
+    optics/asm_improve_test.go:87-107)."""
+    n, wl = 100, 632.8e-9
+    ex, dx = gaussian_tilted(oracle, n, 0.004, wl, 4e-4, 0.003)
+    ref, _, wo, so = oracle.propagate(ex, None, dx, 0.03, method, oracle.ctx_for(wl))
+    out, _, wg, sg = wos.propagate(ex, None, dx, 0.03, method, wos.ctx_for(wl))
+    err = rel_l2(out, ref)
+    print(f"{method} n=100 FP64 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    assert sg.method_used == so.method_used
+    assert wg.as_dict().keys() == wo.as_dict().keys()
+
+
+def config2_input(oracle, seed=7):
+    """1000^2, dx=1.5um, 532nm: x-polarised plane wave times an i.i.d. uniform
+    random phase (diffuser hash, one value per pixel) inside a circular
+    aperture of diameter 1.2 mm; Ey = 0 (Polarized)."""
+    n, dx, wl = 1000, 1.5e-6, 532e-9
+    ex = oracle.random_phase(n, seed)
+    oracle.apply_circle_aperture(ex, None, dx, 0.6e-3)
+    ey = np.zeros_like(ex)
+    return ex, ey, n, dx, wl
+
+
+def test_config2_asm_pad_jones_fp64_and_fp32(wos, oracle):
+    ex, ey, n, dx, wl = config2_input(oracle)
+    z = 10e-3
+    rx, ry, wo, so = oracle.propagate(ex, ey, dx, z, "asm_pad", oracle.ctx_for(wl))
+    gx, gy, wg, sg = wos.propagate(ex, ey, dx, z, "asm_pad", wos.ctx_for(wl))
+    err = rel_l2(np.concatenate([gx, gy]), np.concatenate([rx, ry]))
+    print(f"config2 asm_pad FP64 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    assert wg.as_dict().keys() == wo.as_dict().keys()
+    assert abs(sg.p_out - so.p_out) <= 1e-12 * so.p_out
+    fx, fy, _, _ = wos.propagate(ex, ey, dx, z, "asm_pad", wos.ctx_for(wl), precision=wos.FP32)
+    err32 = rel_l2(np.concatenate([fx, fy]), np.concatenate([rx, ry]))
+    print(f"config2 asm_pad FP32 rel-L2 {err32:.3e}", flush=True)
+    assert err32 <= FP32_TOL

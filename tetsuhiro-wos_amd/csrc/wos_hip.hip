@@ -17,6 +17,7 @@
 
 #include "../../include/wos_hip.h"
 #include "wos_engine.hpp"
+#include "wos_bluestein.hpp"
 
 namespace wos {
 
@@ -30,12 +31,28 @@ int default_col_w_f64(int N, int cols);
 int default_col_w_f32(int N, int cols);
 bool col_w_valid_f64(int N, int W);
 bool col_w_valid_f32(int N, int W);
+int launch_rows_bs_f64(int M, const BsRowArgs& A, hipStream_t s);
+int launch_rows_bs_f32(int M, const BsRowArgs& A, hipStream_t s);
+int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
+int launch_cols_bs_f32(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
+int bs_default_w_f64(int M, int cols);
+int bs_default_w_f32(int M, int cols);
+static int launch_rows_bs(int prec, int M, const BsRowArgs& A, hipStream_t s);
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
 bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 int max_pow2_len() { return 8192; }
+// internal power-of-two length of a Bluestein transform of length L
+static int bs_len(int L) {
+  int m = 1;
+  while (m < 2 * L - 1) m <<= 1;
+  return m;
+}
+// lengths the engine transforms: powers of two up to 8192, others through
+// Bluestein with an internal length up to 8192
+static bool len_ok(int L) { return L >= 1 && (pow2(L) ? L <= max_pow2_len() : bs_len(L) <= max_pow2_len()); }
 
 static size_t csize(int prec) { return prec == WOS_FP64 ? 16 : 8; }
 
@@ -174,6 +191,108 @@ int get_twiddles(int n, int prec, const void** out) {
   return 0;
 }
 
+__global__ void k_convert_scale_f64(const cpx<double>* __restrict__ in, cpx<double>* __restrict__ out, int n, double sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = {in[i].x * sc, in[i].y * sc};
+}
+__global__ void k_convert_scale_f32(const cpx<double>* __restrict__ in, cpx<float>* __restrict__ out, int n, double sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = {(float)(in[i].x * sc), (float)(in[i].y * sc)};
+}
+
+int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows);
+
+// Bluestein tables of (L, direction) in precision prec: chirp[L] and
+// FFT_M(b)/M, generated in FP64 on the device (the spectrum by the engine's
+// own M-point FFT) and rounded to prec. Cached.
+static int get_bs_tables(int L, int inverse, int prec, BsTab* out) {
+  Device& d = device();
+  const long long key = ((long long)L * 2 + inverse) * 2 + prec;
+  auto it = d.chirps.find(key);
+  if (it != d.chirps.end()) {
+    out->chirp = it->second;
+    out->bspec = (const char*)it->second + (size_t)L * csize(prec);
+    return 0;
+  }
+  const int M = bs_len(L);
+  if (M > max_pow2_len()) {
+    set_error("Bluestein length not supported: " + std::to_string(L));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  int rc;
+  void* tmp = nullptr;  // FP64 chirp[L] then b[M]
+  WOS_HIP_TRY(hipMalloc(&tmp, (size_t)(L + M) * 16));
+  cpx<double>* c64 = (cpx<double>*)tmp;
+  cpx<double>* b64 = c64 + L;
+  hipLaunchKernelGGL(k_bs_chirp, dim3((L + 255) / 256), dim3(256), 0, d.stream, c64, L, inverse ? 1.0 : -1.0);
+  hipLaunchKernelGGL(k_bs_b0, dim3((M + 255) / 256), dim3(256), 0, d.stream, (const cpx<double>*)c64, b64, L, M);
+  WOS_HIP_TRY(hipGetLastError());
+  const void* tw;
+  if ((rc = get_twiddles(M, WOS_FP64, &tw))) return rc;
+  RowArgs ra = row_args_init();
+  ra.in = b64;
+  ra.out = b64;
+  ra.in_unit = ra.out_unit = M;
+  ra.in_ld = ra.out_ld = M;
+  ra.rows = 1;
+  ra.units = 1;
+  ra.in_cols = M;
+  ra.out_cols = M;
+  ra.tw = tw;
+  if ((rc = launch_rows(WOS_FP64, M, -1, ra, d.stream, 0))) return rc;
+  void* p = nullptr;
+  WOS_HIP_TRY(hipMalloc(&p, (size_t)(L + M) * csize(prec)));
+  if (prec == WOS_FP64) {
+    hipLaunchKernelGGL(k_convert_scale_f64, dim3((L + 255) / 256), dim3(256), 0, d.stream, (const cpx<double>*)c64, (cpx<double>*)p, L, 1.0);
+    hipLaunchKernelGGL(k_convert_scale_f64, dim3((M + 255) / 256), dim3(256), 0, d.stream, (const cpx<double>*)b64, (cpx<double>*)p + L, M, 1.0 / M);
+  } else {
+    hipLaunchKernelGGL(k_convert_scale_f32, dim3((L + 255) / 256), dim3(256), 0, d.stream, (const cpx<double>*)c64, (cpx<float>*)p, L, 1.0);
+    hipLaunchKernelGGL(k_convert_scale_f32, dim3((M + 255) / 256), dim3(256), 0, d.stream, (const cpx<double>*)b64, (cpx<float>*)p + L, M, 1.0 / M);
+  }
+  WOS_HIP_TRY(hipGetLastError());
+  WOS_HIP_TRY(hipStreamSynchronize(d.stream));
+  (void)hipFree(tmp);
+  d.chirps[key] = p;
+  out->chirp = p;
+  out->bspec = (const char*)p + (size_t)L * csize(prec);
+  return 0;
+}
+
+// In-place row transforms of `units` x rows rows of length L (row-major, ld)
+// in either engine path.
+static int rows_any(int prec, void* d, int L, int rows, int ld, long long unit, int units, int inverse, double scale,
+                    hipStream_t s, double* pw_in = nullptr, double* pw_out = nullptr) {
+  int rc;
+  RowArgs ra = row_args_init();
+  ra.pw_in = pw_in;
+  ra.pw_out = pw_out;
+  ra.in = d;
+  ra.out = d;
+  ra.in_unit = ra.out_unit = unit;
+  ra.in_ld = ra.out_ld = ld;
+  ra.rows = rows;
+  ra.units = units;
+  ra.in_cols = L;
+  ra.out_cols = L;
+  if (pow2(L)) {
+    const void* tw;
+    if ((rc = get_twiddles(L, prec, &tw))) return rc;
+    ra.tw = tw;
+    ra.scale_re = scale;
+    return launch_rows(prec, L, inverse ? 1 : -1, ra, s, 0);
+  }
+  const int M = bs_len(L);
+  BsRowArgs A;
+  A.io = ra;
+  A.len = L;
+  A.post = scale;
+  if ((rc = get_bs_tables(L, inverse, prec, &A.tab))) return rc;
+  const void* tw;
+  if ((rc = get_twiddles(M, prec, &tw))) return rc;
+  A.io.tw = tw;
+  return launch_rows_bs(prec, M, A, s);
+}
+
 int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows) {
   return prec == WOS_FP64 ? launch_rows_f64(N, dir, rows, a, s) : launch_rows_f32(N, dir, rows, a, s);
 }
@@ -189,6 +308,15 @@ int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hip
 }
 int default_col_w(int prec, int N, int cols) {
   return prec == WOS_FP64 ? default_col_w_f64(N, cols) : default_col_w_f32(N, cols);
+}
+static int launch_rows_bs(int prec, int M, const BsRowArgs& A, hipStream_t s) {
+  return prec == WOS_FP64 ? launch_rows_bs_f64(M, A, s) : launch_rows_bs_f32(M, A, s);
+}
+static int launch_cols_bs(int prec, int M, int mode, int W, const BsColArgs& A, hipStream_t s) {
+  return prec == WOS_FP64 ? launch_cols_bs_f64(M, mode, W, A, s) : launch_cols_bs_f32(M, mode, W, A, s);
+}
+static int bs_default_w(int prec, int M, int cols) {
+  return prec == WOS_FP64 ? bs_default_w_f64(M, cols) : bs_default_w_f32(M, cols);
 }
 
 // rows handled by one row-pass workgroup (mirrors rows_per_block<N>())
@@ -578,29 +706,14 @@ static void asm_warn(wos_warnings* w, double z, double p_in, double p_out, int z
 // In-place 2-D transform of `units` n x n fields (power-of-two n).
 static int fft2d_dev(int prec, void* d, int n, int units, int inverse) {
   Device& dv = device();
-  if (!pow2(n) || n > max_pow2_len()) {
+  if (!len_ok(n)) {
     set_error("2-D FFT length not supported: " + std::to_string(n));
     return WOS_ERR_UNSUPPORTED;
   }
-  const void* tw;
-  int rc = get_twiddles(n, prec, &tw);
-  if (rc) return rc;
-  const int dir = inverse ? 1 : -1;
   const double sc = inverse ? 1.0 / (double)n : 1.0;
-  RowArgs ra = row_args_init();
-  ra.in = d;
-  ra.out = d;
-  ra.in_unit = ra.out_unit = (long long)n * n;
-  ra.in_ld = ra.out_ld = n;
-  ra.rows = n;
-  ra.units = units;
-  ra.in_col0 = 0;
-  ra.in_cols = n;
-  ra.out_col0 = 0;
-  ra.out_cols = n;
-  ra.scale_re = sc;
-  ra.tw = tw;
-  if ((rc = launch_rows(prec, n, dir, ra, dv.stream))) return rc;
+  int rc;
+  // rows (fft2DCPU order: rows, then columns; inverse scales 1/n per pass)
+  if ((rc = rows_any(prec, d, n, n, n, (long long)n * n, units, inverse, sc, dv.stream))) return rc;
   ColArgs ca;
   memset(&ca, 0, sizeof(ca));
   ca.data = d;
@@ -612,9 +725,28 @@ static int fft2d_dev(int prec, void* d, int n, int units, int inverse) {
   ca.rows = n;
   ca.out_row0 = 0;
   ca.out_rows = n;
-  ca.scale = sc;
-  ca.tw = tw;
-  return launch_cols(prec, n, 0, dir, 0, ca, dv.stream);
+  if (pow2(n)) {
+    const void* tw;
+    if ((rc = get_twiddles(n, prec, &tw))) return rc;
+    ca.scale = sc;
+    ca.tw = tw;
+    return launch_cols(prec, n, 0, inverse ? 1 : -1, 0, ca, dv.stream);
+  }
+  const int M = bs_len(n);
+  BsColArgs A;
+  A.io = ca;
+  A.io.scale = 1.0;
+  A.len = n;
+  A.post0 = sc;
+  A.post1 = 1.0;
+  if ((rc = get_bs_tables(n, inverse, prec, &A.tab0))) return rc;
+  A.tab1 = A.tab0;
+  const void* tw;
+  if ((rc = get_twiddles(M, prec, &tw))) return rc;
+  A.io.tw = tw;
+  const int W = bs_default_w(prec, M, n);
+  A.io.cols = n;
+  return launch_cols_bs(prec, M, 0, W, A, dv.stream);
 }
 
 struct TransferOut {
@@ -660,6 +792,11 @@ struct TransferLaunch {
   Tuning tu;
   RowArgs fwd, inv;
   ColArgs col;
+  // non-power-of-two M: Bluestein kernels with internal length Mb
+  bool bs = false;
+  int Mb = 0;
+  BsRowArgs bfwd, binv;
+  BsColArgs bcol;
 };
 
 static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int n, int units, bool pad,
@@ -667,19 +804,28 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
                           double* pw_out, Tuning tu) {
   const int M = pad ? 2 * n : n;
   const int off = pad ? n / 2 : 0;
-  if (!pow2(M) || M > max_pow2_len()) {
+  if (!len_ok(M)) {
     set_error("transform length not supported: " + std::to_string(M));
     return WOS_ERR_UNSUPPORTED;
   }
-  if (tf_kind != TF_ASM) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
-  tu = resolve_tuning(prec, M, tu);
+  const bool bs = !pow2(M);
+  const int Mk = bs ? bs_len(M) : M;  // kernel FFT length
+  if (bs) {
+    tu.tw = bs_default_w(prec, Mk, M);
+    tu.rows = rows_per_block_host(Mk);
+  } else {
+    if (tf_kind != TF_ASM) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
+    tu = resolve_tuning(prec, M, tu);
+  }
   const void* tw;
-  int rc = get_twiddles(M, prec, &tw);
+  int rc = get_twiddles(Mk, prec, &tw);
   if (rc) return rc;
   L.prec = prec;
   L.M = M;
   L.mode = tf_kind == TF_ASM ? 1 : 2;
   L.tu = tu;
+  L.bs = bs;
+  L.Mb = Mk;
   const long long iunit = (long long)n * M;
   const int tstride = n * tu.tw;
   const int lg = ilog2(tu.tw);
@@ -737,11 +883,46 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   rb.scale_im = post_im * inv;
   rb.pw_out = pw_out;
   rb.tw = tw;
+  if (bs) {
+    // fft2DCPU's per-pass 1/M of the inverse is applied by the Bluestein posts
+    BsTab fwd_t, inv_t;
+    if ((rc = get_bs_tables(M, 0, prec, &fwd_t))) return rc;
+    if ((rc = get_bs_tables(M, 1, prec, &inv_t))) return rc;
+    L.bfwd.io = ra;
+    L.bfwd.len = M;
+    L.bfwd.tab = fwd_t;
+    L.bfwd.post = 1.0;
+    L.bcol.io = ca;
+    L.bcol.len = M;
+    L.bcol.tab0 = fwd_t;
+    L.bcol.tab1 = inv_t;
+    L.bcol.post0 = 1.0;
+    L.bcol.post1 = 1.0 / (double)M;
+    L.binv.io = rb;
+    L.binv.io.scale_re = post_re;
+    L.binv.io.scale_im = post_im;
+    L.binv.len = M;
+    L.binv.tab = inv_t;
+    L.binv.post = 1.0 / (double)M;
+  }
   return 0;
+}
+
+// Set the power-partial outputs of a built transfer.
+static void set_transfer_power(TransferLaunch& L, double* pw_in, double* pw_out) {
+  L.fwd.pw_in = pw_in;
+  L.inv.pw_out = pw_out;
+  L.bfwd.io.pw_in = pw_in;
+  L.binv.io.pw_out = pw_out;
 }
 
 // kernel k of a transfer step: 0 forward rows, 1 columns, 2 inverse rows
 static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s) {
+  if (L.bs) {
+    if (k == 0) return launch_rows_bs(L.prec, L.Mb, L.bfwd, s);
+    if (k == 1) return launch_cols_bs(L.prec, L.Mb, 2, L.tu.tw, L.bcol, s);
+    return launch_rows_bs(L.prec, L.Mb, L.binv, s);
+  }
   if (k == 0) return launch_rows(L.prec, L.M, -1, L.fwd, s, L.tu.rows);
   if (k == 1) return launch_cols(L.prec, L.M, L.mode, 0, L.tu.tw, L.col, s);
   return launch_rows(L.prec, L.M, 1, L.inv, s, L.tu.rows);
@@ -762,28 +943,27 @@ static int transfer_dev(int prec, void* d, int n, int units, bool pad, const std
   Device& dv = device();
   const int M = pad ? 2 * n : n;
   int rc;
-  if (!pow2(M) || M > max_pow2_len()) {
+  if (!len_ok(M)) {
     set_error("transform length not supported: " + std::to_string(M));
     return WOS_ERR_UNSUPPORTED;
   }
   if ((rc = dv.scratch.ensure((size_t)units * n * M * csize(prec)))) return rc;
-  void* inter = dv.scratch.p;
   if ((rc = dv.params.ensure(sizeof(TFUnit) * units))) return rc;
   if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * units))) return rc;
-  Tuning tu;
-  if (tf[0].kind != TF_ASM) tu.tw = default_col_w(prec, M, M);
-  tu = resolve_tuning(prec, M, tu);
-  const int rpb = tu.rows;
+  TransferLaunch L;
+  if ((rc = build_transfer(L, prec, d, dv.scratch.p, n, units, pad, (const TFUnit*)dv.params.p, tf[0].kind, post_re,
+                           post_im, nullptr, nullptr, Tuning())))
+    return rc;
+  const int rpb = L.tu.rows;
   const int bpu = (n + rpb - 1) / rpb;
   const int nblocks = bpu * units;
   if (pw) {
     if ((rc = dv.partial_in.ensure(sizeof(double) * nblocks))) return rc;
     if ((rc = dv.partial_out.ensure(sizeof(double) * nblocks))) return rc;
+    set_transfer_power(L, (double*)dv.partial_in.p, (double*)dv.partial_out.p);
   }
-  if ((rc = enqueue_transfer(prec, d, inter, n, units, pad, (const TFUnit*)dv.params.p, tf[0].kind, post_re, post_im,
-                             pw ? (double*)dv.partial_in.p : nullptr, pw ? (double*)dv.partial_out.p : nullptr,
-                             dv.stream, tu)))
-    return rc;
+  for (int k = 0; k < 3; ++k)
+    if ((rc = launch_transfer_kernel(L, k, dv.stream))) return rc;
   if (pw) {
     std::vector<double> hin(nblocks), hout(nblocks);
     WOS_HIP_TRY(hipMemcpyAsync(hin.data(), dv.partial_in.p, sizeof(double) * nblocks, hipMemcpyDeviceToHost, dv.stream));
@@ -999,7 +1179,7 @@ static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double 
       Device& dv = device();
       const double k = 2.0 * M_PI / wl;
       const int m = 2 * n;
-      if (!pow2(m) || m > max_pow2_len()) {
+      if (!len_ok(m)) {
         set_error("fresnel_ir transform length not supported");
         return WOS_ERR_UNSUPPORTED;
       }
@@ -1147,7 +1327,7 @@ static int fft1d_host(int prec, void* a, int n, int inverse) {
   }
   if (n == 1) return WOS_OK;
   Device& dv = device();
-  if (!pow2(n) || n > max_pow2_len()) {
+  if (!len_ok(n)) {
     set_error("1-D FFT length not supported: " + std::to_string(n));
     return WOS_ERR_UNSUPPORTED;
   }
@@ -1155,20 +1335,7 @@ static int fft1d_host(int prec, void* a, int n, int inverse) {
   int rc;
   if ((rc = dv.field.ensure(bytes))) return rc;
   WOS_HIP_TRY(hipMemcpyAsync(dv.field.p, a, bytes, hipMemcpyHostToDevice, dv.stream));
-  const void* tw;
-  if ((rc = get_twiddles(n, prec, &tw))) return rc;
-  RowArgs ra = row_args_init();
-  ra.in = dv.field.p;
-  ra.out = dv.field.p;
-  ra.in_unit = ra.out_unit = n;
-  ra.in_ld = ra.out_ld = n;
-  ra.rows = 1;
-  ra.units = 1;
-  ra.in_cols = n;
-  ra.out_cols = n;
-  ra.scale_re = inverse ? 1.0 / (double)n : 1.0;
-  ra.tw = tw;
-  if ((rc = launch_rows(prec, n, inverse ? 1 : -1, ra, dv.stream))) return rc;
+  if ((rc = rows_any(prec, dv.field.p, n, 1, n, n, 1, inverse, inverse ? 1.0 / (double)n : 1.0, dv.stream))) return rc;
   WOS_HIP_TRY(hipMemcpyAsync(a, dv.field.p, bytes, hipMemcpyDeviceToHost, dv.stream));
   WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
   return WOS_OK;
@@ -1223,7 +1390,7 @@ int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const 
     set_error("wos_hip_propagate1d: bad arguments");
     return WOS_ERR_INVALID;
   }
-  if (!pow2(n) || n > max_pow2_len()) {
+  if (!len_ok(n)) {
     set_error("1-D propagation length not supported: " + std::to_string(n));
     return WOS_ERR_UNSUPPORTED;
   }
@@ -1233,34 +1400,17 @@ int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const 
   void* d;
   int rc;
   if ((rc = upload_fields(precision, src, 1, (size_t)n, &d))) return rc;
-  const void* tw;
-  if ((rc = get_twiddles(n, precision, &tw))) return rc;
   if ((rc = dv.partial_in.ensure(sizeof(double)))) return rc;
   if ((rc = dv.partial_out.ensure(sizeof(double)))) return rc;
-  RowArgs ra = row_args_init();
-  ra.in = d;
-  ra.out = d;
-  ra.in_unit = ra.out_unit = n;
-  ra.in_ld = ra.out_ld = n;
-  ra.rows = 1;
-  ra.units = 1;
-  ra.in_cols = n;
-  ra.out_cols = n;
-  ra.scale_re = 1.0;
-  ra.pw_in = (double*)dv.partial_in.p;
-  ra.tw = tw;
-  if ((rc = launch_rows(precision, n, -1, ra, dv.stream))) return rc;
+  if ((rc = rows_any(precision, d, n, 1, n, n, 1, 0, 1.0, dv.stream, (double*)dv.partial_in.p, nullptr))) return rc;
   TFUnit u = make_asm_tf(wl, z, *ctx, n, dx, 0, 0);
   if (precision == WOS_FP64)
     hipLaunchKernelGGL(k_tf1d<double>, dim3(grid_for(n)), dim3(256), 0, dv.stream, (cpx<double>*)d, n, dx, u);
   else
     hipLaunchKernelGGL(k_tf1d<float>, dim3(grid_for(n)), dim3(256), 0, dv.stream, (cpx<float>*)d, n, dx, u);
   WOS_HIP_TRY(hipGetLastError());
-  RowArgs rb = ra;
-  rb.pw_in = nullptr;
-  rb.pw_out = (double*)dv.partial_out.p;
-  rb.scale_re = 1.0 / (double)n;
-  if ((rc = launch_rows(precision, n, 1, rb, dv.stream))) return rc;
+  if ((rc = rows_any(precision, d, n, 1, n, n, 1, 1, 1.0 / (double)n, dv.stream, nullptr, (double*)dv.partial_out.p)))
+    return rc;
   double p_in = 0, p_out = 0;
   WOS_HIP_TRY(hipMemcpyAsync(&p_in, dv.partial_in.p, sizeof(double), hipMemcpyDeviceToHost, dv.stream));
   WOS_HIP_TRY(hipMemcpyAsync(&p_out, dv.partial_out.p, sizeof(double), hipMemcpyDeviceToHost, dv.stream));
@@ -1448,7 +1598,7 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
   }
   const bool pad = method == WOS_METHOD_ASM_PAD;
   const int M = pad ? 2 * n : n;
-  if (!pow2(M) || M > max_pow2_len()) {
+  if (!len_ok(M)) {
     set_error("wos_hip_batch_create: transform length not supported");
     return WOS_ERR_UNSUPPORTED;
   }
@@ -1630,6 +1780,16 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* best_w, int* best_rows, 
   WOS_HIP_TRY(hipEventCreate(&e1));
   const int S = (int)csize(b->prec);
   double best = 1e300;
+  if (!pow2(b->M)) {  // Bluestein grids run their fixed launch shape
+    double per;
+    if ((rc = batch_time_runs(b, iters, e0, e1, &per))) return rc;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best_w) *best_w = 0;
+    if (best_rows) *best_rows = 0;
+    if (best_ms) *best_ms = per;
+    return WOS_OK;
+  }
   Tuning bt = resolve_tuning(b->prec, b->M, Tuning());
   const Tuning saved = b->tune;
   for (int W = 1; W <= 16; W *= 2) {
@@ -1664,7 +1824,7 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* best_w, int* best_rows, 
 int wos_hip_batch_set_tuning(wos_batch b, int col_w, int rows) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
-  if (col_w <= 0 || (b->M % col_w) != 0 || !col_w_valid(b->prec, b->M, col_w) ||
+  if (!pow2(b->M) || col_w <= 0 || (b->M % col_w) != 0 || !col_w_valid(b->prec, b->M, col_w) ||
       (rows > 0 && !rows_valid(b->prec, b->M, rows))) {
     set_error("wos_hip_batch_set_tuning: tile width or rows per workgroup not built for this grid");
     return WOS_ERR_INVALID;

@@ -11,4 +11,9 @@ int launch_cols_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream
 }
 int default_col_w_f32(int N, int cols) { return default_col_w_T<float>(N, cols); }
 bool col_w_valid_f32(int N, int W) { return col_w_valid_T<float>(N, W); }
+int launch_rows_bs_f32(int M, const BsRowArgs& A, hipStream_t s) { return launch_rows_bs_T<float>(M, A, s); }
+int launch_cols_bs_f32(int M, int mode, int W, const BsColArgs& A, hipStream_t s) {
+  return launch_cols_bs_T<float>(M, mode, W, A, s);
+}
+int bs_default_w_f32(int M, int cols) { return bs_default_w_T<float>(M, cols); }
 }  // namespace wos
