@@ -93,6 +93,8 @@ int wos_hip_init(void);
 const char* wos_hip_last_error(void);
 /* library build string */
 const char* wos_hip_version(void);
+/* device time (HIP events) of the last split-step pipeline, ms */
+double wos_hip_last_device_ms(void);
 /* frees every cached plan/buffer (device memory returns to the runtime) */
 int wos_hip_release_cache(void);
 
@@ -122,7 +124,10 @@ int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const 
  * cannot run on the device, so the caller samples it on the grid in the
  * reference's order: n_planes == 1 for a z-invariant medium n(x, y), or
  * n_planes == steps with plane s sampled at zc = z*s/steps + dz/2
- * (optics/medium.go:35,55). index is n_planes*n*n complex128. */
+ * (optics/medium.go:35,55). index is n_planes*n*n complex128.
+ * On power-of-two grids the two ASM half steps between screens run as one
+ * fused pass (H(dz/2)^2) and each half step's evanescent_filtered check is
+ * taken from spectral power sums; other grids run the half steps one by one. */
 int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int steps,
                        const double* index, int n_planes, const wos_ctx* ctx, int precision,
                        wos_warnings* w);

@@ -251,3 +251,97 @@ def test_config2_asm_pad_jones_fp64_and_fp32(wos, oracle):
     err32 = rel_l2(np.concatenate([fx, fy]), np.concatenate([rx, ry]))
     print(f"config2 asm_pad FP32 rel-L2 {err32:.3e}", flush=True)
     assert err32 <= FP32_TOL
+
+
+# ---------------------------------------------------------------- split-step BPM
+
+def graded_index(n, dx):
+    """Config-3 medium: n = 1.45 - 0.01 (r/100um)^2 for r < 100um, else 1.44, + 1e-5 i."""
+    x = (np.arange(n) - n / 2) * dx
+    r = np.hypot(x[None, :], x[:, None])
+    return np.where(r < 100e-6, 1.45 - 0.01 * (r / 100e-6) ** 2, 1.44) + 1e-5j
+
+
+def test_split_step_fp32_error(wos, oracle):
+    n, dx, wl = 128, 0.5e-6, 1.55e-6
+    index = graded_index(n, dx)
+    ex = oracle.gaussian(n, dx, 10e-6)
+    ref, _, _ = oracle.split_step(ex, None, dx, 200e-6, 40, index, oracle.ctx_for(wl))
+    out, _, _ = wos.split_step(ex, None, dx, 200e-6, 40, index, wos.ctx_for(wl), precision=wos.FP32)
+    err = rel_l2(out, ref)
+    print(f"split-step FP32 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP32_TOL
+
+
+def test_split_step_z_dependent_planes_and_non_pow2(wos, oracle):
+    """Stratified medium (TestSplitStepStratified, optics/medium_test.go:45-66)
+    with one sampled plane per slice, on a power-of-two grid (fused pipeline)
+    and on a non-power-of-two grid (half steps one by one)."""
+    wl, z, steps = 632.8e-9, 2e-3, 8
+    dz = z / steps
+    for n in (128, 100):
+        dx = 2e-6
+        planes = np.array([np.full((n, n), (1.5 if (z * s / steps + dz / 2) < z / 2 else 1.0) + 2e-6j)
+                           for s in range(steps)])
+        ex = oracle.gaussian(n, dx, 40e-6)
+        ref, _, wo = oracle.split_step(ex, None, dx, z, steps, planes, oracle.ctx_for(wl))
+        out, _, wg = wos.split_step(ex, None, dx, z, steps, planes, wos.ctx_for(wl))
+        err = rel_l2(out, ref)
+        print(f"stratified n={n} FP64 rel-L2 {err:.3e}", flush=True)
+        assert err <= FP64_TOL
+        assert wg.as_dict().keys() == wo.as_dict().keys()
+
+
+def test_split_step_evanescent_warning_parity(wos, oracle):
+    """A field with sub-wavelength content loses power in every half step:
+    the fused pipeline's spectral power checks must fire like the
+    reference's spatial Power() pairs (optics/propagate.go:204-208)."""
+    n, dx, wl = 64, 0.25e-6, 1e-6
+    rng = np.random.default_rng(11)
+    ex = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    index = np.full((n, n), 1.2 + 1e-4j)
+    ref, _, wo = oracle.split_step(ex, None, dx, 2e-6, 4, index, oracle.ctx_for(wl))
+    out, _, wg = wos.split_step(ex, None, dx, 2e-6, 4, index, wos.ctx_for(wl))
+    assert rel_l2(out, ref) <= FP64_TOL
+    do, dg = wo.as_dict(), wg.as_dict()
+    assert do.keys() == dg.keys() and "evanescent_filtered" in do
+    assert dg["evanescent_filtered"][0] == do["evanescent_filtered"][0] == 8
+    assert abs(dg["evanescent_filtered"][1] - do["evanescent_filtered"][1]) <= 1e-9 * do["evanescent_filtered"][1]
+
+
+def test_config3_bpm_short_vs_oracle(wos, oracle):
+    """Config-3 grid and medium (1024^2, dx 0.5um, 1.55um, Gaussian w0 10um),
+    20 of the 400 slices, FP64 vs the oracle's step-by-step BPM."""
+    n, dx, wl, steps = 1024, 0.5e-6, 1.55e-6, 20
+    index = graded_index(n, dx)
+    ex = oracle.gaussian(n, dx, 10e-6)
+    z = steps * 5e-6
+    ref, _, _ = oracle.split_step(ex, None, dx, z, steps, index, oracle.ctx_for(wl))
+    out, _, _ = wos.split_step(ex, None, dx, z, steps, index, wos.ctx_for(wl))
+    err = rel_l2(out, ref)
+    print(f"config3 20 steps FP64 rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+
+
+def test_config3_bpm_full_fp32_vs_fp64(wos, oracle):
+    """Config 3 in full: 400 slices of dz = 5um. FP32 bulk path vs the FP64
+    path (itself pinned to the oracle above) and the absorbed power against
+    exp(-2 k0 Im(n) z) = 0.850316 (the light stays inside the periodic window)."""
+    n, dx, wl, steps = 1024, 0.5e-6, 1.55e-6, 400
+    index = graded_index(n, dx)
+    ex = oracle.gaussian(n, dx, 10e-6)
+    z = steps * 5e-6
+    p0 = np.sum(np.abs(ex) ** 2)
+    o64, _, w64 = wos.split_step(ex, None, dx, z, steps, index, wos.ctx_for(wl))
+    t64 = wos.last_device_ms()
+    o32, _, w32 = wos.split_step(ex, None, dx, z, steps, index, wos.ctx_for(wl), precision=wos.FP32)
+    t32 = wos.last_device_ms()
+    err = rel_l2(o32, o64)
+    ratio = np.sum(np.abs(o64) ** 2) / p0
+    want = np.exp(-2 * (2 * np.pi / wl) * 1e-5 * z)
+    print(f"config3 400 steps: FP32 vs FP64 rel-L2 {err:.3e}; power ratio {ratio:.6f} (closed form {want:.6f}); "
+          f"device ms FP64 {t64:.2f} ({steps / t64 * 1e3:.0f} steps/s) FP32 {t32:.2f} ({steps / t32 * 1e3:.0f} steps/s)",
+          flush=True)
+    assert err <= FP32_TOL
+    assert abs(ratio - want) <= 1e-3
+    assert w32.as_dict().keys() == w64.as_dict().keys()

@@ -63,8 +63,9 @@ struct Device {
 Device& device();
 int ensure_init();
 
-// Twiddle table exp(-2 pi i m / n), m in [0, n), precision prec (0 = FP64).
-int get_twiddles(int n, int prec, const void** out);
+// Stage twiddle tables of an n-point transform in precision prec (0 = FP64);
+// acc = 1: compensated [hi, lo] pairs.
+int get_twiddles(int n, int prec, const void** out, int acc = 0);
 
 // ---- launchers (return 0 or a WOS_ERR_* code) ----
 // Row pass on `units` units of rows x N; rows = rows per workgroup (0: default).

@@ -71,28 +71,41 @@ __device__ __forceinline__ void dft4(cpx<R>& v0, cpx<R>& v1, cpx<R>& v2, cpx<R>&
   v3 = t1 - t3;
 }
 
-// multiply by exp(DIR * 2 pi i m / 8), m = 1, 3
-template <int DIR, int M, typename R>
-__device__ __forceinline__ cpx<R> w8(cpx<R> a) {
-  const R c = (R)W16<R>::H;
-  if constexpr (M == 1) {
-    if constexpr (DIR < 0) return {c * (a.x + a.y), c * (a.y - a.x)};
-    else return {c * (a.x - a.y), c * (a.x + a.y)};
-  } else {  // M == 3
-    if constexpr (DIR < 0) return {c * (a.y - a.x), -c * (a.x + a.y)};
-    else return {-c * (a.x + a.y), c * (a.x - a.y)};
+// c * x with c a compile-time FP64 constant: rounded to R, or (ACC) as an
+// R hi + lo pair, so the product is not biased by the constant's rounding.
+template <bool ACC, typename R>
+__device__ __forceinline__ R cmulk(double c, R x) {
+  const R hi = (R)c;
+  if constexpr (ACC) {
+    const R lo = (R)(c - (double)hi);
+    return fma(hi, x, lo * x);
+  } else {
+    return hi * x;
   }
 }
 
-template <int DIR, typename R>
+// multiply by exp(DIR * 2 pi i m / 8), m = 1, 3
+template <int DIR, int M, typename R, bool ACC = false>
+__device__ __forceinline__ cpx<R> w8(cpx<R> a) {
+  const double c = W16<R>::H;
+  if constexpr (M == 1) {
+    if constexpr (DIR < 0) return {cmulk<ACC>(c, a.x + a.y), cmulk<ACC>(c, a.y - a.x)};
+    else return {cmulk<ACC>(c, a.x - a.y), cmulk<ACC>(c, a.x + a.y)};
+  } else {  // M == 3
+    if constexpr (DIR < 0) return {cmulk<ACC>(c, a.y - a.x), -cmulk<ACC>(c, a.x + a.y)};
+    else return {-cmulk<ACC>(c, a.x + a.y), cmulk<ACC>(c, a.x - a.y)};
+  }
+}
+
+template <int DIR, typename R, bool ACC = false>
 __device__ __forceinline__ void dft8(cpx<R>* v) {
   cpx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
   cpx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
   dft4<DIR>(e0, e1, e2, e3);
   dft4<DIR>(o0, o1, o2, o3);
-  o1 = w8<DIR, 1>(o1);
+  o1 = w8<DIR, 1, R, ACC>(o1);
   o2 = mul_i<DIR>(o2);
-  o3 = w8<DIR, 3>(o3);
+  o3 = w8<DIR, 3, R, ACC>(o3);
   v[0] = e0 + o0;
   v[4] = e0 - o0;
   v[1] = e1 + o1;
@@ -104,27 +117,27 @@ __device__ __forceinline__ void dft8(cpx<R>* v) {
 }
 
 // multiply by exp(DIR * 2 pi i m / 16) for a compile-time m in 0..15
-template <int DIR, int M, typename R>
+template <int DIR, int M, typename R, bool ACC = false>
 __device__ __forceinline__ cpx<R> w16(cpx<R> a) {
   constexpr int m = M & 15;
   if constexpr (m == 0) return a;
   else if constexpr (m == 4) return mul_i<DIR>(a);
   else if constexpr (m == 8) return {-a.x, -a.y};
   else if constexpr (m == 12) return mul_i<-DIR>(a);
-  else if constexpr (m == 2) return w8<DIR, 1>(a);
-  else if constexpr (m == 6) return w8<DIR, 3>(a);
+  else if constexpr (m == 2) return w8<DIR, 1, R, ACC>(a);
+  else if constexpr (m == 6) return w8<DIR, 3, R, ACC>(a);
   else {
     // generic: cos/sin of 2 pi m / 16
     constexpr double C1 = W16<R>::C1, S1 = W16<R>::S1;
     constexpr double cs[16] = {1, C1, 0, S1, 0, -S1, 0, -C1, -1, -C1, 0, -S1, 0, S1, 0, C1};
     constexpr double sn[16] = {0, S1, 0, C1, 1, C1, 0, S1, 0, -S1, 0, -C1, -1, -C1, 0, -S1};
-    const R c = (R)cs[m];
-    const R s = (R)(DIR < 0 ? -sn[m] : sn[m]);
-    return {a.x * c - a.y * s, a.x * s + a.y * c};
+    const double c = cs[m];
+    const double s = DIR < 0 ? -sn[m] : sn[m];
+    return {cmulk<ACC>(c, a.x) - cmulk<ACC>(s, a.y), cmulk<ACC>(s, a.x) + cmulk<ACC>(c, a.y)};
   }
 }
 
-template <int DIR, typename R>
+template <int DIR, typename R, bool ACC = false>
 __device__ __forceinline__ void dft16(cpx<R>* v) {
   // n = 4 n1 + n2: radix-4 over n1 for each n2
   dft4<DIR>(v[0], v[4], v[8], v[12]);
@@ -132,15 +145,15 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   dft4<DIR>(v[2], v[6], v[10], v[14]);
   dft4<DIR>(v[3], v[7], v[11], v[15]);
   // slot 4 k1 + n2 holds X[k1][n2]; twiddle by W16^(n2 k1)
-  v[5] = w16<DIR, 1>(v[5]);
-  v[6] = w16<DIR, 2>(v[6]);
-  v[7] = w16<DIR, 3>(v[7]);
-  v[9] = w16<DIR, 2>(v[9]);
-  v[10] = w16<DIR, 4>(v[10]);
-  v[11] = w16<DIR, 6>(v[11]);
-  v[13] = w16<DIR, 3>(v[13]);
-  v[14] = w16<DIR, 6>(v[14]);
-  v[15] = w16<DIR, 9>(v[15]);
+  v[5] = w16<DIR, 1, R, ACC>(v[5]);
+  v[6] = w16<DIR, 2, R, ACC>(v[6]);
+  v[7] = w16<DIR, 3, R, ACC>(v[7]);
+  v[9] = w16<DIR, 2, R, ACC>(v[9]);
+  v[10] = w16<DIR, 4, R, ACC>(v[10]);
+  v[11] = w16<DIR, 6, R, ACC>(v[11]);
+  v[13] = w16<DIR, 3, R, ACC>(v[13]);
+  v[14] = w16<DIR, 6, R, ACC>(v[14]);
+  v[15] = w16<DIR, 9, R, ACC>(v[15]);
   // radix-4 over n2 for each k1: result y[k1 + 4 k2] lands in slot 4 k1 + k2
   dft4<DIR>(v[0], v[1], v[2], v[3]);
   dft4<DIR>(v[4], v[5], v[6], v[7]);
@@ -156,12 +169,12 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   t = v[11]; v[11] = v[14]; v[14] = t;
 }
 
-template <int RAD, int DIR, typename R>
+template <int RAD, int DIR, typename R, bool ACC = false>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RAD == 2) dft2<DIR>(v);
   else if constexpr (RAD == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
-  else if constexpr (RAD == 8) dft8<DIR>(v);
-  else if constexpr (RAD == 16) dft16<DIR>(v);
+  else if constexpr (RAD == 8) dft8<DIR, R, ACC>(v);
+  else if constexpr (RAD == 16) dft16<DIR, R, ACC>(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,7 +234,9 @@ constexpr int lds_len(int n) { return n + n / 16 + 1; }
 // v[b*RAD + r] holds x[j_b + r N/RAD], j_b = t + b*T. The stage's twiddle
 // table starts at tws + OFF; entry [k][r-1] = exp(-2 pi i r k/(NS RAD)),
 // conjugated for the inverse direction.
-template <typename R, int N, int DIR, int RAD, int NS, int OFF>
+// ACC: the table holds [hi, lo] pairs (2 entries per twiddle) and the
+// product is v*hi + v*lo.
+template <typename R, int N, int DIR, int RAD, int NS, int OFF, bool ACC = false>
 __device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __restrict__ tws) {
   constexpr int V = Plan<N>::V, T = Plan<N>::T;
   constexpr int NB = V / RAD;
@@ -230,14 +245,24 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __
     if constexpr (NS > 1) {
       const int j = t + b * T;
       const int k = j & (NS - 1);
-      const cpx<R>* tk = tws + OFF + k * (RAD - 1);
+      if constexpr (ACC) {
+        const cpx<R>* tk = tws + 2 * (OFF + k * (RAD - 1));
 #pragma unroll
-      for (int r = 1; r < RAD; ++r) {
-        const cpx<R> w = tk[r - 1];
-        v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], w) : cmulc(v[b * RAD + r], w);
+        for (int r = 1; r < RAD; ++r) {
+          const cpx<R> hi = tk[2 * (r - 1)], lo = tk[2 * (r - 1) + 1];
+          const cpx<R> x = v[b * RAD + r];
+          v[b * RAD + r] = DIR < 0 ? cmul(x, hi) + cmul(x, lo) : cmulc(x, hi) + cmulc(x, lo);
+        }
+      } else {
+        const cpx<R>* tk = tws + OFF + k * (RAD - 1);
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) {
+          const cpx<R> w = tk[r - 1];
+          v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], w) : cmulc(v[b * RAD + r], w);
+        }
       }
     }
-    dft<RAD, DIR>(v + b * RAD);
+    dft<RAD, DIR, R, ACC>(v + b * RAD);
   }
 }
 
@@ -288,20 +313,20 @@ __device__ __forceinline__ int slot_pos(int t, int b, int r) {
 // LDS exchanges between them. On entry v holds the inputs of stage S0 in its
 // butterfly order; on exit v holds the outputs of stage S1-1 in the order
 // "slot (b, r) = position t + b T + r N/RAD_last" (valid for the last stage).
-template <typename R, int N, int DIR, int S, int SEND, int ES>
+template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false>
 __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
   if constexpr (S < SEND) {
     constexpr int RAD = DIR < 0 ? Plan<N>::fwd_rad(S) : Plan<N>::inv_rad(S);
     constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
     constexpr int OFF = DIR < 0 ? Plan<N>::tw_off_fwd(S) : Plan<N>::tw_off_inv(S);
-    stage_compute<R, N, DIR, RAD, NS, OFF>(v, t, tw);
+    stage_compute<R, N, DIR, RAD, NS, OFF, ACC>(v, t, tw);
     if constexpr (S + 1 < SEND) {
       constexpr int RADN = DIR < 0 ? Plan<N>::fwd_rad(S + 1) : Plan<N>::inv_rad(S + 1);
       __syncthreads();  // previous readers of lds are done
       stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
       __syncthreads();
       stage_load_lds<R, N, RADN, ES>(v, t, lds);
-      run_stages<R, N, DIR, S + 1, SEND, ES>(v, t, tw, lds);
+      run_stages<R, N, DIR, S + 1, SEND, ES, ACC>(v, t, tw, lds);
     }
   }
 }
@@ -316,9 +341,10 @@ constexpr int last_rad() {
 
 // Full N-point transform of the values in v (input map of first_rad, output
 // map of last_rad).
-template <typename R, int N, int DIR, int ES>
+// ACC: compensated twiddles and constants (tw then points to a [hi, lo] table).
+template <typename R, int N, int DIR, int ES, bool ACC = false>
 __device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
-  run_stages<R, N, DIR, 0, Plan<N>::S, ES>(v, t, tw, lds);
+  run_stages<R, N, DIR, 0, Plan<N>::S, ES, ACC>(v, t, tw, lds);
 }
 
 }  // namespace wos

@@ -154,9 +154,10 @@ static void plan_stages(int n, std::vector<int>& fwd, std::vector<int>& inv) {
 
 // Stage twiddle tables of an n-point transform (forward stages 1..S-1, then
 // inverse stages 1..S-1, the layout Plan<N>::tw_off_* addresses), cached.
-int get_twiddles(int n, int prec, const void** out) {
+// acc: [hi, lo] pairs for the compensated split-step kernels.
+int get_twiddles(int n, int prec, const void** out, int acc) {
   Device& d = device();
-  long long key = (long long)n * 2 + prec;
+  long long key = ((long long)n * 2 + prec) * 2 + (acc ? 1 : 0);
   auto it = d.twiddles.find(key);
   if (it != d.twiddles.end()) {
     *out = it->second;
@@ -177,13 +178,13 @@ int get_twiddles(int n, int prec, const void** out) {
     }
   }
   void* p = nullptr;
-  WOS_HIP_TRY(hipMalloc(&p, (size_t)std::max(off, 1) * csize(prec)));
+  WOS_HIP_TRY(hipMalloc(&p, (size_t)std::max(off, 1) * csize(prec) * (acc ? 2 : 1)));
   for (const St& q : st) {
     const int cnt = q.ns * (q.rad - 1), bs = 256, gs = (cnt + bs - 1) / bs;
     if (prec == WOS_FP64)
-      hipLaunchKernelGGL(k_stage_twiddles<double>, dim3(gs), dim3(bs), 0, d.stream, (cpx<double>*)p, q.off, q.ns, q.rad);
+      hipLaunchKernelGGL(k_stage_twiddles<double>, dim3(gs), dim3(bs), 0, d.stream, (cpx<double>*)p, q.off, q.ns, q.rad, acc);
     else
-      hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, q.off, q.ns, q.rad);
+      hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, q.off, q.ns, q.rad, acc);
     WOS_HIP_TRY(hipGetLastError());
   }
   d.twiddles[key] = p;
@@ -479,6 +480,13 @@ __global__ void k_mul_table(cpx<R>* __restrict__ a, const cpx<double>* __restric
     }
     a[idx] = {(R)r.x, (R)r.y};
   }
+}
+
+// a[u] *= t for complex64 fields and table
+__global__ void k_mul_table32(cpx<float>* __restrict__ a, const cpx<float>* __restrict__ t, size_t nn, int units) {
+  const size_t total = nn * units;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+    a[idx] = cmul(a[idx], t[idx % nn]);
 }
 
 // out[i] += w * Intensity(i) over the components of one field
@@ -1215,6 +1223,132 @@ static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double 
   }
 }
 
+// evanescent_filtered checks of merged split-step propagations from the
+// spectral power sums of one column pass (Parseval: the ratio of the spatial
+// powers before and after a propagation equals the ratio of these sums).
+// halves = 1: one step (|U|^2 -> |HU|^2); 2: two steps (-> |H^2 U|^2).
+// Accumulates into the device warnings in Warnings.Add order.
+__global__ void k_split_warn(const double* __restrict__ pw3, int nblocks, int halves, double z, wos_warnings* dw) {
+  __shared__ double red[3][256];
+  double s[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int q = 0; q < 3; ++q) {  // Neumaier per thread
+      const double x = pw3[(size_t)b * 3 + q];
+      const double t = s[q] + x;
+      c[q] += fabs(s[q]) >= fabs(x) ? (s[q] - t) + x : (x - t) + s[q];
+      s[q] = t;
+    }
+  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = s[q] + c[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[3] = {0, 0, 0};
+    for (int q = 0; q < 3; ++q)
+      for (int i = 0; i < (int)blockDim.x; ++i) tot[q] += red[q][i];
+    for (int h = 0; h < halves; ++h) {
+      const double pin = tot[h], pout = tot[h + 1];
+      if (pin > 0 && pout < pin * (1 - 1e-8) && z > 0) {
+        dw->count[WOS_WARN_EVANESCENT_FILTERED]++;
+        dw->value[WOS_WARN_EVANESCENT_FILTERED] = (pin - pout) / pin;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_warn(wos_warnings* dw) {
+  if (threadIdx.x < WOS_WARN_SLOTS) {
+    dw->count[threadIdx.x] = 0;
+    dw->value[threadIdx.x] = 0;
+  }
+}
+
+static double g_last_device_ms = 0;
+
+// Split-step BPM (optics/medium.go:22-45) on the device for a power-of-two
+// grid: the two ASM half steps between screens are merged into one column
+// pass applying H(dz/2)^2, each screen runs inside the row pass between the
+// inverse and the forward row transform, and every half step's
+// evanescent_filtered check is evaluated on the device from spectral power
+// sums. `screens` holds n_planes complex128 tables t(x, y). Backward (z < 0)
+// warnings are added by the caller.
+static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, double z, int steps, const void* screens,
+                           int n_planes, const wos_ctx& ctx, wos_warnings* w) {
+  Device& dv = device();
+  int rc;
+  const size_t nn = (size_t)n * n;
+  const double dz = z / (double)steps;
+  std::vector<TFUnit> tf(ncomp, make_asm_tf(ctx.wavelength, dz / 2, ctx, n, dx, 0, 0));
+  if ((rc = dv.params.ensure(sizeof(TFUnit) * ncomp))) return rc;
+  if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * ncomp))) return rc;
+  if ((rc = dv.scratch.ensure(ncomp * nn * csize(prec)))) return rc;
+  TransferLaunch L;
+  Tuning tu;
+  tu.tw = default_col_w(prec, n, n);  // the split-step column kernel is built for the default tile
+  if ((rc = build_transfer(L, prec, d, dv.scratch.p, n, ncomp, false, (const TFUnit*)dv.params.p, TF_ASM, 1.0, 0.0,
+                           nullptr, nullptr, tu)))
+    return rc;
+  const int ncolblk = (n / L.tu.tw) * ncomp;
+  if ((rc = dv.hist.ensure(sizeof(double) * 3 * ncolblk + 256))) return rc;
+  double* pw3 = (double*)dv.hist.p;
+  wos_warnings* dw = (wos_warnings*)((char*)dv.hist.p + sizeof(double) * 3 * ncolblk);
+  dw = (wos_warnings*)(((uintptr_t)dw + 15) & ~(uintptr_t)15);
+  if ((rc = dv.hist.ensure((char*)(dw + 1) - (char*)dv.hist.p))) return rc;
+  hipLaunchKernelGGL(k_zero_warn, dim3(1), dim3(64), 0, dv.stream, dw);
+  // the screen row pass: tiled intermediate in place, inverse -> screen -> forward
+  RowArgs scr = L.inv;
+  scr.out = const_cast<void*>(L.inv.in);
+  scr.out_unit = L.inv.in_unit;
+  scr.out_tw = L.inv.in_tw;
+  scr.out_tstride = L.inv.in_tstride;
+  scr.out_ld = 0;
+  scr.out_col0 = 0;
+  scr.out_cols = n;
+  scr.pw_out = nullptr;
+  ColArgs col = L.col;
+  col.pw3 = w ? pw3 : nullptr;
+  if (prec == WOS_FP32) {  // the slice kernels use compensated twiddles
+    const void* twa;
+    if ((rc = get_twiddles(n, prec, &twa, 1))) return rc;
+    scr.tw = twa;
+    col.tw = twa;
+  }
+  const bool want = w != nullptr;
+  hipEvent_t e0, e1;
+  WOS_HIP_TRY(hipEventCreate(&e0));
+  WOS_HIP_TRY(hipEventCreate(&e1));
+  WOS_HIP_TRY(hipEventRecord(e0, dv.stream));
+  if ((rc = launch_rows(prec, n, -1, L.fwd, dv.stream, L.tu.rows))) return rc;
+  for (int s = 0; s <= steps; ++s) {
+    col.tf_square = (s == 0 || s == steps) ? 0 : 1;
+    if ((rc = launch_cols(prec, n, 3, 0, L.tu.tw, col, dv.stream))) return rc;
+    if (want) {
+      hipLaunchKernelGGL(k_split_warn, dim3(1), dim3(256), 0, dv.stream, (const double*)pw3, ncolblk,
+                         col.tf_square ? 2 : 1, dz / 2, dw);
+      WOS_HIP_TRY(hipGetLastError());
+    }
+    if (s == steps) break;
+    scr.screen = (const char*)screens + (n_planes == 1 ? 0 : (size_t)s * nn * 16);
+    if ((rc = launch_rows(prec, n, 1, scr, dv.stream, L.tu.rows))) return rc;
+  }
+  if ((rc = launch_rows(prec, n, 1, L.inv, dv.stream, L.tu.rows))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e1, dv.stream));
+  if (want) {
+    wos_warnings hw;
+    WOS_HIP_TRY(hipMemcpyAsync(&hw, dw, sizeof(hw), hipMemcpyDeviceToHost, dv.stream));
+    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+    for (int c = 0; c < WOS_WARN_SLOTS; ++c) {
+      w->count[c] += hw.count[c];
+      if (hw.value[c] != 0) w->value[c] = hw.value[c];
+    }
+  }
+  WOS_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  g_last_device_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
 // Upload host complex128 arrays into the field buffer in `prec`.
 static int upload_fields(int prec, const double* const* src, int ncomp, size_t nn, void** dptr) {
   Device& dv = device();
@@ -1286,6 +1420,8 @@ int wos_hip_device_name(char* buf, int len) {
 int wos_hip_init(void) { return ensure_init(); }
 
 const char* wos_hip_last_error(void) { return g_err.c_str(); }
+
+double wos_hip_last_device_ms(void) { return g_last_device_ms; }
 
 const char* wos_hip_version(void) { return "libwos_hip 0.1 (gfx950, Stockham radix-16/8/4/2, LDS-staged)"; }
 
@@ -1445,8 +1581,9 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
   if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
   const double dz = z / (double)steps;
   const double k = 2.0 * M_PI / ctx->wavelength;
-  // screen tables t(x, y) per plane, FP64
-  if ((rc = dv.screens.ensure((size_t)n_planes * nn * 16))) return rc;
+  // screen tables t(x, y) per plane: FP64, plus a rounded copy for FP32
+  const size_t tb = (size_t)n_planes * nn * 16;
+  if ((rc = dv.screens.ensure(tb + (precision == WOS_FP32 ? (size_t)n_planes * nn * 8 : 0)))) return rc;
   if ((rc = dv.work.ensure(nn * 16))) return rc;
   for (int p = 0; p < n_planes; ++p) {
     WOS_HIP_TRY(hipMemcpyAsync(dv.work.p, index + (size_t)p * nn * 2, nn * 16, hipMemcpyHostToDevice, dv.stream));
@@ -1455,13 +1592,36 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
     WOS_HIP_TRY(hipGetLastError());
     WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
   }
+  const void* tables = dv.screens.p;
+  if (precision == WOS_FP32) {
+    tables = (char*)dv.screens.p + tb;
+    const size_t tot = (size_t)n_planes * nn;
+    hipLaunchKernelGGL(k_z2c, dim3(grid_for(tot)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.screens.p,
+                       (cpx<float>*)tables, tot);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  // merged device pipeline unless a half step needs the wrap diagnostic
+  // (its quick check, optics/propagate.go:251, passes for BPM slices)
+  const bool quick_ok = std::fabs(dz / 2) * ctx->wavelength / (2 * dx) <= (double)n * dx;
+  if (pow2(n) && quick_ok && !getenv("WOS_SPLIT_STEP_REFERENCE_ORDER")) {
+    if ((rc = split_step_fast(precision, d, n, ncomp, dx, z, steps, dv.screens.p, n_planes, *ctx, w))) return rc;
+    if (w && dz / 2 < 0) {  // asmWarn's backward diagnostics, one per half step
+      int zero, reg;
+      double alpha, limit;
+      evanescent_policy(*ctx, dz / 2, &zero, &reg, &alpha, &limit);
+      for (int i = 0; i < 2 * steps; ++i) asm_warn(w, dz / 2, 0, 0, zero, reg);
+    }
+    return download_fields(precision, d, dst, ncomp, nn);
+  }
   for (int s = 0; s < steps; ++s) {
     if ((rc = propagate_dev(precision, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, *ctx, w, nullptr))) return rc;
-    const cpx<double>* t = (const cpx<double>*)dv.screens.p + (n_planes == 1 ? 0 : (size_t)s * nn);
+    const void* t = (const char*)tables + (n_planes == 1 ? 0 : (size_t)s * nn * csize(precision));
     if (precision == WOS_FP64)
-      hipLaunchKernelGGL(k_mul_table<double>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<double>*)d, t, nn, ncomp);
+      hipLaunchKernelGGL(k_mul_table<double>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<double>*)d,
+                         (const cpx<double>*)t, nn, ncomp);
     else
-      hipLaunchKernelGGL(k_mul_table<float>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<float>*)d, t, nn, ncomp);
+      hipLaunchKernelGGL(k_mul_table32, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<float>*)d,
+                         (const cpx<float>*)t, nn, ncomp);
     WOS_HIP_TRY(hipGetLastError());
     if ((rc = propagate_dev(precision, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, *ctx, w, nullptr))) return rc;
   }

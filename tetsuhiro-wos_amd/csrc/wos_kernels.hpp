@@ -283,13 +283,19 @@ __device__ __forceinline__ void block_sum_store(double v, double* red, double* o
 // The angle -2 r k/(NS RAD) (in units of pi) is exact: NS*RAD is a power of two.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_stage_twiddles(cpx<R>* tw, int off, int ns, int rad) {
+__global__ void k_stage_twiddles(cpx<R>* tw, int off, int ns, int rad, int acc) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ns * (rad - 1)) return;
   const int k = e / (rad - 1), r = e % (rad - 1) + 1;
   double s, c;
   sincospi(-2.0 * (double)(r * k) / (double)(ns * rad), &s, &c);
-  tw[off + e] = {(R)c, (R)s};
+  const cpx<R> hi = {(R)c, (R)s};
+  if (acc) {  // [hi, lo] pair: lo = rounded remainder of the FP64 twiddle
+    tw[2 * (off + e)] = hi;
+    tw[2 * (off + e) + 1] = {(R)(c - (double)hi.x), (R)(s - (double)hi.y)};
+  } else {
+    tw[off + e] = hi;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -318,6 +324,7 @@ struct RowArgs {
   double* pw_in;                // per-block sum |in|^2 (nullable)
   double* pw_out;               // per-block sum |out|^2 (nullable)
   const void* tw;               // N-point twiddles
+  const void* screen;           // SCR kernels: N x N row-major complex128 phase screen
 };
 
 // RowArgs with row-major addressing on both sides and no embed window.
@@ -341,10 +348,15 @@ __device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int 
 
 // Threads are interleaved over the ROWS rows of a workgroup (row = tid %
 // ROWS), so one wavefront covers 64/ROWS consecutive positions of ROWS rows.
-template <typename R, int N, int DIR, int ROWS>
+// SCR = 1 (with DIR = +1): inverse row FFT -> scale -> multiply by the
+// phase screen -> forward row FFT, the split-step BPM screen between two
+// propagation steps (the inverse output map is the forward input map, so
+// the data stays in registers).
+template <typename R, int N, int DIR, int ROWS, int SCR = 0>
 __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   constexpr int T = Plan<N>::T, V = Plan<N>::V;
-  constexpr int R0 = first_rad<N, DIR>(), RL = last_rad<N, DIR>();
+  static_assert(SCR == 0 || DIR > 0, "the screen follows an inverse transform");
+  constexpr int R0 = first_rad<N, DIR>(), RL = SCR ? last_rad<N, -1>() : last_rad<N, DIR>();
   constexpr int NT = ROWS * T;
   __shared__ cpx<R> lds[ROWS * lds_len(N)];
   __shared__ double red[(NT + 63) / 64];
@@ -385,7 +397,8 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     for (int i = 0; i < V; ++i) pin += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
     block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
   }
-  fft_block<R, N, DIR, 1>(v, t, tw, lds + lr * lds_len(N));
+  constexpr bool ACC = SCR == 1 && sizeof(R) == 4;  // split-step slices: compensated twiddles
+  fft_block<R, N, DIR, 1, ACC>(v, t, tw, lds + lr * lds_len(N));
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   if (a.scale_im == 0) {
     if (a.scale_re != 1) {
@@ -395,6 +408,21 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   } else {
 #pragma unroll
     for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);
+  }
+  if constexpr (SCR == 1) {
+    constexpr int RS = last_rad<N, +1>();
+    static_assert(RS == first_rad<N, -1>(), "screen map");
+    // FP64 screen applied in FP64: the same table multiplies every slice
+    const cpx<double>* scr = (const cpx<double>*)a.screen + (size_t)row * N;
+#pragma unroll
+    for (int b = 0; b < V / RS; ++b)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const cpx<double> x = {(double)v[b * RS + r].x, (double)v[b * RS + r].y};
+        const cpx<double> y = cmul(x, scr[slot_pos<N, RS>(t, b, r)]);
+        v[b * RS + r] = {(R)y.x, (R)y.y};
+      }
+    fft_block<R, N, -1, 1, ACC>(v, t, tw, lds + lr * lds_len(N));
   }
   if (full_out) {
 #pragma unroll
@@ -428,7 +456,8 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
 // ---------------------------------------------------------------------------
 // Column pass over a tile of W adjacent columns. MODE 0: plain transform
 // (direction DIR, output scale); MODE 1: forward -> asm TF -> inverse (the
-// ASM step); MODE 2: forward -> any TF kind -> inverse. With `tiled` the
+// ASM step); MODE 2: forward -> any TF kind -> inverse; MODE 3: the
+// split-step variant of MODE 1 (see below). With `tiled` the
 // tile is one contiguous block (row stride W, tiles tstride apart); otherwise
 // rows are ld apart in a row-major array.
 // ---------------------------------------------------------------------------
@@ -446,6 +475,8 @@ struct ColArgs {
   int xcd_remap;       // spread consecutive tiles over one XCD
   const TFUnit* tf;    // per unit (MODE 1, 2)
   const void* tw;
+  int tf_square;       // MODE 3: apply H^2 (two merged propagation steps)
+  double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2 (nullable)
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
@@ -492,8 +523,9 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
         v[b * R0 + r] = x;
       }
   }
-  fft_block<R, N, DIR0, W>(v, t, tw, lds + c);
-  if constexpr (MODE >= 1) {
+  constexpr bool ACC = MODE == 3 && sizeof(R) == 4;  // split-step slices: compensated twiddles
+  fft_block<R, N, DIR0, W, ACC>(v, t, tw, lds + c);
+  if constexpr (MODE == 1 || MODE == 2) {
     constexpr int RL = last_rad<N, -1>();
     if (active) {
       const TFUnit u = a.tf[unit];
@@ -525,7 +557,56 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
           }
       }
     }
-    fft_block<R, N, +1, W>(v, t, tw, lds + c);
+  }
+  if constexpr (MODE == 3) {
+    // Split-step column pass: H (tf_square = 0) or H^2 (two merged half
+    // steps), H evaluated exactly in FP64 and applied in FP64 arithmetic for
+    // both precisions -- the same H multiplies the field every slice, so an
+    // FP32-rounded H would accumulate its rounding linearly over the slices.
+    // Optional spectral power sums |U|^2, |H U|^2, |H^2 U|^2 per workgroup.
+    constexpr int RL = last_rad<N, -1>();
+    double s0 = 0, s1 = 0, s2 = 0;
+    const bool want_pw = a.pw3 != nullptr;
+    if (active) {
+      const TFUnit u = a.tf[unit];
+#pragma unroll
+      for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const cpx<double> h = asm_tf_fp64_at(u, col, slot_pos<N, RL>(t, b, r));
+          const cpx<double> x = {(double)v[b * RL + r].x, (double)v[b * RL + r].y};
+          const cpx<double> y = cmul(x, h);
+          cpx<double> out = y;
+          if (a.tf_square) out = cmul(y, h);
+          if (want_pw) {
+            s0 += x.x * x.x + x.y * x.y;
+            s1 += y.x * y.x + y.y * y.y;
+            s2 += out.x * out.x + out.y * out.y;
+          }
+          v[b * RL + r] = {(R)out.x, (R)out.y};
+        }
+    }
+    if (want_pw) {  // uniform
+      __shared__ double red3[3][(W * Plan<N>::T + 63) / 64];
+      double vals[3] = {s0, s1, s2};
+      const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double x = vals[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red3[q][wave] = x;
+      }
+      __syncthreads();
+      if (tid < 3) {
+        double x = 0;
+        for (int w = 0; w < (W * Plan<N>::T + 63) / 64; ++w) x += red3[tid][w];
+        a.pw3[(size_t)blockIdx.x * 3 + tid] = x;
+      }
+    }
+  }
+  if constexpr (MODE >= 1) {
+    fft_block<R, N, +1, W, ACC>(v, t, tw, lds + c);
   }
   constexpr int DIRL = MODE >= 1 ? +1 : DIR;
   constexpr int RL = last_rad<N, DIRL>();

@@ -30,7 +30,7 @@ EXPORTS = [
     "wos_hip_batch_create", "wos_hip_batch_destroy", "wos_hip_batch_upload",
     "wos_hip_batch_download", "wos_hip_batch_fill_schell", "wos_hip_batch_capture",
     "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
-    "wos_hip_batch_autotune", "wos_hip_batch_set_tuning",
+    "wos_hip_batch_autotune", "wos_hip_batch_set_tuning", "wos_hip_last_device_ms",
 ]
 
 
@@ -83,6 +83,7 @@ def load(path=None):
     L.wos_hip_init.restype = I
     L.wos_hip_last_error.restype = C.c_char_p
     L.wos_hip_version.restype = C.c_char_p
+    L.wos_hip_last_device_ms.restype = C.c_double
     for nm in ("wos_hip_fft2d_z", "wos_hip_fft1d_z", "wos_hip_fft2d_c", "wos_hip_fft1d_c"):
         getattr(L, nm).argtypes = [P, I, I]
         getattr(L, nm).restype = I
@@ -122,6 +123,11 @@ def _p(a):
 
 def _c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def last_device_ms():
+    """Device time of the last split-step pipeline (HIP events), ms."""
+    return load().wos_hip_last_device_ms()
 
 
 def device_count():
