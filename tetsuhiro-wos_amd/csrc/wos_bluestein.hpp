@@ -167,7 +167,7 @@ __global__ __launch_bounds__(W* Plan<M>::T) void k_cols_bs(const BsColArgs A) {
   bluestein_block<R, M, W>(v, t, L, A.tab0, (R)A.post0, tw, lds + c);
   if constexpr (MODE == 2) {
     if (active) {
-      const TFUnit u = a.tf[unit];
+      const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
 #pragma unroll
       for (int b = 0; b < V / R0; ++b)
 #pragma unroll

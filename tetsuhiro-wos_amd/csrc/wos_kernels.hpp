@@ -176,7 +176,7 @@ __device__ __forceinline__ cpx<float> asm_tf_fp32_prop(const TF32Col& c, int iy,
 }
 
 // Evanescent factor of the FP32 path at row iy (rho > 1), else 1.
-__device__ __noinline__ float asm_tf_fp32_evan(const TFUnit& u, const TF32Col& c, int iy) {
+__device__ __noinline__ float asm_tf_fp32_evan(const TFUnit& u, TF32Col c, int iy) {
   const double ay = fma((double)(iy <= u.M / 2 ? iy : iy - u.M), c.cwl, c.coff);
   const double x = fma(-ay, ay, c.one_m_ax2);
   if (x >= 0) return 1.f;
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
   if constexpr (MODE == 1 || MODE == 2) {
     constexpr int RL = last_rad<N, -1>();
     if (active) {
-      const TFUnit u = a.tf[unit];
+      const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
       if constexpr (MODE == 1 && sizeof(R) == 4) {
         const TF32Col cc = tf32_col(u, col);
         bool evan = false;
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
     double s0 = 0, s1 = 0, s2 = 0;
     const bool want_pw = a.pw3 != nullptr;
     if (active) {
-      const TFUnit u = a.tf[unit];
+      const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
 #pragma unroll
       for (int b = 0; b < V / RL; ++b)
 #pragma unroll
