@@ -29,6 +29,18 @@ Z = 20e-3
 UNITS = 64
 W0, SIGMA_C, SEED = 300e-6, 50e-6, 20240607
 HBM_PEAK_GBS = 8000.0
+# MI355X VALU issue ceiling: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
+VALU_WAVE_INSTR_PER_S = 614.4e9
+# HBM bytes per full-batch launch (64 x 2048^2), rocprofv3 PMC: 2 x FETCH_SIZE + WRITE_SIZE
+# (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md
+PMC_HBM_BYTES = {
+    "fp32": {"row_fft_fwd": (1048756 * 2 + 2097152) * 1024, "col_fft_tf_ifft": (1049539 * 2 + 2097152) * 1024,
+             "row_fft_inv": (1048794 * 2 + 2097152) * 1024},
+    "fp64": {"row_fft_fwd": (2097621 * 2 + 4194305) * 1024, "col_fft_tf_ifft": (2115560 * 2 + 4194306) * 1024,
+             "row_fft_inv": (2097595 * 2 + 4194304) * 1024},
+}
+# VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
+PMC_VALU = {"fp32": {"row_fft_fwd": 821 * 262144, "col_fft_tf_ifft": 1992 * 262144, "row_fft_inv": 871 * 262144}}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
 
@@ -109,13 +121,23 @@ def main():
     props64 = UNITS / (r64["ms_per_step"] * 1e-3)
     err32 = max(rel_l2(r32["unit0"], r64["unit0"]), rel_l2(r32["unit_last"], r64["unit_last"]))
 
-    # dominant kernel of the FP32 pipeline: one read + one write of the batch per launch
+    # Per-kernel roofline of the FP32 pipeline. Algorithmic bytes of a launch:
+    # one read + one write of the 64 fields (each pass must stream the batch
+    # through HBM once); the fused column kernel does the work of two of the
+    # four passes of an unfused forward + inverse 2-D FFT with one of them.
     names = ["row_fft_fwd", "col_fft_tf_ifft", "row_fft_inv"]
     k = r32["kernel_ms"]
     dom = int(np.argmax(k))
     bytes_launch = 2.0 * UNITS * N * N * 8
-    achieved = bytes_launch / (k[dom] * 1e-3) / 1e9
-    # whole step: algorithmic bytes 8*M^2*S per propagation
+    per_kernel = []
+    for i, nm in enumerate(names):
+        gbs = bytes_launch / (k[i] * 1e-3) / 1e9
+        per_kernel.append({"kernel": nm, "ms": round(k[i], 4), "achieved_GBs": round(gbs, 1),
+                           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": PMC_HBM_BYTES["fp32"][nm],
+                           "valu_issue_frac": round(PMC_VALU["fp32"][nm] / VALU_WAVE_INSTR_PER_S / (k[i] * 1e-3), 3)})
+    achieved = per_kernel[dom]["achieved_GBs"]
+    # whole step: algorithmic bytes 8*M^2*S per propagation (two 2-D FFTs x
+    # two out-of-cache passes x read + write)
     step_gbs = props32 * 8.0 * N * N * 8 / 1e9
 
     line = {
@@ -139,10 +161,11 @@ def main():
         "autotune": r32["tune"],
         "fp32_over_fp64": round(props32 / props64, 3),
         "rel_l2_fp32_vs_fp64": err32,
-        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel_ms": [round(x, 4) for x in k],
+                     "traffic": PMC_HBM_BYTES["fp32"][names[dom]],
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "kernels": per_kernel,
                      "step_algorithmic_GBs": round(step_gbs, 1),
                      "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(ROWS* Plan<M>::T) void k_rows_bs(const BsRowArgs A)
   constexpr int T = Plan<M>::T, V = Plan<M>::V;
   constexpr int R0 = first_rad<M, -1>();
   constexpr int NT = ROWS * T;
-  __shared__ cpx<R> lds[ROWS * lds_len(M)];
+  constexpr int LS = row_lds_stride<R>(M, ROWS);
+  __shared__ cpx<R> lds[ROWS * LS];
   __shared__ double red[(NT + 63) / 64];
   const int tid = threadIdx.x;
   const int lr = tid % ROWS, t = tid / ROWS;
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(ROWS* Plan<M>::T) void k_rows_bs(const BsRowArgs A)
     for (int i = 0; i < V; ++i) pin += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
     block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
   }
-  bluestein_block<R, M, 1>(v, t, L, A.tab, (R)A.post, (const cpx<R>*)a.tw, lds + lr * lds_len(M));
+  bluestein_block<R, M, 1>(v, t, L, A.tab, (R)A.post, (const cpx<R>*)a.tw, lds + lr * LS);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   if (a.scale_im != 0 || a.scale_re != 1) {
 #pragma unroll

@@ -230,6 +230,24 @@ struct Plan {
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 constexpr int lds_len(int n) { return n + n / 16 + 1; }
 
+// LDS stride between the transforms of a multi-row workgroup (row = tid %
+// rows). LDS serves 256 bytes (S elements) per cycle; the lanes of one such
+// group are S/rows transform positions of each of the rows. Offsetting row
+// r by r*S/rows elements (mod S) puts the rows of a group in disjoint bank
+// ranges for both the unit-stride and the stride-17 (padded radix-16) access
+// of a stage; with rows >= S the lanes of a group are rows at one position
+// and an odd stride keeps them apart.
+template <typename R>
+constexpr int row_lds_stride(int n, int rows) {
+  constexpr int S = 256 / (int)sizeof(cpx<R>);
+  int l = lds_len(n);
+  if (rows <= 1) return l;
+  const int want = rows >= S ? 1 : S / rows;
+  const int mod = rows >= S ? 2 : S;
+  while (l % mod != want % mod) ++l;
+  return l;
+}
+
 // Apply twiddles and the radix-RAD DFT to the butterflies a thread owns.
 // v[b*RAD + r] holds x[j_b + r N/RAD], j_b = t + b*T. The stage's twiddle
 // table starts at tws + OFF; entry [k][r-1] = exp(-2 pi i r k/(NS RAD)),

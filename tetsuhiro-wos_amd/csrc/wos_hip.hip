@@ -794,7 +794,7 @@ static Tuning resolve_tuning(int prec, int M, Tuning t) {
 // The three launches of one transfer step (forward rows, fused columns,
 // inverse rows) on `units` fields of n x n: power-of-two grid side M = n, or
 // M = 2n zero-padded with the centre crop. inter is the column-tiled
-// intermediate of units * n * M elements and must not alias d.
+// intermediate of units * inter_unit_elems(n, M) elements and must not alias d.
 struct TransferLaunch {
   int prec = 0, M = 0, mode = 1;
   Tuning tu;
@@ -806,6 +806,12 @@ struct TransferLaunch {
   BsRowArgs bfwd, binv;
   BsColArgs bcol;
 };
+
+// Column-tiled intermediate: tiles of tw columns, each n*tw elements plus
+// kTilePad, so the tiles a row pass touches together do not start at
+// power-of-two distances (which concentrate on the same HBM channels).
+constexpr int kTilePad = 32;
+static inline long long inter_unit_elems(int n, int M) { return (long long)n * M + (long long)M * kTilePad; }
 
 static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int n, int units, bool pad,
                           const TFUnit* d_tf, int tf_kind, double post_re, double post_im, double* pw_in,
@@ -834,8 +840,8 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   L.tu = tu;
   L.bs = bs;
   L.Mb = Mk;
-  const long long iunit = (long long)n * M;
-  const int tstride = n * tu.tw;
+  const long long iunit = inter_unit_elems(n, M);
+  const int tstride = n * tu.tw + kTilePad;
   const int lg = ilog2(tu.tw);
   RowArgs& ra = L.fwd;
   ra = row_args_init();
@@ -955,7 +961,7 @@ static int transfer_dev(int prec, void* d, int n, int units, bool pad, const std
     set_error("transform length not supported: " + std::to_string(M));
     return WOS_ERR_UNSUPPORTED;
   }
-  if ((rc = dv.scratch.ensure((size_t)units * n * M * csize(prec)))) return rc;
+  if ((rc = dv.scratch.ensure((size_t)units * inter_unit_elems(n, M) * csize(prec)))) return rc;
   if ((rc = dv.params.ensure(sizeof(TFUnit) * units))) return rc;
   if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * units))) return rc;
   TransferLaunch L;
@@ -1279,7 +1285,7 @@ static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, doubl
   std::vector<TFUnit> tf(ncomp, make_asm_tf(ctx.wavelength, dz / 2, ctx, n, dx, 0, 0));
   if ((rc = dv.params.ensure(sizeof(TFUnit) * ncomp))) return rc;
   if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * ncomp))) return rc;
-  if ((rc = dv.scratch.ensure(ncomp * nn * csize(prec)))) return rc;
+  if ((rc = dv.scratch.ensure(ncomp * inter_unit_elems(n, n) * csize(prec)))) return rc;
   TransferLaunch L;
   Tuning tu;
   tu.tw = default_col_w(prec, n, n);  // the split-step column kernel is built for the default tile
@@ -1713,7 +1719,7 @@ struct wos_batch_s {
   double dx = 0;
   hipStream_t stream = nullptr;
   void* fields = nullptr;   // units * n * n, row-major
-  void* inter = nullptr;    // units * n * M, column-tiled intermediate
+  void* inter = nullptr;    // units * inter_unit_elems(n, M), column-tiled intermediate
   TFUnit* d_tf = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1790,7 +1796,7 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
     return fail(WOS_ERR_NOMEM);
   }
   (void)hipMemset(b->fields, 0, nn * n_units * csize(precision));
-  if ((e = hipMalloc(&b->inter, (size_t)n * M * n_units * csize(precision))) != hipSuccess) {
+  if ((e = hipMalloc(&b->inter, (size_t)inter_unit_elems(n, M) * n_units * csize(precision))) != hipSuccess) {
     b->inter = nullptr;
     set_error(std::string("hipMalloc(batch intermediate): ") + hipGetErrorString(e));
     return fail(WOS_ERR_NOMEM);

@@ -358,7 +358,8 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   static_assert(SCR == 0 || DIR > 0, "the screen follows an inverse transform");
   constexpr int R0 = first_rad<N, DIR>(), RL = SCR ? last_rad<N, -1>() : last_rad<N, DIR>();
   constexpr int NT = ROWS * T;
-  __shared__ cpx<R> lds[ROWS * lds_len(N)];
+  constexpr int LS = row_lds_stride<R>(N, ROWS);
+  __shared__ cpx<R> lds[ROWS * LS];
   __shared__ double red[(NT + 63) / 64];
   const int tid = threadIdx.x;
   const int lr = tid % ROWS, t = tid / ROWS;
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
   }
   constexpr bool ACC = SCR == 1 && sizeof(R) == 4;  // split-step slices: compensated twiddles
-  fft_block<R, N, DIR, 1, ACC>(v, t, tw, lds + lr * lds_len(N));
+  fft_block<R, N, DIR, 1, ACC>(v, t, tw, lds + lr * LS);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   if (a.scale_im == 0) {
     if (a.scale_re != 1) {
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
         const cpx<double> y = cmul(x, scr[slot_pos<N, RS>(t, b, r)]);
         v[b * RS + r] = {(R)y.x, (R)y.y};
       }
-    fft_block<R, N, -1, 1, ACC>(v, t, tw, lds + lr * lds_len(N));
+    fft_block<R, N, -1, 1, ACC>(v, t, tw, lds + lr * LS);
   }
   if (full_out) {
 #pragma unroll
