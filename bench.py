@@ -109,7 +109,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(every unit is independent; see the sharding note in DESIGN.md)", file=sys.stderr)
+              "(the units of a batch are independent; several GPUs would each take a share of them)", file=sys.stderr)
         sys.exit(2)
 
     import wos_hip as wos

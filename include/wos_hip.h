@@ -185,6 +185,31 @@ void* wos_hip_batch_device_ptr(wos_batch b);
  * timed back to back over `iters` launches. kernel_ms may be NULL. */
 int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, double* kernel_ms);
 
+/* ---- phase-mask design operator (no reference counterpart) --------------
+ * Forward/Backward pair behind a Go operator for phase-mask optimisation. For
+ * each of `units` masks theta (n x n, radians): U = ASM_z(exp(i theta)) with
+ * asmTF (optics/propagate.go:159-181) on the fft2D conventions
+ * (optics/fft.go:209-242), I = |U|^2, S = sum I, and the loss
+ * L = sum (I/S - T)^2 against one target T (n x n, real). The forward keeps U
+ * on the device as the checkpoint of the backward, which returns
+ * dL/dtheta = -2 Im(exp(i theta) conj(A^H (dL/dI . U))) with A^H the same
+ * propagation under the conjugate transfer function. Sums are FP64 and
+ * deterministic; precision selects the field arithmetic. */
+typedef struct wos_mask_s* wos_mask;
+
+int wos_hip_mask_create(wos_mask* out, int n, int units, double dx, double z, const wos_ctx* ctx, int precision,
+                        const double* target);
+int wos_hip_mask_destroy(wos_mask m);
+/* theta: units*n*n doubles (unit-major, row-major); loss: units doubles (may be NULL) */
+int wos_hip_mask_forward(wos_mask m, const double* theta, double* loss);
+/* gradient of the last forward's losses: units*n*n doubles (may be NULL to
+ * leave it on the device, see wos_hip_mask_device_grad); repeatable */
+int wos_hip_mask_backward(wos_mask m, double* grad);
+/* the forward's checkpoint U of one unit as complex128 n*n */
+int wos_hip_mask_field(wos_mask m, int unit, double* out);
+/* device pointer of the gradient (units*n*n doubles) */
+void* wos_hip_mask_device_grad(wos_mask m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -828,7 +828,8 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     tu.tw = bs_default_w(prec, Mk, M);
     tu.rows = rows_per_block_host(Mk);
   } else {
-    if (tf_kind != TF_ASM) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
+    const bool lean = tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM;
+    if (!lean) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
     tu = resolve_tuning(prec, M, tu);
   }
   const void* tw;
@@ -836,7 +837,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   if (rc) return rc;
   L.prec = prec;
   L.M = M;
-  L.mode = tf_kind == TF_ASM ? 1 : 2;
+  L.mode = (tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM) ? 1 : 2;
   L.tu = tu;
   L.bs = bs;
   L.Mb = Mk;
@@ -2080,3 +2081,5 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 }
 
 }  // extern "C"
+
+#include "wos_adjoint.inc"

@@ -158,7 +158,9 @@ __device__ __forceinline__ TF32Col tf32_col(const TFUnit& u, int col) {
   c.one_m_ax2 = fma(-ax, ax, 1.0);
   c.cwl = u.wl * u.inv_den;
   c.coff = u.wl * u.fy_off;
-  c.kq = u.kz * 0.63661977236758134308;
+  // conjugate transfer function (adjoint): the propagating phase changes
+  // sign, the real evanescent factor does not
+  c.kq = (u.kind == TF_CONJ_ASM ? -u.kz : u.kz) * 0.63661977236758134308;
   return c;
 }
 
@@ -546,14 +548,19 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
               v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan(u, cc, slot_pos<N, RL>(t, b, r)));
         }
       } else {
+        const bool conj = MODE == 1 && u.kind == TF_CONJ_ASM;
 #pragma unroll
         for (int b = 0; b < V / RL; ++b)
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
             const int p = slot_pos<N, RL>(t, b, r);
             cpx<R> h;
-            if constexpr (MODE == 1) h = asm_tf_fp64_at(u, col, p);
-            else h = tf_eval<R>(u, col, p);
+            if constexpr (MODE == 1) {
+              h = asm_tf_fp64_at(u, col, p);
+              if (conj) h.y = -h.y;
+            } else {
+              h = tf_eval<R>(u, col, p);
+            }
             v[b * RL + r] = cmul(v[b * RL + r], h);
           }
       }

@@ -31,6 +31,8 @@ EXPORTS = [
     "wos_hip_batch_download", "wos_hip_batch_fill_schell", "wos_hip_batch_capture",
     "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
     "wos_hip_batch_autotune", "wos_hip_batch_set_tuning", "wos_hip_last_device_ms",
+    "wos_hip_mask_create", "wos_hip_mask_destroy", "wos_hip_mask_forward", "wos_hip_mask_backward",
+    "wos_hip_mask_field", "wos_hip_mask_device_grad",
 ]
 
 
@@ -107,6 +109,13 @@ def load(path=None):
     L.wos_hip_batch_time.argtypes = [P, I, I, C.POINTER(D), P]
     L.wos_hip_batch_autotune.argtypes = [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D)]
     L.wos_hip_batch_set_tuning.argtypes = [P, I, I]
+    L.wos_hip_mask_create.argtypes = [C.POINTER(P), I, I, D, D, C.POINTER(Ctx), I, P]
+    L.wos_hip_mask_destroy.argtypes = [P]
+    L.wos_hip_mask_forward.argtypes = [P, P, P]
+    L.wos_hip_mask_backward.argtypes = [P, P]
+    L.wos_hip_mask_field.argtypes = [P, I, P]
+    L.wos_hip_mask_device_grad.argtypes = [P]
+    L.wos_hip_mask_device_grad.restype = P
     _lib = L
     return L
 
@@ -288,3 +297,47 @@ class Batch:
         _check(load().wos_hip_batch_time(self.h, warmup, iters, C.byref(ms), _p(k) if kernels else None),
                "batch_time")
         return ms.value, (k if kernels else None)
+
+
+class MaskOp:
+    """Phase-mask design operator (wos_mask): forward loss of `units` masks
+    and the adjoint gradient, the Forward/Backward pair a Go optimiser calls."""
+
+    def __init__(self, n, units, dx, z, ctx, target, precision=FP64):
+        self.n, self.units = n, units
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        assert t.shape == (n, n)
+        h = C.c_void_p()
+        _check(load().wos_hip_mask_create(C.byref(h), n, units, dx, z, C.byref(ctx), precision, _p(t)),
+               "mask_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(load().wos_hip_mask_destroy(self.h), "mask_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, theta):
+        """theta: (units, n, n) radians -> losses (units,)"""
+        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(self.units, self.n, self.n)
+        loss = np.zeros(self.units, dtype=np.float64)
+        _check(load().wos_hip_mask_forward(self.h, _p(th), _p(loss)), "mask_forward")
+        return loss
+
+    def backward(self):
+        """dL/dtheta of the last forward, (units, n, n)"""
+        g = np.zeros((self.units, self.n, self.n), dtype=np.float64)
+        _check(load().wos_hip_mask_backward(self.h, _p(g)), "mask_backward")
+        return g
+
+    def field(self, unit):
+        out = np.zeros((self.n, self.n), dtype=np.complex128)
+        _check(load().wos_hip_mask_field(self.h, unit, _p(out)), "mask_field")
+        return out
+
