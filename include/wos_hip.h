@@ -185,6 +185,51 @@ void* wos_hip_batch_device_ptr(wos_batch b);
  * timed back to back over `iters` launches. kernel_ms may be NULL. */
 int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, double* kernel_ms);
 
+/* ---- heterogeneous pipeline (no reference counterpart: batching is new) --
+ * Stages of batched asm / asm_pad propagations, each with its own grid size
+ * and unit count, run back to back on one stream and captured into one
+ * hipGraph. Stage inputs are kept (outputs go to separate buffers), so each
+ * run recomputes the same outputs. A stage with src_units == 1 propagates one
+ * source field for every unit (PropagatePolychromatic, optics/polychromatic.go:
+ * 18-43, or Propagate3D, optics/volume.go:8-18); with src_units == units each
+ * unit has its own source (independent Propagate calls). Units with z == 0
+ * return their source (optics/propagate.go:66-68). The pipeline emits no
+ * warnings. */
+typedef struct wos_pipeline_s* wos_pipeline;
+
+int wos_hip_pipeline_create(wos_pipeline* out, int precision);
+int wos_hip_pipeline_destroy(wos_pipeline p);
+/* add a stage of `units` propagations (unit u at wavelengths[u] over zs[u]);
+ * *stage receives its index */
+int wos_hip_pipeline_add(wos_pipeline p, int n, double dx, int method, int units, int src_units,
+                         const double* wavelengths, const double* zs, const wos_ctx* ctx, int* stage);
+/* the stage also writes sum_u weights[u] |U_u|^2 (n*n float64) in unit order,
+ * the incoherent sum of PropagatePolychromatic */
+int wos_hip_pipeline_set_intensity(wos_pipeline p, int stage, const double* weights);
+/* copy source src_index of a stage from a complex128 host array of n*n */
+int wos_hip_pipeline_upload(wos_pipeline p, int stage, int src_index, const double* ex);
+/* fill a stage's sources with Gaussian-Schell realisations first_realisation,
+ * first_realisation+1, ... of `seed` generated on the device */
+int wos_hip_pipeline_fill_schell(wos_pipeline p, int stage, double w0, double sigma_c, uint64_t seed,
+                                 int first_realisation);
+/* per-stage autotune as wos_hip_batch_autotune; stage_ms (may be NULL) gets
+ * each stage's best time per run */
+int wos_hip_pipeline_autotune(wos_pipeline p, int iters, double* stage_ms);
+/* capture every stage into one hipGraph; later runs replay it */
+int wos_hip_pipeline_capture(wos_pipeline p);
+/* enqueue `iters` runs on the pipeline's stream (asynchronous) */
+int wos_hip_pipeline_run(wos_pipeline p, int iters);
+int wos_hip_pipeline_sync(wos_pipeline p);
+/* output field of one unit of a stage as complex128 n*n */
+int wos_hip_pipeline_download(wos_pipeline p, int stage, int unit, double* ex);
+/* source field src_index of a stage as complex128 n*n */
+int wos_hip_pipeline_download_source(wos_pipeline p, int stage, int src_index, double* ex);
+/* intensity output of a stage (n*n float64) */
+int wos_hip_pipeline_intensity(wos_pipeline p, int stage, double* out);
+/* HIP-event timing: ms_run = mean time of one run; stage_ms[s] (may be NULL)
+ * = mean time of stage s enqueued alone */
+int wos_hip_pipeline_time(wos_pipeline p, int warmup, int iters, double* ms_run, double* stage_ms);
+
 /* ---- phase-mask design operator (no reference counterpart) --------------
  * Forward/Backward pair behind a Go operator for phase-mask optimisation. For
  * each of `units` masks theta (n x n, radians): U = ASM_z(exp(i theta)) with

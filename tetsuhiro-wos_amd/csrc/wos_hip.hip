@@ -1714,6 +1714,38 @@ int wos_hip_volume(const double* ex, const double* ey, int n, double dx, const d
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
+// Gaussian-Schell realisations u0 .. u0+count-1 of `seed` (optics/coherence.go:22-80
+// with the counter-based normal generator) into dst (count fields of n x n,
+// precision prec), generated in FP64 on the device stream; synchronous.
+static int schell_fill(int prec, int n, double dx, double w0, double sigma_c, uint64_t seed, int u0, int count,
+                       void* dst) {
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  int rc;
+  if ((rc = dv.aux.ensure(nn * 16))) return rc;    // sqrt(S)
+  if ((rc = dv.work.ensure(nn * 16))) return rc;   // one realisation
+  hipLaunchKernelGGL(k_schell_mu, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, n, dx, sigma_c);
+  WOS_HIP_TRY(hipGetLastError());
+  if ((rc = fft2d_dev(WOS_FP64, dv.aux.p, n, 1, 0))) return rc;
+  hipLaunchKernelGGL(k_schell_sqrt, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, nn);
+  WOS_HIP_TRY(hipGetLastError());
+  for (int u = 0; u < count; ++u) {
+    hipLaunchKernelGGL(k_schell_noise, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.work.p,
+                       (const cpx<double>*)dv.aux.p, nn, seed, (uint64_t)(u0 + u));
+    WOS_HIP_TRY(hipGetLastError());
+    if ((rc = fft2d_dev(WOS_FP64, dv.work.p, n, 1, 1))) return rc;
+    if (prec == WOS_FP64)
+      hipLaunchKernelGGL(k_schell_envelope<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<double>*)dv.work.p, (cpx<double>*)dst + u * nn, n, dx, w0);
+    else
+      hipLaunchKernelGGL(k_schell_envelope<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<double>*)dv.work.p, (cpx<float>*)dst + u * nn, n, dx, w0);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  return WOS_OK;
+}
+
 struct wos_batch_s {
   int n = 0, M = 0, method = 0, prec = 0, units = 0;
   bool pad = false;
@@ -1892,33 +1924,8 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
     set_error("wos_hip_batch_fill_schell: width and coherence must be > 0");
     return WOS_ERR_INVALID;
   }
-  Device& dv = device();
-  const int n = b->n;
-  const size_t nn = (size_t)n * n;
-  int rc;
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
-  if ((rc = dv.aux.ensure(nn * 16))) return rc;    // sqrt(S)
-  if ((rc = dv.work.ensure(nn * 16))) return rc;   // one realisation
-  hipLaunchKernelGGL(k_schell_mu, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, n, b->dx, sigma_c);
-  WOS_HIP_TRY(hipGetLastError());
-  if ((rc = fft2d_dev(WOS_FP64, dv.aux.p, n, 1, 0))) return rc;
-  hipLaunchKernelGGL(k_schell_sqrt, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, nn);
-  WOS_HIP_TRY(hipGetLastError());
-  for (int u = 0; u < b->units; ++u) {
-    hipLaunchKernelGGL(k_schell_noise, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.work.p,
-                       (const cpx<double>*)dv.aux.p, nn, seed, (uint64_t)u);
-    WOS_HIP_TRY(hipGetLastError());
-    if ((rc = fft2d_dev(WOS_FP64, dv.work.p, n, 1, 1))) return rc;
-    if (b->prec == WOS_FP64)
-      hipLaunchKernelGGL(k_schell_envelope<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
-                         (const cpx<double>*)dv.work.p, (cpx<double>*)b->fields + u * nn, n, b->dx, w0);
-    else
-      hipLaunchKernelGGL(k_schell_envelope<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
-                         (const cpx<double>*)dv.work.p, (cpx<float>*)b->fields + u * nn, n, b->dx, w0);
-    WOS_HIP_TRY(hipGetLastError());
-  }
-  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
-  return WOS_OK;
+  return schell_fill(b->prec, b->n, b->dx, w0, sigma_c, seed, 0, b->units, b->fields);
 }
 
 // One run of the batch timed with HIP events (mean over iters after a warm-up).
@@ -2083,3 +2090,4 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 }  // extern "C"
 
 #include "wos_adjoint.inc"
+#include "wos_pipeline.inc"

@@ -33,6 +33,11 @@ EXPORTS = [
     "wos_hip_batch_autotune", "wos_hip_batch_set_tuning", "wos_hip_last_device_ms",
     "wos_hip_mask_create", "wos_hip_mask_destroy", "wos_hip_mask_forward", "wos_hip_mask_backward",
     "wos_hip_mask_field", "wos_hip_mask_device_grad",
+    "wos_hip_pipeline_create", "wos_hip_pipeline_destroy", "wos_hip_pipeline_add",
+    "wos_hip_pipeline_set_intensity", "wos_hip_pipeline_upload", "wos_hip_pipeline_fill_schell",
+    "wos_hip_pipeline_autotune", "wos_hip_pipeline_capture", "wos_hip_pipeline_run", "wos_hip_pipeline_sync",
+    "wos_hip_pipeline_download", "wos_hip_pipeline_download_source", "wos_hip_pipeline_intensity",
+    "wos_hip_pipeline_time",
 ]
 
 
@@ -116,6 +121,20 @@ def load(path=None):
     L.wos_hip_mask_field.argtypes = [P, I, P]
     L.wos_hip_mask_device_grad.argtypes = [P]
     L.wos_hip_mask_device_grad.restype = P
+    L.wos_hip_pipeline_create.argtypes = [C.POINTER(P), I]
+    L.wos_hip_pipeline_destroy.argtypes = [P]
+    L.wos_hip_pipeline_add.argtypes = [P, I, D, I, I, I, P, P, C.POINTER(Ctx), C.POINTER(I)]
+    L.wos_hip_pipeline_set_intensity.argtypes = [P, I, P]
+    L.wos_hip_pipeline_upload.argtypes = [P, I, I, P]
+    L.wos_hip_pipeline_fill_schell.argtypes = [P, I, D, D, C.c_uint64, I]
+    L.wos_hip_pipeline_autotune.argtypes = [P, I, P]
+    L.wos_hip_pipeline_capture.argtypes = [P]
+    L.wos_hip_pipeline_run.argtypes = [P, I]
+    L.wos_hip_pipeline_sync.argtypes = [P]
+    L.wos_hip_pipeline_download.argtypes = [P, I, I, P]
+    L.wos_hip_pipeline_download_source.argtypes = [P, I, I, P]
+    L.wos_hip_pipeline_intensity.argtypes = [P, I, P]
+    L.wos_hip_pipeline_time.argtypes = [P, I, I, C.POINTER(D), P]
     _lib = L
     return L
 
@@ -340,4 +359,90 @@ class MaskOp:
         out = np.zeros((self.n, self.n), dtype=np.complex128)
         _check(load().wos_hip_mask_field(self.h, unit, _p(out)), "mask_field")
         return out
+
+
+class Pipeline:
+    """Heterogeneous device pipeline (wos_pipeline): stages of batched
+    propagations of different grid sizes replayed as one hipGraph."""
+
+    def __init__(self, precision=FP64):
+        self.precision = precision
+        self.stages = []
+        h = C.c_void_p()
+        _check(load().wos_hip_pipeline_create(C.byref(h), precision), "pipeline_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(load().wos_hip_pipeline_destroy(self.h), "pipeline_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, n, dx, wavelengths, zs, ctx, method="asm", src_units=None):
+        wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+        z = np.ascontiguousarray(zs, dtype=np.float64)
+        assert wl.size == z.size
+        units = wl.size
+        src = units if src_units is None else src_units
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        st = C.c_int()
+        _check(load().wos_hip_pipeline_add(self.h, n, dx, m, units, src, _p(wl), _p(z), C.byref(ctx),
+                                           C.byref(st)), "pipeline_add")
+        self.stages.append((n, units))
+        return st.value
+
+    def set_intensity(self, stage, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        _check(load().wos_hip_pipeline_set_intensity(self.h, stage, _p(w)), "pipeline_set_intensity")
+
+    def upload(self, stage, src_index, ex):
+        ex = _c128(ex)
+        _check(load().wos_hip_pipeline_upload(self.h, stage, src_index, _p(ex)), "pipeline_upload")
+
+    def fill_schell(self, stage, w0, sigma_c, seed, first=0):
+        _check(load().wos_hip_pipeline_fill_schell(self.h, stage, w0, sigma_c, seed, first), "pipeline_fill_schell")
+
+    def autotune(self, iters=3):
+        ms = np.zeros(len(self.stages), dtype=np.float64)
+        _check(load().wos_hip_pipeline_autotune(self.h, iters, _p(ms)), "pipeline_autotune")
+        return ms
+
+    def capture(self):
+        _check(load().wos_hip_pipeline_capture(self.h), "pipeline_capture")
+
+    def run(self, iters=1):
+        _check(load().wos_hip_pipeline_run(self.h, iters), "pipeline_run")
+
+    def sync(self):
+        _check(load().wos_hip_pipeline_sync(self.h), "pipeline_sync")
+
+    def download(self, stage, unit):
+        n = self.stages[stage][0]
+        out = np.zeros((n, n), dtype=np.complex128)
+        _check(load().wos_hip_pipeline_download(self.h, stage, unit, _p(out)), "pipeline_download")
+        return out
+
+    def source(self, stage, index):
+        n = self.stages[stage][0]
+        out = np.zeros((n, n), dtype=np.complex128)
+        _check(load().wos_hip_pipeline_download_source(self.h, stage, index, _p(out)), "pipeline_download_source")
+        return out
+
+    def intensity(self, stage):
+        n = self.stages[stage][0]
+        out = np.zeros((n, n), dtype=np.float64)
+        _check(load().wos_hip_pipeline_intensity(self.h, stage, _p(out)), "pipeline_intensity")
+        return out
+
+    def time(self, warmup, iters, stages=False):
+        ms = C.c_double()
+        st = np.zeros(len(self.stages), dtype=np.float64)
+        _check(load().wos_hip_pipeline_time(self.h, warmup, iters, C.byref(ms), _p(st) if stages else None),
+               "pipeline_time")
+        return ms.value, (st if stages else None)
 
