@@ -76,6 +76,99 @@ def run_gpu(wos, precision, steps, warmup):
     return out
 
 
+def pcie_inclusive(wos, inp, wl, precision, reps=5):
+    """Drop-in rate: one 2048^2 Propagate (asm) per call from and to host
+    complex128 memory (upload, device pipeline with power sums, download)."""
+    import ctypes
+    ctx = wos.ctx_for(wl)
+    lib = wos.load()
+    a = np.array(inp, dtype=np.complex128, order="C")   # the caller's slice, propagated in place
+    w, st = wos.Warnings(), wos.PropStats()
+
+    def call():
+        rc = lib.wos_hip_propagate(a.ctypes.data, None, N, DX, Z, wos.METHODS["asm"], ctypes.byref(ctx), precision,
+                                   ctypes.byref(w), ctypes.byref(st))
+        if rc:
+            raise RuntimeError(lib.wos_hip_last_error().decode())
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    return reps / (time.perf_counter() - t0)
+
+
+def bench_bpm(wos):
+    """Split-step BPM: 1024^2, dx 0.5 um, 1.55 um, 400 slices of 5 um through
+    the graded-index medium, Gaussian w0 10 um; device time per slice."""
+    n, dx, wl, steps = 1024, 0.5e-6, 1.55e-6, 400
+    x = (np.arange(n) - n / 2) * dx
+    r = np.hypot(x[None, :], x[:, None])
+    index = np.where(r < 100e-6, 1.45 - 0.01 * (r / 100e-6) ** 2, 1.44) + 1e-5j
+    ex = np.exp(-(r / 10e-6) ** 2).astype(np.complex128)
+    out = {}
+    res = {}
+    for name, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
+        o, _, _ = wos.split_step(ex, None, dx, steps * 5e-6, steps, index, wos.ctx_for(wl), precision=prec)
+        ms = wos.last_device_ms()
+        res[name] = o
+        out[name] = {"steps_per_s": round(steps / (ms * 1e-3), 1), "device_ms": round(ms, 3)}
+    out["rel_l2_fp32_vs_fp64"] = rel_l2(res["fp32"], res["fp64"])
+    out["power_ratio_fp64"] = float(np.sum(np.abs(res["fp64"]) ** 2) / np.sum(np.abs(ex) ** 2))
+    return out
+
+
+def bench_pipeline(wos, steps):
+    """One hipGraph: 64 x 2048^2 polychromatic intensity, 128 z-planes at
+    1024^2, 32 mixed fields (384^2, 1000^2, 1536^2); Gaussian-Schell inputs."""
+    out = {}
+    for name, prec in (("fp32", wos.FP32), ("fp64", wos.FP64)):
+        p = wos.Pipeline(prec)
+        ctx = wos.ctx_for(550e-9)
+        s = p.add(2048, 2e-6, np.linspace(400e-9, 700e-9, 64), np.full(64, 20e-3), ctx, "asm", src_units=1)
+        p.set_intensity(s, np.ones(64))
+        p.fill_schell(s, W0, SIGMA_C, SEED, 0)
+        s = p.add(1024, 2e-6, np.full(128, 532e-9), np.linspace(0, 20e-3, 128), ctx, "asm", src_units=1)
+        p.fill_schell(s, W0, SIGMA_C, SEED, 1)
+        for k, n in enumerate((384, 1000, 1536)):
+            cnt = len(range(k, 32, 3))
+            s = p.add(n, 4.096e-3 / n, np.full(cnt, 532e-9), np.full(cnt, 20e-3), ctx, "asm")
+            p.fill_schell(s, W0, SIGMA_C, SEED, 2 + 100 * k)
+        p.autotune(3)
+        p.capture()
+        ms, st = p.time(2, max(steps, 10), stages=True)
+        out[name] = {"ms_per_graph_run": round(ms, 4), "props_per_s": round(224 / (ms * 1e-3), 1),
+                     "stage_ms": [round(x, 4) for x in st]}
+        p.close()
+    return out
+
+
+def bench_adjoint(wos):
+    """Phase-mask design: 8 masks of 1024^2, forward loss + adjoint gradient."""
+    n, units, dx, wl, z = 1024, 8, 4e-6, 532e-9, 20e-3
+    x = (np.arange(n) - n / 2) * dx
+    t = np.zeros((n, n))
+    for k in range(16):
+        a = 2 * np.pi * k / 16
+        t += np.exp(-((x[None, :] - 800e-6 * np.cos(a)) ** 2 + (x[:, None] - 800e-6 * np.sin(a)) ** 2) / (2 * 12e-6 ** 2))
+    t /= t.sum()
+    theta = np.random.default_rng(1).uniform(0, 2 * np.pi, (units, n, n)).astype(np.float32).astype(np.float64)
+    out = {}
+    for name, prec in (("fp32", wos.FP32), ("fp64", wos.FP64)):
+        op = wos.MaskOp(n, units, dx, z, wos.ctx_for(wl), t, precision=prec)
+        op.forward(theta)
+        op.backward()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.forward(theta)
+            op.backward()
+        el = (time.perf_counter() - t0) / reps
+        out[name] = {"ms_forward_backward_8_masks": round(el * 1e3, 2), "note": "host theta in, host gradient out"}
+        op.close()
+    return out
+
+
 def cpu_baseline(inp, wl, budget_s=12.0):
     """Restated reference CPU path (the FP64 oracle: radix-2 FFT over all host
     threads, serial TF loop and Power sums) on one 2048^2 unit of the same batch."""
@@ -169,6 +262,13 @@ def main():
                      "step_algorithmic_GBs": round(step_gbs, 1),
                      "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }
+    line["pcie_inclusive"] = {
+        "unit": "props/s", "workload": "one 2048^2 asm Propagate per call, complex128 host buffers in and out",
+        "fp64": round(pcie_inclusive(wos, r64["unit0_in"], r64["wl0"], wos.FP64), 1),
+        "fp32": round(pcie_inclusive(wos, r64["unit0_in"], r64["wl0"], wos.FP32), 1)}
+    line["bpm_1024_400_slices"] = bench_bpm(wos)
+    line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps)
+    line["mask_adjoint"] = bench_adjoint(wos)
     if not args.no_cpu:
         ref, cb = cpu_baseline(r64["unit0_in"], r64["wl0"])
         line["cpu_baseline"] = cb
