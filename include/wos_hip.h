@@ -82,6 +82,12 @@ typedef struct wos_prop_stats {
     int32_t reserved;
 } wos_prop_stats;
 
+/* PlaneStats of ComputeStats (optics/metrics.go:8-21), same field order. */
+typedef struct wos_plane_stats {
+    double power, peak, centroid_x, centroid_y, rms_x, rms_y, strehl;
+    double intensity_min, intensity_max, phase_min, phase_max;
+} wos_plane_stats;
+
 /* ---- device / library ---- */
 /* replaces wos_device_count (optics/gpu.go:28-33) */
 int wos_hip_device_count(void);
@@ -229,6 +235,18 @@ int wos_hip_pipeline_intensity(wos_pipeline p, int stage, double* out);
 /* HIP-event timing: ms_run = mean time of one run; stage_ms[s] (may be NULL)
  * = mean time of stage s enqueued alone */
 int wos_hip_pipeline_time(wos_pipeline p, int warmup, int iters, double* ms_run, double* stage_ms);
+
+/* ---- plane statistics ----
+ * ComputeStats (optics/metrics.go:31-91): power, peak, intensity range,
+ * centroid, RMS widths, Strehl (when aperture and distance are > 0) and the
+ * phase range of Ex. Sums are compensated FP64 (deterministic). */
+int wos_hip_compute_stats(const double* ex, const double* ey, int n, double dx, double wl, double strehl_aperture,
+                          double strehl_distance, wos_plane_stats* out);
+/* the same for a device-resident field: a batch unit, a pipeline output */
+int wos_hip_batch_stats(wos_batch b, int unit, double wl, double strehl_aperture, double strehl_distance,
+                        wos_plane_stats* out);
+int wos_hip_pipeline_stats(wos_pipeline p, int stage, int unit, double wl, double strehl_aperture,
+                           double strehl_distance, wos_plane_stats* out);
 
 /* ---- phase-mask design operator (no reference counterpart) --------------
  * Forward/Backward pair behind a Go operator for phase-mask optimisation. For

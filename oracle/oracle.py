@@ -47,7 +47,7 @@ class Stats(C.Structure):
 class PlaneStats(C.Structure):
     _fields_ = [(k, C.c_double) for k in
                 ("power", "peak", "centroid_x", "centroid_y", "rms_x", "rms_y",
-                 "strehl", "intensity_min", "intensity_max")]
+                 "strehl", "intensity_min", "intensity_max", "phase_min", "phase_max")]
 
 
 def ctx_for(wl, evanescent="decay", limit=0.0, backward_regularize=False, alpha=0.0):

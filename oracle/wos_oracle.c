@@ -839,12 +839,13 @@ void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx,
     }
 }
 
-/* ComputeStats (optics/metrics.go:31-91), without the phase range. */
+/* ComputeStats (optics/metrics.go:31-91). */
 void or_compute_stats(const or_cplx* ex, const or_cplx* ey, int n, double dx, double wl,
                       double strehl_aperture, double strehl_distance, or_plane_stats* s) {
     memset(s, 0, sizeof(*s));
     double p = 0, px = 0, py = 0;
     s->intensity_min = INFINITY;
+    double ph_min = INFINITY, ph_max = -INFINITY;
     double peak = -1.0;
     for (int j = 0; j < n; j++) {
         double y = coord(j, n, dx);
@@ -857,10 +858,15 @@ void or_compute_stats(const or_cplx* ex, const or_cplx* ey, int n, double dx, do
             if (wv < s->intensity_min) s->intensity_min = wv;
             if (wv > s->intensity_max) s->intensity_max = wv;
             if (wv > peak) peak = wv;
+            double ph = atan2(ex[idx].im, ex[idx].re);
+            if (ph < ph_min) ph_min = ph;
+            if (ph > ph_max) ph_max = ph;
         }
     }
     s->power = p * dx * dx;
     s->peak = peak;
+    s->phase_min = ph_min;
+    s->phase_max = ph_max;
     if (p <= 0) return;
     s->centroid_x = px / p;
     s->centroid_y = py / p;

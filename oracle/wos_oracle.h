@@ -118,10 +118,10 @@ void or_schell_sqrt_spectrum(or_cplx* sqrt_s, int n, double dx, double sigma_c);
 void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx,
                            double w0, uint64_t seed, uint64_t realization);
 
-/* optics/metrics.go:31-91 (power, peak, centroid, rms, strehl). */
+/* optics/metrics.go:31-91: PlaneStats (field order of the Go struct). */
 typedef struct {
     double power, peak, centroid_x, centroid_y, rms_x, rms_y, strehl;
-    double intensity_min, intensity_max;
+    double intensity_min, intensity_max, phase_min, phase_max;
 } or_plane_stats;
 void or_compute_stats(const or_cplx* ex, const or_cplx* ey, int n, double dx, double wl,
                       double strehl_aperture, double strehl_distance, or_plane_stats* s);

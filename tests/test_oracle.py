@@ -347,3 +347,33 @@ def test_mask_adjoint_matches_central_differences(oracle):
         tm[j, i] -= h
         fd = (oracle.mask_loss_grad(tp, T, dx, z, c, False)[0] - oracle.mask_loss_grad(tm, T, dx, z, c, False)[0]) / (2 * h)
         assert abs(fd - g[j, i]) <= 1e-4 * max(abs(fd), 1e-3 * np.max(np.abs(g)))
+
+
+# ---------------------------------------------------------------- plane statistics
+
+def test_compute_stats_closed_forms(oracle):
+    """ComputeStats (optics/metrics.go:31-91) on an offset Gaussian with a
+    constant phase: power pi w0^2/2, centroid (x0, y0), RMS widths w0/2 (the
+    intensity exp(-2 r^2/w0^2) has sigma w0/2), peak 1, phase range = the
+    phase, Strehl = peak / (P pi R^2 / (wl^2 z^2)); a second polarisation
+    component adds its intensity."""
+    n, dx, w0 = 256, 1e-6, 20e-6
+    x0, y0, ph0 = 10e-6, -6e-6, 0.7
+    x = (np.arange(n) - n / 2) * dx
+    X, Y = x[None, :], x[:, None]
+    ex = np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / w0 ** 2) * np.exp(1j * ph0)
+    wl, ap, dist = 632.8e-9, 1e-3, 0.1
+    s = oracle.compute_stats(ex, None, dx, wl, ap, dist)
+    P = math.pi * w0 ** 2 / 2
+    assert abs(s.power - P) <= 1e-12 * P
+    assert abs(s.centroid_x - x0) <= 1e-15 and abs(s.centroid_y - y0) <= 1e-15
+    assert abs(s.rms_x - w0 / 2) <= 1e-12 * w0 and abs(s.rms_y - w0 / 2) <= 1e-12 * w0
+    assert abs(s.peak - 1.0) <= 1e-15 and s.intensity_max == s.peak
+    imin = float(np.min(np.exp(-2 * ((X - x0) ** 2 + (Y - y0) ** 2) / w0 ** 2)))   # far corner
+    assert abs(s.intensity_min - imin) <= 1e-12 * imin
+    assert abs(s.phase_min - ph0) <= 1e-15 and abs(s.phase_max - ph0) <= 1e-15
+    ideal = P * math.pi * ap ** 2 / (wl ** 2 * dist ** 2)
+    assert abs(s.strehl - 1.0 / ideal) <= 1e-12 / ideal
+    s2 = oracle.compute_stats(ex, 0.5 * ex, dx, wl)
+    assert abs(s2.power - 1.25 * P) <= 1e-12 * P and s2.strehl == 0.0
+    assert abs(s2.centroid_x - x0) <= 1e-15

@@ -37,12 +37,19 @@ EXPORTS = [
     "wos_hip_pipeline_set_intensity", "wos_hip_pipeline_upload", "wos_hip_pipeline_fill_schell",
     "wos_hip_pipeline_autotune", "wos_hip_pipeline_capture", "wos_hip_pipeline_run", "wos_hip_pipeline_sync",
     "wos_hip_pipeline_download", "wos_hip_pipeline_download_source", "wos_hip_pipeline_intensity",
-    "wos_hip_pipeline_time",
+    "wos_hip_pipeline_time", "wos_hip_compute_stats", "wos_hip_batch_stats", "wos_hip_pipeline_stats",
 ]
 
 
 class WosError(RuntimeError):
     pass
+
+
+class PlaneStats(C.Structure):
+    """wos_plane_stats: PlaneStats of ComputeStats."""
+    _fields_ = [(k, C.c_double) for k in
+                ("power", "peak", "centroid_x", "centroid_y", "rms_x", "rms_y", "strehl",
+                 "intensity_min", "intensity_max", "phase_min", "phase_max")]
 
 
 class Ctx(C.Structure):
@@ -135,6 +142,9 @@ def load(path=None):
     L.wos_hip_pipeline_download_source.argtypes = [P, I, I, P]
     L.wos_hip_pipeline_intensity.argtypes = [P, I, P]
     L.wos_hip_pipeline_time.argtypes = [P, I, I, C.POINTER(D), P]
+    L.wos_hip_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
+    L.wos_hip_batch_stats.argtypes = [P, I, D, D, D, C.POINTER(PlaneStats)]
+    L.wos_hip_pipeline_stats.argtypes = [P, I, I, D, D, D, C.POINTER(PlaneStats)]
     _lib = L
     return L
 
@@ -203,6 +213,16 @@ def propagate(ex, ey, dx, z, method, ctx, precision=FP64):
     _check(load().wos_hip_propagate(_p(ex), _p(ey), ex.shape[0], dx, z, m, C.byref(ctx), precision,
                                     C.byref(w), C.byref(st)), "propagate")
     return ex, ey, w, st
+
+
+def compute_stats(ex, ey, dx, wl, strehl_aperture=0.0, strehl_distance=0.0):
+    """ComputeStats of a host field on the device."""
+    ex = _c128(ex)
+    ey = None if ey is None else _c128(ey)
+    st = PlaneStats()
+    _check(load().wos_hip_compute_stats(_p(ex), _p(ey), ex.shape[0], dx, wl, strehl_aperture, strehl_distance,
+                                        C.byref(st)), "compute_stats")
+    return st
 
 
 def propagate1d(a, dx, z, wl, ctx, precision=FP64):
@@ -309,6 +329,12 @@ class Batch:
 
     def sync(self):
         _check(load().wos_hip_batch_sync(self.h), "batch_sync")
+
+    def stats(self, unit, wl, strehl_aperture=0.0, strehl_distance=0.0):
+        st = PlaneStats()
+        _check(load().wos_hip_batch_stats(self.h, unit, wl, strehl_aperture, strehl_distance, C.byref(st)),
+               "batch_stats")
+        return st
 
     def time(self, warmup, iters, kernels=False):
         ms = C.c_double()
@@ -432,6 +458,12 @@ class Pipeline:
         out = np.zeros((n, n), dtype=np.complex128)
         _check(load().wos_hip_pipeline_download_source(self.h, stage, index, _p(out)), "pipeline_download_source")
         return out
+
+    def stats(self, stage, unit, wl, strehl_aperture=0.0, strehl_distance=0.0):
+        st = PlaneStats()
+        _check(load().wos_hip_pipeline_stats(self.h, stage, unit, wl, strehl_aperture, strehl_distance,
+                                             C.byref(st)), "pipeline_stats")
+        return st
 
     def intensity(self, stage):
         n = self.stages[stage][0]
