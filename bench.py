@@ -61,7 +61,7 @@ def run_gpu(wos, precision, steps, warmup):
     b.run(1)
     b.sync()
     out = {"unit0_in": inp0, "unit0": b.download(0), "unit_last": b.download(UNITS - 1),
-           "wl0": wls[0], "tune": {"col_w": tune[0], "rows": tune[1], "ms_per_run": round(tune[2], 4)}}
+           "wl0": wls[0], "tune": dict(tune[0], ms_per_run=round(tune[1], 4))}
     b.run(warmup)
     b.sync()
     t0 = time.perf_counter()

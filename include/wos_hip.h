@@ -168,16 +168,17 @@ int wos_hip_batch_download(wos_batch b, int unit, double* ex);
  * (optics/coherence.go:22-80 with a counter-based normal generator; unit u
  * is realisation u of `seed`) */
 int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t seed);
-/* Autotune the launch shape of this batch: every built column-tile width W
- * (columns per column-pass workgroup, also the tile width of the
- * intermediate layout) against every built rows-per-workgroup of the row
- * passes, each timed over `iters` runs with HIP events on the batch stream;
- * the fastest is kept (a captured graph is dropped). The runs propagate the
- * batch's fields: refill them afterwards. Outputs may be NULL. */
-int wos_hip_batch_autotune(wos_batch b, int iters, int* best_w, int* best_rows, double* best_ms);
-/* Set the launch shape directly (e.g. from a cached autotune table): col_w a
- * built column-tile width, rows a built rows-per-workgroup (0: default). */
-int wos_hip_batch_set_tuning(wos_batch b, int col_w, int rows);
+/* Autotune the launch shape of this batch over every built combination of:
+ * the column-tile width of the intermediate layout (tile_w), the columns per
+ * column-pass workgroup (col_w: tile_w, tile_w/2 or tile_w/4), the rows per
+ * row-pass workgroup and the row groups of the pipelined row kernels (0: one
+ * group, 2, 4). Each is timed over `iters` runs with HIP events on the batch
+ * stream; the fastest is kept (a captured graph is dropped). The runs
+ * propagate the batch's fields: refill them afterwards. shape (may be NULL)
+ * receives {tile_w, col_w, rows, groups}; best_ms (may be NULL) its time. */
+int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
+/* Set the launch shape directly (e.g. from a cached autotune table). */
+int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups);
 /* capture one run into a hipGraph; later runs replay it */
 int wos_hip_batch_capture(wos_batch b);
 /* enqueue `iters` runs on the batch's stream (asynchronous) */

@@ -345,3 +345,40 @@ def test_config3_bpm_full_fp32_vs_fp64(wos, oracle):
     assert err <= FP32_TOL
     assert abs(ratio - want) <= 1e-3
     assert w32.as_dict().keys() == w64.as_dict().keys()
+
+
+# ---------------------------------------------------------------- launch shapes
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_launch_shapes_bit_identical(wos, precision):
+    """Layout tile width, column and row workgroup shapes change only the
+    data movement: the one-group kernels give bit-identical results. The
+    pipelined row kernel (row groups > 0) is another instantiation of the
+    same arithmetic, where the compiler may fuse a complex product's
+    multiply-adds the other way round: it must agree to rounding level."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    n = 1024
+    b = wos.Batch(n, 2e-6, np.array([450e-9, 633e-9]), np.array([5e-3, 9e-3]), wos.ctx_for(550e-9), "asm", prec)
+    # (layout tile width, columns per column workgroup, rows per row workgroup, row groups)
+    shapes = [(0, 0, 0, 0), (1, 1, 1, 0), (2, 2, 8, 0), (4, 4, 4, 2), (4, 2, 2, 4), (8, 8, 1, 2), (2, 1, 1, 4),
+              (8, 8, 8, 4), (8, 4, 2, 2), (8, 2, 4, 0), (16, 4, 1, 4)]
+    if prec == wos.FP32:
+        shapes.append((16, 16, 4, 4))   # 16-column workgroups of 1024-point columns fit LDS in FP32 only
+    ref = None
+    for tw, cw, rows, groups in shapes:
+        if tw:
+            b.set_tuning(tw, cw, rows, groups)
+        b.fill_schell(100e-6, 30e-6, 3)
+        b.run(1)
+        b.sync()
+        out = [b.download(u) for u in range(2)]
+        if ref is None:
+            ref = out
+        else:
+            for u in range(2):
+                if groups == 0:
+                    assert np.array_equal(out[u], ref[u]), (tw, cw, rows, groups, u)
+                else:
+                    e = rel_l2(out[u], ref[u])
+                    assert e <= (1e-15 if prec == wos.FP64 else 1e-6), (tw, cw, rows, groups, u, e)
+    b.close()

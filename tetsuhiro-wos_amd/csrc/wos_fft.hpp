@@ -365,4 +365,83 @@ __device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __rest
   run_stages<R, N, DIR, 0, Plan<N>::S, ES, ACC>(v, t, tw, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Twiddles held in registers. A kernel that transforms several rows per
+// thread loads its twiddles once and keeps no twiddle loads inside the
+// transform, so loads it issues for the next rows stay in flight across the
+// whole transform (vmcnt is an in-order counter: waiting on a twiddle load
+// issued after them would wait for them too).
+// ---------------------------------------------------------------------------
+template <typename R, int N, int DIR>
+struct TwReg {
+  static constexpr int V = Plan<N>::V;
+  static constexpr int rad(int s) { return DIR < 0 ? Plan<N>::fwd_rad(s) : Plan<N>::inv_rad(s); }
+  // register index of stage s's first twiddle (stages 1 .. S-1)
+  static constexpr int base(int s) {
+    int c = 0;
+    for (int i = 1; i < s; ++i) c += (V / rad(i)) * (rad(i) - 1);
+    return c;
+  }
+  static constexpr int total() { return base(Plan<N>::S); }
+  cpx<R> w[total() > 0 ? total() : 1];
+};
+
+template <typename R, int N, int DIR, int S>
+__device__ __forceinline__ void tw_load_stages(TwReg<R, N, DIR>& tr, int t, const cpx<R>* __restrict__ tws) {
+  if constexpr (S < Plan<N>::S) {
+    constexpr int RAD = TwReg<R, N, DIR>::rad(S);
+    constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
+    constexpr int OFF = DIR < 0 ? Plan<N>::tw_off_fwd(S) : Plan<N>::tw_off_inv(S);
+    constexpr int B = TwReg<R, N, DIR>::base(S);
+    constexpr int NB = Plan<N>::V / RAD;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int k = (t + b * Plan<N>::T) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) tr.w[B + b * (RAD - 1) + r - 1] = tws[OFF + k * (RAD - 1) + r - 1];
+    }
+    tw_load_stages<R, N, DIR, S + 1>(tr, t, tws);
+  }
+}
+
+template <typename R, int N, int DIR>
+__device__ __forceinline__ void tw_load(TwReg<R, N, DIR>& tr, int t, const cpx<R>* __restrict__ tws) {
+  tw_load_stages<R, N, DIR, 1>(tr, t, tws);
+}
+
+template <typename R, int N, int DIR, int S, int ES>
+__device__ __forceinline__ void run_stages_reg(cpx<R>* v, int t, const TwReg<R, N, DIR>& tr, cpx<R>* lds) {
+  if constexpr (S < Plan<N>::S) {
+    constexpr int RAD = TwReg<R, N, DIR>::rad(S);
+    constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
+    constexpr int NB = Plan<N>::V / RAD;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if constexpr (NS > 1) {
+        constexpr int B = TwReg<R, N, DIR>::base(S);
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) {
+          const cpx<R> w = tr.w[B + b * (RAD - 1) + r - 1];
+          v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], w) : cmulc(v[b * RAD + r], w);
+        }
+      }
+      dft<RAD, DIR>(v + b * RAD);
+    }
+    if constexpr (S + 1 < Plan<N>::S) {
+      constexpr int RADN = TwReg<R, N, DIR>::rad(S + 1);
+      __syncthreads();
+      stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
+      __syncthreads();
+      stage_load_lds<R, N, RADN, ES>(v, t, lds);
+      run_stages_reg<R, N, DIR, S + 1, ES>(v, t, tr, lds);
+    }
+  }
+}
+
+// fft_block with register twiddles (same maps as fft_block).
+template <typename R, int N, int DIR, int ES>
+__device__ __forceinline__ void fft_block_reg(cpx<R>* v, int t, const TwReg<R, N, DIR>& tr, cpx<R>* lds) {
+  run_stages_reg<R, N, DIR, 0, ES>(v, t, tr, lds);
+}
+
 }  // namespace wos

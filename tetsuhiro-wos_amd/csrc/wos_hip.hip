@@ -768,6 +768,8 @@ struct Tuning {
   int tw = 0;
   int rows = 0;
   int xcd_remap = 0;
+  int groups = 0;  // row groups per workgroup of the pipelined row kernels (<= 1: plain)
+  int col_w = 0;   // columns per column-pass workgroup, dividing tw (0: tw)
 };
 
 // Fill unset fields: the tile width and rows per workgroup that make one
@@ -782,6 +784,7 @@ static Tuning resolve_tuning(int prec, int M, Tuning t) {
         break;
       }
   }
+  if (t.col_w <= 0 || t.col_w > t.tw || (t.tw % t.col_w) != 0 || !col_w_valid(prec, M, t.col_w)) t.col_w = t.tw;
   if (t.rows <= 0) {
     const int rd = rows_per_block_host(M);
     int want = std::max(rd, 128 / (S * std::max(1, t.tw)));
@@ -843,6 +846,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   L.Mb = Mk;
   const long long iunit = inter_unit_elems(n, M);
   const int tstride = n * tu.tw + kTilePad;
+  const int groups = bs ? 0 : tu.groups;
   const int lg = ilog2(tu.tw);
   RowArgs& ra = L.fwd;
   ra = row_args_init();
@@ -862,11 +866,13 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ra.scale_re = 1.0;
   ra.pw_in = pw_in;
   ra.tw = tw;
+  ra.groups = groups;
   ColArgs& ca = L.col;
   memset(&ca, 0, sizeof(ca));
   ca.data = inter;
   ca.unit = iunit;
   ca.tiled = 1;
+  ca.tlg = lg;
   ca.tstride = tstride;
   ca.cols = M;
   ca.units = units;
@@ -898,6 +904,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   rb.scale_im = post_im * inv;
   rb.pw_out = pw_out;
   rb.tw = tw;
+  rb.groups = groups;
   if (bs) {
     // fft2DCPU's per-pass 1/M of the inverse is applied by the Bluestein posts
     BsTab fwd_t, inv_t;
@@ -939,7 +946,7 @@ static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s)
     return launch_rows_bs(L.prec, L.Mb, L.binv, s);
   }
   if (k == 0) return launch_rows(L.prec, L.M, -1, L.fwd, s, L.tu.rows);
-  if (k == 1) return launch_cols(L.prec, L.M, L.mode, 0, L.tu.tw, L.col, s);
+  if (k == 1) return launch_cols(L.prec, L.M, L.mode, 0, L.bs ? L.tu.tw : L.tu.col_w, L.col, s);
   return launch_rows(L.prec, L.M, 1, L.inv, s, L.tu.rows);
 }
 
@@ -1943,7 +1950,43 @@ static int batch_time_runs(wos_batch_s* b, int iters, hipEvent_t e0, hipEvent_t 
   return 0;
 }
 
-int wos_hip_batch_autotune(wos_batch b, int iters, int* best_w, int* best_rows, double* best_ms) {
+// Launch shapes the autotuners try on a power-of-two grid of side M: layout
+// tile width tw, column-workgroup width col_w (tw, tw/2 or tw/4), rows per
+// row-pass workgroup, row groups of the pipelined row kernels.
+static std::vector<Tuning> tuning_candidates(int prec, int M, int n) {
+  std::vector<Tuning> out;
+  const int S = (int)csize(prec);
+  for (int tw = 1; tw <= 16; tw *= 2) {
+    if ((M % tw) != 0) continue;
+    for (int cw = tw; cw >= 1 && cw * 4 >= tw; cw /= 2) {
+      if (!col_w_valid(prec, M, cw)) continue;
+      for (int rows = 1; rows <= 16; rows *= 2) {
+        if (!rows_valid(prec, M, rows)) continue;
+        if (rows * tw * S < 64) continue;  // chunks under half a line: never competitive
+        for (int groups : {0, 2, 4}) {
+          if (groups && (M < 256 || n % (rows * groups) != 0)) continue;
+          Tuning t;
+          t.tw = tw;
+          t.col_w = cw;
+          t.rows = rows;
+          t.groups = groups;
+          out.push_back(t);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+static void tuning_out(const Tuning& t, int* shape) {
+  if (!shape) return;
+  shape[0] = t.tw;
+  shape[1] = t.col_w;
+  shape[2] = t.rows;
+  shape[3] = t.groups;
+}
+
+int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) {
   WOS_ENTER();
   if (!b || iters <= 0) return WOS_ERR_INVALID;
   int rc;
@@ -1952,61 +1995,51 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* best_w, int* best_rows, 
   hipEvent_t e0, e1;
   WOS_HIP_TRY(hipEventCreate(&e0));
   WOS_HIP_TRY(hipEventCreate(&e1));
-  const int S = (int)csize(b->prec);
   double best = 1e300;
+  Tuning bt = b->tune;
   if (!pow2(b->M)) {  // Bluestein grids run their fixed launch shape
-    double per;
-    if ((rc = batch_time_runs(b, iters, e0, e1, &per))) return rc;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (best_w) *best_w = 0;
-    if (best_rows) *best_rows = 0;
-    if (best_ms) *best_ms = per;
-    return WOS_OK;
-  }
-  Tuning bt = resolve_tuning(b->prec, b->M, Tuning());
-  const Tuning saved = b->tune;
-  for (int W = 1; W <= 16; W *= 2) {
-    if ((b->M % W) != 0 || !col_w_valid(b->prec, b->M, W)) continue;
-    for (int rows = 1; rows <= 16; rows *= 2) {
-      if (!rows_valid(b->prec, b->M, rows)) continue;
-      if (rows * W * S < 64) continue;  // chunks under half a line: never competitive
-      Tuning t;
-      t.tw = W;
-      t.rows = rows;
+    rc = batch_time_runs(b, iters, e0, e1, &best);
+    bt = Tuning();
+  } else {
+    rc = 0;
+    const Tuning saved = b->tune;
+    for (const Tuning& t : tuning_candidates(b->prec, b->M, b->n)) {
       b->tune = t;
       double per;
-      if ((rc = batch_time_runs(b, iters, e0, e1, &per))) {
-        b->tune = saved;
-        return rc;
-      }
+      if ((rc = batch_time_runs(b, iters, e0, e1, &per))) break;
       if (per < best) {
         best = per;
         bt = t;
       }
     }
+    b->tune = rc ? saved : bt;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->tune = bt;
-  if (best_w) *best_w = bt.tw;
-  if (best_rows) *best_rows = bt.rows;
+  if (rc) return rc;
+  tuning_out(bt, shape);
   if (best_ms) *best_ms = best;
   return WOS_OK;
 }
 
-int wos_hip_batch_set_tuning(wos_batch b, int col_w, int rows) {
+int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
-  if (!pow2(b->M) || col_w <= 0 || (b->M % col_w) != 0 || !col_w_valid(b->prec, b->M, col_w) ||
-      (rows > 0 && !rows_valid(b->prec, b->M, rows))) {
-    set_error("wos_hip_batch_set_tuning: tile width or rows per workgroup not built for this grid");
+  if (!pow2(b->M) || tile_w <= 0 || (b->M % tile_w) != 0 || col_w <= 0 || (tile_w % col_w) != 0 ||
+      !col_w_valid(b->prec, b->M, col_w) || (rows > 0 && !rows_valid(b->prec, b->M, rows)) ||
+      !(groups == 0 || groups == 1 || groups == 2 || groups == 4)) {
+    set_error("wos_hip_batch_set_tuning: tile width " + std::to_string(tile_w) + ", column width " +
+              std::to_string(col_w) + ", rows " + std::to_string(rows) + " or row groups " + std::to_string(groups) +
+              " not built for a grid of " + std::to_string(b->M));
     return WOS_ERR_INVALID;
   }
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
   batch_drop_graph(b);
-  b->tune.tw = col_w;
+  b->tune = Tuning();
+  b->tune.tw = tile_w;
+  b->tune.col_w = col_w;
   b->tune.rows = rows;
+  b->tune.groups = groups;
   return WOS_OK;
 }
 

@@ -185,6 +185,15 @@ __device__ __noinline__ float asm_tf_fp32_evan(const TFUnit& u, TF32Col c, int i
   return (float)asm_evanescent(u, u.k * u.az * sqrt(-x));
 }
 
+// The FP32-path asmTF at row iy, evanescent rows included (their factor is
+// real: the propagating part returns 1 there).
+__device__ __forceinline__ cpx<float> asm_tf_fp32_full(const TFUnit& u, const TF32Col& c, int iy) {
+  bool evan = false;
+  cpx<float> h = asm_tf_fp32_prop(c, iy, u.M, &evan);
+  if (evan) h = {asm_tf_fp32_evan(u, c, iy), 0.f};
+  return h;
+}
+
 __device__ __noinline__ cpx<float> asm_tf_fp32_generic(const TFUnit& u, double fx, double fy) {
   double ax = u.wl * fx, ay = u.wl * fy;
   double rho2 = ax * ax + ay * ay;
@@ -327,6 +336,7 @@ struct RowArgs {
   double* pw_out;               // per-block sum |out|^2 (nullable)
   const void* tw;               // N-point twiddles
   const void* screen;           // SCR kernels: N x N row-major complex128 phase screen
+  int groups;                   // row groups per workgroup of the pipelined kernel (<= 1: plain)
 };
 
 // RowArgs with row-major addressing on both sides and no embed window.
@@ -456,6 +466,62 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   }
 }
 
+// Pipelined row pass for whole rows (no embed window, no power sums, no
+// screen): each workgroup transforms G groups of ROWS rows. The loads of
+// group g+1 are issued before the transform of group g, and the twiddles sit
+// in registers, so those loads stay in flight across the transform and the
+// stores: a CU keeps HBM requests outstanding while it computes, where the
+// one-group kernel alternates between loading and computing.
+template <typename R, int N, int DIR, int ROWS, int G>
+__global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a) {
+  constexpr int T = Plan<N>::T, V = Plan<N>::V;
+  constexpr int R0 = first_rad<N, DIR>(), RL = last_rad<N, DIR>();
+  constexpr int LS = row_lds_stride<R>(N, ROWS);
+  __shared__ cpx<R> lds[ROWS * LS];
+  const int tid = threadIdx.x;
+  const int lr = tid % ROWS, t = tid / ROWS;
+  const int bpu = a.rows / (ROWS * G);
+  const int unit = blockIdx.x / bpu;
+  const int row0 = (blockIdx.x % bpu) * ROWS * G + lr;
+  const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
+  cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
+  TwReg<R, N, DIR> tr;
+  tw_load<R, N, DIR>(tr, t, (const cpx<R>*)a.tw);
+  const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
+  const bool scale = a.scale_im != 0 || a.scale_re != 1;
+  cpx<R> v[V], nx[V];
+#pragma unroll
+  for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      v[b * R0 + r] = in[tiled_offset(slot_pos<N, R0>(t, b, r), row0, a.in_ld, a.in_tw, a.in_tstride)];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int row = row0 + g * ROWS;
+    if (g + 1 < G) {
+#pragma unroll
+      for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+          nx[b * R0 + r] = in[tiled_offset(slot_pos<N, R0>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride)];
+    }
+    fft_block_reg<R, N, DIR, 1>(v, t, tr, lds + lr * LS);
+    if (scale) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);
+    }
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r)
+        out[tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride)] = v[b * RL + r];
+    if (g + 1 < G) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = nx[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Column pass over a tile of W adjacent columns. MODE 0: plain transform
 // (direction DIR, output scale); MODE 1: forward -> asm TF -> inverse (the
@@ -468,7 +534,7 @@ struct ColArgs {
   void* data;          // in place
   long long unit;      // elements between units
   int ld;              // elements between rows (row-major)
-  int tiled;           // column-tiled layout with tile width W
+  int tiled;           // column-tiled layout (tile width 2^tlg, a multiple of W); 0: row-major
   int tstride;         // elements between tiles (tiled)
   int cols;            // columns per unit (multiple of W)
   int units;
@@ -480,6 +546,7 @@ struct ColArgs {
   const void* tw;
   int tf_square;       // MODE 3: apply H^2 (two merged propagation steps)
   double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2 (nullable)
+  int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
@@ -500,7 +567,10 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
   const int unit = bid / tpu;
   const int tile = bid % tpu;
   const int col = tile * W + c;
-  cpx<R>* base = (cpx<R>*)a.data + (size_t)unit * a.unit + (a.tiled ? tile * a.tstride + c : col);
+  // layout tile of 2^tlg columns; the workgroup's W columns are one part of it
+  const int tl = a.tiled ? 1 << a.tlg : 0;
+  cpx<R>* base =
+      (cpx<R>*)a.data + (size_t)unit * a.unit + (a.tiled ? (col >> a.tlg) * a.tstride + (col & (tl - 1)) : col);
   const cpx<R>* tw = (const cpx<R>*)a.tw;
   const bool active = unit < a.units;
   // uniform fast path: contiguous tile, every transform position present
@@ -513,9 +583,9 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = base[slot_pos<N, R0>(t, b, r) * W];
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = base[slot_pos<N, R0>(t, b, r) * tl];
   } else {
-    const int ld = a.tiled ? W : a.ld;
+    const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
@@ -532,7 +602,48 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
     constexpr int RL = last_rad<N, -1>();
     if (active) {
       const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
-      if constexpr (MODE == 1 && sizeof(R) == 4) {
+      // Without a spectral shift H(fx, -fy) = H(fx, fy), bit for bit (the
+      // frequency of row N-p is the exact negative of row p's). Thread t
+      // holds rows p = t + T m (slot (b, r): m = b + r V/RL); row N - p sits
+      // in thread T - t at m' = V-1-m (thread 0: its own m' = V-m). Every
+      // thread evaluates H for its rows m < V/2, passes them on through LDS
+      // (free between the two transforms) and takes the rest from its
+      // partner: each wave evaluates half as many transfer-function values.
+      const bool mirror = MODE == 1 && T >= 2 && V >= 2 && u.fy_off == 0.0;
+      if (mirror) {
+        constexpr int NB = V / RL, V2 = V / 2;
+        cpx<R>* hx = lds + c;
+        const int pt = (T - t) & (T - 1);
+        const TF32Col cc = tf32_col(u, col);  // FP32 path only
+        const bool conj = u.kind == TF_CONJ_ASM;
+        auto tf_at = [&](int p) -> cpx<R> {
+          if constexpr (sizeof(R) == 4) {
+            return asm_tf_fp32_full(u, cc, p);
+          } else {
+            cpx<double> h = asm_tf_fp64_at(u, col, p);
+            if (conj) h.y = -h.y;
+            return h;
+          }
+        };
+        __syncthreads();  // all waves are done reading the forward transform's last exchange
+#pragma unroll
+        for (int m = 0; m < V2; ++m) {
+          const cpx<R> h = tf_at(t + T * m);
+          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
+          hx[(m * T + t) * W] = h;
+        }
+        if (t == 0) {  // row N/2 (Nyquist) is its own mirror
+          constexpr int m = V2;
+          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = V2; m < V; ++m) {
+          if (t == 0 && m == V2) continue;
+          const int ms = t == 0 ? V - m : V - 1 - m;
+          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
+        }
+      } else if constexpr (MODE == 1 && sizeof(R) == 4) {
         const TF32Col cc = tf32_col(u, col);
         bool evan = false;
 #pragma unroll
@@ -627,9 +738,9 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) base[slot_pos<N, RL>(t, b, r) * W] = v[b * RL + r];
+      for (int r = 0; r < RL; ++r) base[slot_pos<N, RL>(t, b, r) * tl] = v[b * RL + r];
   } else {
-    const int ld = a.tiled ? W : a.ld;
+    const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
