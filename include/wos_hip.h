@@ -171,14 +171,16 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
 /* Autotune the launch shape of this batch over every built combination of:
  * the column-tile width of the intermediate layout (tile_w), the columns per
  * column-pass workgroup (col_w: tile_w, tile_w/2 or tile_w/4), the rows per
- * row-pass workgroup and the row groups of the pipelined row kernels (0: one
- * group, 2, 4). Each is timed over `iters` runs with HIP events on the batch
- * stream; the fastest is kept (a captured graph is dropped). The runs
- * propagate the batch's fields: refill them afterwards. shape (may be NULL)
- * receives {tile_w, col_w, rows, groups}; best_ms (may be NULL) its time. */
+ * row-pass workgroup, the row groups of the pipelined row kernels (0: one
+ * group, 2, 4) and, for one-group row kernels, split LDS exchanges (real and
+ * imaginary parts in two rounds through half the LDS). Each is timed over
+ * `iters` runs with HIP events on the batch stream; the fastest is kept (a
+ * captured graph is dropped). The runs propagate the batch's fields: refill
+ * them afterwards. shape (may be NULL) receives {tile_w, col_w, rows,
+ * groups, split}; best_ms (may be NULL) its time. */
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
 /* Set the launch shape directly (e.g. from a cached autotune table). */
-int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups);
+int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split);
 /* capture one run into a hipGraph; later runs replay it */
 int wos_hip_batch_capture(wos_batch b);
 /* enqueue `iters` runs on the batch's stream (asynchronous) */

@@ -237,9 +237,9 @@ constexpr int lds_len(int n) { return n + n / 16 + 1; }
 // ranges for both the unit-stride and the stride-17 (padded radix-16) access
 // of a stage; with rows >= S the lanes of a group are rows at one position
 // and an odd stride keeps them apart.
-template <typename R>
-constexpr int row_lds_stride(int n, int rows) {
-  constexpr int S = 256 / (int)sizeof(cpx<R>);
+template <typename E>
+constexpr int lds_stride_elems(int n, int rows) {
+  constexpr int S = 256 / (int)sizeof(E);
   int l = lds_len(n);
   if (rows <= 1) return l;
   const int want = rows >= S ? 1 : S / rows;
@@ -247,6 +247,8 @@ constexpr int row_lds_stride(int n, int rows) {
   while (l % mod != want % mod) ++l;
   return l;
 }
+template <typename R>
+constexpr int row_lds_stride(int n, int rows) { return lds_stride_elems<cpx<R>>(n, rows); }
 
 // Apply twiddles and the radix-RAD DFT to the butterflies a thread owns.
 // v[b*RAD + r] holds x[j_b + r N/RAD], j_b = t + b*T. The stage's twiddle
@@ -305,6 +307,41 @@ __device__ __forceinline__ void stage_store_lds(const cpx<R>* v, int t, cpx<R>* 
   }
 }
 
+// One component (COMP 0: real, 1: imaginary) of a stage's outputs / the
+// next stage's inputs, through an LDS array of R: the exchange of a split
+// transform takes two rounds but half the LDS.
+template <typename R, int N, int RAD, int NS, int ES, int COMP>
+__device__ __forceinline__ void stage_store_lds_c(const cpx<R>* v, int t, R* lds) {
+  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int NB = V / RAD;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+    const int k = j & (NS - 1);
+    R* p = lds + lpad((j - k) * RAD + k) * ES;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) p[(r * NS + ((r * NS) >> 4)) * ES] = COMP == 0 ? v[b * RAD + r].x : v[b * RAD + r].y;
+  }
+}
+
+template <typename R, int N, int RAD, int ES, int COMP>
+__device__ __forceinline__ void stage_load_lds_c(cpx<R>* v, int t, const R* lds) {
+  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int NB = V / RAD;
+  constexpr int X = N / RAD;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+    const R* p = lds + lpad(j) * ES;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) {
+      const R x = p[(r * X + ((r * X) >> 4)) * ES];
+      if constexpr (COMP == 0) v[b * RAD + r].x = x;
+      else v[b * RAD + r].y = x;
+    }
+  }
+}
+
 // Read the next stage's inputs from LDS: v[b*RAD + r] = x[j_b + r N/RAD].
 template <typename R, int N, int RAD, int ES>
 __device__ __forceinline__ void stage_load_lds(cpx<R>* v, int t, const cpx<R>* lds) {
@@ -331,7 +368,7 @@ __device__ __forceinline__ int slot_pos(int t, int b, int r) {
 // LDS exchanges between them. On entry v holds the inputs of stage S0 in its
 // butterfly order; on exit v holds the outputs of stage S1-1 in the order
 // "slot (b, r) = position t + b T + r N/RAD_last" (valid for the last stage).
-template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false>
+template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false, bool SPLIT = false>
 __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
   if constexpr (S < SEND) {
     constexpr int RAD = DIR < 0 ? Plan<N>::fwd_rad(S) : Plan<N>::inv_rad(S);
@@ -340,11 +377,23 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __res
     stage_compute<R, N, DIR, RAD, NS, OFF, ACC>(v, t, tw);
     if constexpr (S + 1 < SEND) {
       constexpr int RADN = DIR < 0 ? Plan<N>::fwd_rad(S + 1) : Plan<N>::inv_rad(S + 1);
-      __syncthreads();  // previous readers of lds are done
-      stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
-      __syncthreads();
-      stage_load_lds<R, N, RADN, ES>(v, t, lds);
-      run_stages<R, N, DIR, S + 1, SEND, ES, ACC>(v, t, tw, lds);
+      if constexpr (SPLIT) {  // lds holds lds_len(N) values of R per transform
+        R* lr = (R*)lds;
+        __syncthreads();  // previous readers of lds are done
+        stage_store_lds_c<R, N, RAD, NS, ES, 0>(v, t, lr);
+        __syncthreads();
+        stage_load_lds_c<R, N, RADN, ES, 0>(v, t, lr);
+        __syncthreads();
+        stage_store_lds_c<R, N, RAD, NS, ES, 1>(v, t, lr);
+        __syncthreads();
+        stage_load_lds_c<R, N, RADN, ES, 1>(v, t, lr);
+      } else {
+        __syncthreads();  // previous readers of lds are done
+        stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
+        __syncthreads();
+        stage_load_lds<R, N, RADN, ES>(v, t, lds);
+      }
+      run_stages<R, N, DIR, S + 1, SEND, ES, ACC, SPLIT>(v, t, tw, lds);
     }
   }
 }
@@ -360,53 +409,87 @@ constexpr int last_rad() {
 // Full N-point transform of the values in v (input map of first_rad, output
 // map of last_rad).
 // ACC: compensated twiddles and constants (tw then points to a [hi, lo] table).
-template <typename R, int N, int DIR, int ES, bool ACC = false>
+// SPLIT: exchanges real and imaginary parts in two rounds through half the
+// LDS (lds then holds lds_len(N) values of R per transform, not complex).
+template <typename R, int N, int DIR, int ES, bool ACC = false, bool SPLIT = false>
 __device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
-  run_stages<R, N, DIR, 0, Plan<N>::S, ES, ACC>(v, t, tw, lds);
+  run_stages<R, N, DIR, 0, Plan<N>::S, ES, ACC, SPLIT>(v, t, tw, lds);
 }
 
 // ---------------------------------------------------------------------------
-// Twiddles held in registers. A kernel that transforms several rows per
-// thread loads its twiddles once and keeps no twiddle loads inside the
-// transform, so loads it issues for the next rows stay in flight across the
-// whole transform (vmcnt is an in-order counter: waiting on a twiddle load
-// issued after them would wait for them too).
+// Twiddles held in registers as seeds. A kernel that transforms several rows
+// per thread keeps no twiddle loads inside the transform, so loads it issues
+// for the next rows stay in flight across the whole transform (vmcnt is an
+// in-order counter: waiting on a twiddle load issued after them would wait
+// for them too). Per butterfly block a thread keeps w^1 and, for radix 8 and
+// 16, w^4 in FP64 (w = exp(-2 pi i k/(NS RAD))); the other powers are FP64
+// products of these (at most four deep, error ~1e-15) rounded once to R.
 // ---------------------------------------------------------------------------
 template <typename R, int N, int DIR>
 struct TwReg {
   static constexpr int V = Plan<N>::V;
   static constexpr int rad(int s) { return DIR < 0 ? Plan<N>::fwd_rad(s) : Plan<N>::inv_rad(s); }
-  // register index of stage s's first twiddle (stages 1 .. S-1)
+  static constexpr int seeds(int s) { return rad(s) >= 8 ? 2 : 1; }
+  // register index of stage s's first seed (stages 1 .. S-1)
   static constexpr int base(int s) {
     int c = 0;
-    for (int i = 1; i < s; ++i) c += (V / rad(i)) * (rad(i) - 1);
+    for (int i = 1; i < s; ++i) c += (V / rad(i)) * seeds(i);
     return c;
   }
   static constexpr int total() { return base(Plan<N>::S); }
-  cpx<R> w[total() > 0 ? total() : 1];
+  cpx<double> w[total() > 0 ? total() : 1];
 };
 
 template <typename R, int N, int DIR, int S>
-__device__ __forceinline__ void tw_load_stages(TwReg<R, N, DIR>& tr, int t, const cpx<R>* __restrict__ tws) {
+__device__ __forceinline__ void tw_load_stages(TwReg<R, N, DIR>& tr, int t, const cpx<double>* __restrict__ tws) {
   if constexpr (S < Plan<N>::S) {
     constexpr int RAD = TwReg<R, N, DIR>::rad(S);
     constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
     constexpr int OFF = DIR < 0 ? Plan<N>::tw_off_fwd(S) : Plan<N>::tw_off_inv(S);
     constexpr int B = TwReg<R, N, DIR>::base(S);
+    constexpr int SE = TwReg<R, N, DIR>::seeds(S);
     constexpr int NB = Plan<N>::V / RAD;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int k = (t + b * Plan<N>::T) & (NS - 1);
-#pragma unroll
-      for (int r = 1; r < RAD; ++r) tr.w[B + b * (RAD - 1) + r - 1] = tws[OFF + k * (RAD - 1) + r - 1];
+      tr.w[B + b * SE] = tws[OFF + k * (RAD - 1)];               // w^1
+      if constexpr (SE == 2) tr.w[B + b * SE + 1] = tws[OFF + k * (RAD - 1) + 3];  // w^4
     }
     tw_load_stages<R, N, DIR, S + 1>(tr, t, tws);
   }
 }
 
+// tws: the FP64 stage twiddle table of N points (any kernel precision)
 template <typename R, int N, int DIR>
-__device__ __forceinline__ void tw_load(TwReg<R, N, DIR>& tr, int t, const cpx<R>* __restrict__ tws) {
+__device__ __forceinline__ void tw_load(TwReg<R, N, DIR>& tr, int t, const cpx<double>* __restrict__ tws) {
   tw_load_stages<R, N, DIR, 1>(tr, t, tws);
+}
+
+// powers w^1 .. w^(RAD-1) of a block from its seeds, rounded to R
+template <typename R, int RAD>
+__device__ __forceinline__ void tw_expand(const cpx<double>* sd, cpx<R>* w) {
+  const cpx<double> w1 = sd[0];
+  w[1] = {(R)w1.x, (R)w1.y};
+  if constexpr (RAD >= 4) {
+    const cpx<double> w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    w[2] = {(R)w2.x, (R)w2.y};
+    w[3] = {(R)w3.x, (R)w3.y};
+    if constexpr (RAD >= 8) {
+      const cpx<double> w4 = sd[1];
+      const cpx<double> w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+      w[4] = {(R)w4.x, (R)w4.y};
+      w[5] = {(R)w5.x, (R)w5.y};
+      w[6] = {(R)w6.x, (R)w6.y};
+      w[7] = {(R)w7.x, (R)w7.y};
+      if constexpr (RAD == 16) {
+        const cpx<double> w8 = cmul(w4, w4), w12 = cmul(w8, w4);
+        const cpx<double> p[8] = {w8, cmul(w8, w1), cmul(w8, w2), cmul(w8, w3),
+                                  w12, cmul(w12, w1), cmul(w12, w2), cmul(w12, w3)};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[8 + i] = {(R)p[i].x, (R)p[i].y};
+      }
+    }
+  }
 }
 
 template <typename R, int N, int DIR, int S, int ES>
@@ -419,11 +502,12 @@ __device__ __forceinline__ void run_stages_reg(cpx<R>* v, int t, const TwReg<R, 
     for (int b = 0; b < NB; ++b) {
       if constexpr (NS > 1) {
         constexpr int B = TwReg<R, N, DIR>::base(S);
+        constexpr int SE = TwReg<R, N, DIR>::seeds(S);
+        cpx<R> w[RAD];
+        tw_expand<R, RAD>(tr.w + B + b * SE, w);
 #pragma unroll
-        for (int r = 1; r < RAD; ++r) {
-          const cpx<R> w = tr.w[B + b * (RAD - 1) + r - 1];
-          v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], w) : cmulc(v[b * RAD + r], w);
-        }
+        for (int r = 1; r < RAD; ++r)
+          v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], w[r]) : cmulc(v[b * RAD + r], w[r]);
       }
       dft<RAD, DIR>(v + b * RAD);
     }
@@ -438,7 +522,7 @@ __device__ __forceinline__ void run_stages_reg(cpx<R>* v, int t, const TwReg<R, 
   }
 }
 
-// fft_block with register twiddles (same maps as fft_block).
+// fft_block with register twiddle seeds (same maps as fft_block).
 template <typename R, int N, int DIR, int ES>
 __device__ __forceinline__ void fft_block_reg(cpx<R>* v, int t, const TwReg<R, N, DIR>& tr, cpx<R>* lds) {
   run_stages_reg<R, N, DIR, 0, ES>(v, t, tr, lds);

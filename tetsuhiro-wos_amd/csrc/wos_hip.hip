@@ -770,6 +770,7 @@ struct Tuning {
   int xcd_remap = 0;
   int groups = 0;  // row groups per workgroup of the pipelined row kernels (<= 1: plain)
   int col_w = 0;   // columns per column-pass workgroup, dividing tw (0: tw)
+  int split = 0;   // one-group row kernels exchange real and imaginary parts separately
 };
 
 // Fill unset fields: the tile width and rows per workgroup that make one
@@ -835,9 +836,10 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     if (!lean) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
     tu = resolve_tuning(prec, M, tu);
   }
-  const void* tw;
+  const void *tw, *tw64;
   int rc = get_twiddles(Mk, prec, &tw);
   if (rc) return rc;
+  if ((rc = get_twiddles(Mk, WOS_FP64, &tw64))) return rc;
   L.prec = prec;
   L.M = M;
   L.mode = (tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM) ? 1 : 2;
@@ -847,6 +849,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   const long long iunit = inter_unit_elems(n, M);
   const int tstride = n * tu.tw + kTilePad;
   const int groups = bs ? 0 : tu.groups;
+  const int split = bs ? 0 : tu.split;
   const int lg = ilog2(tu.tw);
   RowArgs& ra = L.fwd;
   ra = row_args_init();
@@ -866,7 +869,9 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ra.scale_re = 1.0;
   ra.pw_in = pw_in;
   ra.tw = tw;
+  ra.tw64 = tw64;
   ra.groups = groups;
+  ra.split = split;
   ColArgs& ca = L.col;
   memset(&ca, 0, sizeof(ca));
   ca.data = inter;
@@ -904,7 +909,9 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   rb.scale_im = post_im * inv;
   rb.pw_out = pw_out;
   rb.tw = tw;
+  rb.tw64 = tw64;
   rb.groups = groups;
+  rb.split = split;
   if (bs) {
     // fft2DCPU's per-pass 1/M of the inverse is applied by the Bluestein posts
     BsTab fwd_t, inv_t;
@@ -1965,12 +1972,15 @@ static std::vector<Tuning> tuning_candidates(int prec, int M, int n) {
         if (rows * tw * S < 64) continue;  // chunks under half a line: never competitive
         for (int groups : {0, 2, 4}) {
           if (groups && (M < 256 || n % (rows * groups) != 0)) continue;
-          Tuning t;
-          t.tw = tw;
-          t.col_w = cw;
-          t.rows = rows;
-          t.groups = groups;
-          out.push_back(t);
+          for (int split = 0; split <= (groups == 0 && M >= 256 ? 1 : 0); ++split) {
+            Tuning t;
+            t.tw = tw;
+            t.col_w = cw;
+            t.rows = rows;
+            t.groups = groups;
+            t.split = split;
+            out.push_back(t);
+          }
         }
       }
     }
@@ -1984,6 +1994,7 @@ static void tuning_out(const Tuning& t, int* shape) {
   shape[1] = t.col_w;
   shape[2] = t.rows;
   shape[3] = t.groups;
+  shape[4] = t.split;
 }
 
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) {
@@ -2022,7 +2033,7 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) 
   return WOS_OK;
 }
 
-int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups) {
+int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
   if (!pow2(b->M) || tile_w <= 0 || (b->M % tile_w) != 0 || col_w <= 0 || (tile_w % col_w) != 0 ||
@@ -2040,6 +2051,7 @@ int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int g
   b->tune.col_w = col_w;
   b->tune.rows = rows;
   b->tune.groups = groups;
+  b->tune.split = split ? 1 : 0;
   return WOS_OK;
 }
 

@@ -337,6 +337,8 @@ struct RowArgs {
   const void* tw;               // N-point twiddles
   const void* screen;           // SCR kernels: N x N row-major complex128 phase screen
   int groups;                   // row groups per workgroup of the pipelined kernel (<= 1: plain)
+  int split;                    // one-group kernel with split (real, imaginary) LDS exchanges
+  const void* tw64;             // FP64 twiddles of the N points (pipelined kernel)
 };
 
 // RowArgs with row-major addressing on both sides and no embed window.
@@ -364,13 +366,18 @@ __device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int 
 // phase screen -> forward row FFT, the split-step BPM screen between two
 // propagation steps (the inverse output map is the forward input map, so
 // the data stays in registers).
-template <typename R, int N, int DIR, int ROWS, int SCR = 0>
+// SPLIT: the stage exchanges move real and imaginary parts in two rounds
+// through half the LDS, so more workgroups fit a CU.
+template <typename R, int N, int DIR, int ROWS, int SCR = 0, bool SPLIT = false>
 __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
   constexpr int T = Plan<N>::T, V = Plan<N>::V;
   static_assert(SCR == 0 || DIR > 0, "the screen follows an inverse transform");
   constexpr int R0 = first_rad<N, DIR>(), RL = SCR ? last_rad<N, -1>() : last_rad<N, DIR>();
   constexpr int NT = ROWS * T;
-  constexpr int LS = row_lds_stride<R>(N, ROWS);
+  // per-row LDS stride in complex values (SPLIT: half of a stride in values
+  // of R, rounded up so rows start complex-aligned; the multi-row strides
+  // are even already)
+  constexpr int LS = SPLIT ? (lds_stride_elems<R>(N, ROWS) + 1) / 2 : row_lds_stride<R>(N, ROWS);
   __shared__ cpx<R> lds[ROWS * LS];
   __shared__ double red[(NT + 63) / 64];
   const int tid = threadIdx.x;
@@ -411,7 +418,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
   }
   constexpr bool ACC = SCR == 1 && sizeof(R) == 4;  // split-step slices: compensated twiddles
-  fft_block<R, N, DIR, 1, ACC>(v, t, tw, lds + lr * LS);
+  fft_block<R, N, DIR, 1, ACC, SPLIT>(v, t, tw, lds + lr * LS);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   if (a.scale_im == 0) {
     if (a.scale_re != 1) {
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
         const cpx<double> y = cmul(x, scr[slot_pos<N, RS>(t, b, r)]);
         v[b * RS + r] = {(R)y.x, (R)y.y};
       }
-    fft_block<R, N, -1, 1, ACC>(v, t, tw, lds + lr * LS);
+    fft_block<R, N, -1, 1, ACC, SPLIT>(v, t, tw, lds + lr * LS);
   }
   if (full_out) {
 #pragma unroll
@@ -486,7 +493,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
   const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
   cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
   TwReg<R, N, DIR> tr;
-  tw_load<R, N, DIR>(tr, t, (const cpx<R>*)a.tw);
+  tw_load<R, N, DIR>(tr, t, (const cpx<double>*)a.tw64);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   const bool scale = a.scale_im != 0 || a.scale_re != 1;
   cpx<R> v[V], nx[V];
