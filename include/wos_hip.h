@@ -138,6 +138,11 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
                        const double* index, int n_planes, const wos_ctx* ctx, int precision,
                        wos_warnings* w);
 
+/* Launch shape of the split-step slices on an n-point power-of-two grid:
+ * column tile width (the default, or narrower from n = 512) and rows per row
+ * workgroup (0: default); tile_w = 0 restores the built-in choice. */
+int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows);
+
 /* PropagatePolychromatic (optics/polychromatic.go:18-43): out (n*n float64)
  * = sum_s weight_s * |U_s|^2; the input field is not modified. */
 int wos_hip_polychromatic(const double* ex, const double* ey, int n, double dx, double z,

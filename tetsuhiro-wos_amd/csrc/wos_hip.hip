@@ -1245,33 +1245,59 @@ static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double 
 }
 
 // evanescent_filtered checks of merged split-step propagations from the
-// spectral power sums of one column pass (Parseval: the ratio of the spatial
-// powers before and after a propagation equals the ratio of these sums).
-// halves = 1: one step (|U|^2 -> |HU|^2); 2: two steps (-> |H^2 U|^2).
-// Accumulates into the device warnings in Warnings.Add order.
-__global__ void k_split_warn(const double* __restrict__ pw3, int nblocks, int halves, double z, wos_warnings* dw) {
+// spectral power sums of the column passes (Parseval: the ratio of the
+// spatial powers before and after a propagation equals the ratio of these
+// sums): |U|^2 -> |HU|^2 for one step, -> |H^2 U|^2 for two merged steps.
+// Spectral power totals of every slice of a run: one workgroup per slice
+// sums that slice's per-workgroup partials (pw3 + s*3*nblocks) into
+// tot[3*s .. 3*s+2] (Neumaier per thread, fixed-order block sum).
+__global__ void k_split_totals(const double* __restrict__ pw3, int nblocks, double* __restrict__ tot) {
   __shared__ double red[3][256];
+  const double* p = pw3 + (size_t)blockIdx.x * 3 * nblocks;
   double s[3] = {0, 0, 0}, c[3] = {0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-    for (int q = 0; q < 3; ++q) {  // Neumaier per thread
-      const double x = pw3[(size_t)b * 3 + q];
+    for (int q = 0; q < 3; ++q) {
+      const double x = p[(size_t)b * 3 + q];
       const double t = s[q] + x;
       c[q] += fabs(s[q]) >= fabs(x) ? (s[q] - t) + x : (x - t) + s[q];
       s[q] = t;
     }
   for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = s[q] + c[q];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double tot[3] = {0, 0, 0};
-    for (int q = 0; q < 3; ++q)
-      for (int i = 0; i < (int)blockDim.x; ++i) tot[q] += red[q][i];
+  if (threadIdx.x < 3) {
+    double a = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) a += red[threadIdx.x][i];
+    tot[(size_t)blockIdx.x * 3 + threadIdx.x] = a;
+  }
+}
+
+// The evanescent_filtered checks of every half step, in order (one thread):
+// slice s has 1 half step if it is the first or the last, else 2.
+__global__ void k_split_warn_all(const double* __restrict__ tot, int slices, double z, wos_warnings* dw) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int sl = 0; sl < slices; ++sl) {
+    const double* t = tot + (size_t)sl * 3;
+    const int halves = (sl == 0 || sl == slices - 1) ? 1 : 2;
     for (int h = 0; h < halves; ++h) {
-      const double pin = tot[h], pout = tot[h + 1];
+      const double pin = t[h], pout = t[h + 1];
       if (pin > 0 && pout < pin * (1 - 1e-8) && z > 0) {
         dw->count[WOS_WARN_EVANESCENT_FILTERED]++;
         dw->value[WOS_WARN_EVANESCENT_FILTERED] = (pin - pout) / pin;
       }
     }
+  }
+}
+
+// H(dz/2) of the split-step column pass, FP64, evaluated once per run by the
+// column kernel's own function and stored in its W-column tile order:
+// element (row p, column c) at (c / W) * n * W + p * W + c % W.
+__global__ void k_h_table(const TFUnit* __restrict__ u, int n, int W, cpx<double>* __restrict__ h) {
+  const size_t nn = (size_t)n * n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+    const int tile = (int)(i / ((size_t)n * W));
+    const int rem = (int)(i % ((size_t)n * W));
+    const int row = rem / W, col = tile * W + rem % W;
+    h[i] = asm_tf_fp64_at(*u, col, row);
   }
 }
 
@@ -1283,6 +1309,24 @@ __global__ void k_zero_warn(wos_warnings* dw) {
 }
 
 static double g_last_device_ms = 0;
+
+// Launch shape of the split-step slices: column tile width and rows per row
+// workgroup. One field of n^2 is small for 256 CUs, so narrower shapes give
+// more workgroups; wos_hip_split_step_set_shape overrides the choice.
+static std::map<long long, Tuning> g_split_shapes;
+
+static Tuning split_step_shape(int prec, int n) {
+  auto it = g_split_shapes.find((long long)n * 2 + prec);
+  if (it != g_split_shapes.end()) return it->second;
+  Tuning t;
+  t.tw = default_col_w(prec, n, n);
+  if (n >= 512 && t.tw > 4) t.tw = 4;  // 1024^2: 4-column tiles with 4 (FP32) or 2 (FP64) rows measured best
+  if (n >= 512) {
+    const int rows = prec == WOS_FP32 ? 4 : 2;
+    if (rows_valid(prec, n, rows)) t.rows = rows;
+  }
+  return t;
+}
 
 // Split-step BPM (optics/medium.go:22-45) on the device for a power-of-two
 // grid: the two ASM half steps between screens are merged into one column
@@ -1302,18 +1346,26 @@ static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, doubl
   if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * ncomp))) return rc;
   if ((rc = dv.scratch.ensure(ncomp * inter_unit_elems(n, n) * csize(prec)))) return rc;
   TransferLaunch L;
-  Tuning tu;
-  tu.tw = default_col_w(prec, n, n);  // the split-step column kernel is built for the default tile
+  Tuning tu = split_step_shape(prec, n);
   if ((rc = build_transfer(L, prec, d, dv.scratch.p, n, ncomp, false, (const TFUnit*)dv.params.p, TF_ASM, 1.0, 0.0,
                            nullptr, nullptr, tu)))
     return rc;
   const int ncolblk = (n / L.tu.tw) * ncomp;
-  if ((rc = dv.hist.ensure(sizeof(double) * 3 * ncolblk + 256))) return rc;
+  // power partials of every slice (read by one warning pass after the loop)
+  const size_t pw_bytes = sizeof(double) * 3 * (size_t)ncolblk * (steps + 1);
+  const size_t tot_bytes = sizeof(double) * 3 * (size_t)(steps + 1);
+  if ((rc = dv.hist.ensure(pw_bytes + tot_bytes + 256))) return rc;
   double* pw3 = (double*)dv.hist.p;
-  wos_warnings* dw = (wos_warnings*)((char*)dv.hist.p + sizeof(double) * 3 * ncolblk);
+  double* tot = (double*)((char*)dv.hist.p + pw_bytes);
+  wos_warnings* dw = (wos_warnings*)((char*)dv.hist.p + pw_bytes + tot_bytes);
   dw = (wos_warnings*)(((uintptr_t)dw + 15) & ~(uintptr_t)15);
   if ((rc = dv.hist.ensure((char*)(dw + 1) - (char*)dv.hist.p))) return rc;
   hipLaunchKernelGGL(k_zero_warn, dim3(1), dim3(64), 0, dv.stream, dw);
+  // H(dz/2) once per run, in the column kernel's tile order
+  if ((rc = dv.ir.ensure(nn * 16))) return rc;
+  hipLaunchKernelGGL(k_h_table, dim3(std::min<unsigned>(grid_for(nn), 4096)), dim3(256), 0, dv.stream,
+                     (const TFUnit*)dv.params.p, n, L.tu.tw, (cpx<double>*)dv.ir.p);
+  WOS_HIP_TRY(hipGetLastError());
   // the screen row pass: tiled intermediate in place, inverse -> screen -> forward
   RowArgs scr = L.inv;
   scr.out = const_cast<void*>(L.inv.in);
@@ -1326,6 +1378,7 @@ static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, doubl
   scr.pw_out = nullptr;
   ColArgs col = L.col;
   col.pw3 = w ? pw3 : nullptr;
+  col.htab = dv.ir.p;
   if (prec == WOS_FP32) {  // the slice kernels use compensated twiddles
     const void* twa;
     if ((rc = get_twiddles(n, prec, &twa, 1))) return rc;
@@ -1340,17 +1393,18 @@ static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, doubl
   if ((rc = launch_rows(prec, n, -1, L.fwd, dv.stream, L.tu.rows))) return rc;
   for (int s = 0; s <= steps; ++s) {
     col.tf_square = (s == 0 || s == steps) ? 0 : 1;
+    col.pw3 = w ? pw3 + (size_t)s * 3 * ncolblk : nullptr;
     if ((rc = launch_cols(prec, n, 3, 0, L.tu.tw, col, dv.stream))) return rc;
-    if (want) {
-      hipLaunchKernelGGL(k_split_warn, dim3(1), dim3(256), 0, dv.stream, (const double*)pw3, ncolblk,
-                         col.tf_square ? 2 : 1, dz / 2, dw);
-      WOS_HIP_TRY(hipGetLastError());
-    }
     if (s == steps) break;
     scr.screen = (const char*)screens + (n_planes == 1 ? 0 : (size_t)s * nn * 16);
     if ((rc = launch_rows(prec, n, 1, scr, dv.stream, L.tu.rows))) return rc;
   }
   if ((rc = launch_rows(prec, n, 1, L.inv, dv.stream, L.tu.rows))) return rc;
+  if (want) {
+    hipLaunchKernelGGL(k_split_totals, dim3(steps + 1), dim3(256), 0, dv.stream, (const double*)pw3, ncolblk, tot);
+    hipLaunchKernelGGL(k_split_warn_all, dim3(1), dim3(64), 0, dv.stream, (const double*)tot, steps + 1, dz / 2, dw);
+    WOS_HIP_TRY(hipGetLastError());
+  }
   WOS_HIP_TRY(hipEventRecord(e1, dv.stream));
   if (want) {
     wos_warnings hw;
@@ -1579,6 +1633,30 @@ int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const 
 // PropagateSplitStep (optics/medium.go:22-45): Strang slices
 // [ASM dz/2, medium phase dz, ASM dz/2], each ASM half step a full Propagate
 // with its own diagnostics, as the reference executes them.
+int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows) {
+  WOS_ENTER();
+  if (!valid_prec(precision) || n <= 0 || !pow2(n)) {
+    set_error("wos_hip_split_step_set_shape: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  const long long key = (long long)n * 2 + precision;
+  if (tile_w == 0) {
+    g_split_shapes.erase(key);
+    return WOS_OK;
+  }
+  const int dw = default_col_w(precision, n, n);
+  if (!(tile_w == dw || (n >= 512 && tile_w >= 2 && tile_w < dw)) || (n % tile_w) != 0 ||
+      !col_w_valid(precision, n, tile_w) || (rows > 0 && !rows_valid(precision, n, rows))) {
+    set_error("wos_hip_split_step_set_shape: column tile or rows not built for the split-step kernels");
+    return WOS_ERR_INVALID;
+  }
+  Tuning t;
+  t.tw = tile_w;
+  t.rows = rows;
+  g_split_shapes[key] = t;
+  return WOS_OK;
+}
+
 int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int steps, const double* index,
                        int n_planes, const wos_ctx* ctx, int precision, wos_warnings* w) {
   WOS_ENTER();

@@ -554,6 +554,7 @@ struct ColArgs {
   int tf_square;       // MODE 3: apply H^2 (two merged propagation steps)
   double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2 (nullable)
   int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
+  const void* htab;    // MODE 3: FP64 H table in W-column tile order (nullable: evaluate H)
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
@@ -695,11 +696,14 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
     const bool want_pw = a.pw3 != nullptr;
     if (active) {
       const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
+      // H of this column from the table (same values as evaluating it here)
+      const cpx<double>* ht = a.htab ? (const cpx<double>*)a.htab + (size_t)(col / W) * N * W + (col % W) : nullptr;
 #pragma unroll
       for (int b = 0; b < V / RL; ++b)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
-          const cpx<double> h = asm_tf_fp64_at(u, col, slot_pos<N, RL>(t, b, r));
+          const int p = slot_pos<N, RL>(t, b, r);
+          const cpx<double> h = ht ? ht[(size_t)p * W] : asm_tf_fp64_at(u, col, p);
           const cpx<double> x = {(double)v[b * RL + r].x, (double)v[b * RL + r].y};
           const cpx<double> y = cmul(x, h);
           cpx<double> out = y;

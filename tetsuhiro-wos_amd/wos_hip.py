@@ -26,7 +26,7 @@ EXPORTS = [
     "wos_hip_device_count", "wos_hip_device_name", "wos_hip_init", "wos_hip_last_error",
     "wos_hip_version", "wos_hip_release_cache", "wos_hip_fft2d_z", "wos_hip_fft1d_z",
     "wos_hip_fft2d_c", "wos_hip_fft1d_c", "wos_hip_propagate", "wos_hip_propagate1d",
-    "wos_hip_split_step", "wos_hip_polychromatic", "wos_hip_volume",
+    "wos_hip_split_step", "wos_hip_split_step_set_shape", "wos_hip_polychromatic", "wos_hip_volume",
     "wos_hip_batch_create", "wos_hip_batch_destroy", "wos_hip_batch_upload",
     "wos_hip_batch_download", "wos_hip_batch_fill_schell", "wos_hip_batch_capture",
     "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
@@ -105,6 +105,7 @@ def load(path=None):
                                     C.POINTER(PropStats)]
     L.wos_hip_propagate1d.argtypes = [P, I, D, D, D, C.POINTER(Ctx), I, C.POINTER(Warnings)]
     L.wos_hip_split_step.argtypes = [P, P, I, D, D, I, P, I, C.POINTER(Ctx), I, C.POINTER(Warnings)]
+    L.wos_hip_split_step_set_shape.argtypes = [I, I, I, I]
     L.wos_hip_polychromatic.argtypes = [P, P, I, D, D, P, P, I, I, C.POINTER(Ctx), I, P,
                                         C.POINTER(Warnings)]
     L.wos_hip_volume.argtypes = [P, P, I, D, P, I, I, C.POINTER(Ctx), I, P, P, C.POINTER(Warnings)]
@@ -242,6 +243,11 @@ def split_step(ex, ey, dx, z, steps, index, ctx, precision=FP64):
     _check(load().wos_hip_split_step(_p(ex), _p(ey), ex.shape[0], dx, z, steps, _p(index), planes,
                                      C.byref(ctx), precision, C.byref(w)), "split_step")
     return ex, ey, w
+
+
+def split_step_set_shape(precision, n, tile_w, rows=0):
+    """Launch shape of the split-step slices (tile_w = 0: built-in choice)."""
+    _check(load().wos_hip_split_step_set_shape(precision, n, tile_w, rows), "split_step_set_shape")
 
 
 def polychromatic(ex, ey, dx, z, wavelengths, weights, method, ctx, precision=FP64):
