@@ -281,6 +281,25 @@ int wos_hip_mask_field(wos_mask m, int unit, double* out);
 /* device pointer of the gradient (units*n*n doubles) */
 void* wos_hip_mask_device_grad(wos_mask m);
 
+/* ---- multi-plane phase design through a split-step medium (no reference
+ * counterpart) ------------------------------------------------------------
+ * exp(i theta) (n x n) goes through `steps` slices of PropagateSplitStep
+ * (optics/medium.go:22-45) over z, with index planes sampled as for
+ * wos_hip_split_step, and the screen of slice s multiplied by exp(i phi_s)
+ * (design phase planes); the loss is the mask operator's. The forward keeps
+ * the field before every screen on the device (steps checkpoints of n*n);
+ * the backward runs the slices in reverse with the conjugate transfer
+ * function and conjugate screens and returns dL/dtheta and dL/dphi_s. */
+typedef struct wos_bpm_mask_s* wos_bpm_mask;
+
+int wos_hip_bpm_mask_create(wos_bpm_mask* out, int n, double dx, double z, int steps, const double* index,
+                            int n_planes, const wos_ctx* ctx, int precision, const double* target);
+int wos_hip_bpm_mask_destroy(wos_bpm_mask m);
+/* theta: n*n; phi: steps*n*n design phases or NULL (none); loss may be NULL */
+int wos_hip_bpm_mask_forward(wos_bpm_mask m, const double* theta, const double* phi, double* loss);
+/* gradients of the last forward's loss: grad_theta n*n, grad_phi steps*n*n (either may be NULL) */
+int wos_hip_bpm_mask_backward(wos_bpm_mask m, double* grad_theta, double* grad_phi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,8 @@ def lib():
         L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
         L.or_mask_loss_grad.argtypes = [P, P, I, D, D, C.POINTER(Ctx), P]
         L.or_mask_loss_grad.restype = D
+        L.or_bpm_mask_loss_grad.argtypes = [P, P, P, I, P, I, D, D, I, C.POINTER(Ctx), P, P]
+        L.or_bpm_mask_loss_grad.restype = D
         _lib = L
     return _lib
 
@@ -283,6 +285,22 @@ def mask_loss_grad(theta, target, dx, z, ctx, want_grad=True):
     g = np.zeros_like(theta) if want_grad else None
     L = lib().or_mask_loss_grad(_p(theta), _p(target), n, dx, z, C.byref(ctx), _p(g))
     return L, g
+
+
+def bpm_mask_loss_grad(theta, phi, index, target, dx, z, steps, ctx, want_grad=True):
+    """Multi-plane phase design through a split-step medium; returns
+    (loss, grad_theta, grad_phi)."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    n = theta.shape[0]
+    index = _c128(index)
+    planes = 1 if index.ndim == 2 else index.shape[0]
+    ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    gt = np.zeros_like(theta) if want_grad else None
+    gp = np.zeros((steps, n, n)) if want_grad else None
+    L = lib().or_bpm_mask_loss_grad(_p(theta), _p(ph), _p(index), planes, _p(target), n, dx, z, steps,
+                                    C.byref(ctx), _p(gt), _p(gp))
+    return L, gt, gp
 
 
 # ---- deterministic generators restated from the reference's tests ----

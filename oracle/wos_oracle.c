@@ -945,3 +945,98 @@ double or_mask_loss_grad(const double* theta, const double* target, int n, doubl
     free(u);
     return L;
 }
+
+/* ------------------------------------------------------------------ */
+/* Multi-plane phase design through a split-step medium (no reference   */
+/* counterpart). Forward: u = exp(i theta); per slice s (PropagateSplit- */
+/* Step, optics/medium.go:22-45): u = A u; checkpoint u; u = (u t_s)     */
+/* exp(i phi_s) with t_s the medium screen (optics/medium.go:49-65);     */
+/* u = A u; A = ASM over dz/2. Loss as or_mask_loss_grad. Backward:      */
+/* lambda = dL/dI . U; per slice in reverse: lambda = A^H lambda;        */
+/* dL/dphi_s = -2 Im(conj(lambda) v), v = (ck_s t_s) exp(i phi_s);       */
+/* lambda = conj(t_s exp(i phi_s)) lambda; lambda = A^H lambda;          */
+/* dL/dtheta = -2 Im(conj(lambda) u0).                                   */
+/* ------------------------------------------------------------------ */
+static void transfer_asm_conj(or_cplx* a, int n, double dx, const asm_tf* t) {
+    or_fft2d(a, n, 0);
+    for (int j = 0; j < n; j++) {
+        double fy = or_freq(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            size_t idx = (size_t)j * n + i;
+            or_cplx h = asm_tf_eval(t, or_freq(i, n, dx), fy);
+            h.im = -h.im;
+            a[idx] = cmul(a[idx], h);
+        }
+    }
+    or_fft2d(a, n, 1);
+}
+
+static or_cplx medium_screen(or_cplx nv, double k, double dz) {
+    double ph = k * (nv.re - 1) * dz;
+    return cmul(cre(exp(-k * nv.im * dz)), cexpI(ph));
+}
+
+double or_bpm_mask_loss_grad(const double* theta, const double* phi, const or_cplx* index, int n_planes,
+                             const double* target, int n, double dx, double z, int steps, const or_ctx* ctx,
+                             double* grad_theta, double* grad_phi) {
+    size_t nn = (size_t)n * n;
+    double dz = z / (double)steps;
+    double k = 2.0 * M_PI / ctx->wavelength;
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, dz / 2, &zero, &reg, &alpha, &limit);
+    asm_tf t = make_asm_tf(ctx->wavelength, dz / 2, zero, reg, alpha, limit);
+    or_cplx* u = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    or_cplx* ck = (or_cplx*)malloc(sizeof(or_cplx) * nn * (size_t)steps);
+    for (size_t i = 0; i < nn; i++) u[i] = cexpI(theta[i]);
+    for (int s = 0; s < steps; s++) {
+        const or_cplx* ix = index + (n_planes == 1 ? 0 : (size_t)s * nn);
+        transfer_asm(u, n, dx, &t);
+        memcpy(ck + (size_t)s * nn, u, sizeof(or_cplx) * nn);
+        for (size_t i = 0; i < nn; i++) {
+            or_cplx v = cmul(u[i], medium_screen(ix[i], k, dz));
+            u[i] = phi ? cmul(v, cexpI(phi[(size_t)s * nn + i])) : v;
+        }
+        transfer_asm(u, n, dx, &t);
+    }
+    double S = 0;
+    for (size_t i = 0; i < nn; i++) S += u[i].re * u[i].re + u[i].im * u[i].im;
+    double L = 0, gsum = 0;
+    for (size_t i = 0; i < nn; i++) {
+        double I = u[i].re * u[i].re + u[i].im * u[i].im;
+        double d = I / S - target[i];
+        L += d * d;
+        gsum += 2 * d * (I / S);
+    }
+    if (grad_theta || grad_phi) {
+        for (size_t i = 0; i < nn; i++) {
+            double I = u[i].re * u[i].re + u[i].im * u[i].im;
+            double wq = (2 * (I / S - target[i]) - gsum) / S;
+            u[i].re *= wq;
+            u[i].im *= wq;
+        }
+        for (int s = steps - 1; s >= 0; s--) {
+            const or_cplx* ix = index + (n_planes == 1 ? 0 : (size_t)s * nn);
+            transfer_asm_conj(u, n, dx, &t);
+            for (size_t i = 0; i < nn; i++) {
+                or_cplx ts = medium_screen(ix[i], k, dz);
+                or_cplx e = phi ? cexpI(phi[(size_t)s * nn + i]) : cre(1.0);
+                or_cplx v = cmul(cmul(ck[(size_t)s * nn + i], ts), e);
+                if (grad_phi) grad_phi[(size_t)s * nn + i] = -2 * (u[i].re * v.im - u[i].im * v.re);
+                or_cplx m = cmul(ts, e);
+                m.im = -m.im;
+                u[i] = cmul(u[i], m);
+            }
+            transfer_asm_conj(u, n, dx, &t);
+        }
+        if (grad_theta)
+            for (size_t i = 0; i < nn; i++) {
+                or_cplx u0 = cexpI(theta[i]);
+                grad_theta[i] = -2 * (u[i].re * u0.im - u[i].im * u0.re);
+            }
+    }
+    free(u);
+    free(ck);
+    return L;
+}
+

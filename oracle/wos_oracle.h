@@ -132,6 +132,16 @@ void or_compute_stats(const or_cplx* ex, const or_cplx* ey, int n, double dx, do
 double or_mask_loss_grad(const double* theta, const double* target, int n, double dx,
                          double z, const or_ctx* ctx, double* grad);
 
+/* Multi-plane phase design through a split-step medium (no reference
+ * counterpart): exp(i theta) through `steps` slices of PropagateSplitStep
+ * whose screen s is multiplied by exp(i phi_s) (phi may be NULL: no design
+ * phases); loss as or_mask_loss_grad; gradients by reversed slices with
+ * conjugate operators and the checkpointed fields before each screen.
+ * grad_theta (n*n) and grad_phi (steps*n*n) may be NULL. */
+double or_bpm_mask_loss_grad(const double* theta, const double* phi, const or_cplx* index, int n_planes,
+                             const double* target, int n, double dx, double z, int steps, const or_ctx* ctx,
+                             double* grad_theta, double* grad_phi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,3 +111,90 @@ def test_config5_mask_design_fd_and_cpu_adjoint(wos, oracle):
     assert el <= 1e-5 and eg <= 1e-4
     op32.close()
     op.close()
+
+
+# ---------------------------------------------------------------- multi-plane phase design through BPM
+
+def bpm_case(n, dx, steps, seed):
+    rng = np.random.default_rng(seed)
+    theta = rng.uniform(0, 2 * math.pi, (n, n))
+    phi = rng.uniform(0, 2 * math.pi, (steps, n, n))
+    x = (np.arange(n) - n / 2) * dx
+    r = np.hypot(x[None, :], x[:, None])
+    index = np.array([np.where(r < n * dx * (0.1 + 0.05 * s), 1.5, 1.3) + 1e-4j for s in range(steps)])
+    target = np.exp(-((x[None, :] - 4 * dx) ** 2 + x[:, None] ** 2) / (2 * (3 * dx) ** 2))
+    return theta, phi, index, target / target.sum()
+
+
+@pytest.mark.parametrize("n", [64, 100])
+def test_bpm_mask_operator_matches_cpu_adjoint(wos, oracle, n):
+    """Loss and gradients (input phase and every design phase plane) vs the
+    oracle's FP64 BPM adjoint; n = 100 runs the Bluestein path. FP32 error
+    against FP64 reported."""
+    dx, wl, steps, z = 1e-6, 1.0e-6, 4, 80e-6
+    theta, phi, index, target = bpm_case(n, dx, steps, 2)
+    L, gt, gp = oracle.bpm_mask_loss_grad(theta, phi, index, target, dx, z, steps, oracle.ctx_for(wl))
+    op = wos.BpmMaskOp(n, dx, z, steps, index, wos.ctx_for(wl), target)
+    lg = op.forward(theta, phi)
+    g_t, g_p = op.backward()
+    et, ep = rel_l2(g_t, gt), rel_l2(g_p, gp)
+    print(f"bpm mask n={n}: loss {lg:.9e} vs {L:.9e}, grad theta {et:.2e}, grad phi {ep:.2e}", flush=True)
+    assert abs(lg - L) <= 1e-12 * L and et <= 1e-10 and ep <= 1e-10
+    op32 = wos.BpmMaskOp(n, dx, z, steps, index, wos.ctx_for(wl), target, precision=wos.FP32)
+    l32 = op32.forward(theta, phi)
+    t32, p32 = op32.backward()
+    print(f"bpm mask n={n} FP32: loss {abs(l32 - L) / L:.2e}, grad theta {rel_l2(t32, gt):.2e}, "
+          f"grad phi {rel_l2(p32, gp):.2e}", flush=True)
+    assert abs(l32 - L) <= 1e-5 * L and rel_l2(t32, gt) <= 1e-4 and rel_l2(p32, gp) <= 1e-4
+    # no design phases: the forward is PropagateSplitStep of exp(i theta)
+    l0 = op.forward(theta)
+    out, _, _ = oracle.split_step(np.exp(1j * theta), None, dx, z, steps, index, oracle.ctx_for(wl))
+    I = np.abs(out) ** 2
+    assert abs(l0 - np.sum((I / I.sum() - target) ** 2)) <= 1e-12 * l0
+    op.close()
+    op32.close()
+
+
+def test_bpm_mask_fd_1024(wos, oracle):
+    """1024^2 (config-3 grid, 20 slices of 5 um through a graded index) with
+    design phase planes: GPU FP64 gradients vs central differences of the GPU
+    forward on seeded pixels of theta and of three phase planes, and vs the
+    oracle's adjoint."""
+    n, dx, wl, steps = 1024, 0.5e-6, 1.55e-6, 20
+    z = steps * 5e-6
+    rng = np.random.default_rng(31)
+    theta = rng.uniform(0, 0.5, (n, n))
+    phi = rng.uniform(0, 0.5, (steps, n, n))
+    x = (np.arange(n) - n / 2) * dx
+    r = np.hypot(x[None, :], x[:, None])
+    index = np.where(r < 100e-6, 1.45 - 0.01 * (r / 100e-6) ** 2, 1.44) + 1e-5j
+    target = np.exp(-((x[None, :] - 8e-6) ** 2 + x[:, None] ** 2) / (2 * (4e-6) ** 2))
+    target /= target.sum()
+    op = wos.BpmMaskOp(n, dx, z, steps, index, wos.ctx_for(wl), target)
+    L0 = op.forward(theta, phi)
+    gt, gp = op.backward()
+    Lc, ct, cp = oracle.bpm_mask_loss_grad(theta, phi, index, target, dx, z, steps, oracle.ctx_for(wl))
+    print(f"bpm mask 1024: loss {L0:.9e} vs CPU {Lc:.9e}; grad theta {rel_l2(gt, ct):.2e}, grad phi {rel_l2(gp, cp):.2e}",
+          flush=True)
+    assert abs(L0 - Lc) <= 1e-12 * Lc and rel_l2(gt, ct) <= 1e-10 and rel_l2(gp, cp) <= 1e-10
+    h = 1e-3
+    gmax = max(np.max(np.abs(gt)), np.max(np.abs(gp)))
+    worst = 0.0
+    # pixels near the beam, where the gradient is not negligible
+    picks = [("t", None, 512, 520), ("t", None, 500, 512), ("p", 0, 515, 505), ("p", 9, 512, 530), ("p", 19, 520, 512)]
+    for kind, s, j, i in picks:
+        tp, tm, pp, pm = theta.copy(), theta.copy(), phi.copy(), phi.copy()
+        if kind == "t":
+            tp[j, i] += h
+            tm[j, i] -= h
+            g = gt[j, i]
+        else:
+            pp[s, j, i] += h
+            pm[s, j, i] -= h
+            g = gp[s, j, i]
+        fd = (op.forward(tp, pp) - op.forward(tm, pm)) / (2 * h)
+        err = abs(fd - g) / max(abs(fd), 1e-3 * gmax)
+        worst = max(worst, err)
+        assert err <= 1e-4, (kind, s, j, i, fd, g)
+    print(f"bpm mask 1024 FD: worst relative error {worst:.2e}", flush=True)
+    op.close()

@@ -377,3 +377,47 @@ def test_compute_stats_closed_forms(oracle):
     s2 = oracle.compute_stats(ex, 0.5 * ex, dx, wl)
     assert abs(s2.power - 1.25 * P) <= 1e-12 * P and s2.strehl == 0.0
     assert abs(s2.centroid_x - x0) <= 1e-15
+
+
+def test_bpm_mask_adjoint_matches_central_differences(oracle):
+    """Multi-plane phase design through a split-step medium: gradients with
+    respect to the input phase and to each design phase plane vs central
+    differences of the FP64 forward; with no design phases the forward is
+    PropagateSplitStep of exp(i theta)."""
+    n, dx, wl, steps, z = 32, 1e-6, 1.0e-6, 3, 60e-6
+    rng = np.random.default_rng(9)
+    theta = rng.uniform(0, 2 * math.pi, (n, n))
+    phi = rng.uniform(0, 2 * math.pi, (steps, n, n))
+    x = (np.arange(n) - n / 2) * dx
+    r = np.hypot(x[None, :], x[:, None])
+    index = np.array([np.where(r < (6 + 2 * s) * 1e-6, 1.5, 1.2) + 1e-4j for s in range(steps)])
+    T = np.exp(-((x[None, :] - 3e-6) ** 2 + x[:, None] ** 2) / (2 * (3e-6) ** 2))
+    T /= T.sum()
+    c = ctx(oracle, wl)
+    L0, gt, gp = oracle.bpm_mask_loss_grad(theta, phi, index, T, dx, z, steps, c)
+    h = 1e-5
+
+    def fd(mod):
+        tp, pp = theta.copy(), phi.copy()
+        tm, pm = theta.copy(), phi.copy()
+        mod(tp, pp, h)
+        mod(tm, pm, -h)
+        return (oracle.bpm_mask_loss_grad(tp, pp, index, T, dx, z, steps, c, False)[0] -
+                oracle.bpm_mask_loss_grad(tm, pm, index, T, dx, z, steps, c, False)[0]) / (2 * h)
+
+    gmax = max(np.max(np.abs(gt)), np.max(np.abs(gp)))
+    for (j, i) in [(3, 5), (16, 17), (30, 2)]:
+        def mt(t, p, d, j=j, i=i):
+            t[j, i] += d
+        f = fd(mt)
+        assert abs(f - gt[j, i]) <= 1e-4 * max(abs(f), 1e-3 * gmax)
+    for (s, j, i) in [(0, 4, 9), (1, 15, 16), (2, 20, 11)]:
+        def mp(t, p, d, s=s, j=j, i=i):
+            p[s, j, i] += d
+        f = fd(mp)
+        assert abs(f - gp[s, j, i]) <= 1e-4 * max(abs(f), 1e-3 * gmax)
+    # no design phases: the forward field is PropagateSplitStep's
+    L1, _, _ = oracle.bpm_mask_loss_grad(theta, None, index, T, dx, z, steps, c, False)
+    out, _, _ = oracle.split_step(np.exp(1j * theta), None, dx, z, steps, index, c)
+    I = np.abs(out) ** 2
+    assert abs(L1 - np.sum((I / I.sum() - T) ** 2)) <= 1e-12 * L1

@@ -38,6 +38,7 @@ EXPORTS = [
     "wos_hip_pipeline_autotune", "wos_hip_pipeline_capture", "wos_hip_pipeline_run", "wos_hip_pipeline_sync",
     "wos_hip_pipeline_download", "wos_hip_pipeline_download_source", "wos_hip_pipeline_intensity",
     "wos_hip_pipeline_time", "wos_hip_compute_stats", "wos_hip_batch_stats", "wos_hip_pipeline_stats",
+    "wos_hip_bpm_mask_create", "wos_hip_bpm_mask_destroy", "wos_hip_bpm_mask_forward", "wos_hip_bpm_mask_backward",
 ]
 
 
@@ -146,6 +147,10 @@ def load(path=None):
     L.wos_hip_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
     L.wos_hip_batch_stats.argtypes = [P, I, D, D, D, C.POINTER(PlaneStats)]
     L.wos_hip_pipeline_stats.argtypes = [P, I, I, D, D, D, C.POINTER(PlaneStats)]
+    L.wos_hip_bpm_mask_create.argtypes = [C.POINTER(P), I, D, D, I, P, I, C.POINTER(Ctx), I, P]
+    L.wos_hip_bpm_mask_destroy.argtypes = [P]
+    L.wos_hip_bpm_mask_forward.argtypes = [P, P, P, P]
+    L.wos_hip_bpm_mask_backward.argtypes = [P, P, P]
     _lib = L
     return L
 
@@ -484,4 +489,42 @@ class Pipeline:
         _check(load().wos_hip_pipeline_time(self.h, warmup, iters, C.byref(ms), _p(st) if stages else None),
                "pipeline_time")
         return ms.value, (st if stages else None)
+
+
+class BpmMaskOp:
+    """Multi-plane phase design through a split-step medium (wos_bpm_mask)."""
+
+    def __init__(self, n, dx, z, steps, index, ctx, target, precision=FP64):
+        self.n, self.steps = n, steps
+        ix = _c128(index)
+        planes = 1 if ix.ndim == 2 else ix.shape[0]
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        h = C.c_void_p()
+        _check(load().wos_hip_bpm_mask_create(C.byref(h), n, dx, z, steps, _p(ix), planes, C.byref(ctx), precision,
+                                              _p(t)), "bpm_mask_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(load().wos_hip_bpm_mask_destroy(self.h), "bpm_mask_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, theta, phi=None):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        loss = C.c_double()
+        _check(load().wos_hip_bpm_mask_forward(self.h, _p(th), _p(ph), C.byref(loss)), "bpm_mask_forward")
+        return loss.value
+
+    def backward(self):
+        gt = np.zeros((self.n, self.n))
+        gp = np.zeros((self.steps, self.n, self.n))
+        _check(load().wos_hip_bpm_mask_backward(self.h, _p(gt), _p(gp)), "bpm_mask_backward")
+        return gt, gp
 
