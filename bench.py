@@ -34,13 +34,13 @@ VALU_WAVE_INSTR_PER_S = 614.4e9
 # HBM bytes per full-batch launch (64 x 2048^2), rocprofv3 PMC: 2 x FETCH_SIZE + WRITE_SIZE
 # (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md
 PMC_HBM_BYTES = {
-    "fp32": {"row_fft_fwd": (1048756 * 2 + 2097152) * 1024, "col_fft_tf_ifft": (1049539 * 2 + 2097152) * 1024,
-             "row_fft_inv": (1048794 * 2 + 2097152) * 1024},
-    "fp64": {"row_fft_fwd": (2097621 * 2 + 4194305) * 1024, "col_fft_tf_ifft": (2115560 * 2 + 4194306) * 1024,
-             "row_fft_inv": (2097595 * 2 + 4194304) * 1024},
+    "fp32": {"row_fft_fwd": (1048758 * 2 + 2097152) * 1024, "col_fft_tf_ifft": (1049123 * 2 + 2097152) * 1024,
+             "row_fft_inv": (1048762 * 2 + 2097152) * 1024},
+    "fp64": {"row_fft_fwd": (2104635 * 2 + 4194305) * 1024, "col_fft_tf_ifft": (2119539 * 2 + 4194310) * 1024,
+             "row_fft_inv": (2102526 * 2 + 4194304) * 1024},
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
-PMC_VALU = {"fp32": {"row_fft_fwd": 821 * 262144, "col_fft_tf_ifft": 1992 * 262144, "row_fft_inv": 871 * 262144}}
+PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1774 * 262144, "row_fft_inv": 872 * 262144}}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
 
@@ -49,13 +49,17 @@ def rel_l2(a, b):
     return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / den))
 
 
-def run_gpu(wos, precision, steps, warmup):
+def run_gpu(wos, precision, steps, warmup, shape=None):
     wls = np.linspace(400e-9, 700e-9, UNITS)
     zs = np.full(UNITS, Z)
     b = wos.Batch(N, DX, wls, zs, wos.ctx_for(550e-9), "asm", precision)
     b.fill_schell(W0, SIGMA_C, SEED)
     inp0 = b.download(0)
-    tune = b.autotune(5)
+    if shape:   # a fixed launch shape: tile_w, col_w, rows, row groups, split
+        b.set_tuning(*shape)
+        tune = (dict(zip(("tile_w", "col_w", "rows", "row_groups", "split_lds"), shape)), b.time(1, 5)[0])
+    else:
+        tune = b.autotune(5)
     b.fill_schell(W0, SIGMA_C, SEED)   # the tuning launches ran on the data
     b.capture()
     b.run(1)
@@ -199,7 +203,12 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline batch (no other workloads)")
+    ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split")
+    ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split")
     args = ap.parse_args()
+    shape32 = [int(x) for x in args.shape32.split(",")] if args.shape32 else None
+    shape64 = [int(x) for x in args.shape64.split(",")] if args.shape64 else None
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the units of a batch are independent; several GPUs would each take a share of them)", file=sys.stderr)
@@ -207,8 +216,8 @@ def main():
 
     import wos_hip as wos
     wos.init()
-    r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup)
-    r64 = run_gpu(wos, wos.FP64, args.steps, args.warmup)
+    r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup, shape32)
+    r64 = run_gpu(wos, wos.FP64, args.steps, args.warmup, shape64)
 
     props32 = UNITS / (r32["ms_per_step"] * 1e-3)
     props64 = UNITS / (r64["ms_per_step"] * 1e-3)
@@ -262,6 +271,9 @@ def main():
                      "step_algorithmic_GBs": round(step_gbs, 1),
                      "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }
+    if args.no_extra:
+        print(json.dumps(line), flush=True)
+        return
     line["pcie_inclusive"] = {
         "unit": "props/s", "workload": "one 2048^2 asm Propagate per call, complex128 host buffers in and out",
         "fp64": round(pcie_inclusive(wos, r64["unit0_in"], r64["wl0"], wos.FP64), 1),
