@@ -1,0 +1,78 @@
+"""Edge cases of the C ABI on the GPU: the smallest and largest grids, the
+largest Bluestein length, z = 0, and the error returns for invalid and
+unsupported arguments (nothing is computed for them)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def test_one_point_transforms_are_identity(wos):
+    a = np.array([[3.0 - 2.0j]])
+    for inv in (False, True):
+        assert np.array_equal(wos.fft2d(a, inverse=inv), a)
+        assert np.array_equal(wos.fft1d(a.ravel(), inverse=inv), a.ravel())
+
+
+def test_largest_power_of_two_grid(wos, oracle):
+    """8192^2 complex128 (1 GiB): forward and inverse against the oracle."""
+    n = 8192
+    rng = np.random.default_rng(8)
+    a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 0.5
+    g = wos.fft2d(a)
+    r = oracle.fft2d(a)
+    e = rel_l2(g, r)
+    print(f"8192^2 fft2d FP64 rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-12
+    back = wos.fft2d(g, inverse=True)
+    assert rel_l2(back, a) <= 1e-12
+
+
+def test_largest_bluestein_length(wos):
+    """L = 4095 (internal 8192-point transforms) in 1-D against numpy."""
+    n = 4095
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel_l2(wos.fft1d(a), np.fft.fft(a)) <= 1e-12
+    assert rel_l2(wos.fft1d(a, inverse=True), np.fft.ifft(a)) <= 1e-12
+
+
+def test_zero_distance_is_a_no_op(wos):
+    rng = np.random.default_rng(5)
+    ex = rng.standard_normal((64, 64)) + 1j * rng.standard_normal((64, 64))
+    for m in ("asm", "asm_pad", "fresnel_tf", "fresnel_ir"):
+        out, _, w, _ = wos.propagate(ex, None, 1e-6, 0.0, m, wos.ctx_for(633e-9))
+        assert np.array_equal(out, ex) and not w.as_dict()
+
+
+def test_invalid_and_unsupported_arguments(wos):
+    lib = wos.load()
+    a = np.zeros(16, dtype=np.complex128)
+    ctx = wos.ctx_for(633e-9)
+    # invalid: null data, non-positive sizes, bad method, bad precision, bad dx
+    assert lib.wos_hip_fft2d_z(None, 4, 0) == 2
+    assert lib.wos_hip_fft2d_z(a.ctypes.data, 0, 0) == 2
+    assert lib.wos_hip_fft1d_z(a.ctypes.data, -3, 0) == 2
+    b = np.ones((4, 4), dtype=np.complex128)
+    assert lib.wos_hip_propagate(b.ctypes.data, None, 4, 1e-6, 1e-3, 99, C.byref(ctx), 0, None, None) in (2, 4)
+    assert lib.wos_hip_propagate(b.ctypes.data, None, 4, 1e-6, 1e-3, 0, C.byref(ctx), 7, None, None) == 2
+    assert lib.wos_hip_propagate(b.ctypes.data, None, 4, 0.0, 1e-3, 0, C.byref(ctx), 0, None, None) == 2
+    assert np.array_equal(b, np.ones((4, 4)))                    # nothing written
+    # unsupported: beyond the largest transform
+    big = np.zeros(1, dtype=np.complex128)
+    assert lib.wos_hip_fft2d_z(big.ctypes.data, 16384, 0) == 4
+    assert lib.wos_hip_fft1d_z(big.ctypes.data, 8193, 0) == 4
+    assert b"not supported" in lib.wos_hip_last_error()
+    # a handle-based call with a bad shape reports instead of launching
+    bt = wos.Batch(256, 1e-6, [633e-9], [1e-3], ctx, "asm", wos.FP32)
+    with pytest.raises(wos.WosError):
+        bt.set_tuning(3, 3, 1, 0)
+    with pytest.raises(wos.WosError):
+        bt.set_tuning(4, 4, 3, 0)
+    bt.close()

@@ -1519,6 +1519,11 @@ static int fft2d_host(int prec, void* a, int n, int inverse) {
     set_error("fft2d: bad arguments");
     return WOS_ERR_INVALID;
   }
+  if (n == 1) return WOS_OK;  // the 1-point transform is the identity in both directions
+  if (!len_ok(n)) {
+    set_error("2-D FFT size not supported: " + std::to_string(n));
+    return WOS_ERR_UNSUPPORTED;
+  }
   Device& dv = device();
   const size_t nn = (size_t)n * n;
   const size_t bytes = nn * csize(prec);
