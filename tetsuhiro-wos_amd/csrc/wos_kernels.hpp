@@ -364,6 +364,27 @@ constexpr int rows_per_block() {
   return Plan<N>::T >= 256 ? 1 : 256 / Plan<N>::T;
 }
 
+// Streaming accesses of the FFT passes: every field element is read once and
+// written once per pass, so loads and stores are marked nontemporal (the
+// stores gain in every kernel; nontemporal loads slow the pipelined row
+// kernel's prefetch, which uses gld<false>).
+template <bool NT = true, typename R>
+__device__ __forceinline__ cpx<R> gld(const cpx<R>* p) {
+  if constexpr (NT) {
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    const v2 x = __builtin_nontemporal_load((const v2*)p);
+    return {x.x, x.y};
+  } else {
+    return *p;
+  }
+}
+template <typename R>
+__device__ __forceinline__ void gst(cpx<R>* p, cpx<R> v) {
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  const v2 x = {v.x, v.y};
+  __builtin_nontemporal_store(x, (v2*)p);
+}
+
 __device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int tstride) {
   if (lg < 0) return row * ld + q;
   return (q >> lg) * tstride + (row << lg) + (q & ((1 << lg) - 1));
@@ -408,7 +429,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
       for (int r = 0; r < R0; ++r)
-        v[b * R0 + r] = in[tiled_offset(slot_pos<N, R0>(t, b, r), row, a.in_ld, a.in_tw, a.in_tstride)];
+        v[b * R0 + r] = gld(in + tiled_offset(slot_pos<N, R0>(t, b, r), row, a.in_ld, a.in_tw, a.in_tstride));
   } else {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
@@ -458,7 +479,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r)
-        out[tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride)] = v[b * RL + r];
+        gst(out + tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
   } else {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
@@ -510,7 +531,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
   for (int b = 0; b < V / R0; ++b)
 #pragma unroll
     for (int r = 0; r < R0; ++r)
-      v[b * R0 + r] = in[tiled_offset(slot_pos<N, R0>(t, b, r), row0, a.in_ld, a.in_tw, a.in_tstride)];
+      v[b * R0 + r] = gld<false>(in + tiled_offset(slot_pos<N, R0>(t, b, r), row0, a.in_ld, a.in_tw, a.in_tstride));
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int row = row0 + g * ROWS;
@@ -519,7 +540,8 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
       for (int b = 0; b < V / R0; ++b)
 #pragma unroll
         for (int r = 0; r < R0; ++r)
-          nx[b * R0 + r] = in[tiled_offset(slot_pos<N, R0>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride)];
+          nx[b * R0 + r] =
+              gld<false>(in + tiled_offset(slot_pos<N, R0>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride));
     }
     fft_block_reg<R, N, DIR, 1>(v, t, tr, lds + lr * LS);
     if (scale) {
@@ -530,7 +552,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r)
-        out[tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride)] = v[b * RL + r];
+        gst(out + tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
     if (g + 1 < G) {
 #pragma unroll
       for (int i = 0; i < V; ++i) v[i] = nx[i];
@@ -602,12 +624,12 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = base[slot_pos<N, R0>(t, b, r) * tl];
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0>(t, b, r) * tl);
   } else if (fastw) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = base[slot_pos<N, R0>(t, b, r) * W];
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0>(t, b, r) * W);
   } else {
     const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
@@ -765,12 +787,12 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) base[slot_pos<N, RL>(t, b, r) * W] = v[b * RL + r];
+      for (int r = 0; r < RL; ++r) gst(base + slot_pos<N, RL>(t, b, r) * W, v[b * RL + r]);
   } else if (fast) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) base[slot_pos<N, RL>(t, b, r) * tl] = v[b * RL + r];
+      for (int r = 0; r < RL; ++r) gst(base + slot_pos<N, RL>(t, b, r) * tl, v[b * RL + r]);
   } else {
     const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
