@@ -102,6 +102,23 @@ def pcie_inclusive(wos, inp, wl, precision, reps=5):
     return reps / (time.perf_counter() - t0)
 
 
+def fft2d_dropin_ms(wos, inp, reps=10):
+    """Drop-in gpuFFT2D stand-in: one 2048^2 complex128 2-D FFT per call on a
+    host buffer (upload, transform, download), alternating forward and
+    inverse so the values stay bounded."""
+    import ctypes
+    lib = wos.load()
+    a = np.array(inp, dtype=np.complex128, order="C")
+    for d in (0, 1):
+        if lib.wos_hip_fft2d_z(a.ctypes.data, N, d):
+            raise RuntimeError(lib.wos_hip_last_error().decode())
+    t0 = time.perf_counter()
+    for i in range(reps):
+        if lib.wos_hip_fft2d_z(a.ctypes.data, N, i & 1):
+            raise RuntimeError(lib.wos_hip_last_error().decode())
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
 def bench_bpm(wos):
     """Split-step BPM: 1024^2, dx 0.5 um, 1.55 um, 400 slices of 5 um through
     the graded-index medium, Gaussian w0 10 um; device time per slice."""
@@ -277,7 +294,10 @@ def main():
     line["pcie_inclusive"] = {
         "unit": "props/s", "workload": "one 2048^2 asm Propagate per call, complex128 host buffers in and out",
         "fp64": round(pcie_inclusive(wos, r64["unit0_in"], r64["wl0"], wos.FP64), 1),
-        "fp32": round(pcie_inclusive(wos, r64["unit0_in"], r64["wl0"], wos.FP32), 1)}
+        "fp32": round(pcie_inclusive(wos, r64["unit0_in"], r64["wl0"], wos.FP32), 1),
+        "fft2d_z_2048_ms": round(fft2d_dropin_ms(wos, r64["unit0_in"]), 3),
+        "fft2d_note": "one complex128 2-D FFT of a host buffer per call, transfers included; the reference "
+                      "publishes 45 ms for this call through cuFFT on a Tesla T4 (README.md:113)"}
     line["bpm_1024_400_slices"] = bench_bpm(wos)
     line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps)
     line["mask_adjoint"] = bench_adjoint(wos)

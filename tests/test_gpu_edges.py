@@ -34,6 +34,24 @@ def test_largest_power_of_two_grid(wos, oracle):
     assert rel_l2(back, a) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [1000, 3000])
+def test_host_transfers_round_trip(wos, n):
+    """Host <-> device copies go through a ring of eight 8 MiB pinned chunks:
+    n = 1000 (16 MB, one full chunk and a partial one) and n = 3000 (144 MB,
+    18 chunks, every slot reused) come back bit-exact in FP64, and equal to
+    the float32 rounding in FP32."""
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    for prec in (wos.FP64, wos.FP32):
+        b = wos.Batch(n, 1e-6, [633e-9, 532e-9], [1e-3, 1e-3], wos.ctx_for(633e-9), "asm", prec)
+        b.upload(1, a)
+        b.upload(0, -a)
+        got1, got0 = b.download(1), b.download(0)
+        want = a if prec == wos.FP64 else a.astype(np.complex64).astype(np.complex128)
+        assert np.array_equal(got1, want) and np.array_equal(got0, -want), (n, prec)
+        b.close()
+
+
 def test_largest_bluestein_length(wos):
     """L = 4095 (internal 8192-point transforms) in 1-D against numpy."""
     n = 4095

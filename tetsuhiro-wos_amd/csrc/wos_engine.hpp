@@ -63,6 +63,15 @@ struct Device {
 Device& device();
 int ensure_init();
 
+// Transfers between pageable host memory (a Go slice, a numpy array) and the
+// device, in the order of stream s. They go through a ring of pinned staging
+// chunks: host threads copy a chunk into (or out of) the ring while the DMA
+// engine moves the previous one, instead of the runtime's own pageable
+// staging. host_to_dev returns once the source has been read (the DMA may
+// still be in flight); dev_to_host returns when the data is in dst.
+int host_to_dev(void* dst, const void* src, size_t bytes, hipStream_t s);
+int dev_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // Stage twiddle tables of an n-point transform in precision prec (0 = FP64);
 // acc = 1: compensated [hi, lo] pairs.
 int get_twiddles(int n, int prec, const void** out, int acc = 0);
