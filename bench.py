@@ -136,13 +136,19 @@ def bench_bpm(wos):
         out[name] = {"steps_per_s": round(steps / (ms * 1e-3), 1), "device_ms": round(ms, 3)}
     out["rel_l2_fp32_vs_fp64"] = rel_l2(res["fp32"], res["fp64"])
     out["power_ratio_fp64"] = float(np.sum(np.abs(res["fp64"]) ** 2) / np.sum(np.abs(ex) ** 2))
-    return out
+    # a Strang step is two ASM half steps: 2 x 8 n^2 S algorithmic bytes
+    for name, S in (("fp64", 16), ("fp32", 8)):
+        out[name]["hbm_frac"] = round(out[name]["steps_per_s"] * 2 * 8 * n * n * S / (HBM_PEAK_GBS * 1e9), 4)
+    check20 = {}
+    for name, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
+        check20[name], _, _ = wos.split_step(ex, None, dx, 20 * 5e-6, 20, index, wos.ctx_for(wl), precision=prec)
+    return out, check20
 
 
 def bench_pipeline(wos, steps):
     """One hipGraph: 64 x 2048^2 polychromatic intensity, 128 z-planes at
     1024^2, 32 mixed fields (384^2, 1000^2, 1536^2); Gaussian-Schell inputs."""
-    out = {}
+    out, res = {}, {}
     for name, prec in (("fp32", wos.FP32), ("fp64", wos.FP64)):
         p = wos.Pipeline(prec)
         ctx = wos.ctx_for(550e-9)
@@ -158,9 +164,17 @@ def bench_pipeline(wos, steps):
         p.autotune(3)
         p.capture()
         ms, st = p.time(2, max(steps, 10), stages=True)
+        p.run(1)
+        p.sync()
+        res[name] = (p.intensity(0), p.download(3, 10))
+        S = 8 if prec == wos.FP32 else 16
+        B = 8 * S * (64 * 2048 ** 2 + 128 * 1024 ** 2 + 11 * 384 ** 2 + 11 * 1000 ** 2 + 10 * 1536 ** 2)
         out[name] = {"ms_per_graph_run": round(ms, 4), "props_per_s": round(224 / (ms * 1e-3), 1),
-                     "stage_ms": [round(x, 4) for x in st]}
+                     "stage_ms": [round(x, 4) for x in st],
+                     "hbm_frac": round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
         p.close()
+    out["rel_l2_fp32_vs_fp64_intensity"] = rel_l2(res["fp32"][0], res["fp64"][0])
+    out["rel_l2_fp32_vs_fp64_1000sq_field"] = rel_l2(res["fp32"][1], res["fp64"][1])
     return out
 
 
@@ -175,10 +189,11 @@ def bench_adjoint(wos):
     t /= t.sum()
     theta = np.random.default_rng(1).uniform(0, 2 * np.pi, (units, n, n)).astype(np.float32).astype(np.float64)
     out = {}
+    keep = {"theta0": theta[0], "grad0": {}, "loss0": {}}
     for name, prec in (("fp32", wos.FP32), ("fp64", wos.FP64)):
         op = wos.MaskOp(n, units, dx, z, wos.ctx_for(wl), t, precision=prec)
-        op.forward(theta)
-        op.backward()
+        keep["loss0"][name] = float(op.forward(theta)[0])
+        keep["grad0"][name] = op.backward()[0]
         reps = 5
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -187,6 +202,151 @@ def bench_adjoint(wos):
         el = (time.perf_counter() - t0) / reps
         out[name] = {"ms_forward_backward_8_masks": round(el * 1e3, 2), "note": "host theta in, host gradient out"}
         op.close()
+    out["grad_rel_l2_fp32_vs_fp64"] = rel_l2(keep["grad0"]["fp32"], keep["grad0"]["fp64"])
+    return out, keep
+
+
+def config12_inputs():
+    """Inputs of configurations 1 and 2 in closed form.
+    1: 512^2, dx 2 um, 632.8 nm: unit-amplitude Gaussian (w0 200 um) through
+       a thin lens f = 50 mm, phase (-k/(2f)) r^2; asm over 50 mm.
+    2: 1000^2, dx 1.5 um, 532 nm: x-polarised plane wave times a seeded i.i.d.
+       uniform [0, 2 pi) phase per pixel inside a circle of diameter 1.2 mm,
+       Ey = 0 (Polarized, both components propagate); asm_pad over 10 mm."""
+    n, dx, wl = 512, 2e-6, 632.8e-9
+    x = (np.arange(n) - n / 2) * dx
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    k = 2 * np.pi / wl
+    c1 = np.exp(-r2 / (200e-6) ** 2) * np.exp(1j * ((-k / (2 * 0.05)) * r2))
+    n2, dx2 = 1000, 1.5e-6
+    x2 = (np.arange(n2) - n2 / 2) * dx2
+    inside = np.hypot(x2[None, :], x2[:, None]) <= 0.6e-3
+    phase = np.random.default_rng(7).uniform(0, 2 * np.pi, (n2, n2))
+    c2 = np.where(inside, np.exp(1j * phase), 0)
+    return {"config1": (c1, None, n, dx, wl, 0.05, "asm"),
+            "config2": (c2, np.zeros_like(c2), n2, dx2, 532e-9, 10e-3, "asm_pad")}
+
+
+def bench_config12(wos):
+    """Configurations 1 and 2 as device-resident batches (one unit per field
+    component, replayed from a hipGraph) and as drop-in Propagate calls on
+    host buffers. Algorithmic bytes per component: 8 M^2 S (M = 2n for
+    asm_pad)."""
+    out, fields = {}, {}
+    for name, (ex, ey, n, dx, wl, z, method) in config12_inputs().items():
+        comps = [ex] if ey is None else [ex, ey]
+        M = 2 * n if method == "asm_pad" else n
+        res = {"grid": n, "method": method, "components": len(comps)}
+        for pname, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
+            b = wos.Batch(n, dx, [wl] * len(comps), [z] * len(comps), wos.ctx_for(wl), method, prec)
+            b.autotune(3)
+            for u, f in enumerate(comps):
+                b.upload(u, f)
+            b.capture()
+            b.run(1)
+            b.sync()
+            fields[(name, pname)] = [b.download(u) for u in range(len(comps))]
+            ms, _ = b.time(3, 30)
+            b.close()
+            S = 16 if prec == wos.FP64 else 8
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                wos.propagate(ex, ey, dx, z, method, wos.ctx_for(wl), precision=prec)
+            res[pname] = {"props_per_s": round(len(comps) / (ms * 1e-3), 1), "device_ms": round(ms, 4),
+                          "hbm_frac": round(len(comps) * 8 * M * M * S / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                          "dropin_ms_per_call": round((time.perf_counter() - t0) * 1e3 / reps, 2)}
+        a32, a64 = np.concatenate(fields[(name, "fp32")]), np.concatenate(fields[(name, "fp64")])
+        res["rel_l2_fp32_vs_fp64"] = rel_l2(a32, a64)
+        out[name] = res
+    return out, fields
+
+
+def cpu_timed(fn, reps, warm=1):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_configs(cfg12, fields, bpm, bpm_check20, adj):
+    """Restated reference CPU path (the FP64 oracle) on each configuration:
+    the checker for the GPU outputs and the CPU timing beside them (median
+    of the runs after one warm-up; all host threads up to 16, and 8 threads
+    for configuration 1 as in the reference's published 8-core figures)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    out = {"cpu_model": cpu_model(), "threads": threads}
+    for name, (ex, ey, n, dx, wl, z, method) in config12_inputs().items():
+        O.set_threads(threads)
+        ctx = O.ctx_for(wl)
+        ref = O.propagate(ex, ey, dx, z, method, ctx)
+        refc = np.concatenate([ref[0]] if ey is None else [ref[0], ref[1]])
+        reps = 5 if name == "config1" else 2
+        t = cpu_timed(lambda: O.propagate(ex, ey, dx, z, method, ctx), reps)
+        r = {"cpu_ms_per_propagate": round(t * 1e3, 1),
+             "rel_l2_fp64_vs_oracle": rel_l2(np.concatenate(fields[(name, "fp64")]), refc),
+             "rel_l2_fp32_vs_oracle": rel_l2(np.concatenate(fields[(name, "fp32")]), refc)}
+        if name == "config1" and threads > 8:
+            O.set_threads(8)
+            r["cpu_ms_per_propagate_8_threads"] = round(cpu_timed(lambda: O.propagate(ex, ey, dx, z, method, ctx), 5) * 1e3, 1)
+            O.set_threads(threads)
+        r["gpu_speedup_fp64"] = round(t * cfg12[name]["fp64"]["props_per_s"] / cfg12[name]["components"], 1)
+        out[name] = r
+    # configuration 3: a 5-slice sample of the 400-slice run
+    n, dx, wl = 1024, 0.5e-6, 1.55e-6
+    x = (np.arange(n) - n / 2) * dx
+    rr = np.hypot(x[None, :], x[:, None])
+    index = np.where(rr < 100e-6, 1.45 - 0.01 * (rr / 100e-6) ** 2, 1.44) + 1e-5j
+    ex = np.exp(-(rr / 10e-6) ** 2).astype(np.complex128)
+    ctx = O.ctx_for(wl)
+    t = cpu_timed(lambda: O.split_step(ex, None, dx, 5 * 5e-6, 5, index, ctx), 2)
+    ref, _, _ = O.split_step(ex, None, dx, 20 * 5e-6, 20, index, ctx)
+    out["config3"] = {"cpu_steps_per_s": round(5 / t, 2), "sample": "5 slices of 5 um",
+                      "gpu_speedup_fp64": round(bpm["fp64"]["steps_per_s"] * t / 5, 1),
+                      "rel_l2_fp64_vs_oracle_20_slices": rel_l2(bpm_check20["fp64"], ref),
+                      "rel_l2_fp32_vs_oracle_20_slices": rel_l2(bpm_check20["fp32"], ref)}
+    # configuration 4: one unit of each grid, weighted by the unit counts of the graph
+    ctx = O.ctx_for(532e-9)
+    per = {}
+    for n, dx in ((2048, 2e-6), (1024, 2e-6), (384, 4.096e-3 / 384), (1000, 4.096e-3 / 1000), (1536, 4.096e-3 / 1536)):
+        f = np.exp(-((np.arange(n) - n / 2) * dx / W0) ** 2)
+        f = (f[None, :] * f[:, None]).astype(np.complex128)
+        per[n] = cpu_timed(lambda: O.propagate(f, None, dx, 20e-3, "asm", ctx), 1)
+    total = 64 * per[2048] + 128 * per[1024] + 11 * per[384] + 11 * per[1000] + 10 * per[1536]
+    out["config4"] = {"cpu_s_per_224_props": round(total, 2), "cpu_props_per_s": round(224 / total, 2),
+                      "sample": "one propagation per grid size, weighted 64/128/11/11/10"}
+    # configuration 5: one mask's forward + adjoint, times 8
+    n, dx, wl, z = 1024, 4e-6, 532e-9, 20e-3
+    xx = (np.arange(n) - n / 2) * dx
+    tg = np.zeros((n, n))
+    for k in range(16):
+        a = 2 * np.pi * k / 16
+        tg += np.exp(-((xx[None, :] - 800e-6 * np.cos(a)) ** 2 + (xx[:, None] - 800e-6 * np.sin(a)) ** 2) / (2 * 12e-6 ** 2))
+    tg /= tg.sum()
+    ctx = O.ctx_for(wl)
+    loss0, grad0 = O.mask_loss_grad(adj["theta0"], tg, dx, z, ctx)
+    t = cpu_timed(lambda: O.mask_loss_grad(adj["theta0"], tg, dx, z, ctx), 2)
+    out["config5"] = {"cpu_ms_forward_backward_8_masks": round(8 * t * 1e3, 1), "sample": "one mask, times 8",
+                      "grad_rel_l2_fp64_vs_oracle": rel_l2(adj["grad0"]["fp64"], grad0),
+                      "grad_rel_l2_fp32_vs_oracle": rel_l2(adj["grad0"]["fp32"], grad0),
+                      "loss_rel_fp64_vs_oracle": abs(adj["loss0"]["fp64"] - loss0) / loss0}
     return out
 
 
@@ -298,14 +458,19 @@ def main():
         "fft2d_z_2048_ms": round(fft2d_dropin_ms(wos, r64["unit0_in"]), 3),
         "fft2d_note": "one complex128 2-D FFT of a host buffer per call, transfers included; the reference "
                       "publishes 45 ms for this call through cuFFT on a Tesla T4 (README.md:113)"}
-    line["bpm_1024_400_slices"] = bench_bpm(wos)
+    cfg12, fields = bench_config12(wos)
+    line["config1_512_asm"] = cfg12["config1"]
+    line["config2_1000_asm_pad_jones"] = cfg12["config2"]
+    line["bpm_1024_400_slices"], bpm20 = bench_bpm(wos)
     line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps)
-    line["mask_adjoint"] = bench_adjoint(wos)
+    line["mask_adjoint"], adj = bench_adjoint(wos)
     if not args.no_cpu:
         ref, cb = cpu_baseline(r64["unit0_in"], r64["wl0"])
+        cb["cpu_model"] = cpu_model()
         line["cpu_baseline"] = cb
         line["rel_l2_fp64_vs_oracle"] = rel_l2(r64["unit0"], ref)
         line["rel_l2_fp32_vs_oracle"] = rel_l2(r32["unit0"], ref)
+        line["cpu_configs"] = cpu_configs(cfg12, fields, line["bpm_1024_400_slices"], bpm20, adj)
     print(json.dumps(line), flush=True)
 
 
