@@ -262,6 +262,53 @@ def bench_config12(wos):
     return out, fields
 
 
+def train_steps(f, m, ctx, shape):
+    """A train of runTrain's element kinds on one field: lens, propagation
+    (each followed by the band limit 0.9/0.05), soft circular aperture,
+    correlated diffuser, polariser (the field becomes a vector field),
+    quarter-wave plate, two more propagations."""
+    c, s = np.cos(0.5), np.sin(0.5)
+    pol = (c * c, s * c, s * c, s * s)
+    e1, e2 = np.exp(1j * np.pi / 4), np.exp(-1j * np.pi / 4)
+    c2, s2 = np.cos(0.2), np.sin(0.2)
+    ret = ((c2 * c2) * e1 + (s2 * s2) * e2, (s2 * c2) * (e1 - e2), (s2 * c2) * (e1 - e2), (s2 * s2) * e1 + (c2 * c2) * e2)
+    f.lens(633e-9, 0.2, aperture=1.8e-3)
+    f.propagate(0.02, "asm", ctx)
+    f.bandlimit(0.9, 0.05)
+    f.aperture(shape)
+    f.diffuser(sigma=0.8, corr=10e-6, amp=1.0, seed=3)
+    f.propagate(0.01, "asm", ctx)
+    f.bandlimit(0.9, 0.05)
+    f.jones(*pol)
+    f.jones(*ret)
+    f.propagate(0.01, "asm", ctx)
+    f.bandlimit(0.9, 0.05)
+
+
+def bench_train(wos, n=2048, dx=2e-6, reps=5):
+    """The element train above on a device-resident 2048^2 field (no host
+    round trip between elements), upload and download outside the timing."""
+    x = (np.arange(n) - n / 2) * dx
+    g = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (0.8e-3) ** 2).astype(np.complex128)
+    out, res = {}, {}
+    for name, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
+        shape = wos.make_shape("circle", radius=1.5e-3, edge_sigma=5e-6)
+        f = wos.Field.from_host(g, dx=dx, precision=prec)
+        train_steps(f, wos, wos.ctx_for(633e-9), shape)
+        res[name] = f.download()
+        ts = []
+        for _ in range(reps):
+            f.upload(g)
+            t0 = time.perf_counter()
+            train_steps(f, wos, wos.ctx_for(633e-9), shape)
+            ts.append(time.perf_counter() - t0)
+        f.close()
+        out[name] = {"ms_per_train": round(float(np.median(ts)) * 1e3, 3)}
+    out["elements_per_train"] = 11
+    out["rel_l2_fp32_vs_fp64"] = rel_l2(np.concatenate(res["fp32"]), np.concatenate(res["fp64"]))
+    return out, g, res["fp64"]
+
+
 def cpu_timed(fn, reps, warm=1):
     for _ in range(warm):
         fn()
@@ -348,6 +395,21 @@ def cpu_configs(cfg12, fields, bpm, bpm_check20, adj):
                       "grad_rel_l2_fp32_vs_oracle": rel_l2(adj["grad0"]["fp32"], grad0),
                       "loss_rel_fp64_vs_oracle": abs(adj["loss0"]["fp64"] - loss0) / loss0}
     return out
+
+
+def cpu_train(g, out64, dx=2e-6):
+    """The element train on the oracle (the checker and the CPU time beside
+    the device-resident train)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.set_threads(min(16, os.cpu_count() or 1))
+    shape = O.make_shape("circle", radius=1.5e-3, edge_sigma=5e-6)
+    t0 = time.perf_counter()
+    f = O.OField(g, dx=dx)
+    train_steps(f, O, O.ctx_for(633e-9), shape)
+    t = time.perf_counter() - t0
+    return {"cpu_ms_per_train": round(t * 1e3, 1),
+            "rel_l2_fp64_vs_oracle": rel_l2(np.concatenate(out64), np.concatenate([f.ex, f.ey]))}
 
 
 def cpu_baseline(inp, wl, budget_s=12.0):
@@ -464,6 +526,7 @@ def main():
     line["bpm_1024_400_slices"], bpm20 = bench_bpm(wos)
     line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps)
     line["mask_adjoint"], adj = bench_adjoint(wos)
+    line["element_train_2048"], train_in, train_out64 = bench_train(wos)
     if not args.no_cpu:
         ref, cb = cpu_baseline(r64["unit0_in"], r64["wl0"])
         cb["cpu_model"] = cpu_model()
@@ -471,6 +534,7 @@ def main():
         line["rel_l2_fp64_vs_oracle"] = rel_l2(r64["unit0"], ref)
         line["rel_l2_fp32_vs_oracle"] = rel_l2(r32["unit0"], ref)
         line["cpu_configs"] = cpu_configs(cfg12, fields, line["bpm_1024_400_slices"], bpm20, adj)
+        line["cpu_configs"]["element_train_2048"] = cpu_train(train_in, train_out64)
     print(json.dumps(line), flush=True)
 
 

@@ -256,6 +256,75 @@ int wos_hip_batch_stats(wos_batch b, int unit, double wl, double strehl_aperture
 int wos_hip_pipeline_stats(wos_pipeline p, int stage, int unit, double wl, double strehl_aperture,
                            double strehl_distance, wos_plane_stats* out);
 
+/* ---- device-resident fields: the element chain of a train ----------------
+ * runTrain (optics/simulator.go:397-500) applies elements and propagations to
+ * one Field in turn. A wos_field keeps that Field (optics/field.go:9-40) in
+ * HBM between the steps: Ex and Ey (Ey allocated also for a scalar field, as
+ * Field does; `polarized` says whether it takes part), so a train makes no
+ * host round trip between elements. Elements compute in FP64 in the
+ * reference's operation order for both precisions (an FP32 field is rounded
+ * once per element). All calls are synchronous. */
+typedef struct wos_field_s* wos_field;
+
+int wos_hip_field_create(wos_field* out, int n, double dx, int polarized, int precision);
+int wos_hip_field_destroy(wos_field f);
+/* Field.Clone (optics/field.go:42-52) */
+int wos_hip_field_clone(wos_field f, wos_field* out);
+/* complex128 host arrays of n*n; ey NULL: zeros */
+int wos_hip_field_upload(wos_field f, const double* ex, const double* ey);
+/* either pointer may be NULL */
+int wos_hip_field_download(wos_field f, double* ex, double* ey);
+/* Field.Polarized: 1 once a Jones element has made a scalar field a vector field */
+int wos_hip_field_polarized(wos_field f);
+/* device pointer of Ex (Ey follows n*n elements later) */
+void* wos_hip_field_device_ptr(wos_field f);
+/* Propagate (optics/propagate.go:65-102) on the resident field, Ey only if
+ * polarized; warnings and stats as wos_hip_propagate */
+int wos_hip_field_propagate(wos_field f, double z, int method, const wos_ctx* ctx, wos_warnings* w,
+                            wos_prop_stats* st);
+/* lensEl.Apply (optics/elements.go:195-220): phase (-k/(2f)) r^2 about
+ * (x0, y0), zero outside `aperture` (radius, 0: none) */
+int wos_hip_field_lens(wos_field f, double wavelength, double focal, double aperture, double x0, double y0);
+
+/* Aperture outline: the parsed shapeGeom (optics/shape.go:28-48, 57-160).
+ * Triangle, polygon, star and custom outlines are WOS_SHAPE_POLYGON with their
+ * vertices (regularPolygonVertices / starVertices / parseVertices,
+ * optics/elements.go:316-338, 397-425, formed by the caller). */
+#define WOS_SHAPE_CIRCLE 0
+#define WOS_SHAPE_SLIT 1
+#define WOS_SHAPE_SQUARE 2
+#define WOS_SHAPE_RECTANGLE 3
+#define WOS_SHAPE_ELLIPSE 4
+#define WOS_SHAPE_SUPERELLIPSE 5
+#define WOS_SHAPE_RING 6
+#define WOS_SHAPE_DOUBLE_SLIT 7
+#define WOS_SHAPE_CROSS 8
+#define WOS_SHAPE_POLYGON 9
+typedef struct {
+  int32_t kind, n_verts;
+  double radius, width, height, a, b, order, rin, rout, inner, length, separation, rotation, edge_sigma;
+  const double* verts; /* n_verts (x, y) pairs, host memory */
+} wos_shape;
+/* apertureEl.Apply (optics/elements.go:288-310): hard edge, or the erf
+ * smooth step when edge_sigma > 0, of the signed distance about (x0, y0) */
+int wos_hip_field_aperture(wos_field f, const wos_shape* shape, double x0, double y0);
+/* diffuserEl.Apply (optics/elements.go:722-783): hashNoise01 phase on the lab
+ * lattice anchored at (x0, y0), Gaussian correlation `correlation` (m, 0:
+ * white), rescaled to standard deviation `sigma`, times `amplitude` */
+int wos_hip_field_diffuser(wos_field f, double sigma, double correlation, double amplitude, int64_t seed, double x0,
+                           double y0);
+/* Field.ApplyJones (optics/field.go:109-131), abcd = {a.re, a.im, b.re, b.im,
+ * c.re, c.im, d.re, d.im}; polarizer, retarder, rotator and custom_jones
+ * (optics/elements.go:1018-1085) are this with their matrices */
+int wos_hip_field_jones(wos_field f, const double* abcd);
+/* Field.ScaleAmplitude (optics/field.go:134-143), the beamsplitter's arms */
+int wos_hip_field_scale(wos_field f, double re, double im);
+/* Field.ApplyBandlimit (optics/field.go:264-294); fraction outside (0, 1): no-op */
+int wos_hip_field_bandlimit(wos_field f, double fraction, double sigma);
+/* ComputeStats (optics/metrics.go:31-91) of the resident field */
+int wos_hip_field_stats(wos_field f, double wl, double strehl_aperture, double strehl_distance,
+                        wos_plane_stats* out);
+
 /* ---- phase-mask design operator (no reference counterpart) --------------
  * Forward/Backward pair behind a Go operator for phase-mask optimisation. For
  * each of `units` masks theta (n x n, radians): U = ASM_z(exp(i theta)) with

@@ -59,6 +59,98 @@ def ctx_for(wl, evanescent="decay", limit=0.0, backward_regularize=False, alpha=
 _lib = None
 
 
+class Cplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
+               "double_slit": 7, "cross": 8, "polygon": 9}
+
+
+class Shape(C.Structure):
+    """or_shape: a parsed aperture outline (optics/shape.go:28-48)."""
+    _fields_ = [("kind", C.c_int32), ("n_verts", C.c_int32)] + \
+        [(k, C.c_double) for k in ("radius", "width", "height", "a", "b", "order", "rin", "rout", "inner",
+                                   "length", "separation", "rotation", "edge_sigma")] + [("verts", C.c_void_p)]
+
+
+def make_shape(kind, verts=None, **params):
+    """Shape of `kind` ("circle" ... "cross", or "polygon" with an (m, 2)
+    vertex array); unset parameters take the reference's defaults
+    (optics/shape.go:51-55)."""
+    d = {"radius": 1e-3, "width": 2e-3, "height": 2e-3, "a": 1e-3, "b": 2e-3, "order": 2, "rin": 5e-4,
+         "rout": 1e-3, "inner": 5e-4, "length": 4e-3, "separation": 1e-3, "rotation": 0, "edge_sigma": 0}
+    d.update(params)
+    g = Shape(kind=SHAPE_KINDS[kind], n_verts=0, **d)
+    if verts is not None:
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        g.n_verts = v.shape[0]
+        g.verts = v.ctypes.data
+        g._keep = v
+    return g
+
+
+def regular_polygon(n, radius, phi0):
+    v = np.zeros((n, 2))
+    lib().or_regular_polygon(n, radius, phi0, v.ctypes.data)
+    return v
+
+
+def star(n, outer, inner, phi0):
+    v = np.zeros((2 * n, 2))
+    lib().or_star(n, outer, inner, phi0, v.ctypes.data)
+    return v
+
+
+def shape_signed_distance(g, dx, dy):
+    return lib().or_shape_signed_distance(C.byref(g), dx, dy)
+
+
+class OField:
+    """Field (optics/field.go:9-40) on the host for the element chain: ey is
+    always allocated; `polarized` says whether it takes part."""
+
+    def __init__(self, ex, ey=None, dx=1e-6, polarized=None):
+        self.ex = np.array(ex, dtype=np.complex128, order="C")
+        self.ey = np.zeros_like(self.ex) if ey is None else np.array(ey, dtype=np.complex128, order="C")
+        self.n = self.ex.shape[0]
+        self.dx = dx
+        self.polarized = (ey is not None) if polarized is None else bool(polarized)
+
+    def clone(self):
+        return OField(self.ex, self.ey, self.dx, self.polarized)
+
+    def lens(self, wl, f, aperture=0.0, x0=0.0, y0=0.0):
+        lib().or_apply_lens_at(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, wl, f, aperture, x0, y0)
+
+    def aperture(self, g, x0=0.0, y0=0.0):
+        lib().or_apply_aperture(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, C.byref(g), x0, y0)
+
+    def diffuser(self, sigma=np.pi, corr=2e-5, amp=1.0, seed=1, x0=0.0, y0=0.0):
+        lib().or_apply_diffuser(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, sigma, corr, amp,
+                                seed, x0, y0)
+
+    def jones(self, a, b, c, d):
+        pol = C.c_int(int(self.polarized))
+        cx = [Cplx(complex(v).real, complex(v).imag) for v in (a, b, c, d)]
+        lib().or_apply_jones(_p(self.ex), _p(self.ey), C.byref(pol), self.n, *cx)
+        self.polarized = bool(pol.value)
+
+    def scale(self, s):
+        s = complex(s)
+        lib().or_scale_amplitude(_p(self.ex), _p(self.ey), int(self.polarized), self.n, Cplx(s.real, s.imag))
+
+    def bandlimit(self, fraction, sigma):
+        lib().or_bandlimit(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, fraction, sigma)
+
+    def propagate(self, z, method, ctx):
+        ex, ey, w, st = propagate(self.ex, self.ey if self.polarized else None, self.dx, z, method, ctx)
+        self.ex = np.ascontiguousarray(ex)
+        if self.polarized:
+            self.ey = np.ascontiguousarray(ey)
+        return w, st
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -105,6 +197,16 @@ def lib():
         L.or_schell_sqrt_spectrum.argtypes = [P, I, D, D]
         L.or_schell_realization.argtypes = [P, P, I, D, D, C.c_uint64, C.c_uint64]
         L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
+        L.or_apply_lens_at.argtypes = [P, P, I, I, D, D, D, D, D, D]
+        L.or_shape_signed_distance.argtypes = [C.POINTER(Shape), D, D]
+        L.or_shape_signed_distance.restype = D
+        L.or_regular_polygon.argtypes = [I, D, D, P]
+        L.or_star.argtypes = [I, D, D, D, P]
+        L.or_apply_aperture.argtypes = [P, P, I, I, D, C.POINTER(Shape), D, D]
+        L.or_apply_diffuser.argtypes = [P, P, I, I, D, D, D, D, C.c_int64, D, D]
+        L.or_apply_jones.argtypes = [P, P, C.POINTER(I), I, Cplx, Cplx, Cplx, Cplx]
+        L.or_scale_amplitude.argtypes = [P, P, I, I, Cplx]
+        L.or_bandlimit.argtypes = [P, P, I, I, D, D, D]
         L.or_mask_loss_grad.argtypes = [P, P, I, D, D, C.POINTER(Ctx), P]
         L.or_mask_loss_grad.restype = D
         L.or_bpm_mask_loss_grad.argtypes = [P, P, P, I, P, I, D, D, I, C.POINTER(Ctx), P, P]

@@ -760,6 +760,192 @@ void or_apply_circle_aperture(or_cplx* ex, or_cplx* ey, int n, double dx, double
     }
 }
 
+/* lensEl.Apply with its aperture and offset (optics/elements.go:195-220) */
+void or_apply_lens_at(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double wl, double f,
+                      double aperture, double x0, double y0) {
+    double k = 2.0 * M_PI / wl;
+    double c = -k / (2 * f);
+    double ap = aperture;
+    for (int j = 0; j < n; j++) {
+        double dy = coord(j, n, dx) - y0;
+        for (int i = 0; i < n; i++) {
+            double ddx = coord(i, n, dx) - x0;
+            double r2 = ddx * ddx + dy * dy;
+            size_t idx = (size_t)j * n + i;
+            if (ap > 0 && r2 > ap * ap) {
+                ex[idx] = cre(0);
+                if (polarized) ey[idx] = cre(0);
+                continue;
+            }
+            or_cplx t = cexpI(c * r2);
+            ex[idx] = cmul(ex[idx], t);
+            if (polarized) ey[idx] = cmul(ey[idx], t);
+        }
+    }
+}
+
+/* regularPolygonVertices, starVertices (optics/elements.go:316-338) */
+void or_regular_polygon(int n, double radius, double phi0, double* verts) {
+    for (int i = 0; i < n; i++) {
+        double a = phi0 + 2 * M_PI * (double)i / (double)n;
+        verts[2 * i] = radius * cos(a);
+        verts[2 * i + 1] = radius * sin(a);
+    }
+}
+void or_star(int n, double outer, double inner, double phi0, double* verts) {
+    for (int i = 0; i < 2 * n; i++) {
+        double a = phi0 + M_PI * (double)i / (double)n;
+        double r = (i % 2 == 1) ? inner : outer;
+        verts[2 * i] = r * cos(a);
+        verts[2 * i + 1] = r * sin(a);
+    }
+}
+
+/* pointInPolygon, distToSegment, polygonSignedDistance (optics/elements.go:342-395) */
+static int point_in_polygon(const double* p, int n, double x, double y) {
+    if (n < 3) return 0;
+    int inside = 0;
+    int j = n - 1;
+    for (int i = 0; i < n; i++) {
+        double xi = p[2 * i], yi = p[2 * i + 1], xj = p[2 * j], yj = p[2 * j + 1];
+        if ((yi > y) != (yj > y)) {
+            double xint = (xj - xi) * (y - yi) / (yj - yi) + xi;
+            if (x < xint) inside = !inside;
+        }
+        j = i;
+    }
+    return inside;
+}
+static double dist_to_segment(double px, double py, double ax, double ay, double bx, double by) {
+    double abx = bx - ax, aby = by - ay;
+    double apx = px - ax, apy = py - ay;
+    double t = (apx * abx + apy * aby) / (abx * abx + aby * aby);
+    if (t < 0) t = 0;
+    else if (t > 1) t = 1;
+    return or_hypot(px - (ax + t * abx), py - (ay + t * aby));
+}
+static double polygon_signed_distance(const double* p, int n, double x, double y) {
+    if (n < 3) return -1e300;
+    double d = INFINITY;
+    for (int i = 0; i < n; i++) {
+        int k = (i + 1) % n;
+        double dd = dist_to_segment(x, y, p[2 * i], p[2 * i + 1], p[2 * k], p[2 * k + 1]);
+        if (dd < d) d = dd;
+    }
+    if (!point_in_polygon(p, n, x, y)) d = -d;
+    return d;
+}
+
+/* shapeGeom.signedDistance (optics/shape.go:163-198) */
+double or_shape_signed_distance(const or_shape* g, double dx, double dy) {
+    double cr = cos(g->rotation), sr = sin(g->rotation);
+    double u = cr * dx + sr * dy;
+    double v = -sr * dx + cr * dy;
+    switch (g->kind) {
+    case 0: return g->radius - or_hypot(dx, dy);
+    case 1: return fmin(g->width / 2 - fabs(u), g->height / 2 - fabs(v));
+    case 2: return fmin(g->width / 2 - fabs(u), g->width / 2 - fabs(v));
+    case 3: return fmin(g->width / 2 - fabs(u), g->height / 2 - fabs(v));
+    case 4: return (1 - or_hypot(u / g->a, v / g->b)) * fmin(g->a, g->b);
+    case 9: return polygon_signed_distance(g->verts, g->n_verts, u, v);
+    case 6: {
+        double r = or_hypot(dx, dy);
+        return fmin(r - g->rin, g->rout - r);
+    }
+    case 7: {
+        double d1 = fmin(g->height / 2 - fabs(v), g->width / 2 - fabs(u - g->sep / 2));
+        double d2 = fmin(g->height / 2 - fabs(v), g->width / 2 - fabs(u + g->sep / 2));
+        return fmax(d1, d2);
+    }
+    case 8: {
+        double hw = g->width / 2, hl = g->length / 2;
+        return fmax(fmin(hw - fabs(u), hl - fabs(v)), fmin(hl - fabs(u), hw - fabs(v)));
+    }
+    case 5: {
+        double rn = pow(fabs(u) / g->a, g->order) + pow(fabs(v) / g->b, g->order);
+        return (1 - rn) * fmin(g->a, g->b);
+    }
+    }
+    return -1e300;
+}
+
+/* apertureEl.Apply (optics/elements.go:288-310): step or smoothStep
+ * (optics/elements.go:143-153) of the signed distance */
+void or_apply_aperture(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, const or_shape* g,
+                       double x0, double y0) {
+    double sig = g->edge_sigma;
+    for (int j = 0; j < n; j++) {
+        double dy = coord(j, n, dx) - y0;
+        for (int i = 0; i < n; i++) {
+            double ddx = coord(i, n, dx) - x0;
+            double d = or_shape_signed_distance(g, ddx, dy);
+            or_cplx t = sig > 0 ? cre(0.5 * (1 + erf(d / (M_SQRT2 * sig)))) : cre(d >= 0 ? 1.0 : 0.0);
+            size_t idx = (size_t)j * n + i;
+            ex[idx] = cmul(ex[idx], t);
+            if (polarized) ey[idx] = cmul(ey[idx], t);
+        }
+    }
+}
+
+/* Field.ApplyJones (optics/field.go:109-131) */
+void or_apply_jones(or_cplx* ex, or_cplx* ey, int* polarized, int n, or_cplx a, or_cplx b, or_cplx c,
+                    or_cplx d) {
+    size_t nn = (size_t)n * n;
+    if (!*polarized) {
+        if (b.re == 0 && b.im == 0 && c.re == 0 && c.im == 0 && d.re == a.re && d.im == a.im) {
+            for (size_t i = 0; i < nn; i++) ex[i] = cmul(ex[i], a);
+            return;
+        }
+        *polarized = 1;
+        for (size_t i = 0; i < nn; i++) {
+            or_cplx e = ex[i];
+            ex[i] = cmul(a, e);
+            ey[i] = cmul(c, e);
+        }
+        return;
+    }
+    for (size_t i = 0; i < nn; i++) {
+        or_cplx e = ex[i], f = ey[i];
+        ex[i] = cadd(cmul(a, e), cmul(b, f));
+        ey[i] = cadd(cmul(c, e), cmul(d, f));
+    }
+}
+
+/* Field.ScaleAmplitude (optics/field.go:134-143) */
+void or_scale_amplitude(or_cplx* ex, or_cplx* ey, int polarized, int n, or_cplx s) {
+    size_t nn = (size_t)n * n;
+    for (size_t i = 0; i < nn; i++) ex[i] = cmul(ex[i], s);
+    if (polarized)
+        for (size_t i = 0; i < nn; i++) ey[i] = cmul(ey[i], s);
+}
+
+/* Field.ApplyBandlimit (optics/field.go:264-294) */
+static void bandlimit_cut(or_cplx* a, int n, double dx, double f0, double sig) {
+    or_fft2d(a, n, 0);
+    for (int j = 0; j < n; j++) {
+        double fy = or_freq(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            double fx = or_freq(i, n, dx);
+            double fr = or_hypot(fx, fy);
+            if (fr > f0) {
+                double d = (fr - f0) / sig;
+                size_t idx = (size_t)j * n + i;
+                a[idx] = cmul(a[idx], cre(exp(-d * d)));
+            }
+        }
+    }
+    or_fft2d(a, n, 1);
+}
+void or_bandlimit(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double fraction, double sigma) {
+    if (fraction >= 1 || fraction <= 0) return;
+    double fnyq = 1 / (2 * dx);
+    double f0 = fraction * fnyq;
+    double sig = sigma * fnyq;
+    if (sig <= 0) sig = fnyq * 0.01;
+    bandlimit_cut(ex, n, dx, f0, sig);
+    if (polarized) bandlimit_cut(ey, n, dx, f0, sig);
+}
+
 /* hashNoise01 (optics/elements.go:710-721) */
 double or_hash_noise01(int64_t x, int64_t y, int64_t seed) {
     uint64_t h = (uint64_t)seed * 0x9E3779B97F4A7C15ULL ^ (uint64_t)x * 0xC2B2AE3D27D4EB4FULL ^
@@ -770,6 +956,58 @@ double or_hash_noise01(int64_t x, int64_t y, int64_t seed) {
     h *= 0x94D049BB133111EBULL;
     h ^= h >> 31;
     return (double)(h >> 11) / (double)(1ULL << 53);
+}
+
+/* diffuserEl.Apply (optics/elements.go:722-783): hash noise on the lab
+ * lattice, Gaussian smoothing in the frequency domain, rescale to the
+ * requested standard deviation (serial sums), then amp * cexpI(phase). */
+void or_apply_diffuser(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double sigma,
+                       double corr, double amp, int64_t seed, double x0, double y0) {
+    size_t nn = (size_t)n * n;
+    double* phase = (double*)malloc(nn * sizeof(double));
+    double ddx = dx;
+    if (ddx <= 0) ddx = 1;
+    for (int j = 0; j < n; j++) {
+        int64_t gy = (int64_t)round((coord(j, n, dx) + y0) / ddx);
+        for (int i = 0; i < n; i++) {
+            int64_t gx = (int64_t)round((coord(i, n, dx) + x0) / ddx);
+            phase[(size_t)j * n + i] = or_hash_noise01(gx, gy, seed) * 2 * M_PI - M_PI;
+        }
+    }
+    if (corr > 0) {
+        or_cplx* z = (or_cplx*)malloc(nn * sizeof(or_cplx));
+        for (size_t i = 0; i < nn; i++) z[i] = cre(phase[i]);
+        or_fft2d(z, n, 0);
+        for (int j = 0; j < n; j++) {
+            double fy = or_freq(j, n, dx);
+            for (int i = 0; i < n; i++) {
+                double fx = or_freq(i, n, dx);
+                double f2 = fx * fx + fy * fy;
+                size_t idx = (size_t)j * n + i;
+                z[idx] = cmul(z[idx], cre(exp(-2 * M_PI * M_PI * corr * corr * f2)));
+            }
+        }
+        or_fft2d(z, n, 1);
+        for (size_t i = 0; i < nn; i++) phase[i] = z[i].re;
+        free(z);
+    }
+    double m = 0, s2 = 0;
+    for (size_t i = 0; i < nn; i++) {
+        m += phase[i];
+        s2 += phase[i] * phase[i];
+    }
+    m /= (double)nn;
+    s2 = s2 / (double)nn - m * m;
+    if (s2 > 0) {
+        double sc = sigma / sqrt(s2);
+        for (size_t i = 0; i < nn; i++) phase[i] = (phase[i] - m) * sc;
+    }
+    for (size_t i = 0; i < nn; i++) {
+        or_cplx t = cmul(cre(amp), cexpI(phase[i]));
+        ex[i] = cmul(ex[i], t);
+        if (polarized) ey[i] = cmul(ey[i], t);
+    }
+    free(phase);
 }
 
 /* One i.i.d. uniform phase per pixel from the diffuser hash, lattice cell

@@ -111,6 +111,33 @@ void or_apply_circle_aperture(or_cplx* ex, or_cplx* ey, int n, double dx, double
 double or_hash_noise01(int64_t x, int64_t y, int64_t seed);
 void or_random_phase(or_cplx* ex, int n, int64_t seed);           /* exp(i 2pi u(i,j)) */
 
+/* The element chain of a train (optics/simulator.go:397-500). Fields follow
+ * Field (optics/field.go:9-40): ey is always allocated; *polarized says
+ * whether it takes part. */
+void or_apply_lens_at(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double wl, double f,
+                      double aperture, double x0, double y0);             /* lensEl.Apply */
+/* Aperture outline (optics/shape.go:28-48): the parsed shapeGeom. kind:
+ * 0 circle, 1 slit, 2 square, 3 rectangle, 4 ellipse, 5 superellipse,
+ * 6 ring, 7 double_slit, 8 cross, 9 polygon (triangle, polygon, star and
+ * custom outlines: their vertices). */
+typedef struct {
+    int32_t kind, n_verts;
+    double radius, width, height, a, b, order, rin, rout, inner, length, sep, rotation, edge_sigma;
+    const double* verts;    /* n_verts (x, y) pairs */
+} or_shape;
+double or_shape_signed_distance(const or_shape* g, double dx, double dy);   /* shapeGeom.signedDistance */
+void or_regular_polygon(int n, double radius, double phi0, double* verts);   /* regularPolygonVertices */
+void or_star(int n, double outer, double inner, double phi0, double* verts); /* starVertices (2n points) */
+void or_apply_aperture(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, const or_shape* g,
+                       double x0, double y0);                             /* apertureEl.Apply */
+void or_apply_diffuser(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double sigma,
+                       double corr, double amp, int64_t seed, double x0, double y0);  /* diffuserEl.Apply */
+void or_apply_jones(or_cplx* ex, or_cplx* ey, int* polarized, int n, or_cplx a, or_cplx b, or_cplx c,
+                    or_cplx d);                                           /* Field.ApplyJones */
+void or_scale_amplitude(or_cplx* ex, or_cplx* ey, int polarized, int n, or_cplx s);  /* Field.ScaleAmplitude */
+void or_bandlimit(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double fraction,
+                  double sigma);                                          /* Field.ApplyBandlimit */
+
 /* optics/coherence.go with a counter-based normal generator in place of
  * Go's math/rand stream (that stream is not reproducible without Go). */
 double or_normal(uint64_t seed, uint64_t counter);

@@ -51,6 +51,11 @@ int main(void) {
   printf("%d %d %d %d %d %d\n", WOS_METHOD_ASM, WOS_METHOD_ASM_PAD, WOS_METHOD_ASM_SHIFT,
          WOS_METHOD_ASM_SHIFT_PAD, WOS_METHOD_FRESNEL_TF, WOS_METHOD_FRESNEL_IR);
   printf("%d %d\n", WOS_FP64, WOS_FP32);
+  printf("%zu %zu %zu %zu %zu\n", sizeof(wos_shape), offsetof(wos_shape, radius), offsetof(wos_shape, separation),
+         offsetof(wos_shape, edge_sigma), offsetof(wos_shape, verts));
+  printf("%d %d %d %d %d %d %d %d %d %d\n", WOS_SHAPE_CIRCLE, WOS_SHAPE_SLIT, WOS_SHAPE_SQUARE, WOS_SHAPE_RECTANGLE,
+         WOS_SHAPE_ELLIPSE, WOS_SHAPE_SUPERELLIPSE, WOS_SHAPE_RING, WOS_SHAPE_DOUBLE_SLIT, WOS_SHAPE_CROSS,
+         WOS_SHAPE_POLYGON);
   return 0;
 }
 ''')
@@ -67,6 +72,12 @@ int main(void) {
                                                   ("asm", "asm_pad", "asm_shift", "asm_shift_pad",
                                                    "fresnel_tf", "fresnel_ir")]
     assert [int(x) for x in lines[4].split()] == [wos_hip.FP64, wos_hip.FP32]
+    S = wos_hip.Shape
+    assert [int(x) for x in lines[5].split()] == [C.sizeof(S), S.radius.offset, S.separation.offset,
+                                                  S.edge_sigma.offset, S.verts.offset]
+    assert [int(x) for x in lines[6].split()] == [wos_hip.SHAPE_KINDS[k] for k in
+                                                  ("circle", "slit", "square", "rectangle", "ellipse",
+                                                   "superellipse", "ring", "double_slit", "cross", "polygon")]
 
 
 def test_oracle_structs_match_library_structs():
@@ -77,6 +88,9 @@ def test_oracle_structs_match_library_structs():
     assert C.sizeof(O.Warnings) == C.sizeof(wos_hip.Warnings)
     assert O.WARN_CODES == wos_hip.WARN_CODES
     assert O.METHODS == wos_hip.METHODS
+    assert C.sizeof(O.Shape) == C.sizeof(wos_hip.Shape)
+    assert [f[0] for f in O.Shape._fields_] == [f[0] for f in wos_hip.Shape._fields_]
+    assert O.SHAPE_KINDS == wos_hip.SHAPE_KINDS
 
 
 def test_no_device_reports_error_without_computing():
@@ -94,3 +108,5 @@ def test_no_device_reports_error_without_computing():
     assert b"device" in lib.wos_hip_last_error()
     with pytest.raises(wos_hip.WosError):
         wos_hip.propagate(a, None, 1e-6, 1e-3, "asm", wos_hip.ctx_for(632.8e-9))
+    with pytest.raises(wos_hip.WosError):
+        wos_hip.Field(8, 1e-6)

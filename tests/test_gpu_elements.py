@@ -1,0 +1,161 @@
+"""Device-resident element chain (wos_field) against the oracle's
+restatements: every element on its own, then whole trains with
+propagations, a beamsplitter's cloned arm and the band limit, in FP64
+(<= 1e-12) and FP32 (<= 1e-5)."""
+import math
+
+import numpy as np
+import pytest
+
+from test_oracle_elements import jones_polarizer, jones_retarder, jones_rotator
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def field(n, seed, polarized=False):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) - n / 2) / n
+    env = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / 0.1)
+    ex = env * np.exp(2j * np.pi * rng.random((n, n)))
+    ey = env * np.exp(2j * np.pi * rng.random((n, n))) * 0.5 if polarized else None
+    return ex, ey
+
+
+def compare(wos, oracle, n, dx, polarized, op, seed=1, tol64=FP64_TOL):
+    """Apply op(f) to a device field (FP64 and FP32) and to the oracle field."""
+    ex, ey = field(n, seed, polarized)
+    o = oracle.OField(ex, ey, dx=dx)
+    op(o, oracle)
+    ro = np.concatenate([o.ex, o.ey]) if o.polarized else o.ex
+    errs = {}
+    for prec in (wos.FP64, wos.FP32):
+        f = wos.Field.from_host(ex, ey, dx=dx, precision=prec)
+        op(f, wos)
+        gx, gy = f.download()
+        assert f.polarized == o.polarized
+        got = np.concatenate([gx, gy]) if f.polarized else gx
+        if not f.polarized:
+            assert np.array_equal(gy, np.zeros_like(gy)) or ey is not None
+        errs[prec] = rel_l2(got, ro)
+        f.close()
+    assert errs[wos.FP64] <= tol64 and errs[wos.FP32] <= FP32_TOL, errs
+    return errs
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+def test_lens(wos, oracle, polarized):
+    e = compare(wos, oracle, 128, 2e-6, polarized, lambda f, m: f.lens(633e-9, 0.03, aperture=100e-6, x0=6e-6,
+                                                                        y0=-4e-6))
+    print(f"lens polarized={polarized}: FP64 {e[wos.FP64]:.2e} FP32 {e[wos.FP32]:.2e}", flush=True)
+
+
+SHAPES = [
+    ("circle", dict(radius=70e-6), None),
+    ("slit", dict(width=30.5e-6, height=150e-6, rotation=0.3), None),
+    ("square", dict(width=90.5e-6, rotation=-0.2), None),
+    ("rectangle", dict(width=120.3e-6, height=50.7e-6), None),
+    ("ellipse", dict(a=80e-6, b=40e-6, rotation=0.7), None),
+    ("superellipse", dict(a=80e-6, b=60e-6, order=4.5), None),
+    ("ring", dict(rin=30e-6, rout=90e-6), None),
+    ("double_slit", dict(width=10.5e-6, height=100e-6, separation=40e-6), None),
+    ("cross", dict(width=20.5e-6, length=150e-6, rotation=0.4), None),
+    ("polygon", dict(rotation=0.1), ("polygon", 6, 90e-6)),
+    ("polygon", dict(), ("star", 5, 100e-6)),
+    ("circle", dict(radius=70e-6, edge_sigma=5e-6), None),
+]
+
+
+@pytest.mark.parametrize("idx", range(len(SHAPES)))
+def test_aperture_outlines(wos, oracle, idx):
+    kind, params, verts = SHAPES[idx]
+    v = None
+    if verts is not None:
+        v = (oracle.regular_polygon(verts[1], verts[2], math.pi / verts[1]) if verts[0] == "polygon"
+             else oracle.star(verts[1], verts[2], verts[2] / 2.5, math.pi / 2))
+
+    e = compare(wos, oracle, 128, 2e-6, idx % 2 == 1,
+                lambda f, m: f.aperture(m.make_shape(kind, verts=v, **params), x0=5e-6, y0=-3e-6))
+    print(f"aperture {kind} {params}: FP64 {e[wos.FP64]:.2e} FP32 {e[wos.FP32]:.2e}", flush=True)
+    assert e[wos.FP64] <= 1e-14  # the masks agree pixel for pixel
+
+
+@pytest.mark.parametrize("n,corr", [(128, 0.0), (128, 6e-6), (100, 6e-6)])
+def test_diffuser(wos, oracle, n, corr):
+    e = compare(wos, oracle, n, 2e-6, n == 128, lambda f, m: f.diffuser(sigma=1.3, corr=corr, amp=0.8, seed=11,
+                                                                        x0=8e-6, y0=2e-6))
+    print(f"diffuser n={n} corr={corr}: FP64 {e[wos.FP64]:.2e} FP32 {e[wos.FP32]:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+@pytest.mark.parametrize("which", ["polarizer", "retarder", "rotator", "scalar", "custom"])
+def test_jones(wos, oracle, polarized, which):
+    m = {"polarizer": jones_polarizer(0.6, 0.9), "retarder": jones_retarder(math.pi / 2, math.pi / 4),
+         "rotator": jones_rotator(0.35), "scalar": (0.5 - 0.5j, 0, 0, 0.5 - 0.5j),
+         "custom": (1 + 2j, 0.5, -0.25j, 3)}[which]
+    e = compare(wos, oracle, 64, 1e-6, polarized, lambda f, mod: f.jones(*m))
+    assert e[wos.FP64] <= 1e-15
+
+
+@pytest.mark.parametrize("n", [128, 100])
+def test_scale_and_bandlimit(wos, oracle, n):
+    compare(wos, oracle, n, 1e-6, True, lambda f, m: f.scale(0.3 + 0.4j))
+    for pol in (False, True):
+        e = compare(wos, oracle, n, 1e-6, pol, lambda f, m: f.bandlimit(0.7, 0.05))
+        print(f"bandlimit n={n} polarized={pol}: FP64 {e[wos.FP64]:.2e} FP32 {e[wos.FP32]:.2e}", flush=True)
+    compare(wos, oracle, n, 1e-6, False, lambda f, m: f.bandlimit(1.0, 0.05))
+
+
+def run_train(f, m, ctx, bl):
+    """A train with the element kinds of runTrain: lens, propagation (each
+    followed by the band limit), aperture, diffuser, a polariser that makes
+    the field a vector field, a retarder, and propagations after it."""
+    f.lens(633e-9, 0.05, aperture=220e-6)
+    f.propagate(0.01, "asm", ctx)
+    f.bandlimit(*bl)
+    f.aperture(m.make_shape("circle", radius=150e-6, edge_sigma=4e-6))
+    f.diffuser(sigma=0.8, corr=6e-6, amp=1.0, seed=3)
+    f.propagate(0.004, "asm_pad", ctx)
+    f.bandlimit(*bl)
+    f.jones(*jones_polarizer(0.5))
+    f.jones(*jones_retarder(math.pi / 2, 0.2))
+    f.propagate(0.006, "asm", ctx)
+    f.bandlimit(*bl)
+
+
+@pytest.mark.parametrize("n", [256, 200])
+def test_train_matches_oracle(wos, oracle, n):
+    dx = 2e-6
+    ex, _ = field(n, 7)
+    bl = (0.9, 0.05)
+    o = oracle.OField(ex, dx=dx)
+    run_train(o, oracle, oracle.ctx_for(633e-9), bl)
+    # the beamsplitter's reflected arm: a clone scaled by i sqrt(r) exp(i phase)
+    arm_o = o.clone()
+    arm_o.scale(1j * math.sqrt(0.3) * complex(math.cos(0.2), math.sin(0.2)))
+    o.scale(math.sqrt(0.7))
+    arm_o.propagate(0.003, "asm", oracle.ctx_for(633e-9))
+    ref = np.concatenate([o.ex, o.ey, arm_o.ex, arm_o.ey])
+    so = oracle.compute_stats(o.ex, o.ey, dx, 633e-9)
+    for prec, tol in ((wos.FP64, FP64_TOL), (wos.FP32, FP32_TOL)):
+        f = wos.Field.from_host(ex, dx=dx, precision=prec)
+        run_train(f, wos, wos.ctx_for(633e-9), bl)
+        arm = f.clone()
+        arm.scale(1j * math.sqrt(0.3) * complex(math.cos(0.2), math.sin(0.2)))
+        f.scale(math.sqrt(0.7))
+        arm.propagate(0.003, "asm", wos.ctx_for(633e-9))
+        assert f.polarized and arm.polarized
+        got = np.concatenate([*f.download(), *arm.download()])
+        e = rel_l2(got, ref)
+        sg = f.stats(633e-9)
+        print(f"train n={n} prec={prec}: rel-L2 {e:.2e}, power {sg.power:.9e} vs {so.power:.9e}", flush=True)
+        assert e <= tol
+        assert abs(sg.power - so.power) <= (1e-12 if prec == wos.FP64 else 1e-5) * so.power
+        f.close()
+        arm.close()

@@ -1,0 +1,229 @@
+"""Pins of the oracle's element-chain restatements (lens, aperture outlines,
+diffuser, Jones elements, amplitude scale, band limit) by closed forms,
+cross-checks between independent outlines and physical invariants, so that
+a wrong sign, offset or formula in one of them fails here."""
+import math
+
+import numpy as np
+import pytest
+
+
+def ones(n):
+    return np.ones((n, n), dtype=np.complex128)
+
+
+def mask(oracle, n, dx, g, x0=0.0, y0=0.0):
+    f = oracle.OField(ones(n), dx=dx)
+    f.aperture(g, x0, y0)
+    return f.ex.real
+
+
+def test_lens_inverse_offset_aperture_and_focus(oracle):
+    n, dx, wl = 128, 2e-6, 633e-9
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    f = oracle.OField(a, dx=dx)
+    f.lens(wl, 0.05)
+    f.lens(wl, -0.05)  # a lens of -f undoes it
+    assert np.max(np.abs(f.ex - a)) <= 1e-13
+    # an offset lens is the centred lens translated by whole pixels
+    c = oracle.OField(ones(n), dx=dx)
+    c.lens(wl, 0.02)
+    o = oracle.OField(ones(n), dx=dx)
+    o.lens(wl, 0.02, x0=3 * dx, y0=-2 * dx)
+    # X(i) - 3 dx = X(i - 3), Y(j) + 2 dx = Y(j + 2)
+    assert np.max(np.abs(o.ex[:-2, 3:] - c.ex[2:, :-3])) <= 1e-12
+    # the aperture zeroes r > radius and leaves the modulus inside
+    ap = oracle.OField(ones(n), dx=dx)
+    ap.lens(wl, 0.02, aperture=40 * dx)
+    x = (np.arange(n) - n / 2) * dx
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    out = r2 > (40 * dx) ** 2
+    assert np.all(ap.ex[out] == 0) and np.allclose(np.abs(ap.ex[~out]), 1, atol=1e-15)
+    # a converging lens focuses a Gaussian (w0) at z = f to w_f = wl f / (pi w0):
+    # peak intensity grows by (w0 / w_f)^2
+    n, dx, wl, w0, fl = 256, 2e-6, 0.5e-6, 100e-6, 0.01
+    g = oracle.OField(oracle.gaussian(n, dx, w0), dx=dx)
+    g.lens(wl, fl)
+    g.propagate(fl, "asm", oracle.ctx_for(wl))
+    gain = np.max(np.abs(g.ex) ** 2)
+    want = (w0 / (wl * fl / (math.pi * w0))) ** 2
+    assert abs(gain / want - 1) < 0.02, (gain, want)
+
+
+def test_aperture_outlines_cross_checks(oracle):
+    n, dx = 128, 1e-6
+    R = 37.3e-6
+    circ = mask(oracle, n, dx, oracle.make_shape("circle", radius=R))
+    # the same disk as a ring without hole, an ellipse with a = b, a superellipse of order 2
+    assert np.array_equal(circ, mask(oracle, n, dx, oracle.make_shape("ring", rin=0.0, rout=R)))
+    assert np.array_equal(circ, mask(oracle, n, dx, oracle.make_shape("ellipse", a=R, b=R)))
+    assert np.array_equal(circ, mask(oracle, n, dx, oracle.make_shape("superellipse", a=R, b=R, order=2)))
+    # rectangle pixel count: centres at (i - n/2) dx, |x| <= w/2
+    w, h = 41.3e-6, 17.7e-6
+    rect = mask(oracle, n, dx, oracle.make_shape("rectangle", width=w, height=h))
+    assert rect.sum() == (2 * math.floor(w / 2 / dx) + 1) * (2 * math.floor(h / 2 / dx) + 1)
+    assert np.array_equal(rect, mask(oracle, n, dx, oracle.make_shape("slit", width=w, height=h)))
+    assert np.array_equal(mask(oracle, n, dx, oracle.make_shape("square", width=w)),
+                          mask(oracle, n, dx, oracle.make_shape("rectangle", width=w, height=w)))
+    # a quarter turn swaps width and height
+    assert np.array_equal(rect.T, mask(oracle, n, dx, oracle.make_shape("rectangle", width=w, height=h,
+                                                                        rotation=math.pi / 2)))
+    # the double slit is two rectangles at +-sep/2 (placed with the aperture offset)
+    sep = 30.2e-6
+    ds = mask(oracle, n, dx, oracle.make_shape("double_slit", width=w / 4, height=h, separation=sep))
+    r1 = mask(oracle, n, dx, oracle.make_shape("rectangle", width=w / 4, height=h), x0=sep / 2)
+    r2 = mask(oracle, n, dx, oracle.make_shape("rectangle", width=w / 4, height=h), x0=-sep / 2)
+    assert np.array_equal(ds, np.maximum(r1, r2)) and ds.sum() == r1.sum() + r2.sum()
+    # the cross is the union of a w x L and an L x w bar
+    L = 70.3e-6
+    cr = mask(oracle, n, dx, oracle.make_shape("cross", width=w / 3, length=L))
+    b1 = mask(oracle, n, dx, oracle.make_shape("rectangle", width=w / 3, height=L))
+    b2 = mask(oracle, n, dx, oracle.make_shape("rectangle", width=L, height=w / 3))
+    assert np.array_equal(cr, np.maximum(b1, b2))
+    # a regular 4-gon rotated by pi/4 is the square of half-width R/sqrt(2)
+    sq_poly = mask(oracle, n, dx, oracle.make_shape("polygon", verts=oracle.regular_polygon(4, R, math.pi / 4)))
+    assert np.array_equal(sq_poly, mask(oracle, n, dx, oracle.make_shape("square", width=R * math.sqrt(2))))
+    # the polygon signed distance is the Euclidean distance to the nearest edge
+    g = oracle.make_shape("polygon", verts=oracle.regular_polygon(4, R, math.pi / 4))
+    hw = R / math.sqrt(2)
+    assert abs(oracle.shape_signed_distance(g, 0.0, 0.0) - hw) < 1e-18
+    assert abs(oracle.shape_signed_distance(g, hw + 3e-6, hw + 4e-6) + 5e-6) < 1e-18
+    # a star lies between its inner and outer polygons
+    st = mask(oracle, n, dx, oracle.make_shape("polygon", verts=oracle.star(5, R, R / 2, math.pi / 2)))
+    inner = mask(oracle, n, dx, oracle.make_shape("circle", radius=R / 2 * math.cos(math.pi / 5) * 0.99))
+    outer = circ
+    assert np.all(st >= inner) and np.all(st <= outer) and inner.sum() < st.sum() < outer.sum()
+    # the smooth edge: t(0) = 1/2 and t(d) + t(-d) = 1
+    sig = 2e-6
+    g = oracle.make_shape("circle", radius=R, edge_sigma=sig)
+    t = mask(oracle, n, dx, g)
+    x = (np.arange(n) - n / 2) * dx
+    d = R - np.hypot(x[None, :], x[:, None])
+    want = 0.5 * (1 + np.array([math.erf(v / (math.sqrt(2) * sig)) for v in d.ravel()]).reshape(d.shape))
+    assert np.max(np.abs(t - want)) < 1e-14
+
+
+def test_diffuser_statistics_anchoring_and_correlation(oracle):
+    n, dx = 128, 1e-6
+    rng = np.random.default_rng(2)
+    a = np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n))) * (1 + rng.random((n, n)))
+
+    def phase(corr, x0=0.0, sigma=0.3, seed=5):
+        f = oracle.OField(ones(n), dx=dx)
+        f.diffuser(sigma=sigma, corr=corr, amp=1.0, seed=seed, x0=x0)
+        return np.angle(f.ex)
+
+    # phase only at amp = 1, amplitude scales by amp
+    f = oracle.OField(a, dx=dx)
+    f.diffuser(sigma=1.0, corr=0.0, amp=1.0, seed=3)
+    assert np.max(np.abs(np.abs(f.ex) - np.abs(a))) <= 1e-14
+    f = oracle.OField(a, dx=dx)
+    f.diffuser(sigma=1.0, corr=3e-6, amp=0.5, seed=3)
+    assert np.max(np.abs(np.abs(f.ex) - 0.5 * np.abs(a))) <= 1e-14
+    # the screen is rescaled to zero mean and the requested standard deviation
+    for corr in (0.0, 4e-6):
+        p = phase(corr)
+        assert abs(p.mean()) < 1e-13 and abs(p.std() - 0.3) < 1e-12
+    # white noise is uncorrelated between neighbours, the smoothed screen is not
+    p0, p1 = phase(0.0), phase(5e-6)
+    rho = lambda p: np.corrcoef(p[:, :-1].ravel(), p[:, 1:].ravel())[0, 1]
+    assert abs(rho(p0)) < 0.05 and rho(p1) > 0.9
+    # the screen belongs to the lab lattice: an element moved by +5 cells shows
+    # the cells 5 columns further on, up to the window's own rescaling
+    ps = phase(0.0, x0=5 * dx)
+    A = np.vstack([p0[:, 5:].ravel(), np.ones(p0[:, 5:].size)]).T
+    coef, res, _, _ = np.linalg.lstsq(A, ps[:, :-5].ravel(), rcond=None)
+    assert abs(coef[0] - 1) < 0.05 and np.max(np.abs(A @ coef - ps[:, :-5].ravel())) < 1e-12
+
+
+def jones_polarizer(angle, t=1.0):
+    c, s = math.cos(angle), math.sin(angle)
+    return t * c * c, t * s * c, t * s * c, t * s * s
+
+
+def jones_retarder(d, th):
+    c, s = math.cos(th), math.sin(th)
+    e1, e2 = complex(math.cos(d / 2), math.sin(d / 2)), complex(math.cos(-d / 2), math.sin(-d / 2))
+    a = (c * c) * e1 + (s * s) * e2
+    b = (s * c) * (e1 - e2)
+    dd = (s * s) * e1 + (c * c) * e2
+    return a, b, b, dd
+
+
+def jones_rotator(th):
+    c, s = math.cos(th), math.sin(th)
+    return c, -s, s, c
+
+
+def test_jones_elements(oracle):
+    n = 32
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    # a scalar matrix keeps a scalar field scalar
+    f = oracle.OField(a)
+    f.jones(2j, 0, 0, 2j)
+    assert not f.polarized and np.max(np.abs(f.ex - 2j * a)) < 1e-14
+    # Malus: a polariser at theta passes cos^2 theta of x-polarised power
+    for th in (0.3, 1.1):
+        f = oracle.OField(a)
+        f.jones(*jones_polarizer(th))
+        assert f.polarized
+        p = np.sum(np.abs(f.ex) ** 2 + np.abs(f.ey) ** 2) / np.sum(np.abs(a) ** 2)
+        assert abs(p - math.cos(th) ** 2) < 1e-14
+    # a rotator turns x polarisation by theta and keeps the power
+    f = oracle.OField(a)
+    f.jones(*jones_rotator(0.4))
+    assert np.max(np.abs(f.ey - math.tan(0.4) * f.ex)) < 1e-13
+    assert abs(np.sum(np.abs(f.ex) ** 2 + np.abs(f.ey) ** 2) / np.sum(np.abs(a) ** 2) - 1) < 1e-14
+    # a quarter-wave plate at 45 degrees makes x polarisation circular
+    f = oracle.OField(a)
+    f.jones(*jones_retarder(math.pi / 2, math.pi / 4))
+    assert np.max(np.abs(np.abs(f.ex) - np.abs(f.ey))) < 1e-13
+    assert np.max(np.abs(np.abs(np.angle(f.ey / f.ex)) - math.pi / 2)) < 1e-12
+    # on a vector field the full 2x2 product
+    b = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    f = oracle.OField(a, b)
+    m = (1 + 2j, 0.5, -0.25j, 3)
+    f.jones(*m)
+    assert np.max(np.abs(f.ex - (m[0] * a + m[1] * b))) < 1e-13
+    assert np.max(np.abs(f.ey - (m[2] * a + m[3] * b))) < 1e-13
+    # amplitude scale on both components of a vector field, Ex only on a scalar one
+    f = oracle.OField(a, b)
+    f.scale(0.6j)
+    assert np.max(np.abs(f.ex - 0.6j * a)) < 1e-14 and np.max(np.abs(f.ey - 0.6j * b)) < 1e-14
+    f = oracle.OField(a)
+    f.scale(2)
+    assert np.array_equal(f.ey, np.zeros_like(a))
+
+
+def test_bandlimit_closed_forms(oracle):
+    n, dx = 128, 1e-6
+    x = (np.arange(n) - n / 2) * dx
+    # disabled outside (0, 1)
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    for fr in (1.0, 0.0, 1.5):
+        f = oracle.OField(a, dx=dx)
+        f.bandlimit(fr, 0.05)
+        assert np.array_equal(f.ex, a)
+    # a beam whose spectrum lies inside the cut-off passes unchanged
+    # (w = 8 um: the window edge holds exp(-64) of the peak, so no wrap-around tail)
+    g = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (8e-6) ** 2).astype(np.complex128)
+    f = oracle.OField(g, dx=dx)
+    f.bandlimit(0.9, 0.05)
+    assert np.max(np.abs(f.ex - g)) < 1e-12
+    # a single bin above the cut-off is scaled by exp(-((f - f0)/sigma)^2)
+    fnyq = 1 / (2 * dx)
+    for k in (58, 60):
+        fx = k / (n * dx)
+        w = np.exp(2j * np.pi * fx * x)[None, :] * np.ones((n, 1))
+        f = oracle.OField(w, dx=dx)
+        f.bandlimit(0.8, 0.05)
+        d = (fx - 0.8 * fnyq) / (0.05 * fnyq)
+        want = np.exp(-d * d) if fx > 0.8 * fnyq else 1.0
+        assert np.max(np.abs(f.ex - want * w)) < 1e-12, k
+    # a polarised field filters both components; a scalar one leaves Ey alone
+    f = oracle.OField(a, 2 * a, dx=dx)
+    f.bandlimit(0.5, 0.1)
+    assert np.max(np.abs(f.ey - 2 * f.ex)) < 1e-12

@@ -2374,3 +2374,4 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 #include "wos_adjoint.inc"
 #include "wos_pipeline.inc"
 #include "wos_stats.inc"
+#include "wos_field.inc"

@@ -39,7 +39,37 @@ EXPORTS = [
     "wos_hip_pipeline_download", "wos_hip_pipeline_download_source", "wos_hip_pipeline_intensity",
     "wos_hip_pipeline_time", "wos_hip_compute_stats", "wos_hip_batch_stats", "wos_hip_pipeline_stats",
     "wos_hip_bpm_mask_create", "wos_hip_bpm_mask_destroy", "wos_hip_bpm_mask_forward", "wos_hip_bpm_mask_backward",
+    "wos_hip_field_create", "wos_hip_field_destroy", "wos_hip_field_clone", "wos_hip_field_upload",
+    "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
+    "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
+    "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats",
 ]
+
+SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
+               "double_slit": 7, "cross": 8, "polygon": 9}
+
+
+class Shape(C.Structure):
+    """wos_shape: a parsed aperture outline (optics/shape.go:28-48)."""
+    _fields_ = [("kind", C.c_int32), ("n_verts", C.c_int32)] + \
+        [(k, C.c_double) for k in ("radius", "width", "height", "a", "b", "order", "rin", "rout", "inner",
+                                   "length", "separation", "rotation", "edge_sigma")] + [("verts", C.c_void_p)]
+
+
+def make_shape(kind, verts=None, **params):
+    """Shape of `kind` ("circle" ... "cross", or "polygon" with an (m, 2)
+    vertex array); unset parameters take the reference's defaults
+    (optics/shape.go:51-55)."""
+    d = {"radius": 1e-3, "width": 2e-3, "height": 2e-3, "a": 1e-3, "b": 2e-3, "order": 2, "rin": 5e-4,
+         "rout": 1e-3, "inner": 5e-4, "length": 4e-3, "separation": 1e-3, "rotation": 0, "edge_sigma": 0}
+    d.update(params)
+    g = Shape(kind=SHAPE_KINDS[kind], n_verts=0, **d)
+    if verts is not None:
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        g.n_verts = v.shape[0]
+        g.verts = v.ctypes.data
+        g._keep = v
+    return g
 
 
 class WosError(RuntimeError):
@@ -151,6 +181,22 @@ def load(path=None):
     L.wos_hip_bpm_mask_destroy.argtypes = [P]
     L.wos_hip_bpm_mask_forward.argtypes = [P, P, P, P]
     L.wos_hip_bpm_mask_backward.argtypes = [P, P, P]
+    L.wos_hip_field_create.argtypes = [P, I, D, I, I]
+    L.wos_hip_field_destroy.argtypes = [P]
+    L.wos_hip_field_clone.argtypes = [P, P]
+    L.wos_hip_field_upload.argtypes = [P, P, P]
+    L.wos_hip_field_download.argtypes = [P, P, P]
+    L.wos_hip_field_polarized.argtypes = [P]
+    L.wos_hip_field_device_ptr.argtypes = [P]
+    L.wos_hip_field_device_ptr.restype = P
+    L.wos_hip_field_propagate.argtypes = [P, D, I, C.POINTER(Ctx), C.POINTER(Warnings), C.POINTER(PropStats)]
+    L.wos_hip_field_lens.argtypes = [P, D, D, D, D, D]
+    L.wos_hip_field_aperture.argtypes = [P, C.POINTER(Shape), D, D]
+    L.wos_hip_field_diffuser.argtypes = [P, D, D, D, C.c_int64, D, D]
+    L.wos_hip_field_jones.argtypes = [P, P]
+    L.wos_hip_field_scale.argtypes = [P, D, D]
+    L.wos_hip_field_bandlimit.argtypes = [P, D, D]
+    L.wos_hip_field_stats.argtypes = [P, D, D, D, C.POINTER(PlaneStats)]
     _lib = L
     return L
 
@@ -219,6 +265,89 @@ def propagate(ex, ey, dx, z, method, ctx, precision=FP64):
     _check(load().wos_hip_propagate(_p(ex), _p(ey), ex.shape[0], dx, z, m, C.byref(ctx), precision,
                                     C.byref(w), C.byref(st)), "propagate")
     return ex, ey, w, st
+
+
+class Field:
+    """Device-resident Field (wos_field): the element chain of a train runs
+    on it without host round trips."""
+
+    def __init__(self, n, dx, polarized=False, precision=FP64, _handle=None):
+        self.n, self.dx, self.precision = n, dx, precision
+        if _handle is not None:
+            self.h = _handle
+            return
+        h = C.c_void_p()
+        _check(load().wos_hip_field_create(C.byref(h), n, dx, int(polarized), precision), "field_create")
+        self.h = h
+
+    @classmethod
+    def from_host(cls, ex, ey=None, dx=1e-6, polarized=None, precision=FP64):
+        ex = _c128(ex)
+        f = cls(ex.shape[0], dx, (ey is not None) if polarized is None else polarized, precision)
+        f.upload(ex, ey)
+        return f
+
+    def close(self):
+        if self.h:
+            _check(load().wos_hip_field_destroy(self.h), "field_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def polarized(self):
+        return bool(load().wos_hip_field_polarized(self.h))
+
+    def clone(self):
+        h = C.c_void_p()
+        _check(load().wos_hip_field_clone(self.h, C.byref(h)), "field_clone")
+        return Field(self.n, self.dx, precision=self.precision, _handle=h)
+
+    def upload(self, ex, ey=None):
+        ex = _c128(ex)
+        ey = None if ey is None else _c128(ey)
+        _check(load().wos_hip_field_upload(self.h, _p(ex), _p(ey)), "field_upload")
+
+    def download(self):
+        ex = np.zeros((self.n, self.n), dtype=np.complex128)
+        ey = np.zeros_like(ex)
+        _check(load().wos_hip_field_download(self.h, _p(ex), _p(ey)), "field_download")
+        return ex, ey
+
+    def propagate(self, z, method, ctx):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        w, st = Warnings(), PropStats()
+        _check(load().wos_hip_field_propagate(self.h, z, m, C.byref(ctx), C.byref(w), C.byref(st)), "field_propagate")
+        return w, st
+
+    def lens(self, wl, f, aperture=0.0, x0=0.0, y0=0.0):
+        _check(load().wos_hip_field_lens(self.h, wl, f, aperture, x0, y0), "field_lens")
+
+    def aperture(self, g, x0=0.0, y0=0.0):
+        _check(load().wos_hip_field_aperture(self.h, C.byref(g), x0, y0), "field_aperture")
+
+    def diffuser(self, sigma=np.pi, corr=2e-5, amp=1.0, seed=1, x0=0.0, y0=0.0):
+        _check(load().wos_hip_field_diffuser(self.h, sigma, corr, amp, seed, x0, y0), "field_diffuser")
+
+    def jones(self, a, b, c, d):
+        m = np.array([complex(v) for v in (a, b, c, d)], dtype=np.complex128)
+        _check(load().wos_hip_field_jones(self.h, _p(m)), "field_jones")
+
+    def scale(self, s):
+        s = complex(s)
+        _check(load().wos_hip_field_scale(self.h, s.real, s.imag), "field_scale")
+
+    def bandlimit(self, fraction, sigma):
+        _check(load().wos_hip_field_bandlimit(self.h, fraction, sigma), "field_bandlimit")
+
+    def stats(self, wl, strehl_aperture=0.0, strehl_distance=0.0):
+        st = PlaneStats()
+        _check(load().wos_hip_field_stats(self.h, wl, strehl_aperture, strehl_distance, C.byref(st)), "field_stats")
+        return st
 
 
 def compute_stats(ex, ey, dx, wl, strehl_aperture=0.0, strehl_distance=0.0):
