@@ -145,7 +145,7 @@ def bench_bpm(wos):
     return out, check20
 
 
-def bench_pipeline(wos, steps):
+def bench_pipeline(wos, steps, tune=True):
     """One hipGraph: 64 x 2048^2 polychromatic intensity, 128 z-planes at
     1024^2, 32 mixed fields (384^2, 1000^2, 1536^2); Gaussian-Schell inputs."""
     out, res = {}, {}
@@ -161,7 +161,8 @@ def bench_pipeline(wos, steps):
             cnt = len(range(k, 32, 3))
             s = p.add(n, 4.096e-3 / n, np.full(cnt, 532e-9), np.full(cnt, 20e-3), ctx, "asm")
             p.fill_schell(s, W0, SIGMA_C, SEED, 2 + 100 * k)
-        p.autotune(3)
+        if tune:
+            p.autotune(3)
         p.capture()
         ms, st = p.time(2, max(steps, 10), stages=True)
         p.run(1)
@@ -227,7 +228,7 @@ def config12_inputs():
             "config2": (c2, np.zeros_like(c2), n2, dx2, 532e-9, 10e-3, "asm_pad")}
 
 
-def bench_config12(wos):
+def bench_config12(wos, tune=True):
     """Configurations 1 and 2 as device-resident batches (one unit per field
     component, replayed from a hipGraph) and as drop-in Propagate calls on
     host buffers. Algorithmic bytes per component: 8 M^2 S (M = 2n for
@@ -239,7 +240,8 @@ def bench_config12(wos):
         res = {"grid": n, "method": method, "components": len(comps)}
         for pname, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
             b = wos.Batch(n, dx, [wl] * len(comps), [z] * len(comps), wos.ctx_for(wl), method, prec)
-            b.autotune(3)
+            if tune:
+                b.autotune(3)
             for u, f in enumerate(comps):
                 b.upload(u, f)
             b.capture()
@@ -443,6 +445,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline batch (no other workloads)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="extra workloads at their default launch shapes (profiling: no autotuner launches)")
     ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split")
     ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split")
     args = ap.parse_args()
@@ -520,11 +524,11 @@ def main():
         "fft2d_z_2048_ms": round(fft2d_dropin_ms(wos, r64["unit0_in"]), 3),
         "fft2d_note": "one complex128 2-D FFT of a host buffer per call, transfers included; the reference "
                       "publishes 45 ms for this call through cuFFT on a Tesla T4 (README.md:113)"}
-    cfg12, fields = bench_config12(wos)
+    cfg12, fields = bench_config12(wos, not args.no_tune)
     line["config1_512_asm"] = cfg12["config1"]
     line["config2_1000_asm_pad_jones"] = cfg12["config2"]
     line["bpm_1024_400_slices"], bpm20 = bench_bpm(wos)
-    line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps)
+    line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps, not args.no_tune)
     line["mask_adjoint"], adj = bench_adjoint(wos)
     line["element_train_2048"], train_in, train_out64 = bench_train(wos)
     if not args.no_cpu:

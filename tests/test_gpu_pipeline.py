@@ -166,3 +166,45 @@ def test_config4_heterogeneous_pipeline(wos, oracle):
     print(f"config4 FP32 vs FP64: intensity {e_poly:.2e}, volume planes {e_vol:.2e}, mixed fields {e_mix:.2e}",
           flush=True)
     assert e_poly <= FP32_TOL and e_vol <= FP32_TOL and e_mix <= FP32_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_capture_before_any_run(wos, precision):
+    """A batch or pipeline captured before any eager run, with the twiddle and
+    chirp caches emptied first (so every table is created for it: outside the
+    capture, which allows no allocation), replays exactly like an eager run."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    ctx = wos.ctx_for(550e-9)
+    for n, method in ((512, "asm"), (100, "asm_pad")):
+        srcs = [random_field(n, 40 + u) for u in range(2)]
+        outs = []
+        for capture in (True, False):
+            assert wos.load().wos_hip_release_cache() == 0
+            b = wos.Batch(n, 2e-6, [500e-9, 640e-9], [1e-3, 3e-3], ctx, method, prec)
+            for u, f in enumerate(srcs):
+                b.upload(u, f)
+            if capture:
+                b.capture()
+            b.run(1)
+            b.sync()
+            outs.append([b.download(u) for u in range(2)])
+            b.close()
+        for a, e in zip(*outs):
+            assert np.array_equal(a, e), (n, method)
+    outs = []
+    for capture in (True, False):
+        assert wos.load().wos_hip_release_cache() == 0
+        p = wos.Pipeline(prec)
+        s = p.add(256, 2e-6, np.array([450e-9, 600e-9]), np.array([2e-3, 0.0]), ctx, "asm", src_units=1)
+        p.set_intensity(s, [1.0, 0.5])
+        p.upload(s, 0, random_field(256, 50))
+        s2 = p.add(100, 2e-6, np.array([532e-9]), np.array([1e-3]), ctx, "asm_pad")
+        p.upload(s2, 0, random_field(100, 51))
+        if capture:
+            p.capture()
+        p.run(1)
+        p.sync()
+        outs.append((p.intensity(s), p.download(s, 1), p.download(s2, 0)))
+        p.close()
+    for a, e in zip(*outs):
+        assert np.array_equal(a, e)

@@ -340,6 +340,9 @@ int get_twiddles(int n, int prec, const void** out, int acc) {
       hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, q.off, q.ns, q.rad, acc);
     WOS_HIP_TRY(hipGetLastError());
   }
+  // consumers run on other streams (batches, pipelines): the table is
+  // complete before its first use
+  WOS_HIP_TRY(hipStreamSynchronize(d.stream));
   d.twiddles[key] = p;
   *out = p;
   return 0;
@@ -2297,6 +2300,11 @@ int wos_hip_batch_capture(wos_batch b) {
   if (!b) return WOS_ERR_INVALID;
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
   batch_drop_graph(b);
+  {  // twiddle and chirp tables are created (allocated) outside the capture
+    TransferLaunch L;
+    int rc0 = batch_build(b, L);
+    if (rc0) return rc0;
+  }
   WOS_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = batch_enqueue(b);
   hipGraph_t g = nullptr;
