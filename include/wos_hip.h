@@ -47,6 +47,9 @@ extern "C" {
 #define WOS_METHOD_ASM_SHIFT_PAD 3
 #define WOS_METHOD_FRESNEL_TF 4
 #define WOS_METHOD_FRESNEL_IR 5
+/* propFraunhofer (optics/propagate.go:603-662), even n: the output grid
+ * spacing is wl |z| / (n dx), which the caller sets as the new Field.DX */
+#define WOS_METHOD_FRAUNHOFER 6
 
 /* ---- warning codes (optics/propagate.go:203-264, 515-518, 586-589) ---- */
 #define WOS_WARN_EVANESCENT_FILTERED 0
@@ -55,6 +58,7 @@ extern "C" {
 #define WOS_WARN_ASM_ALIAS_WRAP 3
 #define WOS_WARN_FRESNEL_TF_ALIAS 4
 #define WOS_WARN_FRESNEL_IR_ALIAS 5
+#define WOS_WARN_FRAUNHOFER_NEARFIELD 6
 #define WOS_WARN_SLOTS 8
 
 /* The Context fields the path reads (optics/field.go:237-254). */
@@ -278,6 +282,8 @@ int wos_hip_field_download(wos_field f, double* ex, double* ey);
 int wos_hip_field_polarized(wos_field f);
 /* device pointer of Ex (Ey follows n*n elements later) */
 void* wos_hip_field_device_ptr(wos_field f);
+/* Field.DX: the grid spacing, changed by a fraunhofer propagation */
+double wos_hip_field_dx(wos_field f);
 /* Propagate (optics/propagate.go:65-102) on the resident field, Ey only if
  * polarized; warnings and stats as wos_hip_propagate */
 int wos_hip_field_propagate(wos_field f, double z, int method, const wos_ctx* ctx, wos_warnings* w,

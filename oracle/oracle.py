@@ -13,12 +13,12 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-ASM, ASM_PAD, ASM_SHIFT, ASM_SHIFT_PAD, FRESNEL_TF, FRESNEL_IR = range(6)
+ASM, ASM_PAD, ASM_SHIFT, ASM_SHIFT_PAD, FRESNEL_TF, FRESNEL_IR, FRAUNHOFER = range(7)
 METHODS = {"asm": ASM, "asm_pad": ASM_PAD, "asm_shift": ASM_SHIFT,
            "asm_shift_pad": ASM_SHIFT_PAD, "fresnel_tf": FRESNEL_TF,
-           "fresnel_ir": FRESNEL_IR}
+           "fresnel_ir": FRESNEL_IR, "fraunhofer": FRAUNHOFER}
 WARN_CODES = ["evanescent_filtered", "backward_regularized", "backward_evanescent",
-              "asm_alias_wrap", "fresnel_tf_alias", "fresnel_ir_alias"]
+              "asm_alias_wrap", "fresnel_tf_alias", "fresnel_ir_alias", "fraunhofer_nearfield"]
 
 
 class Ctx(C.Structure):
@@ -148,6 +148,8 @@ class OField:
         self.ex = np.ascontiguousarray(ex)
         if self.polarized:
             self.ey = np.ascontiguousarray(ey)
+        if method == "fraunhofer" and z != 0:  # the far-field grid (optics/propagate.go:661)
+            self.dx = ctx.wavelength * abs(z) / (self.n * self.dx)
         return w, st
 
 

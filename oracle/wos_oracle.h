@@ -42,6 +42,7 @@ enum {
     OR_WARN_ASM_ALIAS_WRAP = 3,
     OR_WARN_FRESNEL_TF_ALIAS = 4,
     OR_WARN_FRESNEL_IR_ALIAS = 5,
+    OR_WARN_FRAUNHOFER_NEARFIELD = 6,
     OR_WARN_COUNT = 8
 };
 
@@ -55,7 +56,7 @@ typedef struct {
 /* Method codes, same as WOS_METHOD_* in include/wos_hip.h. */
 enum {
     OR_ASM = 0, OR_ASM_PAD = 1, OR_ASM_SHIFT = 2, OR_ASM_SHIFT_PAD = 3,
-    OR_FRESNEL_TF = 4, OR_FRESNEL_IR = 5
+    OR_FRESNEL_TF = 4, OR_FRESNEL_IR = 5, OR_FRAUNHOFER = 6
 };
 
 /* Per-call diagnostics of one Propagate (last call for loops). */

@@ -159,3 +159,40 @@ def test_train_matches_oracle(wos, oracle, n):
         assert abs(sg.power - so.power) <= (1e-12 if prec == wos.FP64 else 1e-5) * so.power
         f.close()
         arm.close()
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+def test_fraunhofer_matches_oracle(wos, oracle, polarized):
+    """Propagate with method fraunhofer: the drop-in call and the resident
+    field against the oracle (far field at 0.5 m, then a near-field case that
+    raises fraunhofer_nearfield); the field's grid spacing becomes wl z/(n dx)
+    and a following asm propagation runs on it."""
+    n, dx, wl = 256, 2e-6, 633e-9
+    ex, ey = field(n, 21, polarized)
+    for z in (0.5, 2e-3):
+        rx, ry, wo, _ = oracle.propagate(ex, ey, dx, z, "fraunhofer", oracle.ctx_for(wl))
+        ref = np.concatenate([rx, ry]) if polarized else rx
+        for prec, tol in ((wos.FP64, FP64_TOL), (wos.FP32, FP32_TOL)):
+            gx, gy, wg, _ = wos.propagate(ex, ey, dx, z, "fraunhofer", wos.ctx_for(wl), precision=prec)
+            got = np.concatenate([gx, gy]) if polarized else gx
+            e = rel_l2(got, ref)
+            print(f"fraunhofer z={z} polarized={polarized} prec={prec}: rel-L2 {e:.2e}, warnings {wg.as_dict()}",
+                  flush=True)
+            assert e <= tol
+            assert wg.as_dict().keys() == wo.as_dict().keys()
+            for key, (c, v) in wo.as_dict().items():
+                assert wg.as_dict()[key][0] == c and abs(wg.as_dict()[key][1] - v) <= 1e-9 * abs(v)
+    o = oracle.OField(ex, ey, dx=dx)
+    o.propagate(0.5, "fraunhofer", oracle.ctx_for(wl))
+    o.propagate(0.02, "asm", oracle.ctx_for(wl))
+    f = wos.Field.from_host(ex, ey, dx=dx)
+    f.propagate(0.5, "fraunhofer", wos.ctx_for(wl))
+    assert abs(f.grid_dx - o.dx) <= 1e-18
+    f.propagate(0.02, "asm", wos.ctx_for(wl))
+    gx, gy = f.download()
+    got = np.concatenate([gx, gy]) if polarized else gx
+    ref = np.concatenate([o.ex, o.ey]) if polarized else o.ex
+    assert rel_l2(got, ref) <= FP64_TOL
+    f.close()
+    with pytest.raises(wos.WosError):
+        wos.propagate(np.ones((9, 9)), None, dx, 0.5, "fraunhofer", wos.ctx_for(wl))

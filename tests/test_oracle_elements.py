@@ -227,3 +227,34 @@ def test_bandlimit_closed_forms(oracle):
     f = oracle.OField(a, 2 * a, dx=dx)
     f.bandlimit(0.5, 0.1)
     assert np.max(np.abs(f.ey - 2 * f.ex)) < 1e-12
+
+
+def test_fraunhofer_gaussian_far_field(oracle):
+    """propFraunhofer of a tilted Gaussian equals the closed-form far field
+    U(X, Y) = e^{ikz} (-i/(wl z)) e^{ik(X^2+Y^2)/(2z)} pi w0^2
+    exp(-pi^2 w0^2 ((X/(wl z) - f0)^2 + (Y/(wl z))^2)) on the grid of spacing
+    wl z / (n dx), and warns only in the near field (Fresnel number > 0.5)."""
+    n, dx, wl, w0, z = 256, 2e-6, 0.5e-6, 40e-6, 1.0
+    x = (np.arange(n) - n / 2) * dx
+    f0 = 3 / (n * dx)
+    ex = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / w0 ** 2) * np.exp(2j * np.pi * f0 * x)[None, :]
+    f = oracle.OField(ex, dx=dx)
+    w, _ = f.propagate(z, "fraunhofer", oracle.ctx_for(wl))
+    dxo = wl * z / (n * dx)
+    assert abs(f.dx - dxo) <= 1e-18
+    X = (np.arange(n) - n / 2) * dxo
+    k = 2 * np.pi / wl
+    fx, fy = X[None, :] / (wl * z), X[:, None] / (wl * z)
+    want = (np.exp(1j * k * z) * (-1j / (wl * z)) * np.exp(1j * k * (X[None, :] ** 2 + X[:, None] ** 2) / (2 * z))
+            * np.pi * w0 ** 2 * np.exp(-np.pi ** 2 * w0 ** 2 * ((fx - f0) ** 2 + fy ** 2)))
+    err = np.sqrt(np.sum(np.abs(f.ex - want) ** 2) / np.sum(np.abs(want) ** 2))
+    assert err < 1e-8, err
+    assert not w.as_dict()
+    # near field: F = d^2 / (wl z) with d = 2 rms radius = sqrt(2) w0 for this beam
+    g = oracle.OField(ex, dx=dx)
+    w, _ = g.propagate(1e-3, "fraunhofer", oracle.ctx_for(wl))
+    cnt, val = w.as_dict()["fraunhofer_nearfield"]
+    assert cnt == 1 and abs(val / (2 * w0 ** 2 / (wl * 1e-3)) - 1) < 1e-6
+    # odd grids are refused (the reference's shift leaves a stale column there)
+    with pytest.raises(ValueError):
+        oracle.propagate(np.ones((9, 9)), None, dx, z, "fraunhofer", oracle.ctx_for(wl))

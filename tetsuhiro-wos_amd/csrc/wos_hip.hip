@@ -1276,6 +1276,43 @@ static int ramp_dev(int prec, void* d, int n, int units, double dx, double fxc, 
   return 0;
 }
 
+// propFraunhofer's spectrum post-processing (optics/propagate.go:603-650) for
+// `comps` spectra of n x n (n even): output (i, j) takes bin
+// ((i + n/2) % n, (j + n/2) % n) (the fftshift), times dx^2, negated for odd
+// bin parity (the centred-layout checkerboard), times
+// c0 * cexpI(k (x^2 + y^2) / (2 z)) on the output grid of spacing dxo, with
+// c0 = cexpI(k z) * complex(0, -1/(wl z)) formed on the host.
+template <typename R>
+__global__ void k_fraunhofer(const cpx<R>* __restrict__ spec, cpx<R>* __restrict__ out, int n, int comps, double dx2,
+                             double dxo, double k, double z, double c0r, double c0i) {
+#pragma clang fp contract(off)
+  const size_t nn = (size_t)n * n;
+  const int half = n / 2;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nn * comps; g += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = g / nn, idx = g % nn;
+    const int j = (int)(idx / n), i = (int)(idx % n);
+    const int sj = (j + half) % n, si = (i + half) % n;
+    const cpx<R> a = spec[c * nn + (size_t)sj * n + si];
+    double ar = (double)a.x * dx2 - (double)a.y * 0.0, ai = (double)a.x * 0.0 + (double)a.y * dx2;
+    if ((si + sj) & 1) {
+      ar = -ar;
+      ai = -ai;
+    }
+    const double y = ((double)j - (double)n / 2) * dxo;
+    const double x = ((double)i - (double)n / 2) * dxo;
+    double sq, cq;
+    sincos(k * (x * x + y * y) / (2 * z), &sq, &cq);
+    const double tr = c0r * cq - c0i * sq, ti = c0r * sq + c0i * cq;
+    out[g] = {(R)(ar * tr - ai * ti), (R)(ar * ti + ai * tr)};
+  }
+}
+
+}  // namespace wos
+// ComputeStats on device buffers (wos_stats.inc)
+static int stats_dev(int prec, const void* ex, const void* ey, int n, double dx, double wl, double ap, double dist,
+                     wos_plane_stats* out);
+namespace wos {
+
 // Propagate (optics/propagate.go:65-102) of one field (ncomp components,
 // device resident) by one FFT method.
 static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double z, int method, const wos_ctx& ctx,
@@ -1392,6 +1429,43 @@ static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double 
       std::vector<TFUnit> tf(ncomp, u);
       if ((rc = transfer_dev(prec, d, n, ncomp, true, tf, ncomp, pr, pi, nullptr))) return rc;
       if (std::fabs(z) < (double)n * dx * dx / wl) warn_add(w, WOS_WARN_FRESNEL_IR_ALIAS, 0);
+      return 0;
+    }
+    case WOS_METHOD_FRAUNHOFER: {
+      // propFraunhofer (optics/propagate.go:603-662); the output spacing
+      // wl |z| / (n dx) is the caller's new Field.DX
+      if (n & 1) {
+        set_error("fraunhofer: odd grid sizes are not supported");
+        return WOS_ERR_UNSUPPORTED;
+      }
+      const size_t nn = (size_t)n * n;
+      const double k = 2 * M_PI / wl;
+      const double dxo = wl * std::fabs(z) / ((double)n * dx);
+      // rmsRadius of the input (optics/propagate.go:105-130) = 2 sqrt(rms_x^2 + rms_y^2)
+      wos_plane_stats ps;
+      if ((rc = stats_dev(prec, d, ncomp > 1 ? (const char*)d + nn * csize(prec) : nullptr, n, dx, wl, 0, 0, &ps)))
+        return rc;
+      const double d_in = ps.power > 0 ? 2 * std::sqrt(ps.rms_x * ps.rms_x + ps.rms_y * ps.rms_y) : 0.0;
+      if ((rc = fft2d_dev(prec, d, n, ncomp, 0))) return rc;
+      Device& dv = device();
+      if ((rc = dv.work.ensure(nn * ncomp * csize(prec)))) return rc;
+      // c0 = cexpI(k z) * complex(0, -1/(wl z)), Go complex product order
+      const double m = -1 / (wl * z);
+      const double ckz = std::cos(k * z), skz = std::sin(k * z);
+      const double c0r = ckz * 0.0 - skz * m, c0i = ckz * m + skz * 0.0;
+      const unsigned gs = std::min<unsigned>(grid_for(nn * ncomp), 8192);
+      if (prec == WOS_FP64)
+        hipLaunchKernelGGL(k_fraunhofer<double>, dim3(gs), dim3(256), 0, dv.stream, (const cpx<double>*)d,
+                           (cpx<double>*)dv.work.p, n, ncomp, dx * dx, dxo, k, z, c0r, c0i);
+      else
+        hipLaunchKernelGGL(k_fraunhofer<float>, dim3(gs), dim3(256), 0, dv.stream, (const cpx<float>*)d,
+                           (cpx<float>*)dv.work.p, n, ncomp, dx * dx, dxo, k, z, c0r, c0i);
+      WOS_HIP_TRY(hipGetLastError());
+      WOS_HIP_TRY(hipMemcpyAsync(d, dv.work.p, nn * ncomp * csize(prec), hipMemcpyDeviceToDevice, dv.stream));
+      if (d_in > 0) {
+        const double nf = d_in * d_in / (wl * std::fabs(z));
+        if (nf > 0.5) warn_add(w, WOS_WARN_FRAUNHOFER_NEARFIELD, nf);
+      }
       return 0;
     }
     default:

@@ -19,9 +19,9 @@ LIB_PATH = os.path.join(_HERE, "libwos_hip.so")
 
 FP64, FP32 = 0, 1
 METHODS = {"asm": 0, "asm_pad": 1, "asm_shift": 2, "asm_shift_pad": 3,
-           "fresnel_tf": 4, "fresnel_ir": 5}
+           "fresnel_tf": 4, "fresnel_ir": 5, "fraunhofer": 6}
 WARN_CODES = ["evanescent_filtered", "backward_regularized", "backward_evanescent",
-              "asm_alias_wrap", "fresnel_tf_alias", "fresnel_ir_alias"]
+              "asm_alias_wrap", "fresnel_tf_alias", "fresnel_ir_alias", "fraunhofer_nearfield"]
 EXPORTS = [
     "wos_hip_device_count", "wos_hip_device_name", "wos_hip_init", "wos_hip_last_error",
     "wos_hip_version", "wos_hip_release_cache", "wos_hip_fft2d_z", "wos_hip_fft1d_z",
@@ -42,7 +42,7 @@ EXPORTS = [
     "wos_hip_field_create", "wos_hip_field_destroy", "wos_hip_field_clone", "wos_hip_field_upload",
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
-    "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats",
+    "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
 ]
 
 SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
@@ -197,6 +197,8 @@ def load(path=None):
     L.wos_hip_field_scale.argtypes = [P, D, D]
     L.wos_hip_field_bandlimit.argtypes = [P, D, D]
     L.wos_hip_field_stats.argtypes = [P, D, D, D, C.POINTER(PlaneStats)]
+    L.wos_hip_field_dx.argtypes = [P]
+    L.wos_hip_field_dx.restype = D
     _lib = L
     return L
 
@@ -301,6 +303,11 @@ class Field:
     @property
     def polarized(self):
         return bool(load().wos_hip_field_polarized(self.h))
+
+    @property
+    def grid_dx(self):
+        """Field.DX on the device (a fraunhofer propagation changes it)."""
+        return load().wos_hip_field_dx(self.h)
 
     def clone(self):
         h = C.c_void_p()
