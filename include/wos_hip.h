@@ -126,6 +126,14 @@ int wos_hip_fft1d_c(float* a, int n, int inverse);
 int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int method,
                       const wos_ctx* ctx, int precision, wos_warnings* w, wos_prop_stats* st);
 
+/* PropagateVectorial (optics/vectorial.go:14-80), Propagate's "vectorial"
+ * method: Ex and Ey (both complex128 n*n, Ey zeros for a field that had none)
+ * propagate in place under exp(i kz z) and the evanescent policy; ez receives
+ * the longitudinal component -(kx Ex + ky Ey)/kz of the propagated field.
+ * Emits no warnings, as the reference. */
+int wos_hip_propagate_vectorial(double* ex, double* ey, double* ez, int n, double dx, double z, const wos_ctx* ctx,
+                                int precision);
+
 /* Propagate1D (optics/propagate.go:718-738): 1-D ASM of a length-n complex128 array. */
 int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const wos_ctx* ctx,
                         int precision, wos_warnings* w);

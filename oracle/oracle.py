@@ -200,6 +200,7 @@ def lib():
         L.or_schell_realization.argtypes = [P, P, I, D, D, C.c_uint64, C.c_uint64]
         L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
         L.or_apply_lens_at.argtypes = [P, P, I, I, D, D, D, D, D, D]
+        L.or_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx)]
         L.or_shape_signed_distance.argtypes = [C.POINTER(Shape), D, D]
         L.or_shape_signed_distance.restype = D
         L.or_regular_polygon.argtypes = [I, D, D, P]
@@ -272,6 +273,15 @@ def propagate(ex, ey, dx, z, method, ctx):
     if rc != 0:
         raise ValueError(f"or_propagate rc={rc}")
     return ex, ey, w, st
+
+
+def propagate_vectorial(ex, ey, dx, z, ctx):
+    """PropagateVectorial on copies; ey None: zeros. Returns (ex, ey, ez)."""
+    ex = _c128(ex).copy()
+    ey = np.zeros_like(ex) if ey is None else _c128(ey).copy()
+    ez = np.zeros_like(ex)
+    lib().or_propagate_vectorial(_p(ex), _p(ey), _p(ez), ex.shape[0], dx, z, C.byref(ctx))
+    return ex, ey, ez
 
 
 def propagate1d(a, dx, z, wl, ctx):

@@ -713,6 +713,74 @@ int or_propagate(or_cplx* ex, or_cplx* ey, int n, double dx, double z, int metho
     return 0;
 }
 
+/* Go's complex division (runtime complex128div, Smith's algorithm), for the
+ * divisor forms of PropagateVectorial: (kz, 0) and (0, kzi). */
+static inline or_cplx cdiv(or_cplx n, or_cplx m) {
+    or_cplx r;
+    if (fabs(m.re) >= fabs(m.im)) {
+        double ratio = m.im / m.re;
+        double denom = m.re + ratio * m.im;
+        r.re = (n.re + n.im * ratio) / denom;
+        r.im = (n.im - n.re * ratio) / denom;
+    } else {
+        double ratio = m.re / m.im;
+        double denom = m.im + ratio * m.re;
+        r.re = (n.re * ratio + n.im) / denom;
+        r.im = (n.im * ratio - n.re) / denom;
+    }
+    return r;
+}
+
+/* PropagateVectorial (optics/vectorial.go:14-80) */
+void or_propagate_vectorial(or_cplx* ex, or_cplx* ey, or_cplx* ez, int n, double dx, double z,
+                            const or_ctx* ctx) {
+    double wl = ctx->wavelength;
+    double k = 2 * M_PI / wl;
+    double az = fabs(z);
+    int backward = z < 0;
+    int zero, reg;
+    double alpha, limit;
+    evanescent_policy(ctx, z, &zero, &reg, &alpha, &limit);
+    size_t nn = (size_t)n * n;
+    or_fft2d(ex, n, 0);
+    or_fft2d(ey, n, 0);
+    memset(ez, 0, nn * sizeof(or_cplx));
+    for (int j = 0; j < n; j++) {
+        double fy = or_freq(j, n, dx);
+        double ky = 2 * M_PI * fy;
+        for (int i = 0; i < n; i++) {
+            double fx = or_freq(i, n, dx);
+            double kx = 2 * M_PI * fx;
+            double rho = wl * or_hypot(fx, fy);
+            size_t idx = (size_t)j * n + i;
+            or_cplx H;
+            or_cplx s = cadd(cmul(cre(kx), ex[idx]), cmul(cre(ky), ey[idx]));
+            or_cplx ns = {-s.re, -s.im};
+            if (rho <= 1) {
+                double kz = k * sqrt((1 - rho) * (1 + rho));
+                H = cexpI(kz * z);
+                if (kz > 0) ez[idx] = cdiv(ns, cre(kz));
+            } else {
+                double decay = k * az * sqrt((rho - 1) * (rho + 1));
+                if (zero || (limit > 0 && decay > limit)) H = cre(0);
+                else if (backward && reg) {
+                    double a = exp(decay);
+                    H = cre(a / (1 + alpha * alpha * a * a));
+                } else if (backward) H = cre(exp(decay));
+                else H = cre(exp(-decay));
+                double kzi = k * sqrt((rho - 1) * (rho + 1));
+                if (kzi > 0) ez[idx] = cdiv(ns, (or_cplx){0.0, kzi});
+            }
+            ex[idx] = cmul(ex[idx], H);
+            ey[idx] = cmul(ey[idx], H);
+            ez[idx] = cmul(ez[idx], H);
+        }
+    }
+    or_fft2d(ex, n, 1);
+    or_fft2d(ey, n, 1);
+    or_fft2d(ez, n, 1);
+}
+
 /* Propagate1D (optics/propagate.go:718-738) */
 int or_propagate1d(or_cplx* a, int n, double dx, double z, double wl, const or_ctx* ctx,
                    or_warnings* w) {

@@ -196,3 +196,17 @@ def test_fraunhofer_matches_oracle(wos, oracle, polarized):
     f.close()
     with pytest.raises(wos.WosError):
         wos.propagate(np.ones((9, 9)), None, dx, 0.5, "fraunhofer", wos.ctx_for(wl))
+
+
+@pytest.mark.parametrize("n,dx,wl,z", [(128, 0.2e-6, 0.5e-6, 4e-6), (100, 1e-6, 633e-9, -20e-6), (256, 2e-6, 633e-9, 0.02)])
+def test_vectorial_matches_oracle(wos, oracle, n, dx, wl, z):
+    """PropagateVectorial (Ex, Ey in place, Ez out) against the oracle, with
+    evanescent bins (dx < wl/2), a backward step and a Bluestein grid."""
+    ex, ey = field(n, 31, True)
+    ctx_o, ctx_g = oracle.ctx_for(wl), wos.ctx_for(wl)
+    ref = np.concatenate(oracle.propagate_vectorial(ex, ey, dx, z, ctx_o))
+    for prec, tol in ((wos.FP64, FP64_TOL), (wos.FP32, FP32_TOL)):
+        got = np.concatenate(wos.propagate_vectorial(ex, ey, dx, z, ctx_g, precision=prec))
+        e = rel_l2(got, ref)
+        print(f"vectorial n={n} z={z} prec={prec}: rel-L2 {e:.2e}", flush=True)
+        assert e <= tol

@@ -258,3 +258,40 @@ def test_fraunhofer_gaussian_far_field(oracle):
     # odd grids are refused (the reference's shift leaves a stale column there)
     with pytest.raises(ValueError):
         oracle.propagate(np.ones((9, 9)), None, dx, z, "fraunhofer", oracle.ctx_for(wl))
+
+
+def test_vectorial_divergence_free_and_transverse(oracle):
+    """PropagateVectorial: Ex and Ey propagate like the scalar ASM (up to the
+    rounding order of the phase), the output satisfies kx Ex + ky Ey + kz Ez
+    = 0 on every bin (kz imaginary on evanescent bins), and a tilted plane
+    wave gets Ez = -(kx/kz) Ex."""
+    n, dx, wl, z = 64, 0.2e-6, 0.5e-6, 3e-6
+    rng = np.random.default_rng(9)
+    x = (np.arange(n) - n / 2) * dx
+    env = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (3e-6) ** 2)
+    ex = env * np.exp(2j * np.pi * rng.random((n, n)))
+    ey = 0.5 * env * np.exp(2j * np.pi * rng.random((n, n)))
+    ctx = oracle.ctx_for(wl)
+    vx, vy, vz = oracle.propagate_vectorial(ex, ey, dx, z, ctx)
+    sx, sy, _, _ = oracle.propagate(ex, ey, dx, z, "asm", ctx)
+    rel = lambda a, b: np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2))
+    assert rel(vx, sx) < 1e-12 and rel(vy, sy) < 1e-12
+    f = np.fft.fftfreq(n, dx)
+    kx, ky = 2 * np.pi * f[None, :], 2 * np.pi * f[:, None]
+    k = 2 * np.pi / wl
+    rho2 = (wl * f[None, :]) ** 2 + (wl * f[:, None]) ** 2
+    kz = np.where(rho2 <= 1, k * np.sqrt(np.abs(1 - rho2)) + 0j, 1j * k * np.sqrt(np.abs(rho2 - 1)))
+    Fx, Fy, Fz = np.fft.fft2(vx), np.fft.fft2(vy), np.fft.fft2(vz)
+    div = kx * Fx + ky * Fy + kz * Fz
+    scale = np.max(np.abs(kx * Fx) + np.abs(ky * Fy))
+    assert np.any(rho2 > 1) and np.max(np.abs(div)) / scale < 1e-12
+    # ... evanescent bins included, where Ez carries the decayed field
+    ev = rho2 > 1
+    assert np.max(np.abs(div[ev])) / np.max(np.abs(kx * Fx)[ev]) < 1e-9
+    # a tilted x-polarised plane wave (bin 5): Ez = -(kx / kz) Ex
+    fx = 5 / (n * dx)
+    w = np.exp(2j * np.pi * fx * x)[None, :] * np.ones((n, 1))
+    px, py, pz = oracle.propagate_vectorial(w, None, dx, z, ctx)
+    kxx = 2 * np.pi * fx
+    kzz = k * np.sqrt(1 - (wl * fx) ** 2)
+    assert np.max(np.abs(pz + (kxx / kzz) * px)) < 1e-12 and np.max(np.abs(py)) == 0

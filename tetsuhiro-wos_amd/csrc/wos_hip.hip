@@ -1307,6 +1307,59 @@ __global__ void k_fraunhofer(const cpx<R>* __restrict__ spec, cpx<R>* __restrict
   }
 }
 
+// PropagateVectorial's spectral step (optics/vectorial.go:34-75) on the
+// spectra of Ex, Ey: Ez = -(kx Ex + ky Ey)/kz (kz = i kzi on evanescent bins,
+// Go's complex division of a real or an imaginary divisor), then all three
+// times H = cexpI(kz z) or the evanescent policy's real factor.
+template <typename R>
+__global__ void k_vectorial(cpx<R>* __restrict__ ex, cpx<R>* __restrict__ ey, cpx<R>* __restrict__ ez, int n,
+                            double dx, double wl, double k, double z, int zero, int reg, double alpha, double limit) {
+#pragma clang fp contract(off)
+  const size_t nn = (size_t)n * n;
+  const double den = (double)n * dx;
+  const double az = fabs(z);
+  const bool backward = z < 0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n), i = (int)(idx % n);
+    const double fy = (double)(j <= n / 2 ? j : j - n) / den;
+    const double fx = (double)(i <= n / 2 ? i : i - n) / den;
+    const double ky = 2 * M_PI * fy, kx = 2 * M_PI * fx;
+    const double rho = wl * go_hypot(fx, fy);
+    const double exr = (double)ex[idx].x, exi = (double)ex[idx].y, eyr = (double)ey[idx].x, eyi = (double)ey[idx].y;
+    // -(complex(kx,0)*Ex + complex(ky,0)*Ey)
+    const double nr = -((kx * exr - 0.0 * exi) + (ky * eyr - 0.0 * eyi));
+    const double ni = -((kx * exi + 0.0 * exr) + (ky * eyi + 0.0 * eyr));
+    double hr, hi, zr = 0, zi = 0;
+    if (rho <= 1) {
+      const double kz = k * sqrt((1 - rho) * (1 + rho));
+      sincos(kz * z, &hi, &hr);
+      if (kz > 0) {  // n / complex(kz, 0)
+        const double ratio = 0.0 / kz, denom = kz + ratio * 0.0;
+        zr = (nr + ni * ratio) / denom;
+        zi = (ni - nr * ratio) / denom;
+      }
+    } else {
+      const double decay = k * az * sqrt((rho - 1) * (rho + 1));
+      if (zero || (limit > 0 && decay > limit)) hr = 0;
+      else if (backward && reg) {
+        const double a = exp(decay);
+        hr = a / (1 + alpha * alpha * a * a);
+      } else if (backward) hr = exp(decay);
+      else hr = exp(-decay);
+      hi = 0;
+      const double kzi = k * sqrt((rho - 1) * (rho + 1));
+      if (kzi > 0) {  // n / complex(0, kzi)
+        const double ratio = 0.0 / kzi, denom = kzi + ratio * 0.0;
+        zr = (nr * ratio + ni) / denom;
+        zi = (ni * ratio - nr) / denom;
+      }
+    }
+    ex[idx] = {(R)(exr * hr - exi * hi), (R)(exr * hi + exi * hr)};
+    ey[idx] = {(R)(eyr * hr - eyi * hi), (R)(eyr * hi + eyi * hr)};
+    ez[idx] = {(R)(zr * hr - zi * hi), (R)(zr * hi + zi * hr)};
+  }
+}
+
 }  // namespace wos
 // ComputeStats on device buffers (wos_stats.inc)
 static int stats_dev(int prec, const void* ex, const void* ey, int n, double dx, double wl, double ap, double dist,
@@ -1827,6 +1880,47 @@ int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int me
   return download_fields(precision, d, dst, ncomp, nn);
 }
 
+
+// PropagateVectorial (optics/vectorial.go:14-80)
+int wos_hip_propagate_vectorial(double* ex, double* ey, double* ez, int n, double dx, double z, const wos_ctx* ctx,
+                                int precision) {
+  WOS_ENTER();
+  if (!ex || !ey || !ez || n <= 0 || !ctx || !valid_prec(precision) || !(dx > 0) || !(ctx->wavelength > 0)) {
+    set_error("wos_hip_propagate_vectorial: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (!len_ok(n)) {
+    set_error("wos_hip_propagate_vectorial: grid size not supported: " + std::to_string(n));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const size_t nn = (size_t)n * n;
+  const double* src[2] = {ex, ey};
+  double* dst[2] = {ex, ey};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, 2, nn, &d))) return rc;
+  Device& dv = device();
+  if ((rc = dv.work.ensure(nn * csize(precision)))) return rc;
+  void* dz = dv.work.p;
+  if ((rc = fft2d_dev(precision, d, n, 2, 0))) return rc;
+  int zero, reg;
+  double alpha, limit;
+  evanescent_policy(*ctx, z, &zero, &reg, &alpha, &limit);
+  const double wl = ctx->wavelength, k = 2 * M_PI / wl;
+  const unsigned gs = std::min<unsigned>(grid_for(nn), 8192);
+  if (precision == WOS_FP64)
+    hipLaunchKernelGGL(k_vectorial<double>, dim3(gs), dim3(256), 0, dv.stream, (cpx<double>*)d,
+                       (cpx<double>*)d + nn, (cpx<double>*)dz, n, dx, wl, k, z, zero, reg, alpha, limit);
+  else
+    hipLaunchKernelGGL(k_vectorial<float>, dim3(gs), dim3(256), 0, dv.stream, (cpx<float>*)d, (cpx<float>*)d + nn,
+                       (cpx<float>*)dz, n, dx, wl, k, z, zero, reg, alpha, limit);
+  WOS_HIP_TRY(hipGetLastError());
+  if ((rc = fft2d_dev(precision, d, n, 2, 1))) return rc;
+  if ((rc = fft2d_dev(precision, dz, n, 1, 1))) return rc;
+  if ((rc = download_fields(precision, d, dst, 2, nn))) return rc;
+  double* dzh[1] = {ez};
+  return download_fields(precision, dz, dzh, 1, nn);
+}
 
 // Propagate1D (optics/propagate.go:718-738)
 int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const wos_ctx* ctx, int precision,

@@ -43,6 +43,7 @@ EXPORTS = [
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
+    "wos_hip_propagate_vectorial",
 ]
 
 SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
@@ -197,6 +198,7 @@ def load(path=None):
     L.wos_hip_field_scale.argtypes = [P, D, D]
     L.wos_hip_field_bandlimit.argtypes = [P, D, D]
     L.wos_hip_field_stats.argtypes = [P, D, D, D, C.POINTER(PlaneStats)]
+    L.wos_hip_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx), I]
     L.wos_hip_field_dx.argtypes = [P]
     L.wos_hip_field_dx.restype = D
     _lib = L
@@ -355,6 +357,16 @@ class Field:
         st = PlaneStats()
         _check(load().wos_hip_field_stats(self.h, wl, strehl_aperture, strehl_distance, C.byref(st)), "field_stats")
         return st
+
+
+def propagate_vectorial(ex, ey, dx, z, ctx, precision=FP64):
+    """PropagateVectorial on copies; ey None: zeros. Returns (ex, ey, ez)."""
+    ex = _c128(ex).copy()
+    ey = np.zeros_like(ex) if ey is None else _c128(ey).copy()
+    ez = np.zeros_like(ex)
+    _check(load().wos_hip_propagate_vectorial(_p(ex), _p(ey), _p(ez), ex.shape[0], dx, z, C.byref(ctx), precision),
+           "propagate_vectorial")
+    return ex, ey, ez
 
 
 def compute_stats(ex, ey, dx, wl, strehl_aperture=0.0, strehl_distance=0.0):
