@@ -126,6 +126,13 @@ int wos_hip_fft1d_c(float* a, int n, int inverse);
 int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int method,
                       const wos_ctx* ctx, int precision, wos_warnings* w, wos_prop_stats* st);
 
+/* PropagateUniaxial (optics/birefringent.go:19-61): a uniaxial crystal of
+ * thickness z, optic axis along x. Ex takes the extraordinary transfer
+ * function, Ey (NULL: scalar field, left alone) the ordinary one, at
+ * ctx->wavelength. no, ne <= 0 is an error, as in the reference. */
+int wos_hip_propagate_uniaxial(double* ex, double* ey, int n, double dx, double z, double no, double ne,
+                               const wos_ctx* ctx, int precision);
+
 /* PropagateVectorial (optics/vectorial.go:14-80), Propagate's "vectorial"
  * method: Ex and Ey (both complex128 n*n, Ey zeros for a field that had none)
  * propagate in place under exp(i kz z) and the evanescent policy; ez receives
@@ -292,6 +299,8 @@ int wos_hip_field_polarized(wos_field f);
 void* wos_hip_field_device_ptr(wos_field f);
 /* Field.DX: the grid spacing, changed by a fraunhofer propagation */
 double wos_hip_field_dx(wos_field f);
+/* uniaxialEl.Apply = PropagateUniaxial (optics/elements.go:1102-1104) on the resident field */
+int wos_hip_field_uniaxial(wos_field f, double z, double no, double ne, double wavelength);
 /* Propagate (optics/propagate.go:65-102) on the resident field, Ey only if
  * polarized; warnings and stats as wos_hip_propagate */
 int wos_hip_field_propagate(wos_field f, double z, int method, const wos_ctx* ctx, wos_warnings* w,

@@ -143,6 +143,10 @@ class OField:
     def bandlimit(self, fraction, sigma):
         lib().or_bandlimit(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, fraction, sigma)
 
+    def uniaxial(self, z, no, ne, wl):
+        lib().or_propagate_uniaxial(_p(self.ex), _p(self.ey) if self.polarized else None, self.n, self.dx, z, no,
+                                    ne, wl)
+
     def propagate(self, z, method, ctx):
         ex, ey, w, st = propagate(self.ex, self.ey if self.polarized else None, self.dx, z, method, ctx)
         self.ex = np.ascontiguousarray(ex)
@@ -201,6 +205,7 @@ def lib():
         L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
         L.or_apply_lens_at.argtypes = [P, P, I, I, D, D, D, D, D, D]
         L.or_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx)]
+        L.or_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, D]
         L.or_shape_signed_distance.argtypes = [C.POINTER(Shape), D, D]
         L.or_shape_signed_distance.restype = D
         L.or_regular_polygon.argtypes = [I, D, D, P]
@@ -273,6 +278,14 @@ def propagate(ex, ey, dx, z, method, ctx):
     if rc != 0:
         raise ValueError(f"or_propagate rc={rc}")
     return ex, ey, w, st
+
+
+def propagate_uniaxial(ex, ey, dx, z, no, ne, wl):
+    """PropagateUniaxial on copies (ey None: scalar field)."""
+    ex = _c128(ex).copy()
+    ey = None if ey is None else _c128(ey).copy()
+    lib().or_propagate_uniaxial(_p(ex), _p(ey), ex.shape[0], dx, z, no, ne, wl)
+    return ex, ey
 
 
 def propagate_vectorial(ex, ey, dx, z, ctx):

@@ -713,6 +713,34 @@ int or_propagate(or_cplx* ex, or_cplx* ey, int n, double dx, double z, int metho
     return 0;
 }
 
+/* PropagateUniaxial (optics/birefringent.go:19-61) */
+static void uniaxial_apply(or_cplx* a, int n, double dx, double z, double k0, double no, double ne, int extra) {
+    double az = fabs(z);
+    or_fft2d(a, n, 0);
+    for (int j = 0; j < n; j++) {
+        double fy = or_freq(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            double fx = or_freq(i, n, dx);
+            double arg;
+            if (extra) {
+                double kx = 2 * M_PI * fx, ky = 2 * M_PI * fy;
+                arg = k0 * k0 * ne * ne - (ne * ne / (no * no)) * kx * kx - ky * ky;
+            } else {
+                arg = k0 * k0 * no * no - (2 * M_PI * fx) * (2 * M_PI * fx) - (2 * M_PI * fy) * (2 * M_PI * fy);
+            }
+            or_cplx h = arg < 0 ? cre(exp(-az * sqrt(-arg))) : cexpI(z * sqrt(arg));
+            size_t idx = (size_t)j * n + i;
+            a[idx] = cmul(a[idx], h);
+        }
+    }
+    or_fft2d(a, n, 1);
+}
+void or_propagate_uniaxial(or_cplx* ex, or_cplx* ey, int n, double dx, double z, double no, double ne, double wl) {
+    double k0 = 2 * M_PI / wl;
+    uniaxial_apply(ex, n, dx, z, k0, no, ne, 1);
+    if (ey) uniaxial_apply(ey, n, dx, z, k0, no, ne, 0);
+}
+
 /* Go's complex division (runtime complex128div, Smith's algorithm), for the
  * divisor forms of PropagateVectorial: (kz, 0) and (0, kzi). */
 static inline or_cplx cdiv(or_cplx n, or_cplx m) {

@@ -89,6 +89,9 @@ void or_spectral_centroid(const or_cplx* ex, const or_cplx* ey, int n, double dx
                           double* fx, double* fy);
 int or_propagate1d(or_cplx* a, int n, double dx, double z, double wl,
                    const or_ctx* ctx, or_warnings* w);
+/* PropagateUniaxial (optics/birefringent.go:19-61): Ex extraordinary, Ey
+ * ordinary (ey NULL: scalar field) */
+void or_propagate_uniaxial(or_cplx* ex, or_cplx* ey, int n, double dx, double z, double no, double ne, double wl);
 /* PropagateVectorial (optics/vectorial.go:14-80): ex, ey in/out, ez out */
 void or_propagate_vectorial(or_cplx* ex, or_cplx* ey, or_cplx* ez, int n, double dx, double z,
                             const or_ctx* ctx);

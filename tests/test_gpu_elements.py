@@ -210,3 +210,24 @@ def test_vectorial_matches_oracle(wos, oracle, n, dx, wl, z):
         e = rel_l2(got, ref)
         print(f"vectorial n={n} z={z} prec={prec}: rel-L2 {e:.2e}", flush=True)
         assert e <= tol
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+@pytest.mark.parametrize("n,dx,z", [(128, 0.3e-6, 12e-6), (100, 1e-6, -40e-6)])
+def test_uniaxial_matches_oracle(wos, oracle, polarized, n, dx, z):
+    """PropagateUniaxial (calcite-like no 1.658, ne 1.486) as a drop-in call
+    and as a resident-field element, against the oracle; evanescent bins at
+    dx 0.3 um, a backward step and a Bluestein grid."""
+    wl, no, ne = 633e-9, 1.658, 1.486
+    ex, ey = field(n, 41, polarized)
+    rx, ry = oracle.propagate_uniaxial(ex, ey, dx, z, no, ne, wl)
+    ref = np.concatenate([rx, ry]) if polarized else rx
+    for prec, tol in ((wos.FP64, FP64_TOL), (wos.FP32, FP32_TOL)):
+        gx, gy = wos.propagate_uniaxial(ex, ey, dx, z, no, ne, wos.ctx_for(wl), precision=prec)
+        got = np.concatenate([gx, gy]) if polarized else gx
+        e = rel_l2(got, ref)
+        print(f"uniaxial n={n} z={z} polarized={polarized} prec={prec}: rel-L2 {e:.2e}", flush=True)
+        assert e <= tol
+    e = compare(wos, oracle, n, dx, polarized, lambda f, m: f.uniaxial(z, no, ne, wl), seed=41)
+    with pytest.raises(wos.WosError):
+        wos.propagate_uniaxial(ex, ey, dx, z, 0.0, ne, wos.ctx_for(wl))

@@ -295,3 +295,32 @@ def test_vectorial_divergence_free_and_transverse(oracle):
     kxx = 2 * np.pi * fx
     kzz = k * np.sqrt(1 - (wl * fx) ** 2)
     assert np.max(np.abs(pz + (kxx / kzz) * px)) < 1e-12 and np.max(np.abs(py)) == 0
+
+
+def test_uniaxial_closed_forms(oracle):
+    """PropagateUniaxial: an isotropic crystal (no = ne = n) is the scalar ASM
+    at wavelength wl/n; tilted plane waves pick up exp(i z kz) with the
+    extraordinary kz = sqrt(k0^2 ne^2 - (ne/no)^2 kx^2) on Ex and the
+    ordinary kz = sqrt(k0^2 no^2 - ky^2) on Ey; a scalar field leaves Ey."""
+    n, dx, wl, z = 64, 1e-6, 633e-9, 30e-6
+    rng = np.random.default_rng(12)
+    x = (np.arange(n) - n / 2) * dx
+    env = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (12e-6) ** 2)
+    ex = env * np.exp(2j * np.pi * rng.random((n, n)))
+    ey = env * np.exp(2j * np.pi * rng.random((n, n)))
+    ux, uy = oracle.propagate_uniaxial(ex, ey, dx, z, 1.5, 1.5, wl)
+    sx, sy, _, _ = oracle.propagate(ex, ey, dx, z, "asm", oracle.ctx_for(wl / 1.5))
+    rel = lambda a, b: np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2))
+    assert rel(ux, sx) < 1e-12 and rel(uy, sy) < 1e-12
+    no, ne = 1.54, 1.55
+    k0 = 2 * np.pi / wl
+    fx, fy = 5 / (n * dx), 3 / (n * dx)
+    px = np.exp(2j * np.pi * fx * x)[None, :] * np.ones((n, 1))
+    py = np.exp(2j * np.pi * fy * x)[:, None] * np.ones((1, n))
+    ox, oy = oracle.propagate_uniaxial(px, py, dx, z, no, ne, wl)
+    kze = np.sqrt(k0 ** 2 * ne ** 2 - (ne / no) ** 2 * (2 * np.pi * fx) ** 2)
+    kzo = np.sqrt(k0 ** 2 * no ** 2 - (2 * np.pi * fy) ** 2)
+    assert np.max(np.abs(ox - np.exp(1j * z * kze) * px)) < 1e-11
+    assert np.max(np.abs(oy - np.exp(1j * z * kzo) * py)) < 1e-11
+    sx2, none = oracle.propagate_uniaxial(px, None, dx, z, no, ne, wl)
+    assert none is None and np.array_equal(sx2, ox)

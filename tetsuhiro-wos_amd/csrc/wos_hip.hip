@@ -1360,6 +1360,58 @@ __global__ void k_vectorial(cpx<R>* __restrict__ ex, cpx<R>* __restrict__ ey, cp
   }
 }
 
+// PropagateUniaxial's transfer functions (optics/birefringent.go:38-57) on
+// `comps` spectra: component 0 (Ex) extraordinary, component 1 (Ey) ordinary.
+template <typename R>
+__global__ void k_uniaxial(cpx<R>* __restrict__ a, int n, int comps, double dx, double z, double k0, double no,
+                           double ne) {
+#pragma clang fp contract(off)
+  const size_t nn = (size_t)n * n;
+  const double den = (double)n * dx;
+  const double az = fabs(z);
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nn * comps; g += (size_t)gridDim.x * blockDim.x) {
+    const size_t idx = g % nn;
+    const int j = (int)(idx / n), i = (int)(idx % n);
+    const double fy = (double)(j <= n / 2 ? j : j - n) / den;
+    const double fx = (double)(i <= n / 2 ? i : i - n) / den;
+    double arg;
+    if (g < nn) {
+      const double kx = 2 * M_PI * fx, ky = 2 * M_PI * fy;
+      arg = k0 * k0 * ne * ne - (ne * ne / (no * no)) * kx * kx - ky * ky;
+    } else {
+      arg = k0 * k0 * no * no - (2 * M_PI * fx) * (2 * M_PI * fx) - (2 * M_PI * fy) * (2 * M_PI * fy);
+    }
+    double hr, hi;
+    if (arg < 0) {
+      hr = exp(-az * sqrt(-arg));
+      hi = 0;
+    } else {
+      sincos(z * sqrt(arg), &hi, &hr);
+    }
+    const double ar = (double)a[g].x, ai = (double)a[g].y;
+    a[g] = {(R)(ar * hr - ai * hi), (R)(ar * hi + ai * hr)};
+  }
+}
+
+// PropagateUniaxial (optics/birefringent.go:19-61) of a resident field:
+// ncomp 1 (scalar: Ex only) or 2.
+static int uniaxial_dev(int prec, void* d, int n, int ncomp, double dx, double z, double no, double ne, double wl) {
+  int rc;
+  if ((rc = fft2d_dev(prec, d, n, ncomp, 0))) return rc;
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  const double k0 = 2 * M_PI / wl;
+  const unsigned gs = std::min<unsigned>(grid_for(nn * ncomp), 8192);
+  if (prec == WOS_FP64)
+    hipLaunchKernelGGL(k_uniaxial<double>, dim3(gs), dim3(256), 0, dv.stream, (cpx<double>*)d, n, ncomp, dx, z, k0, no,
+                       ne);
+  else
+    hipLaunchKernelGGL(k_uniaxial<float>, dim3(gs), dim3(256), 0, dv.stream, (cpx<float>*)d, n, ncomp, dx, z, k0, no,
+                       ne);
+  WOS_HIP_TRY(hipGetLastError());
+  return fft2d_dev(prec, d, n, ncomp, 1);
+}
+
 }  // namespace wos
 // ComputeStats on device buffers (wos_stats.inc)
 static int stats_dev(int prec, const void* ex, const void* ey, int n, double dx, double wl, double ap, double dist,
@@ -1880,6 +1932,33 @@ int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int me
   return download_fields(precision, d, dst, ncomp, nn);
 }
 
+
+// PropagateUniaxial (optics/birefringent.go:19-61)
+int wos_hip_propagate_uniaxial(double* ex, double* ey, int n, double dx, double z, double no, double ne,
+                               const wos_ctx* ctx, int precision) {
+  WOS_ENTER();
+  if (!ex || n <= 0 || !ctx || !valid_prec(precision) || !(dx > 0) || !(ctx->wavelength > 0)) {
+    set_error("wos_hip_propagate_uniaxial: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (no <= 0 || ne <= 0) {
+    set_error("uniaxial: n_o and n_e must be > 0");
+    return WOS_ERR_INVALID;
+  }
+  if (!len_ok(n)) {
+    set_error("wos_hip_propagate_uniaxial: grid size not supported: " + std::to_string(n));
+    return WOS_ERR_UNSUPPORTED;
+  }
+  const int ncomp = ey ? 2 : 1;
+  const size_t nn = (size_t)n * n;
+  const double* src[2] = {ex, ey};
+  double* dst[2] = {ex, ey};
+  void* d;
+  int rc;
+  if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
+  if ((rc = uniaxial_dev(precision, d, n, ncomp, dx, z, no, ne, ctx->wavelength))) return rc;
+  return download_fields(precision, d, dst, ncomp, nn);
+}
 
 // PropagateVectorial (optics/vectorial.go:14-80)
 int wos_hip_propagate_vectorial(double* ex, double* ey, double* ez, int n, double dx, double z, const wos_ctx* ctx,

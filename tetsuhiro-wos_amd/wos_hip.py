@@ -43,7 +43,7 @@ EXPORTS = [
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
-    "wos_hip_propagate_vectorial",
+    "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial",
 ]
 
 SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
@@ -199,6 +199,8 @@ def load(path=None):
     L.wos_hip_field_bandlimit.argtypes = [P, D, D]
     L.wos_hip_field_stats.argtypes = [P, D, D, D, C.POINTER(PlaneStats)]
     L.wos_hip_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx), I]
+    L.wos_hip_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, C.POINTER(Ctx), I]
+    L.wos_hip_field_uniaxial.argtypes = [P, D, D, D, D]
     L.wos_hip_field_dx.argtypes = [P]
     L.wos_hip_field_dx.restype = D
     _lib = L
@@ -353,10 +355,22 @@ class Field:
     def bandlimit(self, fraction, sigma):
         _check(load().wos_hip_field_bandlimit(self.h, fraction, sigma), "field_bandlimit")
 
+    def uniaxial(self, z, no, ne, wl):
+        _check(load().wos_hip_field_uniaxial(self.h, z, no, ne, wl), "field_uniaxial")
+
     def stats(self, wl, strehl_aperture=0.0, strehl_distance=0.0):
         st = PlaneStats()
         _check(load().wos_hip_field_stats(self.h, wl, strehl_aperture, strehl_distance, C.byref(st)), "field_stats")
         return st
+
+
+def propagate_uniaxial(ex, ey, dx, z, no, ne, ctx, precision=FP64):
+    """PropagateUniaxial on copies (ey None: scalar field)."""
+    ex = _c128(ex).copy()
+    ey = None if ey is None else _c128(ey).copy()
+    _check(load().wos_hip_propagate_uniaxial(_p(ex), _p(ey), ex.shape[0], dx, z, no, ne, C.byref(ctx), precision),
+           "propagate_uniaxial")
+    return ex, ey
 
 
 def propagate_vectorial(ex, ey, dx, z, ctx, precision=FP64):
