@@ -344,6 +344,27 @@ int wos_hip_field_jones(wos_field f, const double* abcd);
 int wos_hip_field_scale(wos_field f, double re, double im);
 /* Field.ApplyBandlimit (optics/field.go:264-294); fraction outside (0, 1): no-op */
 int wos_hip_field_bandlimit(wos_field f, double fraction, double sigma);
+/* Plane views: the float32 little-endian planes servePlaneData streams
+ * (server/server.go:581-660) for one recorded field, computed on the device
+ * so a plane crosses PCIe as 4 bytes per pixel (server/views.go:131-244,
+ * optics/polar.go:23-96). Masked pixels hold the float32 NaN 0x7FC00000.
+ * mask_cut <= 0 takes the server's defaults (2e-3 of the component's peak
+ * for phases, 1e-6 of the peak S0 for polarisation views). */
+#define WOS_VIEW_TOTAL 0
+#define WOS_VIEW_AMPLITUDE 1
+#define WOS_VIEW_EX 2
+#define WOS_VIEW_EY 3
+#define WOS_VIEW_PHASE_X 4
+#define WOS_VIEW_PHASE_Y 5
+#define WOS_VIEW_POL_INTENSITY 6
+#define WOS_VIEW_POL_AZIMUTH 7
+#define WOS_VIEW_POL_ELLIP 8
+#define WOS_VIEW_POL_S1 9
+#define WOS_VIEW_POL_S2 10
+#define WOS_VIEW_POL_S3 11
+#define WOS_VIEW_POL_DEGREE 12
+/* out: n*n floats (host memory) */
+int wos_hip_field_view(wos_field f, int view, double mask_cut, float* out);
 /* ComputeStats (optics/metrics.go:31-91) of the resident field */
 int wos_hip_field_stats(wos_field f, double wl, double strehl_aperture, double strehl_distance,
                         wos_plane_stats* out);

@@ -206,6 +206,7 @@ def lib():
         L.or_apply_lens_at.argtypes = [P, P, I, I, D, D, D, D, D, D]
         L.or_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx)]
         L.or_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, D]
+        L.or_plane_view.argtypes = [P, P, I, I, D, P]
         L.or_shape_signed_distance.argtypes = [C.POINTER(Shape), D, D]
         L.or_shape_signed_distance.restype = D
         L.or_regular_polygon.argtypes = [I, D, D, P]
@@ -278,6 +279,19 @@ def propagate(ex, ey, dx, z, method, ctx):
     if rc != 0:
         raise ValueError(f"or_propagate rc={rc}")
     return ex, ey, w, st
+
+
+VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity", "pol_azimuth", "pol_ellip",
+         "pol_s1", "pol_s2", "pol_s3", "pol_degree"]
+
+
+def plane_view(ex, ey, view, mask_cut=0.0):
+    """The server's float32 plane view of one field (ey None: scalar)."""
+    ex = _c128(ex)
+    ey = None if ey is None else _c128(ey)
+    out = np.zeros(ex.shape, dtype=np.float32)
+    lib().or_plane_view(_p(ex), _p(ey), ex.shape[0], VIEWS.index(view), mask_cut, out.ctypes.data)
+    return out
 
 
 def propagate_uniaxial(ex, ey, dx, z, no, ne, wl):

@@ -1131,6 +1131,98 @@ void or_bandlimit(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, dou
     if (polarized) bandlimit_cut(ey, n, dx, f0, sig);
 }
 
+/* ---- plane views (server/views.go, optics/polar.go) ---- */
+static double norm2c(or_cplx z) { return z.re * z.re + z.im * z.im; }
+static double wrap_pi(double a) {
+    while (a > M_PI) a -= 2 * M_PI;
+    while (a < -M_PI) a += 2 * M_PI;
+    return a;
+}
+static double wrap_half_pi(double a) {
+    while (a > M_PI / 2) a -= M_PI;
+    while (a < -M_PI / 2) a += M_PI;
+    return a;
+}
+typedef struct { double s0, s1, s2, s3, azimuth, ellipt, degree; } or_stokes;
+/* StokesAt and derive (optics/polar.go:23-83) */
+static or_stokes stokes_at(or_cplx ex, or_cplx ey) {
+    or_stokes st = {0, 0, 0, 0, 0, 0, 0};
+    double ax = or_hypot(ex.re, ex.im), ay = or_hypot(ey.re, ey.im);
+    double px = atan2(ex.im, ex.re), py = atan2(ey.im, ey.re);
+    double delta = wrap_pi(py - px);
+    st.s0 = ax * ax + ay * ay;
+    st.s1 = ax * ax - ay * ay;
+    if (ax > 0 && ay > 0) {
+        st.s2 = 2 * ax * ay * cos(delta);
+        st.s3 = 2 * ax * ay * sin(delta);
+    }
+    if (st.s0 <= 0) return st;
+    st.azimuth = wrap_half_pi(0.5 * atan2(st.s2, st.s1));
+    double r = st.s3 / st.s0;
+    if (r < -1) r = -1;
+    if (r > 1) r = 1;
+    st.ellipt = 0.5 * asin(r);
+    st.degree = or_hypot(or_hypot(st.s1, st.s2), st.s3) / st.s0;
+    return st;
+}
+static void put_f32(float* out, size_t i, double v, int nan) {
+    if (nan) {
+        uint32_t bits = 0x7FC00000u;
+        memcpy(out + i, &bits, 4);
+    } else {
+        out[i] = (float)v;
+    }
+}
+void or_plane_view(const or_cplx* ex, const or_cplx* ey, int n, int view, double mask_cut, float* out) {
+    size_t nn = (size_t)n * n;
+    or_cplx zero = {0, 0};
+    if (view <= 3 || view == 6) {
+        for (size_t i = 0; i < nn; i++) {
+            double vx = norm2c(ex[i]), vy = ey ? norm2c(ey[i]) : 0;
+            double v = view == 2 ? vx : view == 3 ? vy : vx + vy;
+            put_f32(out, i, view == 1 ? sqrt(v) : v, 0);
+        }
+        return;
+    }
+    if (view == 4 || view == 5) {
+        const or_cplx* src = view == 4 ? ex : ey;
+        double mc = mask_cut > 0 ? mask_cut : 2e-3;
+        double peak = 0;
+        for (size_t i = 0; src && i < nn; i++)
+            if (norm2c(src[i]) > peak) peak = norm2c(src[i]);
+        double thr = mc * peak;
+        for (size_t i = 0; i < nn; i++) {
+            if (peak <= 0 || !src || norm2c(src[i]) < thr) put_f32(out, i, 0, 1);
+            else put_f32(out, i, atan2(src[i].im, src[i].re), 0);
+        }
+        return;
+    }
+    double cp = mask_cut > 0 ? mask_cut : 1e-6;
+    double max_s0 = 0;
+    for (size_t i = 0; i < nn; i++) {
+        double s0 = stokes_at(ex[i], ey ? ey[i] : zero).s0;
+        if (s0 > max_s0) max_s0 = s0;
+    }
+    double cut = cp * max_s0;
+    for (size_t i = 0; i < nn; i++) {
+        or_stokes st = stokes_at(ex[i], ey ? ey[i] : zero);
+        if (max_s0 > 0 && st.s0 < cut) {
+            put_f32(out, i, 0, 1);
+            continue;
+        }
+        double v = 0;
+        switch (view) {
+        case 7: v = st.azimuth; break;
+        case 8: v = st.ellipt; break;
+        case 9: v = st.s0 > 0 ? st.s1 / st.s0 : 0; break;
+        case 10: v = st.s0 > 0 ? st.s2 / st.s0 : 0; break;
+        case 11: v = st.s0 > 0 ? st.s3 / st.s0 : 0; break;
+        case 12: v = st.degree; break;
+        }
+        put_f32(out, i, v, 0);
+    }
+}
+
 /* hashNoise01 (optics/elements.go:710-721) */
 double or_hash_noise01(int64_t x, int64_t y, int64_t seed) {
     uint64_t h = (uint64_t)seed * 0x9E3779B97F4A7C15ULL ^ (uint64_t)x * 0xC2B2AE3D27D4EB4FULL ^

@@ -145,6 +145,13 @@ void or_scale_amplitude(or_cplx* ex, or_cplx* ey, int polarized, int n, or_cplx 
 void or_bandlimit(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double fraction,
                   double sigma);                                          /* Field.ApplyBandlimit */
 
+/* servePlaneData's float32 views of one recorded field (server/server.go:581-
+ * 660, server/views.go:131-244, optics/polar.go:23-96): 0 total, 1 amplitude,
+ * 2 ex, 3 ey, 4 phase_x, 5 phase_y, 6 pol_intensity, 7 pol_azimuth,
+ * 8 pol_ellip, 9 pol_s1, 10 pol_s2, 11 pol_s3, 12 pol_degree. ey NULL: scalar
+ * field. Masked pixels are the float32 NaN the server writes (0x7FC00000). */
+void or_plane_view(const or_cplx* ex, const or_cplx* ey, int n, int view, double mask_cut, float* out);
+
 /* optics/coherence.go with a counter-based normal generator in place of
  * Go's math/rand stream (that stream is not reproducible without Go). */
 double or_normal(uint64_t seed, uint64_t counter);

@@ -231,3 +231,26 @@ def test_uniaxial_matches_oracle(wos, oracle, polarized, n, dx, z):
     e = compare(wos, oracle, n, dx, polarized, lambda f, m: f.uniaxial(z, no, ne, wl), seed=41)
     with pytest.raises(wos.WosError):
         wos.propagate_uniaxial(ex, ey, dx, z, 0.0, ne, wos.ctx_for(wl))
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+def test_plane_views_match_oracle(wos, oracle, polarized):
+    """Every float32 plane view of a resident field against the oracle on
+    the same field values: NaN masks identical (bit pattern 0x7FC00000),
+    values within one float32 rounding."""
+    n = 96
+    ex, ey = field(n, 51, polarized)
+    for prec in (wos.FP64, wos.FP32):
+        f = wos.Field.from_host(ex, ey, dx=1e-6, precision=prec)
+        hx, hy = f.download()  # the values the device holds
+        for view in wos.VIEWS:
+            for mc in (0.0, 0.05):
+                got = f.view(view, mc)
+                ref = oracle.plane_view(hx, hy if polarized else None, view, mc)
+                nan = np.isnan(ref)
+                assert np.array_equal(np.isnan(got), nan), (view, mc, prec)
+                assert np.all(got[nan].view(np.uint32) == 0x7FC00000)
+                d = np.abs(got[~nan].astype(np.float64) - ref[~nan].astype(np.float64))
+                scale = np.maximum(np.abs(ref[~nan]).astype(np.float64), 1e-30)
+                assert np.all(d <= 2.5e-7 * scale + 1e-12), (view, mc, prec, float(np.max(d / scale)))
+        f.close()

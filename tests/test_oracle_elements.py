@@ -324,3 +324,38 @@ def test_uniaxial_closed_forms(oracle):
     assert np.max(np.abs(oy - np.exp(1j * z * kzo) * py)) < 1e-11
     sx2, none = oracle.propagate_uniaxial(px, None, dx, z, no, ne, wl)
     assert none is None and np.array_equal(sx2, ox)
+
+
+def test_plane_views_closed_forms(oracle):
+    """The float32 plane views: intensity family and amplitude, masked
+    wrapped phase, and the Stokes views for known polarisation states
+    (linear at theta: s1 = cos 2theta, s2 = sin 2theta, azimuth theta; circular:
+    s3 = +-1, ellipticity +-pi/4; degree 1 for a coherent field)."""
+    n = 32
+    x = (np.arange(n) - n / 2) / n
+    env = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / 0.02)
+    ph = np.exp(1j * 3 * x)[None, :] * np.ones((n, 1))
+    th = 0.3
+    ex, ey = env * np.cos(th) * ph, env * np.sin(th) * ph
+    tot = oracle.plane_view(ex, ey, "total")
+    want = (np.abs(ex) ** 2 + np.abs(ey) ** 2).astype(np.float32)
+    assert np.max(np.abs(tot - want) / want.max()) < 1e-7
+    assert np.allclose(oracle.plane_view(ex, ey, "amplitude") ** 2, tot, rtol=1e-6, atol=0)
+    assert np.array_equal(oracle.plane_view(ex, None, "ey"), np.zeros((n, n), np.float32))
+    assert np.array_equal(oracle.plane_view(ex, ey, "pol_intensity"), tot)
+    # phase: masked (NaN, bits 0x7FC00000) below 2e-3 of the component's peak
+    p = oracle.plane_view(ex, ey, "phase_x")
+    keep = np.abs(ex) ** 2 >= 2e-3 * np.max(np.abs(ex) ** 2)
+    assert np.all(np.isnan(p[~keep])) and np.all(p[~keep].view(np.uint32) == 0x7FC00000)
+    assert np.max(np.abs(p[keep] - np.angle(ex[keep]))) < 1e-6
+    assert np.all(np.isnan(oracle.plane_view(ex, None, "phase_y")))  # no Ey: no phase
+    lit = np.abs(ex) ** 2 + np.abs(ey) ** 2 >= 1e-6 * np.max(np.abs(ex) ** 2 + np.abs(ey) ** 2)
+    for view, v in (("pol_s1", np.cos(2 * th)), ("pol_s2", np.sin(2 * th)), ("pol_s3", 0.0),
+                    ("pol_azimuth", th), ("pol_ellip", 0.0), ("pol_degree", 1.0)):
+        got = oracle.plane_view(ex, ey, view)
+        assert np.all(np.isnan(got[~lit])) and np.max(np.abs(got[lit] - v)) < 1e-6, view
+    # circular light (Ey = +-i Ex): s3 = +-1, ellipticity +-pi/4
+    for s in (1, -1):
+        got3 = oracle.plane_view(env * ph, s * 1j * env * ph, "pol_s3")
+        gote = oracle.plane_view(env * ph, s * 1j * env * ph, "pol_ellip")
+        assert np.max(np.abs(got3[lit] - s)) < 1e-6 and np.max(np.abs(gote[lit] - s * np.pi / 4)) < 1e-6

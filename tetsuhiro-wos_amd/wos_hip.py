@@ -43,8 +43,11 @@ EXPORTS = [
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
-    "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial",
+    "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
 ]
+
+VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity", "pol_azimuth", "pol_ellip",
+         "pol_s1", "pol_s2", "pol_s3", "pol_degree"]
 
 SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
                "double_slit": 7, "cross": 8, "polygon": 9}
@@ -201,6 +204,7 @@ def load(path=None):
     L.wos_hip_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx), I]
     L.wos_hip_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, C.POINTER(Ctx), I]
     L.wos_hip_field_uniaxial.argtypes = [P, D, D, D, D]
+    L.wos_hip_field_view.argtypes = [P, I, D, P]
     L.wos_hip_field_dx.argtypes = [P]
     L.wos_hip_field_dx.restype = D
     _lib = L
@@ -357,6 +361,12 @@ class Field:
 
     def uniaxial(self, z, no, ne, wl):
         _check(load().wos_hip_field_uniaxial(self.h, z, no, ne, wl), "field_uniaxial")
+
+    def view(self, name, mask_cut=0.0):
+        """The server's float32 plane view `name` (VIEWS)."""
+        out = np.zeros((self.n, self.n), dtype=np.float32)
+        _check(load().wos_hip_field_view(self.h, VIEWS.index(name), mask_cut, out.ctypes.data), "field_view")
+        return out
 
     def stats(self, wl, strehl_aperture=0.0, strehl_distance=0.0):
         st = PlaneStats()
