@@ -133,6 +133,16 @@ int wos_hip_propagate(double* ex, double* ey, int n, double dx, double z, int me
 int wos_hip_propagate_uniaxial(double* ex, double* ey, int n, double dx, double z, double no, double ne,
                                const wos_ctx* ctx, int precision);
 
+/* PropagateAnisotropic (optics/berreman.go:13-36): a homogeneous anisotropic
+ * medium of thickness z >= 0 with relative permittivity eps (3x3 complex,
+ * row-major, as 18 doubles re, im), by the Berreman 4x4 method per spectral
+ * bin. Ex and Ey (both n*n complex128) always take part, as in the
+ * reference; the biaxial element (optics/elements.go:1129-1155) passes
+ * diag(nx^2, ny^2, nz^2). Degenerate modes (an isotropic eps) give NaN there,
+ * as in the reference. */
+int wos_hip_propagate_anisotropic(double* ex, double* ey, int n, double dx, double z, const double* eps,
+                                  const wos_ctx* ctx, int precision);
+
 /* PropagateVectorial (optics/vectorial.go:14-80), Propagate's "vectorial"
  * method: Ex and Ey (both complex128 n*n, Ey zeros for a field that had none)
  * propagate in place under exp(i kz z) and the evanescent policy; ez receives
@@ -301,6 +311,9 @@ void* wos_hip_field_device_ptr(wos_field f);
 double wos_hip_field_dx(wos_field f);
 /* uniaxialEl.Apply = PropagateUniaxial (optics/elements.go:1102-1104) on the resident field */
 int wos_hip_field_uniaxial(wos_field f, double z, double no, double ne, double wavelength);
+/* biaxialEl.Apply = PropagateAnisotropic on the resident field (Ey takes
+ * part also for a scalar field, whose Polarized flag is left as is) */
+int wos_hip_field_anisotropic(wos_field f, double z, const double* eps, double wavelength);
 /* Propagate (optics/propagate.go:65-102) on the resident field, Ey only if
  * polarized; warnings and stats as wos_hip_propagate */
 int wos_hip_field_propagate(wos_field f, double z, int method, const wos_ctx* ctx, wos_warnings* w,

@@ -143,6 +143,10 @@ class OField:
     def bandlimit(self, fraction, sigma):
         lib().or_bandlimit(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, fraction, sigma)
 
+    def anisotropic(self, z, eps, wl):
+        e = np.ascontiguousarray(eps, dtype=np.complex128)
+        lib().or_propagate_anisotropic(_p(self.ex), _p(self.ey), self.n, self.dx, z, e.ctypes.data, wl)
+
     def uniaxial(self, z, no, ne, wl):
         lib().or_propagate_uniaxial(_p(self.ex), _p(self.ey) if self.polarized else None, self.n, self.dx, z, no,
                                     ne, wl)
@@ -207,6 +211,7 @@ def lib():
         L.or_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx)]
         L.or_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, D]
         L.or_plane_view.argtypes = [P, P, I, I, D, P]
+        L.or_propagate_anisotropic.argtypes = [P, P, I, D, D, P, D]
         L.or_shape_signed_distance.argtypes = [C.POINTER(Shape), D, D]
         L.or_shape_signed_distance.restype = D
         L.or_regular_polygon.argtypes = [I, D, D, P]
@@ -292,6 +297,15 @@ def plane_view(ex, ey, view, mask_cut=0.0):
     out = np.zeros(ex.shape, dtype=np.float32)
     lib().or_plane_view(_p(ex), _p(ey), ex.shape[0], VIEWS.index(view), mask_cut, out.ctypes.data)
     return out
+
+
+def propagate_anisotropic(ex, ey, dx, z, eps, wl):
+    """PropagateAnisotropic on copies; eps 3x3 complex; ey None: zeros."""
+    ex = _c128(ex).copy()
+    ey = np.zeros_like(ex) if ey is None else _c128(ey).copy()
+    e = np.ascontiguousarray(eps, dtype=np.complex128)
+    lib().or_propagate_anisotropic(_p(ex), _p(ey), ex.shape[0], dx, z, e.ctypes.data, wl)
+    return ex, ey
 
 
 def propagate_uniaxial(ex, ey, dx, z, no, ne, wl):

@@ -741,6 +741,7 @@ void or_propagate_uniaxial(or_cplx* ex, or_cplx* ey, int n, double dx, double z,
     if (ey) uniaxial_apply(ey, n, dx, z, k0, no, ne, 0);
 }
 
+
 /* Go's complex division (runtime complex128div, Smith's algorithm), for the
  * divisor forms of PropagateVectorial: (kz, 0) and (0, kzi). */
 static inline or_cplx cdiv(or_cplx n, or_cplx m) {
@@ -757,6 +758,193 @@ static inline or_cplx cdiv(or_cplx n, or_cplx m) {
         r.im = (n.im * ratio - n.re) / denom;
     }
     return r;
+}
+
+/* ---- Berreman 4x4 (optics/berreman.go) with Go complex128 semantics ---- */
+typedef struct { or_cplx m[4][4]; } mat4;
+static inline or_cplx cneg(or_cplx a) { or_cplx r = {-a.re, -a.im}; return r; }
+/* buildBerreman (optics/berreman.go:90-112) */
+static mat4 build_berreman(const or_cplx* e, double xi, double eta) {
+#define E(i, j) e[3 * (i) + (j)]
+    or_cplx r = cdiv(cre(1), E(2, 2));
+    or_cplx x = cre(xi), y = cre(eta);
+    mat4 D;
+    D.m[0][0] = cmul(cmul(cneg(x), r), E(2, 0));
+    D.m[0][1] = csub(cre(1), cmul(cmul(x, x), r));
+    D.m[0][2] = cmul(cmul(cneg(x), r), E(2, 1));
+    D.m[0][3] = cmul(cmul(cneg(x), r), y);
+    D.m[1][0] = csub(csub(E(0, 0), cmul(y, y)), cmul(cmul(E(0, 2), r), E(2, 0)));
+    D.m[1][1] = cmul(cmul(cneg(E(0, 2)), r), x);
+    D.m[1][2] = csub(cadd(E(0, 1), cmul(y, x)), cmul(cmul(E(0, 2), r), E(2, 1)));
+    D.m[1][3] = cmul(cmul(cneg(E(0, 2)), r), y);
+    D.m[2][0] = cmul(cmul(cneg(y), r), E(2, 0));
+    D.m[2][1] = cmul(cmul(cneg(y), r), x);
+    D.m[2][2] = cmul(cmul(cneg(y), r), E(2, 1));
+    D.m[2][3] = csub(cre(1), cmul(cmul(y, y), r));
+    D.m[3][0] = csub(cadd(E(1, 0), cmul(x, y)), cmul(cmul(E(1, 2), r), E(2, 0)));
+    D.m[3][1] = cmul(cmul(cneg(E(1, 2)), r), x);
+    D.m[3][2] = csub(csub(E(1, 1), cmul(x, x)), cmul(cmul(E(1, 2), r), E(2, 1)));
+    D.m[3][3] = cmul(cmul(cneg(E(1, 2)), r), y);
+#undef E
+    return D;
+}
+static or_cplx trace4(const mat4* A) { return cadd(cadd(cadd(A->m[0][0], A->m[1][1]), A->m[2][2]), A->m[3][3]); }
+static mat4 mat_mul4(const mat4* A, const mat4* B) {
+    mat4 o;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            or_cplx s = cre(0);
+            for (int k = 0; k < 4; k++) s = cadd(s, cmul(A->m[i][k], B->m[k][j]));
+            o.m[i][j] = s;
+        }
+    return o;
+}
+/* addMat4(B, scaleMat4(I, c)) */
+static mat4 add_scaled_identity(const mat4* B, or_cplx c) {
+    mat4 o;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) o.m[i][j] = cadd(B->m[i][j], cmul(cre(i == j ? 1.0 : 0.0), c));
+    return o;
+}
+/* charPoly4, Faddeev-LeVerrier (optics/berreman.go:127-143) */
+static void char_poly4(const mat4* A, or_cplx* c0, or_cplx* c1, or_cplx* c2, or_cplx* c3) {
+    mat4 B = *A, t;
+    *c3 = cneg(trace4(&B));
+    t = add_scaled_identity(&B, *c3);
+    B = mat_mul4(A, &t);
+    *c2 = cdiv(cneg(trace4(&B)), cre(2));
+    t = add_scaled_identity(&B, *c2);
+    B = mat_mul4(A, &t);
+    *c1 = cdiv(cneg(trace4(&B)), cre(3));
+    t = add_scaled_identity(&B, *c1);
+    B = mat_mul4(A, &t);
+    *c0 = cdiv(cneg(trace4(&B)), cre(4));
+}
+static or_cplx poly4(or_cplx z, or_cplx c0, or_cplx c1, or_cplx c2, or_cplx c3) {
+    or_cplx t = cadd(cmul(cmul(cmul(z, z), z), z), cmul(cmul(cmul(c3, z), z), z));
+    t = cadd(t, cmul(cmul(c2, z), z));
+    t = cadd(t, cmul(c1, z));
+    return cadd(t, c0);
+}
+/* roots4, Durand-Kerner (optics/berreman.go:184-210) */
+static void roots4(or_cplx c0, or_cplx c1, or_cplx c2, or_cplx c3, or_cplx* z) {
+    z[0] = (or_cplx){0.4, 0.9};
+    z[1] = (or_cplx){0.4, -0.9};
+    z[2] = (or_cplx){-0.4, 0.9};
+    z[3] = (or_cplx){-0.4, -0.9};
+    for (int iter = 0; iter < 300; iter++) {
+        double max_d = 0;
+        for (int i = 0; i < 4; i++) {
+            or_cplx num = poly4(z[i], c0, c1, c2, c3);
+            or_cplx den = cre(1);
+            for (int j = 0; j < 4; j++)
+                if (j != i) den = cmul(den, csub(z[i], z[j]));
+            or_cplx dz = cdiv(num, den);
+            z[i] = csub(z[i], dz);
+            double d = or_hypot(dz.re, dz.im);
+            if (d > max_d) max_d = d;
+        }
+        if (max_d < 1e-14) break;
+    }
+}
+static or_cplx det3(or_cplx m[3][3]) {
+    or_cplx a = cmul(m[0][0], csub(cmul(m[1][1], m[2][2]), cmul(m[1][2], m[2][1])));
+    or_cplx b = cmul(m[0][1], csub(cmul(m[1][0], m[2][2]), cmul(m[1][2], m[2][0])));
+    or_cplx c = cmul(m[0][2], csub(cmul(m[1][0], m[2][1]), cmul(m[1][1], m[2][0])));
+    return cadd(csub(a, b), c);
+}
+/* nullVector4: the largest-norm adjugate column of A - qI (optics/berreman.go:224-272) */
+static void null_vector4(const mat4* A, or_cplx q, or_cplx* best) {
+    mat4 M = *A;
+    for (int i = 0; i < 4; i++) M.m[i][i] = csub(M.m[i][i], q);
+    double best_n = -1;
+    for (int col = 0; col < 4; col++) {
+        or_cplx v[4];
+        for (int row = 0; row < 4; row++) {
+            or_cplx m[3][3];
+            int ri = 0;
+            for (int r = 0; r < 4; r++) {
+                if (r == row) continue;
+                int ci = 0;
+                for (int c = 0; c < 4; c++) {
+                    if (c == col) continue;
+                    m[ri][ci] = M.m[r][c];
+                    ci++;
+                }
+                ri++;
+            }
+            or_cplx cof = det3(m);
+            if ((row + col) % 2 == 1) cof = cneg(cof);
+            v[row] = cof;
+        }
+        double nrm = 0;
+        for (int k = 0; k < 4; k++) nrm += v[k].re * v[k].re + v[k].im * v[k].im;
+        if (nrm > best_n) {
+            best_n = nrm;
+            for (int k = 0; k < 4; k++) best[k] = v[k];
+        }
+    }
+}
+static int forward_mode(or_cplx q) {
+    if (q.im > 1e-12) return 1;
+    if (q.im < -1e-12) return 0;
+    return q.re > 0;
+}
+/* anisoMode (optics/berreman.go:38-86) */
+static void aniso_mode(const or_cplx* eps, double xi, double eta, double k0, double z, or_cplx* Ex, or_cplx* Ey) {
+    mat4 D = build_berreman(eps, xi, eta);
+    or_cplx c0, c1, c2, c3, roots[4];
+    char_poly4(&D, &c0, &c1, &c2, &c3);
+    roots4(c0, c1, c2, c3, roots);
+    or_cplx qs[2] = {{0, 0}, {0, 0}}, vs[2][4];
+    memset(vs, 0, sizeof(vs));
+    int cnt = 0, used[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++)
+        if (forward_mode(roots[i]) && cnt < 2) {
+            qs[cnt] = roots[i];
+            null_vector4(&D, roots[i], vs[cnt]);
+            used[i] = 1;
+            cnt++;
+        }
+    while (cnt < 2) {
+        double best = -1e300;
+        int bi = -1;
+        for (int i = 0; i < 4; i++)
+            if (!used[i] && roots[i].re > best) {
+                best = roots[i].re;
+                bi = i;
+            }
+        if (bi < 0) break;
+        qs[cnt] = roots[bi];
+        null_vector4(&D, roots[bi], vs[cnt]);
+        used[bi] = 1;
+        cnt++;
+    }
+    or_cplx a = vs[0][0], b = vs[1][0], c = vs[0][2], d = vs[1][2];
+    or_cplx det = csub(cmul(a, d), cmul(b, c));
+    or_cplx m1 = cdiv(csub(cmul(d, *Ex), cmul(b, *Ey)), det);
+    or_cplx m2 = cdiv(cadd(cmul(cneg(c), *Ex), cmul(a, *Ey)), det);
+    or_cplx pf0 = cmul(cexpI(k0 * qs[0].re * z), cre(exp(-k0 * qs[0].im * z)));
+    or_cplx pf1 = cmul(cexpI(k0 * qs[1].re * z), cre(exp(-k0 * qs[1].im * z)));
+    *Ex = cadd(cmul(cmul(m1, pf0), a), cmul(cmul(m2, pf1), b));
+    *Ey = cadd(cmul(cmul(m1, pf0), c), cmul(cmul(m2, pf1), d));
+}
+void or_propagate_anisotropic(or_cplx* ex, or_cplx* ey, int n, double dx, double z, const or_cplx* eps, double wl) {
+    double k0 = 2 * M_PI / wl;
+    or_fft2d(ex, n, 0);
+    or_fft2d(ey, n, 0);
+    for (int j = 0; j < n; j++) {
+        double fy = or_freq(j, n, dx);
+        double ky = 2 * M_PI * fy;
+        for (int i = 0; i < n; i++) {
+            double fx = or_freq(i, n, dx);
+            double kx = 2 * M_PI * fx;
+            size_t idx = (size_t)j * n + i;
+            aniso_mode(eps, kx / k0, ky / k0, k0, z, &ex[idx], &ey[idx]);
+        }
+    }
+    or_fft2d(ex, n, 1);
+    or_fft2d(ey, n, 1);
 }
 
 /* PropagateVectorial (optics/vectorial.go:14-80) */

@@ -92,6 +92,10 @@ int or_propagate1d(or_cplx* a, int n, double dx, double z, double wl,
 /* PropagateUniaxial (optics/birefringent.go:19-61): Ex extraordinary, Ey
  * ordinary (ey NULL: scalar field) */
 void or_propagate_uniaxial(or_cplx* ex, or_cplx* ey, int n, double dx, double z, double no, double ne, double wl);
+/* PropagateAnisotropic (optics/berreman.go:13-36): Berreman 4x4 per spectral
+ * bin for a relative permittivity eps (3x3 complex, row-major); both Ex and
+ * Ey take part, as in the reference. */
+void or_propagate_anisotropic(or_cplx* ex, or_cplx* ey, int n, double dx, double z, const or_cplx* eps, double wl);
 /* PropagateVectorial (optics/vectorial.go:14-80): ex, ey in/out, ez out */
 void or_propagate_vectorial(or_cplx* ex, or_cplx* ey, or_cplx* ez, int n, double dx, double z,
                             const or_ctx* ctx);

@@ -359,3 +359,28 @@ def test_plane_views_closed_forms(oracle):
         got3 = oracle.plane_view(env * ph, s * 1j * env * ph, "pol_s3")
         gote = oracle.plane_view(env * ph, s * 1j * env * ph, "pol_ellip")
         assert np.max(np.abs(got3[lit] - s)) < 1e-6 and np.max(np.abs(gote[lit] - s * np.pi / 4)) < 1e-6
+
+
+def test_berreman_reductions(oracle):
+    """PropagateAnisotropic: a uniaxial tensor diag(ne^2, no^2, no^2) on a
+    field that varies along x only is PropagateUniaxial (exact in the x-z
+    plane); a biaxial tensor at normal incidence gives exp(i k0 nx z) on Ex
+    and exp(i k0 ny z) on Ey. (An isotropic tensor has degenerate modes: the
+    adjugate null vectors vanish and the reference's 2x2 solve gives NaN, so
+    it is no check here.)"""
+    n, dx, wl, z = 32, 1e-6, 633e-9, 20e-6
+    rng = np.random.default_rng(13)
+    x = (np.arange(n) - n / 2) * dx
+    rel = lambda a, b: np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2))
+    no, ne = 1.658, 1.486
+    line = np.exp(-(x / 6e-6) ** 2) * np.exp(2j * np.pi * rng.random(n))
+    fx1, fy1 = line[None, :] * np.ones((n, 1)), 0.5 * line[None, :] * np.ones((n, 1))
+    bx, by = oracle.propagate_anisotropic(fx1, fy1, dx, z, np.diag([ne ** 2, no ** 2, no ** 2]), wl)
+    ux, uy = oracle.propagate_uniaxial(fx1, fy1, dx, z, no, ne, wl)
+    assert rel(bx, ux) < 1e-10 and rel(by, uy) < 1e-10
+    nx, ny, nz = 1.6, 1.5, 1.4
+    k0 = 2 * np.pi / wl
+    bx, by = oracle.propagate_anisotropic(np.ones((n, n)), 2 * np.ones((n, n)), dx, z,
+                                          np.diag([nx ** 2, ny ** 2, nz ** 2]), wl)
+    assert np.max(np.abs(bx - np.exp(1j * k0 * nx * z))) < 1e-12
+    assert np.max(np.abs(by - 2 * np.exp(1j * k0 * ny * z))) < 1e-12

@@ -2630,3 +2630,4 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 #include "wos_pipeline.inc"
 #include "wos_stats.inc"
 #include "wos_field.inc"
+#include "wos_aniso.inc"

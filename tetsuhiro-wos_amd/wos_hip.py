@@ -44,6 +44,7 @@ EXPORTS = [
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
+    "wos_hip_propagate_anisotropic", "wos_hip_field_anisotropic",
 ]
 
 VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity", "pol_azimuth", "pol_ellip",
@@ -205,6 +206,8 @@ def load(path=None):
     L.wos_hip_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, C.POINTER(Ctx), I]
     L.wos_hip_field_uniaxial.argtypes = [P, D, D, D, D]
     L.wos_hip_field_view.argtypes = [P, I, D, P]
+    L.wos_hip_propagate_anisotropic.argtypes = [P, P, I, D, D, P, C.POINTER(Ctx), I]
+    L.wos_hip_field_anisotropic.argtypes = [P, D, P, D]
     L.wos_hip_field_dx.argtypes = [P]
     L.wos_hip_field_dx.restype = D
     _lib = L
@@ -362,6 +365,10 @@ class Field:
     def uniaxial(self, z, no, ne, wl):
         _check(load().wos_hip_field_uniaxial(self.h, z, no, ne, wl), "field_uniaxial")
 
+    def anisotropic(self, z, eps, wl):
+        e = np.ascontiguousarray(eps, dtype=np.complex128)
+        _check(load().wos_hip_field_anisotropic(self.h, z, e.ctypes.data, wl), "field_anisotropic")
+
     def view(self, name, mask_cut=0.0):
         """The server's float32 plane view `name` (VIEWS)."""
         out = np.zeros((self.n, self.n), dtype=np.float32)
@@ -372,6 +379,16 @@ class Field:
         st = PlaneStats()
         _check(load().wos_hip_field_stats(self.h, wl, strehl_aperture, strehl_distance, C.byref(st)), "field_stats")
         return st
+
+
+def propagate_anisotropic(ex, ey, dx, z, eps, ctx, precision=FP64):
+    """PropagateAnisotropic on copies; eps 3x3 complex; ey None: zeros."""
+    ex = _c128(ex).copy()
+    ey = np.zeros_like(ex) if ey is None else _c128(ey).copy()
+    e = np.ascontiguousarray(eps, dtype=np.complex128)
+    _check(load().wos_hip_propagate_anisotropic(_p(ex), _p(ey), ex.shape[0], dx, z, e.ctypes.data, C.byref(ctx),
+                                                precision), "propagate_anisotropic")
+    return ex, ey
 
 
 def propagate_uniaxial(ex, ey, dx, z, no, ne, ctx, precision=FP64):
