@@ -202,19 +202,27 @@ int wos_hip_batch_download(wos_batch b, int unit, double* ex);
  * (optics/coherence.go:22-80 with a counter-based normal generator; unit u
  * is realisation u of `seed`) */
 int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t seed);
-/* Autotune the launch shape of this batch over every built combination of:
- * the column-tile width of the intermediate layout (tile_w), the columns per
- * column-pass workgroup (col_w: tile_w, tile_w/2 or tile_w/4), the rows per
- * row-pass workgroup, the row groups of the pipelined row kernels (0: one
- * group, 2, 4) and, for one-group row kernels, split LDS exchanges (real and
- * imaginary parts in two rounds through half the LDS). Each is timed over
- * `iters` runs with HIP events on the batch stream; the fastest is kept (a
- * captured graph is dropped). The runs propagate the batch's fields: refill
- * them afterwards. shape (may be NULL) receives {tile_w, col_w, rows,
- * groups, split}; best_ms (may be NULL) its time. */
+/* Autotune the launch shape of this batch. Shapes are combinations of: the
+ * column-tile width of the intermediate layout (tile_w); for the row passes
+ * the FFT stage schedule (radix: 16 = radix-16 stages with 16 values per
+ * thread; 8 = radix-8 stages with 8 values per thread and twice the
+ * threads, built for power-of-two grids of 256..8192), the rows per
+ * workgroup, the row groups of the pipelined kernels (0: one group, 2, 4)
+ * and, for radix-16 one-group kernels, split LDS exchanges (real and
+ * imaginary parts in two rounds through half the LDS); for the column pass
+ * its own stage schedule (col_radix) and the columns per workgroup (col_w:
+ * tile_w, tile_w/2 or tile_w/4). For each tile_w the row and column shapes
+ * are searched apart, each kernel timed over `iters` launches with HIP
+ * events on the batch stream, and the tile_w with the smallest sum wins (a
+ * captured graph is dropped). The launches propagate the batch's fields:
+ * refill them afterwards. shape (may be NULL, else 7 ints) receives
+ * {tile_w, col_w, rows, groups, split, radix, col_radix}; best_ms (may be
+ * NULL) the time of one whole run at that shape. */
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
-/* Set the launch shape directly (e.g. from a cached autotune table). */
-int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split);
+/* Set the launch shape directly (e.g. from a cached autotune table);
+ * radix / col_radix 0 mean 16. */
+int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
+                             int col_radix);
 /* capture one run into a hipGraph; later runs replay it */
 int wos_hip_batch_capture(wos_batch b);
 /* enqueue `iters` runs on the batch's stream (asynchronous) */

@@ -370,7 +370,7 @@ def test_batch_launch_shapes_bit_identical(wos, precision):
     ref = None
     for tw, cw, rows, groups, split in shapes:
         if tw:
-            b.set_tuning(tw, cw, rows, groups, split)
+            b.set_tuning(tw, cw, rows, groups, split, 16)
         b.fill_schell(100e-6, 30e-6, 3)
         b.run(1)
         b.sync()
@@ -384,4 +384,60 @@ def test_batch_launch_shapes_bit_identical(wos, precision):
                 else:
                     e = rel_l2(out[u], ref[u])
                     assert e <= (1e-15 if prec == wos.FP64 else 1e-6), (tw, cw, rows, groups, split, u, e)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("n,method", [(256, "asm"), (1024, "asm"), (512, "asm_pad"), (2048, "asm")])
+def test_batch_radix8_schedule(wos, oracle, precision, n, method):
+    """The radix-8 stage schedule (8 values per thread, radix-8 stages and a
+    radix-4/2 remainder) is another factorisation of the same DFTs: under
+    each launch shape it builds, for the row passes, the column pass or
+    both, every unit of a batch run matches the radix-16 run to rounding
+    and, up to 1024 points, the oracle's Propagate at the unit's wavelength
+    and distance (the FP64 and FP32 tolerances). asm_pad covers the
+    zero-padded embed window (M = 2n) of the row passes."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    M = 2 * n if method == "asm_pad" else n
+    dx, wls, zs = 2e-6, np.array([450e-9, 633e-9]), np.array([5e-3, 9e-3])
+    rng = np.random.default_rng(n)
+    srcs = [oracle.gaussian(n, dx, n * dx / 6) * np.exp(1j * rng.uniform(-0.3, 0.3, (n, n))) for _ in range(2)]
+    b = wos.Batch(n, dx, wls, zs, wos.ctx_for(550e-9), method, prec)
+
+    def run(shape):
+        b.set_tuning(*shape)
+        for u, f in enumerate(srcs):
+            b.upload(u, f)
+        b.run(1)
+        b.sync()
+        return [b.download(u) for u in range(2)]
+
+    ref16 = run((4, 4, 0, 0, 0, 16, 16))
+    # (layout tile width, columns per column workgroup, rows per row workgroup, row groups)
+    shapes8 = [(4, 4, 0, 0), (8, 4, 2, 2), (8, 4, 4, 4), (16, 4, 1, 0)]
+    if M >= 1024:   # one- and two-column workgroups: 1024 points and up
+        shapes8 += [(2, 1, 1, 2), (4, 2, 4, 4)]
+    # the row and column passes take their schedules independently
+    runs = [(tw, cw, rows, groups, 0, 8, 8) for tw, cw, rows, groups in shapes8]
+    runs += [(4, 4, 0, 0, 1, 16, 8), (8, 4, 2, 2, 0, 8, 16)]
+    outs = {}
+    for shape in runs:
+        out = outs[shape] = run(shape)
+        for u in range(2):
+            e = rel_l2(out[u], ref16[u])
+            assert e <= (1e-14 if prec == wos.FP64 else 1e-6), (n, method, shape, u, e)
+    if n <= 1024:
+        out = outs[runs[0]]   # radix-8 row and column passes
+        for u in range(2):
+            want, _, _, _ = oracle.propagate(srcs[u], None, dx, zs[u], method, oracle.ctx_for(wls[u]))
+            e = rel_l2(out[u], want)
+            print(f"radix-8 {method} n={n} {precision} unit {u} rel-L2 vs oracle {e:.3e}", flush=True)
+            assert e <= (FP64_TOL if prec == wos.FP64 else FP32_TOL), (n, method, u, e)
+    if n == 256:
+        tune, ms = b.autotune(1)
+        assert tune["radix"] in (8, 16) and tune["col_radix"] in (8, 16) and ms > 0
+        if tune["radix"] == 8:
+            assert tune["split_lds"] == 0
+    with pytest.raises(RuntimeError):
+        b.set_tuning(4, 4, 0, 0, 1, 8, 16)   # split exchanges are radix-16 row kernels only
     b.close()

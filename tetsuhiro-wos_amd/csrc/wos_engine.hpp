@@ -73,14 +73,15 @@ int host_to_dev(void* dst, const void* src, size_t bytes, hipStream_t s);
 int dev_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Stage twiddle tables of an n-point transform in precision prec (0 = FP64);
-// acc = 1: compensated [hi, lo] pairs.
-int get_twiddles(int n, int prec, const void** out, int acc = 0);
+// acc = 1: compensated [hi, lo] pairs; radix: stage schedule (16 or 8).
+int get_twiddles(int n, int prec, const void** out, int acc = 0, int radix = 16);
 
 // ---- launchers (return 0 or a WOS_ERR_* code) ----
 // Row pass on `units` units of rows x N; rows = rows per workgroup (0: default).
+// a.radix = 8 selects the radix-8 schedule's kernels (tables to match).
 int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows = 0);
-bool rows_valid(int prec, int N, int rows);
-bool col_w_valid(int prec, int N, int W);
+bool rows_valid(int prec, int N, int rows, int radix = 16);
+bool col_w_valid(int prec, int N, int W, int radix = 16);
 // Column pass; mode 0 plain (dir), 1 fused forward/TF/inverse. W = 0: default.
 int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int default_col_w(int prec, int N, int cols);

@@ -2,9 +2,10 @@
 //
 // Stockham autosort FFT over a power-of-two length N held by a block of
 // threads. Each thread owns V = 16 complex values (4 butterflies of radix 4,
-// 2 of radix 8, or 1 of radix 16); stages exchange data through LDS, the
-// first stage reads global memory directly and the last stage writes global
-// memory directly, both fully coalesced. Twiddles come from per-stage tables
+// 2 of radix 8, or 1 of radix 16), or V = 8 under the radix-8 schedule;
+// stages exchange data through LDS, the first stage reads global memory
+// directly and the last stage writes global memory directly, both fully
+// coalesced. Twiddles come from per-stage tables
 // generated on the device in FP64: for stage (RAD, Ns) entry [k][r-1] holds
 // exp(-2 pi i r k / (Ns RAD)), so a butterfly reads its RAD-1 twiddles from
 // one base address.
@@ -179,27 +180,32 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
 
 // ---------------------------------------------------------------------------
 // Radix plan for a power-of-two length N: values per thread V, stage radices.
-// Forward: [16, 16, ..., rem]; inverse: [rem, 16, ..., 16], so the last
+// VV selects the schedule: radix-16 stages with 16 values per thread, or
+// radix-8 stages with 8 values per thread (twice the threads per transform,
+// half the registers, one more LDS exchange per 2^12 points).
+// Forward: [VV, VV, ..., rem]; inverse: [rem, VV, ..., VV], so the last
 // forward stage and the first inverse stage share the same butterfly map and
 // a fused forward->multiply->inverse keeps the data in registers.
 // ---------------------------------------------------------------------------
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
-template <int N>
+template <int N, int VV = 16>
 struct Plan {
+  static_assert(VV == 16 || VV == 8, "radix-16 or radix-8 schedule");
   static constexpr int LOG = ilog2(N);
-  static constexpr int V = N >= 16 ? 16 : N;            // values per thread
+  static constexpr int LV = ilog2(VV);
+  static constexpr int V = N >= VV ? VV : N;            // values per thread
   static constexpr int T = N / V;                        // threads per transform
-  static constexpr int N16 = LOG / 4;                    // radix-16 stages
-  static constexpr int REM = 1 << (LOG % 4);             // leftover radix (1: none)
-  static constexpr int S = N16 + (REM > 1 ? 1 : 0);      // number of stages
+  static constexpr int NF = LOG / LV;                    // full-radix (VV) stages
+  static constexpr int REM = 1 << (LOG % LV);            // leftover radix (1: none)
+  static constexpr int S = NF + (REM > 1 ? 1 : 0);       // number of stages
   // radix of stage s, forward order
   static constexpr int fwd_rad(int s) {
-    return N < 16 ? N : (s < N16 ? 16 : REM);
+    return N < VV ? N : (s < NF ? VV : REM);
   }
   // radix of stage s, inverse order
   static constexpr int inv_rad(int s) {
-    return N < 16 ? N : (REM > 1 ? (s == 0 ? REM : 16) : 16);
+    return N < VV ? N : (REM > 1 ? (s == 0 ? REM : VV) : VV);
   }
   // sub-transform length before stage s
   static constexpr int ns_fwd(int s) {
@@ -256,9 +262,9 @@ constexpr int row_lds_stride(int n, int rows) { return lds_stride_elems<cpx<R>>(
 // conjugated for the inverse direction.
 // ACC: the table holds [hi, lo] pairs (2 entries per twiddle) and the
 // product is v*hi + v*lo.
-template <typename R, int N, int DIR, int RAD, int NS, int OFF, bool ACC = false>
+template <typename R, int N, int DIR, int RAD, int NS, int OFF, bool ACC = false, int VV = 16>
 __device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __restrict__ tws) {
-  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int V = Plan<N, VV>::V, T = Plan<N, VV>::T;
   constexpr int NB = V / RAD;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -293,9 +299,9 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int t, const cpx<R>* __
 
 // Write a stage's outputs to LDS (Stockham positions), element stride ES
 // between consecutive positions of one transform (ES = W for column tiles).
-template <typename R, int N, int RAD, int NS, int ES>
+template <typename R, int N, int RAD, int NS, int ES, int VV = 16>
 __device__ __forceinline__ void stage_store_lds(const cpx<R>* v, int t, cpx<R>* lds) {
-  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int V = Plan<N, VV>::V, T = Plan<N, VV>::T;
   constexpr int NB = V / RAD;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -310,9 +316,9 @@ __device__ __forceinline__ void stage_store_lds(const cpx<R>* v, int t, cpx<R>* 
 // One component (COMP 0: real, 1: imaginary) of a stage's outputs / the
 // next stage's inputs, through an LDS array of R: the exchange of a split
 // transform takes two rounds but half the LDS.
-template <typename R, int N, int RAD, int NS, int ES, int COMP>
+template <typename R, int N, int RAD, int NS, int ES, int COMP, int VV = 16>
 __device__ __forceinline__ void stage_store_lds_c(const cpx<R>* v, int t, R* lds) {
-  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int V = Plan<N, VV>::V, T = Plan<N, VV>::T;
   constexpr int NB = V / RAD;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -324,9 +330,9 @@ __device__ __forceinline__ void stage_store_lds_c(const cpx<R>* v, int t, R* lds
   }
 }
 
-template <typename R, int N, int RAD, int ES, int COMP>
+template <typename R, int N, int RAD, int ES, int COMP, int VV = 16>
 __device__ __forceinline__ void stage_load_lds_c(cpx<R>* v, int t, const R* lds) {
-  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int V = Plan<N, VV>::V, T = Plan<N, VV>::T;
   constexpr int NB = V / RAD;
   constexpr int X = N / RAD;
 #pragma unroll
@@ -343,9 +349,9 @@ __device__ __forceinline__ void stage_load_lds_c(cpx<R>* v, int t, const R* lds)
 }
 
 // Read the next stage's inputs from LDS: v[b*RAD + r] = x[j_b + r N/RAD].
-template <typename R, int N, int RAD, int ES>
+template <typename R, int N, int RAD, int ES, int VV = 16>
 __device__ __forceinline__ void stage_load_lds(cpx<R>* v, int t, const cpx<R>* lds) {
-  constexpr int V = Plan<N>::V, T = Plan<N>::T;
+  constexpr int V = Plan<N, VV>::V, T = Plan<N, VV>::T;
   constexpr int NB = V / RAD;
   constexpr int X = N / RAD;
 #pragma unroll
@@ -359,51 +365,51 @@ __device__ __forceinline__ void stage_load_lds(cpx<R>* v, int t, const cpx<R>* l
 
 // Position (within the transform) held by slot (b, r) at the start of the
 // first stage of radix RAD0, and at the end of the last stage of radix RADL.
-template <int N, int RAD>
+template <int N, int RAD, int VV = 16>
 __device__ __forceinline__ int slot_pos(int t, int b, int r) {
-  return t + b * Plan<N>::T + r * (N / RAD);
+  return t + b * Plan<N, VV>::T + r * (N / RAD);
 }
 
 // Run stages [S0, S1) of the forward (DIR<0) or inverse (DIR>0) plan with
 // LDS exchanges between them. On entry v holds the inputs of stage S0 in its
 // butterfly order; on exit v holds the outputs of stage S1-1 in the order
 // "slot (b, r) = position t + b T + r N/RAD_last" (valid for the last stage).
-template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false, bool SPLIT = false>
+template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false, bool SPLIT = false, int VV = 16>
 __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
   if constexpr (S < SEND) {
-    constexpr int RAD = DIR < 0 ? Plan<N>::fwd_rad(S) : Plan<N>::inv_rad(S);
-    constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
-    constexpr int OFF = DIR < 0 ? Plan<N>::tw_off_fwd(S) : Plan<N>::tw_off_inv(S);
-    stage_compute<R, N, DIR, RAD, NS, OFF, ACC>(v, t, tw);
+    constexpr int RAD = DIR < 0 ? Plan<N, VV>::fwd_rad(S) : Plan<N, VV>::inv_rad(S);
+    constexpr int NS = DIR < 0 ? Plan<N, VV>::ns_fwd(S) : Plan<N, VV>::ns_inv(S);
+    constexpr int OFF = DIR < 0 ? Plan<N, VV>::tw_off_fwd(S) : Plan<N, VV>::tw_off_inv(S);
+    stage_compute<R, N, DIR, RAD, NS, OFF, ACC, VV>(v, t, tw);
     if constexpr (S + 1 < SEND) {
-      constexpr int RADN = DIR < 0 ? Plan<N>::fwd_rad(S + 1) : Plan<N>::inv_rad(S + 1);
+      constexpr int RADN = DIR < 0 ? Plan<N, VV>::fwd_rad(S + 1) : Plan<N, VV>::inv_rad(S + 1);
       if constexpr (SPLIT) {  // lds holds lds_len(N) values of R per transform
         R* lr = (R*)lds;
         __syncthreads();  // previous readers of lds are done
-        stage_store_lds_c<R, N, RAD, NS, ES, 0>(v, t, lr);
+        stage_store_lds_c<R, N, RAD, NS, ES, 0, VV>(v, t, lr);
         __syncthreads();
-        stage_load_lds_c<R, N, RADN, ES, 0>(v, t, lr);
+        stage_load_lds_c<R, N, RADN, ES, 0, VV>(v, t, lr);
         __syncthreads();
-        stage_store_lds_c<R, N, RAD, NS, ES, 1>(v, t, lr);
+        stage_store_lds_c<R, N, RAD, NS, ES, 1, VV>(v, t, lr);
         __syncthreads();
-        stage_load_lds_c<R, N, RADN, ES, 1>(v, t, lr);
+        stage_load_lds_c<R, N, RADN, ES, 1, VV>(v, t, lr);
       } else {
         __syncthreads();  // previous readers of lds are done
-        stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
+        stage_store_lds<R, N, RAD, NS, ES, VV>(v, t, lds);
         __syncthreads();
-        stage_load_lds<R, N, RADN, ES>(v, t, lds);
+        stage_load_lds<R, N, RADN, ES, VV>(v, t, lds);
       }
-      run_stages<R, N, DIR, S + 1, SEND, ES, ACC, SPLIT>(v, t, tw, lds);
+      run_stages<R, N, DIR, S + 1, SEND, ES, ACC, SPLIT, VV>(v, t, tw, lds);
     }
   }
 }
 
 // First-stage radix (input map) and last-stage radix (output map).
-template <int N, int DIR>
-constexpr int first_rad() { return DIR < 0 ? Plan<N>::fwd_rad(0) : Plan<N>::inv_rad(0); }
-template <int N, int DIR>
+template <int N, int DIR, int VV = 16>
+constexpr int first_rad() { return DIR < 0 ? Plan<N, VV>::fwd_rad(0) : Plan<N, VV>::inv_rad(0); }
+template <int N, int DIR, int VV = 16>
 constexpr int last_rad() {
-  return DIR < 0 ? Plan<N>::fwd_rad(Plan<N>::S - 1) : Plan<N>::inv_rad(Plan<N>::S - 1);
+  return DIR < 0 ? Plan<N, VV>::fwd_rad(Plan<N, VV>::S - 1) : Plan<N, VV>::inv_rad(Plan<N, VV>::S - 1);
 }
 
 // Full N-point transform of the values in v (input map of first_rad, output
@@ -411,9 +417,9 @@ constexpr int last_rad() {
 // ACC: compensated twiddles and constants (tw then points to a [hi, lo] table).
 // SPLIT: exchanges real and imaginary parts in two rounds through half the
 // LDS (lds then holds lds_len(N) values of R per transform, not complex).
-template <typename R, int N, int DIR, int ES, bool ACC = false, bool SPLIT = false>
+template <typename R, int N, int DIR, int ES, bool ACC = false, bool SPLIT = false, int VV = 16>
 __device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
-  run_stages<R, N, DIR, 0, Plan<N>::S, ES, ACC, SPLIT>(v, t, tw, lds);
+  run_stages<R, N, DIR, 0, Plan<N, VV>::S, ES, ACC, SPLIT, VV>(v, t, tw, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -425,10 +431,10 @@ __device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __rest
 // 16, w^4 in FP64 (w = exp(-2 pi i k/(NS RAD))); the other powers are FP64
 // products of these (at most four deep, error ~1e-15) rounded once to R.
 // ---------------------------------------------------------------------------
-template <typename R, int N, int DIR>
+template <typename R, int N, int DIR, int VV = 16>
 struct TwReg {
-  static constexpr int V = Plan<N>::V;
-  static constexpr int rad(int s) { return DIR < 0 ? Plan<N>::fwd_rad(s) : Plan<N>::inv_rad(s); }
+  static constexpr int V = Plan<N, VV>::V;
+  static constexpr int rad(int s) { return DIR < 0 ? Plan<N, VV>::fwd_rad(s) : Plan<N, VV>::inv_rad(s); }
   static constexpr int seeds(int s) { return rad(s) >= 8 ? 2 : 1; }
   // register index of stage s's first seed (stages 1 .. S-1)
   static constexpr int base(int s) {
@@ -436,33 +442,33 @@ struct TwReg {
     for (int i = 1; i < s; ++i) c += (V / rad(i)) * seeds(i);
     return c;
   }
-  static constexpr int total() { return base(Plan<N>::S); }
+  static constexpr int total() { return base(Plan<N, VV>::S); }
   cpx<double> w[total() > 0 ? total() : 1];
 };
 
-template <typename R, int N, int DIR, int S>
-__device__ __forceinline__ void tw_load_stages(TwReg<R, N, DIR>& tr, int t, const cpx<double>* __restrict__ tws) {
-  if constexpr (S < Plan<N>::S) {
-    constexpr int RAD = TwReg<R, N, DIR>::rad(S);
-    constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
-    constexpr int OFF = DIR < 0 ? Plan<N>::tw_off_fwd(S) : Plan<N>::tw_off_inv(S);
-    constexpr int B = TwReg<R, N, DIR>::base(S);
-    constexpr int SE = TwReg<R, N, DIR>::seeds(S);
-    constexpr int NB = Plan<N>::V / RAD;
+template <typename R, int N, int DIR, int S, int VV = 16>
+__device__ __forceinline__ void tw_load_stages(TwReg<R, N, DIR, VV>& tr, int t, const cpx<double>* __restrict__ tws) {
+  if constexpr (S < Plan<N, VV>::S) {
+    constexpr int RAD = TwReg<R, N, DIR, VV>::rad(S);
+    constexpr int NS = DIR < 0 ? Plan<N, VV>::ns_fwd(S) : Plan<N, VV>::ns_inv(S);
+    constexpr int OFF = DIR < 0 ? Plan<N, VV>::tw_off_fwd(S) : Plan<N, VV>::tw_off_inv(S);
+    constexpr int B = TwReg<R, N, DIR, VV>::base(S);
+    constexpr int SE = TwReg<R, N, DIR, VV>::seeds(S);
+    constexpr int NB = Plan<N, VV>::V / RAD;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int k = (t + b * Plan<N>::T) & (NS - 1);
+      const int k = (t + b * Plan<N, VV>::T) & (NS - 1);
       tr.w[B + b * SE] = tws[OFF + k * (RAD - 1)];               // w^1
       if constexpr (SE == 2) tr.w[B + b * SE + 1] = tws[OFF + k * (RAD - 1) + 3];  // w^4
     }
-    tw_load_stages<R, N, DIR, S + 1>(tr, t, tws);
+    tw_load_stages<R, N, DIR, S + 1, VV>(tr, t, tws);
   }
 }
 
 // tws: the FP64 stage twiddle table of N points (any kernel precision)
-template <typename R, int N, int DIR>
-__device__ __forceinline__ void tw_load(TwReg<R, N, DIR>& tr, int t, const cpx<double>* __restrict__ tws) {
-  tw_load_stages<R, N, DIR, 1>(tr, t, tws);
+template <typename R, int N, int DIR, int VV = 16>
+__device__ __forceinline__ void tw_load(TwReg<R, N, DIR, VV>& tr, int t, const cpx<double>* __restrict__ tws) {
+  tw_load_stages<R, N, DIR, 1, VV>(tr, t, tws);
 }
 
 // powers w^1 .. w^(RAD-1) of a block from its seeds, rounded to R
@@ -492,17 +498,17 @@ __device__ __forceinline__ void tw_expand(const cpx<double>* sd, cpx<R>* w) {
   }
 }
 
-template <typename R, int N, int DIR, int S, int ES>
-__device__ __forceinline__ void run_stages_reg(cpx<R>* v, int t, const TwReg<R, N, DIR>& tr, cpx<R>* lds) {
-  if constexpr (S < Plan<N>::S) {
-    constexpr int RAD = TwReg<R, N, DIR>::rad(S);
-    constexpr int NS = DIR < 0 ? Plan<N>::ns_fwd(S) : Plan<N>::ns_inv(S);
-    constexpr int NB = Plan<N>::V / RAD;
+template <typename R, int N, int DIR, int S, int ES, int VV = 16>
+__device__ __forceinline__ void run_stages_reg(cpx<R>* v, int t, const TwReg<R, N, DIR, VV>& tr, cpx<R>* lds) {
+  if constexpr (S < Plan<N, VV>::S) {
+    constexpr int RAD = TwReg<R, N, DIR, VV>::rad(S);
+    constexpr int NS = DIR < 0 ? Plan<N, VV>::ns_fwd(S) : Plan<N, VV>::ns_inv(S);
+    constexpr int NB = Plan<N, VV>::V / RAD;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       if constexpr (NS > 1) {
-        constexpr int B = TwReg<R, N, DIR>::base(S);
-        constexpr int SE = TwReg<R, N, DIR>::seeds(S);
+        constexpr int B = TwReg<R, N, DIR, VV>::base(S);
+        constexpr int SE = TwReg<R, N, DIR, VV>::seeds(S);
         cpx<R> w[RAD];
         tw_expand<R, RAD>(tr.w + B + b * SE, w);
 #pragma unroll
@@ -511,21 +517,21 @@ __device__ __forceinline__ void run_stages_reg(cpx<R>* v, int t, const TwReg<R, 
       }
       dft<RAD, DIR>(v + b * RAD);
     }
-    if constexpr (S + 1 < Plan<N>::S) {
-      constexpr int RADN = TwReg<R, N, DIR>::rad(S + 1);
+    if constexpr (S + 1 < Plan<N, VV>::S) {
+      constexpr int RADN = TwReg<R, N, DIR, VV>::rad(S + 1);
       __syncthreads();
-      stage_store_lds<R, N, RAD, NS, ES>(v, t, lds);
+      stage_store_lds<R, N, RAD, NS, ES, VV>(v, t, lds);
       __syncthreads();
-      stage_load_lds<R, N, RADN, ES>(v, t, lds);
-      run_stages_reg<R, N, DIR, S + 1, ES>(v, t, tr, lds);
+      stage_load_lds<R, N, RADN, ES, VV>(v, t, lds);
+      run_stages_reg<R, N, DIR, S + 1, ES, VV>(v, t, tr, lds);
     }
   }
 }
 
 // fft_block with register twiddle seeds (same maps as fft_block).
-template <typename R, int N, int DIR, int ES>
-__device__ __forceinline__ void fft_block_reg(cpx<R>* v, int t, const TwReg<R, N, DIR>& tr, cpx<R>* lds) {
-  run_stages_reg<R, N, DIR, 0, ES>(v, t, tr, lds);
+template <typename R, int N, int DIR, int ES, int VV = 16>
+__device__ __forceinline__ void fft_block_reg(cpx<R>* v, int t, const TwReg<R, N, DIR, VV>& tr, cpx<R>* lds) {
+  run_stages_reg<R, N, DIR, 0, ES, VV>(v, t, tr, lds);
 }
 
 }  // namespace wos

@@ -36,6 +36,14 @@ bool col_w_valid_f64(int N, int W);
 bool col_w_valid_f32(int N, int W);
 int launch_rows_bs_f64(int M, const BsRowArgs& A, hipStream_t s);
 int launch_rows_bs_f32(int M, const BsRowArgs& A, hipStream_t s);
+int launch_rows8_f64(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
+int launch_rows8_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
+bool rows_valid8_f64(int N, int rows);
+bool rows_valid8_f32(int N, int rows);
+int launch_cols8_f64(int N, int W, const ColArgs& a, hipStream_t s);
+int launch_cols8_f32(int N, int W, const ColArgs& a, hipStream_t s);
+bool col_w_valid8_f64(int N, int W);
+bool col_w_valid8_f32(int N, int W);
 int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int launch_cols_bs_f32(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int bs_default_w_f64(int M, int cols);
@@ -284,40 +292,43 @@ int ensure_init() {
   return 0;
 }
 
-// Host mirror of Plan<N> (wos_fft.hpp): stage radices of the forward and
-// inverse plans.
-static void plan_stages(int n, std::vector<int>& fwd, std::vector<int>& inv) {
+// Host mirror of Plan<N, radix> (wos_fft.hpp): stage radices of the forward
+// and inverse plans under the radix-16 or radix-8 schedule.
+static void plan_stages(int n, int radix, std::vector<int>& fwd, std::vector<int>& inv) {
   fwd.clear();
   inv.clear();
-  if (n < 16) {
+  if (n < radix) {
     fwd.push_back(n);
     inv.push_back(n);
     return;
   }
-  int lg = 0;
+  int lg = 0, lv = 0;
   while ((1 << lg) < n) ++lg;
-  const int n16 = lg / 4, rem = 1 << (lg % 4);
-  for (int i = 0; i < n16; ++i) fwd.push_back(16);
+  while ((1 << lv) < radix) ++lv;
+  const int nf = lg / lv, rem = 1 << (lg % lv);
+  for (int i = 0; i < nf; ++i) fwd.push_back(radix);
   if (rem > 1) {
     fwd.push_back(rem);
     inv.push_back(rem);
   }
-  for (int i = 0; i < n16; ++i) inv.push_back(16);
+  for (int i = 0; i < nf; ++i) inv.push_back(radix);
 }
 
 // Stage twiddle tables of an n-point transform (forward stages 1..S-1, then
 // inverse stages 1..S-1, the layout Plan<N>::tw_off_* addresses), cached.
-// acc: [hi, lo] pairs for the compensated split-step kernels.
-int get_twiddles(int n, int prec, const void** out, int acc) {
+// acc: [hi, lo] pairs for the compensated split-step kernels; radix: the
+// stage schedule (16 or 8) whose stages the table holds.
+int get_twiddles(int n, int prec, const void** out, int acc, int radix) {
   Device& d = device();
-  long long key = ((long long)n * 2 + prec) * 2 + (acc ? 1 : 0);
+  if (radix != 8) radix = 16;
+  long long key = (((long long)n * 2 + prec) * 2 + (acc ? 1 : 0)) * 2 + (radix == 8 ? 1 : 0);
   auto it = d.twiddles.find(key);
   if (it != d.twiddles.end()) {
     *out = it->second;
     return 0;
   }
   std::vector<int> fwd, inv;
-  plan_stages(n, fwd, inv);
+  plan_stages(n, radix, fwd, inv);
   struct St { int off, ns, rad; };
   std::vector<St> st;
   int off = 0;
@@ -451,16 +462,27 @@ static int rows_any(int prec, void* d, int L, int rows, int ld, long long unit, 
 }
 
 int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows) {
+  if (a.radix == 8)
+    return prec == WOS_FP64 ? launch_rows8_f64(N, dir, rows, a, s) : launch_rows8_f32(N, dir, rows, a, s);
   return prec == WOS_FP64 ? launch_rows_f64(N, dir, rows, a, s) : launch_rows_f32(N, dir, rows, a, s);
 }
-bool rows_valid(int prec, int N, int rows) {
+bool rows_valid(int prec, int N, int rows, int radix) {
+  if (radix == 8) return prec == WOS_FP64 ? rows_valid8_f64(N, rows) : rows_valid8_f32(N, rows);
   return prec == WOS_FP64 ? rows_valid_f64(N, rows) : rows_valid_f32(N, rows);
 }
-bool col_w_valid(int prec, int N, int W) {
+bool col_w_valid(int prec, int N, int W, int radix) {
+  if (radix == 8) return prec == WOS_FP64 ? col_w_valid8_f64(N, W) : col_w_valid8_f32(N, W);
   return prec == WOS_FP64 ? col_w_valid_f64(N, W) : col_w_valid_f32(N, W);
 }
 int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
   if (W <= 0) W = default_col_w(prec, N, a.cols);
+  if (a.radix == 8) {
+    if (mode != 1) {
+      set_error("the radix-8 schedule builds the fused asm column pass only");
+      return WOS_ERR_UNSUPPORTED;
+    }
+    return prec == WOS_FP64 ? launch_cols8_f64(N, W, a, s) : launch_cols8_f32(N, W, a, s);
+  }
   return prec == WOS_FP64 ? launch_cols_f64(N, mode, dir, W, a, s) : launch_cols_f32(N, mode, dir, W, a, s);
 }
 int default_col_w(int prec, int N, int cols) {
@@ -476,9 +498,9 @@ static int bs_default_w(int prec, int M, int cols) {
   return prec == WOS_FP64 ? bs_default_w_f64(M, cols) : bs_default_w_f32(M, cols);
 }
 
-// rows handled by one row-pass workgroup (mirrors rows_per_block<N>())
-static int rows_per_block_host(int N) {
-  int V = N >= 16 ? 16 : N;
+// rows handled by one row-pass workgroup (mirrors rows_per_block<N, radix>())
+static int rows_per_block_host(int N, int radix = 16) {
+  int V = N >= radix ? radix : N;
   int T = N / V;
   return T >= 256 ? 1 : 256 / T;
 }
@@ -927,12 +949,19 @@ struct Tuning {
   int groups = 0;  // row groups per workgroup of the pipelined row kernels (<= 1: plain)
   int col_w = 0;   // columns per column-pass workgroup, dividing tw (0: tw)
   int split = 0;   // one-group row kernels exchange real and imaginary parts separately
+  int radix = 0;      // stage schedule of the row passes: 8 radix-8, else radix-16
+  int col_radix = 0;  // stage schedule of the column pass: 8 radix-8, else radix-16
 };
+
+// The radix-8 schedule is built for the asm transfer kernels from 256 points.
+static bool radix8_ok(int M) { return pow2(M) && M >= 256 && M <= 8192; }
 
 // Fill unset fields: the tile width and rows per workgroup that make one
 // tile chunk of ROWS x TW elements a whole 128-byte line.
 static Tuning resolve_tuning(int prec, int M, Tuning t) {
   const int S = (int)csize(prec);
+  t.radix = t.radix == 8 && radix8_ok(M) ? 8 : 16;
+  t.col_radix = t.col_radix == 8 && radix8_ok(M) ? 8 : 16;
   if (t.tw <= 0) {
     int cand[] = {prec == WOS_FP32 ? 4 : 2, 4, 8, 16, 2, 1};
     for (int w : cand)
@@ -941,12 +970,15 @@ static Tuning resolve_tuning(int prec, int M, Tuning t) {
         break;
       }
   }
-  if (t.col_w <= 0 || t.col_w > t.tw || (t.tw % t.col_w) != 0 || !col_w_valid(prec, M, t.col_w)) t.col_w = t.tw;
+  if (t.col_w <= 0 || t.col_w > t.tw || (t.tw % t.col_w) != 0 || !col_w_valid(prec, M, t.col_w, t.col_radix)) {
+    t.col_w = t.tw;  // the widest column workgroup the schedule builds
+    while (t.col_w > 1 && !col_w_valid(prec, M, t.col_w, t.col_radix)) t.col_w /= 2;
+  }
   if (t.rows <= 0) {
-    const int rd = rows_per_block_host(M);
+    const int rd = rows_per_block_host(M, t.radix);
     int want = std::max(rd, 128 / (S * std::max(1, t.tw)));
-    while (want > rd && !rows_valid(prec, M, want)) want /= 2;
-    t.rows = rows_valid(prec, M, want) ? want : rd;
+    while (want > rd && !rows_valid(prec, M, want, t.radix)) want /= 2;
+    t.rows = rows_valid(prec, M, want, t.radix) ? want : rd;
   }
   return t;
 }
@@ -987,15 +1019,20 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   if (bs) {
     tu.tw = bs_default_w(prec, Mk, M);
     tu.rows = rows_per_block_host(Mk);
+    tu.radix = tu.col_radix = 16;
   } else {
     const bool lean = tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM;
-    if (!lean) tu.tw = default_col_w(prec, M, M);  // generic TFs use the default tile
+    if (!lean) {  // generic TFs use the default tile and schedule
+      tu.tw = default_col_w(prec, M, M);
+      tu.radix = tu.col_radix = 16;
+    }
     tu = resolve_tuning(prec, M, tu);
   }
-  const void *tw, *tw64;
-  int rc = get_twiddles(Mk, prec, &tw);
+  const void *tw, *tw64, *twc;
+  int rc = get_twiddles(Mk, prec, &tw, 0, tu.radix);
   if (rc) return rc;
-  if ((rc = get_twiddles(Mk, WOS_FP64, &tw64))) return rc;
+  if ((rc = get_twiddles(Mk, WOS_FP64, &tw64, 0, tu.radix))) return rc;
+  if ((rc = get_twiddles(Mk, prec, &twc, 0, tu.col_radix))) return rc;
   L.prec = prec;
   L.M = M;
   L.mode = (tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM) ? 1 : 2;
@@ -1028,6 +1065,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ra.tw64 = tw64;
   ra.groups = groups;
   ra.split = split;
+  ra.radix = tu.radix;
   ColArgs& ca = L.col;
   memset(&ca, 0, sizeof(ca));
   ca.data = inter;
@@ -1044,7 +1082,8 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ca.scale = 1.0;
   ca.xcd_remap = tu.xcd_remap;
   ca.tf = d_tf;
-  ca.tw = tw;
+  ca.tw = twc;
+  ca.radix = tu.col_radix;
   RowArgs& rb = L.inv;
   rb = row_args_init();
   rb.in = inter;
@@ -1068,6 +1107,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   rb.tw64 = tw64;
   rb.groups = groups;
   rb.split = split;
+  rb.radix = tu.radix;
   if (bs) {
     // fft2DCPU's per-pass 1/M of the inverse is applied by the Bluestein posts
     BsTab fwd_t, inv_t;
@@ -2444,36 +2484,113 @@ static int batch_time_runs(wos_batch_s* b, int iters, hipEvent_t e0, hipEvent_t 
   return 0;
 }
 
-// Launch shapes the autotuners try on a power-of-two grid of side M: layout
-// tile width tw, column-workgroup width col_w (tw, tw/2 or tw/4), rows per
-// row-pass workgroup, row groups of the pipelined row kernels.
-static std::vector<Tuning> tuning_candidates(int prec, int M, int n) {
-  std::vector<Tuning> out;
+}  // extern "C"
+
+// Time kernel k (0 forward rows, 1 columns, 2 inverse rows) of a built
+// transfer: one warm-up launch, then `iters` launches back to back between
+// two events on stream s.
+static int time_transfer_kernel(const TransferLaunch& L, int k, int iters, hipStream_t s, hipEvent_t e0,
+                                hipEvent_t e1, double* ms_out) {
+  int rc;
+  if ((rc = launch_transfer_kernel(L, k, s))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e0, s));
+  for (int i = 0; i < iters; ++i)
+    if ((rc = launch_transfer_kernel(L, k, s))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e1, s));
+  WOS_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms / iters;
+  return 0;
+}
+
+// Launch-shape search of one transfer step on a power-of-two grid of side M
+// (n rows per unit). The row passes and the column pass share only the
+// intermediate layout's tile width tw, so for each tw they are searched
+// apart, each kernel timed on its own:
+//   row passes (forward + inverse time): stage schedule (radix-16, or
+//     radix-8 from 256 points), rows per workgroup, row groups of the
+//     pipelined kernels, split exchanges (radix-16 one-group kernels);
+//   column pass: stage schedule, columns per workgroup (tw, tw/2 or tw/4).
+// The tw with the smallest sum of its best row and column times wins.
+// build(t, L) builds the three launches of shape t.
+template <typename Build>
+static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const Build& build, Tuning* out) {
   const int S = (int)csize(prec);
-  for (int tw = 1; tw <= 16; tw *= 2) {
-    if ((M % tw) != 0) continue;
-    for (int cw = tw; cw >= 1 && cw * 4 >= tw; cw /= 2) {
-      if (!col_w_valid(prec, M, cw)) continue;
-      for (int rows = 1; rows <= 16; rows *= 2) {
-        if (!rows_valid(prec, M, rows)) continue;
-        if (rows * tw * S < 64) continue;  // chunks under half a line: never competitive
-        for (int groups : {0, 2, 4}) {
-          if (groups && (M < 256 || n % (rows * groups) != 0)) continue;
-          for (int split = 0; split <= (groups == 0 && M >= 256 ? 1 : 0); ++split) {
-            Tuning t;
-            t.tw = tw;
-            t.col_w = cw;
-            t.rows = rows;
-            t.groups = groups;
-            t.split = split;
-            out.push_back(t);
+  hipEvent_t e0, e1;
+  WOS_HIP_TRY(hipEventCreate(&e0));
+  WOS_HIP_TRY(hipEventCreate(&e1));
+  double best = 1e300;
+  Tuning bt;
+  auto search = [&]() -> int {
+    int rc;
+    for (int tw = 1; tw <= 16; tw *= 2) {
+      if ((M % tw) != 0) continue;
+      double brow = 1e300;
+      Tuning tr;
+      for (int radix : {16, 8}) {
+        if (radix == 8 && !radix8_ok(M)) continue;
+        for (int rows = 1; rows <= 16; rows *= 2) {
+          if (!rows_valid(prec, M, rows, radix)) continue;
+          if (rows * tw * S < 64) continue;  // chunks under half a line: never competitive
+          for (int groups : {0, 2, 4}) {
+            if (groups && (M < 256 || n % (rows * groups) != 0)) continue;
+            const int max_split = groups == 0 && M >= 256 && radix == 16 ? 1 : 0;
+            for (int split = 0; split <= max_split; ++split) {
+              Tuning t;
+              t.tw = tw;
+              t.rows = rows;
+              t.groups = groups;
+              t.split = split;
+              t.radix = radix;
+              TransferLaunch L;
+              double f, i;
+              if ((rc = build(t, L))) return rc;
+              if ((rc = time_transfer_kernel(L, 0, iters, s, e0, e1, &f))) return rc;
+              if ((rc = time_transfer_kernel(L, 2, iters, s, e0, e1, &i))) return rc;
+              if (f + i < brow) {
+                brow = f + i;
+                tr = t;
+              }
+            }
           }
         }
       }
+      if (brow >= 1e300) continue;
+      double bcol = 1e300;
+      Tuning tc = tr;
+      for (int radix : {16, 8}) {
+        if (radix == 8 && !radix8_ok(M)) continue;
+        for (int cw = tw; cw >= 1 && cw * 4 >= tw; cw /= 2) {
+          if (!col_w_valid(prec, M, cw, radix)) continue;
+          Tuning t = tr;
+          t.col_w = cw;
+          t.col_radix = radix;
+          TransferLaunch L;
+          double c;
+          if ((rc = build(t, L))) return rc;
+          if ((rc = time_transfer_kernel(L, 1, iters, s, e0, e1, &c))) return rc;
+          if (c < bcol) {
+            bcol = c;
+            tc = t;
+          }
+        }
+      }
+      if (bcol < 1e300 && brow + bcol < best) {
+        best = brow + bcol;
+        bt = tc;
+      }
     }
-  }
-  return out;
+    return 0;
+  };
+  const int rc = search();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) *out = bt;  // no shape timed (tiny grids): the defaults
+  return rc;
 }
+
+extern "C" {
 
 static void tuning_out(const Tuning& t, int* shape) {
   if (!shape) return;
@@ -2482,6 +2599,8 @@ static void tuning_out(const Tuning& t, int* shape) {
   shape[2] = t.rows;
   shape[3] = t.groups;
   shape[4] = t.split;
+  shape[5] = t.radix == 8 ? 8 : 16;
+  shape[6] = t.col_radix == 8 ? 8 : 16;
 }
 
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) {
@@ -2499,36 +2618,43 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) 
     rc = batch_time_runs(b, iters, e0, e1, &best);
     bt = Tuning();
   } else {
-    rc = 0;
-    const Tuning saved = b->tune;
-    for (const Tuning& t : tuning_candidates(b->prec, b->M, b->n)) {
-      b->tune = t;
-      double per;
-      if ((rc = batch_time_runs(b, iters, e0, e1, &per))) break;
-      if (per < best) {
-        best = per;
-        bt = t;
-      }
+    auto build = [b](const Tuning& t, TransferLaunch& L) {
+      return build_transfer(L, b->prec, b->fields, b->inter, b->n, b->units, b->pad, b->d_tf, TF_ASM, 1.0, 0.0,
+                            nullptr, nullptr, t);
+    };
+    rc = tune_transfer(b->prec, b->M, b->n, iters, b->stream, build, &bt);
+    if (!rc) {
+      b->tune = bt;
+      rc = batch_time_runs(b, iters, e0, e1, &best);  // the whole step at the chosen shape
     }
-    b->tune = rc ? saved : bt;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  tuning_out(bt, shape);
+  tuning_out(pow2(b->M) ? resolve_tuning(b->prec, b->M, bt) : bt, shape);
   if (best_ms) *best_ms = best;
   return WOS_OK;
 }
 
-int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split) {
+int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
+                             int col_radix) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
+  if (radix == 0) radix = 16;
+  if (col_radix == 0) col_radix = 16;
+  if (!(radix == 16 || (radix == 8 && radix8_ok(b->M) && !split)) ||
+      !(col_radix == 16 || (col_radix == 8 && radix8_ok(b->M)))) {
+    set_error("wos_hip_batch_set_tuning: row radix " + std::to_string(radix) + (split ? " with split exchanges" : "") +
+              " or column radix " + std::to_string(col_radix) + " not built for a grid of " + std::to_string(b->M));
+    return WOS_ERR_INVALID;
+  }
   if (!pow2(b->M) || tile_w <= 0 || (b->M % tile_w) != 0 || col_w <= 0 || (tile_w % col_w) != 0 ||
-      !col_w_valid(b->prec, b->M, col_w) || (rows > 0 && !rows_valid(b->prec, b->M, rows)) ||
+      !col_w_valid(b->prec, b->M, col_w, col_radix) || (rows > 0 && !rows_valid(b->prec, b->M, rows, radix)) ||
       !(groups == 0 || groups == 1 || groups == 2 || groups == 4)) {
     set_error("wos_hip_batch_set_tuning: tile width " + std::to_string(tile_w) + ", column width " +
               std::to_string(col_w) + ", rows " + std::to_string(rows) + " or row groups " + std::to_string(groups) +
-              " not built for a grid of " + std::to_string(b->M));
+              " not built for a grid of " + std::to_string(b->M) + " under the radix-" + std::to_string(radix) +
+              " row / radix-" + std::to_string(col_radix) + " column schedules");
     return WOS_ERR_INVALID;
   }
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
@@ -2539,6 +2665,8 @@ int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int g
   b->tune.rows = rows;
   b->tune.groups = groups;
   b->tune.split = split ? 1 : 0;
+  b->tune.radix = radix;
+  b->tune.col_radix = col_radix;
   return WOS_OK;
 }
 

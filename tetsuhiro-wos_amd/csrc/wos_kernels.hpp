@@ -348,6 +348,7 @@ struct RowArgs {
   int groups;                   // row groups per workgroup of the pipelined kernel (<= 1: plain)
   int split;                    // one-group kernel with split (real, imaginary) LDS exchanges
   const void* tw64;             // FP64 twiddles of the N points (pipelined kernel)
+  int radix;                    // stage schedule of tw/tw64: 0 or 16 radix-16, 8 radix-8
 };
 
 // RowArgs with row-major addressing on both sides and no embed window.
@@ -359,9 +360,9 @@ inline RowArgs row_args_init() {
   return a;
 }
 
-template <int N>
+template <int N, int VV = 16>
 constexpr int rows_per_block() {
-  return Plan<N>::T >= 256 ? 1 : 256 / Plan<N>::T;
+  return Plan<N, VV>::T >= 256 ? 1 : 256 / Plan<N, VV>::T;
 }
 
 // Streaming accesses of the FFT passes: every field element is read once and
@@ -398,11 +399,11 @@ __device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int 
 // the data stays in registers).
 // SPLIT: the stage exchanges move real and imaginary parts in two rounds
 // through half the LDS, so more workgroups fit a CU.
-template <typename R, int N, int DIR, int ROWS, int SCR = 0, bool SPLIT = false>
-__global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
-  constexpr int T = Plan<N>::T, V = Plan<N>::V;
+template <typename R, int N, int DIR, int ROWS, int SCR = 0, bool SPLIT = false, int VV = 16>
+__global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows(const RowArgs a) {
+  constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
   static_assert(SCR == 0 || DIR > 0, "the screen follows an inverse transform");
-  constexpr int R0 = first_rad<N, DIR>(), RL = SCR ? last_rad<N, -1>() : last_rad<N, DIR>();
+  constexpr int R0 = first_rad<N, DIR, VV>(), RL = SCR ? last_rad<N, -1, VV>() : last_rad<N, DIR, VV>();
   constexpr int NT = ROWS * T;
   // per-row LDS stride in complex values (SPLIT: half of a stride in values
   // of R, rounded up so rows start complex-aligned; the multi-row strides
@@ -429,13 +430,13 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
       for (int r = 0; r < R0; ++r)
-        v[b * R0 + r] = gld(in + tiled_offset(slot_pos<N, R0>(t, b, r), row, a.in_ld, a.in_tw, a.in_tstride));
+        v[b * R0 + r] = gld(in + tiled_offset(slot_pos<N, R0, VV>(t, b, r), row, a.in_ld, a.in_tw, a.in_tstride));
   } else {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
-        const int q = slot_pos<N, R0>(t, b, r) - a.in_col0;
+        const int q = slot_pos<N, R0, VV>(t, b, r) - a.in_col0;
         cpx<R> x = {R(0), R(0)};
         if (active && q >= 0 && q < a.in_cols) x = in[tiled_offset(q, row, a.in_ld, a.in_tw, a.in_tstride)];
         v[b * R0 + r] = x;
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
   }
   constexpr bool ACC = SCR == 1 && sizeof(R) == 4;  // split-step slices: compensated twiddles
-  fft_block<R, N, DIR, 1, ACC, SPLIT>(v, t, tw, lds + lr * LS);
+  fft_block<R, N, DIR, 1, ACC, SPLIT, VV>(v, t, tw, lds + lr * LS);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   if (a.scale_im == 0) {
     if (a.scale_re != 1) {
@@ -460,8 +461,8 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);
   }
   if constexpr (SCR == 1) {
-    constexpr int RS = last_rad<N, +1>();
-    static_assert(RS == first_rad<N, -1>(), "screen map");
+    constexpr int RS = last_rad<N, +1, VV>();
+    static_assert(RS == first_rad<N, -1, VV>(), "screen map");
     // FP64 screen applied in FP64: the same table multiplies every slice
     const cpx<double>* scr = (const cpx<double>*)a.screen + (size_t)row * N;
 #pragma unroll
@@ -469,23 +470,23 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
 #pragma unroll
       for (int r = 0; r < RS; ++r) {
         const cpx<double> x = {(double)v[b * RS + r].x, (double)v[b * RS + r].y};
-        const cpx<double> y = cmul(x, scr[slot_pos<N, RS>(t, b, r)]);
+        const cpx<double> y = cmul(x, scr[slot_pos<N, RS, VV>(t, b, r)]);
         v[b * RS + r] = {(R)y.x, (R)y.y};
       }
-    fft_block<R, N, -1, 1, ACC, SPLIT>(v, t, tw, lds + lr * LS);
+    fft_block<R, N, -1, 1, ACC, SPLIT, VV>(v, t, tw, lds + lr * LS);
   }
   if (full_out) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r)
-        gst(out + tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
+        gst(out + tiled_offset(slot_pos<N, RL, VV>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
   } else {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const int q = slot_pos<N, RL>(t, b, r) - a.out_col0;
+        const int q = slot_pos<N, RL, VV>(t, b, r) - a.out_col0;
         if (active && q >= 0 && q < a.out_cols) out[tiled_offset(q, row, a.out_ld, a.out_tw, a.out_tstride)] = v[b * RL + r];
       }
   }
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const int q = slot_pos<N, RL>(t, b, r) - a.out_col0;
+        const int q = slot_pos<N, RL, VV>(t, b, r) - a.out_col0;
         const cpx<R> y = v[b * RL + r];
         if (active && q >= 0 && q < a.out_cols) pout += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
       }
@@ -509,10 +510,10 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows(const RowArgs a) {
 // in registers, so those loads stay in flight across the transform and the
 // stores: a CU keeps HBM requests outstanding while it computes, where the
 // one-group kernel alternates between loading and computing.
-template <typename R, int N, int DIR, int ROWS, int G>
-__global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a) {
-  constexpr int T = Plan<N>::T, V = Plan<N>::V;
-  constexpr int R0 = first_rad<N, DIR>(), RL = last_rad<N, DIR>();
+template <typename R, int N, int DIR, int ROWS, int G, int VV = 16>
+__global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows_pipe(const RowArgs a) {
+  constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
+  constexpr int R0 = first_rad<N, DIR, VV>(), RL = last_rad<N, DIR, VV>();
   constexpr int LS = row_lds_stride<R>(N, ROWS);
   __shared__ cpx<R> lds[ROWS * LS];
   const int tid = threadIdx.x;
@@ -522,8 +523,8 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
   const int row0 = (blockIdx.x % bpu) * ROWS * G + lr;
   const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
   cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
-  TwReg<R, N, DIR> tr;
-  tw_load<R, N, DIR>(tr, t, (const cpx<double>*)a.tw64);
+  TwReg<R, N, DIR, VV> tr;
+  tw_load<R, N, DIR, VV>(tr, t, (const cpx<double>*)a.tw64);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   const bool scale = a.scale_im != 0 || a.scale_re != 1;
   cpx<R> v[V], nx[V];
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
   for (int b = 0; b < V / R0; ++b)
 #pragma unroll
     for (int r = 0; r < R0; ++r)
-      v[b * R0 + r] = gld<false>(in + tiled_offset(slot_pos<N, R0>(t, b, r), row0, a.in_ld, a.in_tw, a.in_tstride));
+      v[b * R0 + r] = gld<false>(in + tiled_offset(slot_pos<N, R0, VV>(t, b, r), row0, a.in_ld, a.in_tw, a.in_tstride));
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int row = row0 + g * ROWS;
@@ -541,9 +542,9 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
 #pragma unroll
         for (int r = 0; r < R0; ++r)
           nx[b * R0 + r] =
-              gld<false>(in + tiled_offset(slot_pos<N, R0>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride));
+              gld<false>(in + tiled_offset(slot_pos<N, R0, VV>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride));
     }
-    fft_block_reg<R, N, DIR, 1>(v, t, tr, lds + lr * LS);
+    fft_block_reg<R, N, DIR, 1, VV>(v, t, tr, lds + lr * LS);
     if (scale) {
 #pragma unroll
       for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);
@@ -552,7 +553,7 @@ __global__ __launch_bounds__(ROWS* Plan<N>::T) void k_rows_pipe(const RowArgs a)
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r)
-        gst(out + tiled_offset(slot_pos<N, RL>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
+        gst(out + tiled_offset(slot_pos<N, RL, VV>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
     if (g + 1 < G) {
 #pragma unroll
       for (int i = 0; i < V; ++i) v[i] = nx[i];
@@ -586,6 +587,7 @@ struct ColArgs {
   double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2 (nullable)
   int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
   const void* htab;    // MODE 3: FP64 H table in W-column tile order (nullable: evaluate H)
+  int radix;           // stage schedule of tw: 0 or 16 radix-16, 8 radix-8 (MODE 1 only)
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
@@ -595,9 +597,9 @@ __device__ __forceinline__ int remap_xcd(int bid, int nb) {
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-template <typename R, int N, int W, int MODE, int DIR>
-__global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
-  constexpr int T = Plan<N>::T, V = Plan<N>::V;
+template <typename R, int N, int W, int MODE, int DIR, int VV = 16>
+__global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) {
+  constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
   __shared__ cpx<R> lds[lds_len(N) * W];
   const int tid = threadIdx.x;
   const int c = tid % W, t = tid / W;
@@ -618,34 +620,34 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
   const bool fastw = fast && tl == W;
 
   constexpr int DIR0 = MODE >= 1 ? -1 : DIR;
-  constexpr int R0 = first_rad<N, DIR0>();
+  constexpr int R0 = first_rad<N, DIR0, VV>();
   cpx<R> v[V];
   if (fast && !fastw) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0>(t, b, r) * tl);
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0, VV>(t, b, r) * tl);
   } else if (fastw) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0>(t, b, r) * W);
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0, VV>(t, b, r) * W);
   } else {
     const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
-        const int q = slot_pos<N, R0>(t, b, r) - a.in_row0;
+        const int q = slot_pos<N, R0, VV>(t, b, r) - a.in_row0;
         cpx<R> x = {R(0), R(0)};
         if (active && q >= 0 && q < a.rows) x = base[q * ld];
         v[b * R0 + r] = x;
       }
   }
   constexpr bool ACC = MODE == 3 && sizeof(R) == 4;  // split-step slices: compensated twiddles
-  fft_block<R, N, DIR0, W, ACC>(v, t, tw, lds + c);
+  fft_block<R, N, DIR0, W, ACC, false, VV>(v, t, tw, lds + c);
   if constexpr (MODE == 1 || MODE == 2) {
-    constexpr int RL = last_rad<N, -1>();
+    constexpr int RL = last_rad<N, -1, VV>();
     if (active) {
       const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
       // Without a spectral shift H(fx, -fy) = H(fx, fy), bit for bit (the
@@ -696,13 +698,13 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
         for (int b = 0; b < V / RL; ++b)
 #pragma unroll
           for (int r = 0; r < RL; ++r)
-            v[b * RL + r] = cmul(v[b * RL + r], asm_tf_fp32_prop(cc, slot_pos<N, RL>(t, b, r), u.M, &evan));
+            v[b * RL + r] = cmul(v[b * RL + r], asm_tf_fp32_prop(cc, slot_pos<N, RL, VV>(t, b, r), u.M, &evan));
         if (evan) {  // rare: rows past the evanescent cut-off
 #pragma unroll
           for (int b = 0; b < V / RL; ++b)
 #pragma unroll
             for (int r = 0; r < RL; ++r)
-              v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan(u, cc, slot_pos<N, RL>(t, b, r)));
+              v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan(u, cc, slot_pos<N, RL, VV>(t, b, r)));
         }
       } else {
         const bool conj = MODE == 1 && u.kind == TF_CONJ_ASM;
@@ -710,7 +712,7 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
         for (int b = 0; b < V / RL; ++b)
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
-            const int p = slot_pos<N, RL>(t, b, r);
+            const int p = slot_pos<N, RL, VV>(t, b, r);
             cpx<R> h;
             if constexpr (MODE == 1) {
               h = asm_tf_fp64_at(u, col, p);
@@ -729,7 +731,7 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
     // both precisions -- the same H multiplies the field every slice, so an
     // FP32-rounded H would accumulate its rounding linearly over the slices.
     // Optional spectral power sums |U|^2, |H U|^2, |H^2 U|^2 per workgroup.
-    constexpr int RL = last_rad<N, -1>();
+    constexpr int RL = last_rad<N, -1, VV>();
     double s0 = 0, s1 = 0, s2 = 0;
     const bool want_pw = a.pw3 != nullptr;
     if (active) {
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
       for (int b = 0; b < V / RL; ++b)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
-          const int p = slot_pos<N, RL>(t, b, r);
+          const int p = slot_pos<N, RL, VV>(t, b, r);
           const cpx<double> h = ht ? ht[(size_t)p * W] : asm_tf_fp64_at(u, col, p);
           const cpx<double> x = {(double)v[b * RL + r].x, (double)v[b * RL + r].y};
           const cpx<double> y = cmul(x, h);
@@ -755,7 +757,7 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
         }
     }
     if (want_pw) {  // uniform
-      __shared__ double red3[3][(W * Plan<N>::T + 63) / 64];
+      __shared__ double red3[3][(W * Plan<N, VV>::T + 63) / 64];
       double vals[3] = {s0, s1, s2};
       const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -768,16 +770,16 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
       __syncthreads();
       if (tid < 3) {
         double x = 0;
-        for (int w = 0; w < (W * Plan<N>::T + 63) / 64; ++w) x += red3[tid][w];
+        for (int w = 0; w < (W * Plan<N, VV>::T + 63) / 64; ++w) x += red3[tid][w];
         a.pw3[(size_t)blockIdx.x * 3 + tid] = x;
       }
     }
   }
   if constexpr (MODE >= 1) {
-    fft_block<R, N, +1, W, ACC>(v, t, tw, lds + c);
+    fft_block<R, N, +1, W, ACC, false, VV>(v, t, tw, lds + c);
   }
   constexpr int DIRL = MODE >= 1 ? +1 : DIR;
-  constexpr int RL = last_rad<N, DIRL>();
+  constexpr int RL = last_rad<N, DIRL, VV>();
   const R sc = (R)a.scale;
   if (sc != R(1)) {
 #pragma unroll
@@ -787,19 +789,19 @@ __global__ __launch_bounds__(W* Plan<N>::T) void k_cols(const ColArgs a) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) gst(base + slot_pos<N, RL>(t, b, r) * W, v[b * RL + r]);
+      for (int r = 0; r < RL; ++r) gst(base + slot_pos<N, RL, VV>(t, b, r) * W, v[b * RL + r]);
   } else if (fast) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) gst(base + slot_pos<N, RL>(t, b, r) * tl, v[b * RL + r]);
+      for (int r = 0; r < RL; ++r) gst(base + slot_pos<N, RL, VV>(t, b, r) * tl, v[b * RL + r]);
   } else {
     const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const int q = slot_pos<N, RL>(t, b, r) - a.out_row0;
+        const int q = slot_pos<N, RL, VV>(t, b, r) - a.out_row0;
         if (active && q >= 0 && q < a.out_rows) base[q * ld] = v[b * RL + r];
       }
   }
