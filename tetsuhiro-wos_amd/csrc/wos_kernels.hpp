@@ -21,7 +21,7 @@ namespace wos {
 // Transfer functions. The FP64 phase argument is formed exactly as the
 // reference does (no FMA contraction) because kz*z reaches ~5e5 rad.
 // ---------------------------------------------------------------------------
-enum TFKind : int { TF_ASM = 0, TF_FRESNEL = 1, TF_TABLE = 2, TF_NONE = 3, TF_CONJ_ASM = 4 };
+enum TFKind : int { TF_ASM = 0, TF_FRESNEL = 1, TF_TABLE = 2, TF_NONE = 3, TF_CONJ_ASM = 4, TF_BANDLIMIT = 5 };
 
 struct TFUnit {
   double wl, z, k, az;     // wavelength, distance, k = (2 pi)/wl, |z|
@@ -36,6 +36,7 @@ struct TFUnit {
   int kind;                // TFKind
   int zero_ev, reg, backward, pad_;
   const void* table;       // TF_TABLE: M*M complex spectrum (same precision as data)
+  double bl_f0, bl_sig;    // TF_BANDLIMIT: roll-off start and width (1/m)
 };
 
 // Go math.Hypot: max * sqrt(1 + (min/max)^2)
@@ -240,7 +241,18 @@ __device__ __noinline__ cpx<double> asm_tf_fp64_at(const TFUnit& u, int ix, int 
   return asm_tf_fp64(u, fx, fy);
 }
 
-// Any transfer-function kind (asm, conjugate asm, fresnel, table), out of line.
+// ApplyBandlimit's roll-off (optics/field.go:276-289): exp(-d^2) with
+// d = (hypot(fx, fy) - f0)/sig beyond f0, else 1 (Go's order, no FMA).
+__device__ __forceinline__ double bandlimit_factor(const TFUnit& u, double fx, double fy) {
+#pragma clang fp contract(off)
+  const double fr = go_hypot(fx, fy);
+  if (!(fr > u.bl_f0)) return 1.0;
+  const double d = (fr - u.bl_f0) / u.bl_sig;
+  return exp(-d * d);
+}
+
+// Any transfer-function kind (asm, conjugate asm, fresnel, table, band
+// limit), out of line.
 template <typename R>
 __device__ cpx<R> tf_eval(const TFUnit& u, int ix, int iy);
 
@@ -257,6 +269,7 @@ __device__ __noinline__ cpx<double> tf_eval<double>(const TFUnit& u, int ix, int
     }
     case TF_FRESNEL: return fresnel_tf_fp64(u, fx, fy);
     case TF_TABLE: return ((const cpx<double>*)u.table)[(size_t)iy * u.M + ix];
+    case TF_BANDLIMIT: return {bandlimit_factor(u, fx, fy), 0.0};
     default: return {1.0, 0.0};
   }
 }
@@ -273,6 +286,7 @@ __device__ __noinline__ cpx<float> tf_eval<float>(const TFUnit& u, int ix, int i
     }
     case TF_FRESNEL: return fresnel_tf_fp32(u, fx, fy);
     case TF_TABLE: return ((const cpx<float>*)u.table)[(size_t)iy * u.M + ix];
+    case TF_BANDLIMIT: return {(float)bandlimit_factor(u, fx, fy), 0.f};
     default: return {1.f, 0.f};
   }
 }
