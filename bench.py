@@ -266,6 +266,17 @@ def bench_config12(wos, tune=True):
     return out, fields
 
 
+def propagate_bl(f, z, ctx, bl=(0.9, 0.05)):
+    """A propagate element with the band limit: the device field runs it as
+    one call (one fused transfer for asm), the oracle as Propagate then
+    ApplyBandlimit."""
+    if type(f).__name__ == "OField":
+        f.propagate(z, "asm", ctx)
+        f.bandlimit(*bl)
+    else:
+        f.propagate(z, "asm", ctx, bandlimit=bl)
+
+
 def train_steps(f, m, ctx, shape):
     """A train of runTrain's element kinds on one field: lens, propagation
     (each followed by the band limit 0.9/0.05), soft circular aperture,
@@ -277,16 +288,13 @@ def train_steps(f, m, ctx, shape):
     c2, s2 = np.cos(0.2), np.sin(0.2)
     ret = ((c2 * c2) * e1 + (s2 * s2) * e2, (s2 * c2) * (e1 - e2), (s2 * c2) * (e1 - e2), (s2 * s2) * e1 + (c2 * c2) * e2)
     f.lens(633e-9, 0.2, aperture=1.8e-3)
-    f.propagate(0.02, "asm", ctx)
-    f.bandlimit(0.9, 0.05)
+    propagate_bl(f, 0.02, ctx)
     f.aperture(shape)
     f.diffuser(sigma=0.8, corr=10e-6, amp=1.0, seed=3)
-    f.propagate(0.01, "asm", ctx)
-    f.bandlimit(0.9, 0.05)
+    propagate_bl(f, 0.01, ctx)
     f.jones(*pol)
     f.jones(*ret)
-    f.propagate(0.01, "asm", ctx)
-    f.bandlimit(0.9, 0.05)
+    propagate_bl(f, 0.01, ctx)
 
 
 def bench_train(wos, n=2048, dx=2e-6, reps=5):

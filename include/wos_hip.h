@@ -365,6 +365,14 @@ int wos_hip_field_jones(wos_field f, const double* abcd);
 int wos_hip_field_scale(wos_field f, double re, double im);
 /* Field.ApplyBandlimit (optics/field.go:264-294); fraction outside (0, 1): no-op */
 int wos_hip_field_bandlimit(wos_field f, double fraction, double sigma);
+/* A propagate element with the context's band limit (optics/simulator.go:
+ * 495-496): wos_hip_field_propagate, then wos_hip_field_bandlimit. For the
+ * asm method on a power-of-two grid the two run as one transfer per
+ * component (the roll-off applied between H and the inverse column FFT);
+ * warnings and stats are the propagation's, with its output power taken
+ * before the band limit. */
+int wos_hip_field_propagate_bandlimited(wos_field f, double z, int method, const wos_ctx* ctx, double fraction,
+                                        double sigma, wos_warnings* w, wos_prop_stats* st);
 /* Plane views: the float32 little-endian planes servePlaneData streams
  * (server/server.go:581-660) for one recorded field, computed on the device
  * so a plane crosses PCIe as 4 bytes per pixel (server/views.go:131-244,

@@ -161,6 +161,39 @@ def test_train_matches_oracle(wos, oracle, n):
         arm.close()
 
 
+@pytest.mark.parametrize("n,polarized", [(256, False), (256, True), (200, True)])
+def test_propagate_bandlimited_matches_oracle(wos, oracle, n, polarized):
+    """A propagate element with the context's band limit (Propagate, then
+    ApplyBandlimit): one fused transfer per component on power-of-two grids,
+    the two steps on others, against the oracle's two steps. The warnings
+    and stats are the propagation's: dx = 0.25 um keeps evanescent bands on
+    the grid, so evanescent_filtered fires, its value and p_out taken from
+    the power before the band limit (the fused pass sums |H U|^2 over the
+    spectrum)."""
+    dx, wl, z, bl = 0.25e-6, 633e-9, 20e-6, (0.8, 0.05)
+    ex, ey = field(n, 31, polarized)
+    o = oracle.OField(ex, ey, dx=dx)
+    wo, so = o.propagate(z, "asm", oracle.ctx_for(wl))
+    o.bandlimit(*bl)
+    ref = np.concatenate([o.ex, o.ey]) if polarized else o.ex
+    assert "evanescent_filtered" in wo.as_dict()
+    for prec, tol, ptol in ((wos.FP64, FP64_TOL, 1e-12), (wos.FP32, FP32_TOL, 1e-5)):
+        f = wos.Field.from_host(ex, ey, dx=dx, precision=prec)
+        wg, sg = f.propagate(z, "asm", wos.ctx_for(wl), bandlimit=bl)
+        got = np.concatenate(f.download()) if polarized else f.download()[0]
+        e = rel_l2(got, ref)
+        print(f"propagate+bandlimit n={n} polarized={polarized} prec={prec}: rel-L2 {e:.2e}, "
+              f"p_out {sg.p_out:.12e} vs {so.p_out:.12e}", flush=True)
+        assert e <= tol
+        assert abs(sg.p_in - so.p_in) <= ptol * so.p_in and abs(sg.p_out - so.p_out) <= ptol * so.p_out
+        assert wg.as_dict().keys() == wo.as_dict().keys()
+        for code, (cnt, val) in wo.as_dict().items():
+            assert wg.as_dict()[code][0] == cnt
+            if prec == wos.FP64:
+                assert abs(wg.as_dict()[code][1] - val) <= 1e-9 * max(abs(val), 1e-300), code
+        f.close()
+
+
 @pytest.mark.parametrize("polarized", [False, True])
 def test_fraunhofer_matches_oracle(wos, oracle, polarized):
     """Propagate with method fraunhofer: the drop-in call and the resident

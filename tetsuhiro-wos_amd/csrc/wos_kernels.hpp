@@ -21,7 +21,11 @@ namespace wos {
 // Transfer functions. The FP64 phase argument is formed exactly as the
 // reference does (no FMA contraction) because kz*z reaches ~5e5 rad.
 // ---------------------------------------------------------------------------
-enum TFKind : int { TF_ASM = 0, TF_FRESNEL = 1, TF_TABLE = 2, TF_NONE = 3, TF_CONJ_ASM = 4, TF_BANDLIMIT = 5 };
+// TF_ASM_BL: asm followed by ApplyBandlimit on the same grid (one transfer;
+// the column pass also sums |H U|^2 for the propagation's power check).
+enum TFKind : int {
+  TF_ASM = 0, TF_FRESNEL = 1, TF_TABLE = 2, TF_NONE = 3, TF_CONJ_ASM = 4, TF_BANDLIMIT = 5, TF_ASM_BL = 6
+};
 
 struct TFUnit {
   double wl, z, k, az;     // wavelength, distance, k = (2 pi)/wl, |z|
@@ -36,7 +40,7 @@ struct TFUnit {
   int kind;                // TFKind
   int zero_ev, reg, backward, pad_;
   const void* table;       // TF_TABLE: M*M complex spectrum (same precision as data)
-  double bl_f0, bl_sig;    // TF_BANDLIMIT: roll-off start and width (1/m)
+  double bl_f0, bl_sig;    // TF_BANDLIMIT, TF_ASM_BL: roll-off start and width (1/m)
 };
 
 // Go math.Hypot: max * sqrt(1 + (min/max)^2)
@@ -289,6 +293,29 @@ __device__ __noinline__ cpx<float> tf_eval<float>(const TFUnit& u, int ix, int i
     case TF_BANDLIMIT: return {(float)bandlimit_factor(u, fx, fy), 0.f};
     default: return {1.f, 0.f};
   }
+}
+
+// TF_ASM_BL at transform position (ix, iy): the asm transfer function h and
+// ApplyBandlimit's roll-off factor, out of line.
+template <typename R>
+struct TfBand {
+  cpx<R> h;
+  R band;
+};
+template <typename R>
+__device__ TfBand<R> asm_bl_eval(const TFUnit& u, int ix, int iy);
+
+template <>
+__device__ __noinline__ TfBand<double> asm_bl_eval<double>(const TFUnit& u, int ix, int iy) {
+#pragma clang fp contract(off)
+  const double fx = freq_of(ix, u.M, u.den), fy = freq_of(iy, u.M, u.den);
+  return {asm_tf_fp64(u, fx, fy), bandlimit_factor(u, fx, fy)};
+}
+
+template <>
+__device__ __noinline__ TfBand<float> asm_bl_eval<float>(const TFUnit& u, int ix, int iy) {
+  const double fx = freq_of(ix, u.M, u.den), fy = freq_of(iy, u.M, u.den);
+  return {asm_tf_fp32_generic(u, fx, fy), (float)bandlimit_factor(u, fx, fy)};
 }
 
 // ---------------------------------------------------------------------------
@@ -598,7 +625,8 @@ struct ColArgs {
   const TFUnit* tf;    // per unit (MODE 1, 2)
   const void* tw;
   int tf_square;       // MODE 3: apply H^2 (two merged propagation steps)
-  double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2 (nullable)
+  double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2; MODE 2 with
+                       // TF_ASM_BL: per-workgroup sum |H U|^2 (nullable)
   int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
   const void* htab;    // MODE 3: FP64 H table in W-column tile order (nullable: evaluate H)
   int radix;           // stage schedule of tw: 0 or 16 radix-16, 8 radix-8 (MODE 1 only)
@@ -719,6 +747,23 @@ __global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) 
 #pragma unroll
             for (int r = 0; r < RL; ++r)
               v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan(u, cc, slot_pos<N, RL, VV>(t, b, r)));
+        }
+      } else if (MODE == 2 && u.kind == TF_ASM_BL) {
+        // propagation then band limit: U <- B (H U), with the per-workgroup
+        // sum of |H U|^2 (the propagated power, by Parseval) when pw3 is set
+        double s1 = 0;
+#pragma unroll
+        for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            const TfBand<R> hb = asm_bl_eval<R>(u, col, slot_pos<N, RL, VV>(t, b, r));
+            const cpx<R> y = cmul(v[b * RL + r], hb.h);
+            s1 += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+            v[b * RL + r] = cscale(y, hb.band);
+          }
+        if (a.pw3) {  // uniform
+          __shared__ double redc[(W * Plan<N, VV>::T + 63) / 64];
+          block_sum_store<W * Plan<N, VV>::T>(s1, redc, a.pw3, blockIdx.x);
         }
       } else {
         const bool conj = MODE == 1 && u.kind == TF_CONJ_ASM;

@@ -41,6 +41,7 @@ EXPORTS = [
     "wos_hip_bpm_mask_create", "wos_hip_bpm_mask_destroy", "wos_hip_bpm_mask_forward", "wos_hip_bpm_mask_backward",
     "wos_hip_field_create", "wos_hip_field_destroy", "wos_hip_field_clone", "wos_hip_field_upload",
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
+    "wos_hip_field_propagate_bandlimited",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
@@ -199,6 +200,8 @@ def load(path=None):
     L.wos_hip_field_device_ptr.argtypes = [P]
     L.wos_hip_field_device_ptr.restype = P
     L.wos_hip_field_propagate.argtypes = [P, D, I, C.POINTER(Ctx), C.POINTER(Warnings), C.POINTER(PropStats)]
+    L.wos_hip_field_propagate_bandlimited.argtypes = [P, D, I, C.POINTER(Ctx), D, D, C.POINTER(Warnings),
+                                                      C.POINTER(PropStats)]
     L.wos_hip_field_lens.argtypes = [P, D, D, D, D, D]
     L.wos_hip_field_aperture.argtypes = [P, C.POINTER(Shape), D, D]
     L.wos_hip_field_diffuser.argtypes = [P, D, D, D, C.c_int64, D, D]
@@ -340,10 +343,18 @@ class Field:
         _check(load().wos_hip_field_download(self.h, _p(ex), _p(ey)), "field_download")
         return ex, ey
 
-    def propagate(self, z, method, ctx):
+    def propagate(self, z, method, ctx, bandlimit=None):
+        """Propagate; bandlimit=(fraction, sigma) applies ApplyBandlimit after
+        it (one fused transfer for asm on a power-of-two grid)."""
         m = METHODS[method] if isinstance(method, str) else int(method)
         w, st = Warnings(), PropStats()
-        _check(load().wos_hip_field_propagate(self.h, z, m, C.byref(ctx), C.byref(w), C.byref(st)), "field_propagate")
+        if bandlimit is None:
+            _check(load().wos_hip_field_propagate(self.h, z, m, C.byref(ctx), C.byref(w), C.byref(st)),
+                   "field_propagate")
+        else:
+            _check(load().wos_hip_field_propagate_bandlimited(self.h, z, m, C.byref(ctx), float(bandlimit[0]),
+                                                               float(bandlimit[1]), C.byref(w), C.byref(st)),
+                   "field_propagate_bandlimited")
         return w, st
 
     def lens(self, wl, f, aperture=0.0, x0=0.0, y0=0.0):
