@@ -168,9 +168,12 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
                        wos_warnings* w);
 
 /* Launch shape of the split-step slices on an n-point power-of-two grid:
- * column tile width (the default, or narrower from n = 512) and rows per row
- * workgroup (0: default); tile_w = 0 restores the built-in choice. */
-int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows);
+ * column tile width (radix-16 columns: the default, or narrower from
+ * n = 512; radix-8 columns: 2 or more), rows per row workgroup (0: default)
+ * and the FFT stage schedules of the row and column passes (radix / col_radix:
+ * 16, or 8 for 256..8192 points; 0 means 16); tile_w = 0 restores the
+ * built-in choice. */
+int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows, int radix, int col_radix);
 
 /* PropagatePolychromatic (optics/polychromatic.go:18-43): out (n*n float64)
  * = sum_s weight_s * |U_s|^2; the input field is not modified. */

@@ -323,6 +323,40 @@ def test_config3_bpm_short_vs_oracle(wos, oracle):
     assert err <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_split_step_radix8_schedules(wos, oracle, precision):
+    """The split-step slices under the radix-8 schedule (the screen row pass
+    and the H^2 column pass; FP32 with compensated radix-8 twiddles) for the
+    row passes, the column pass or both: config-3 grid and medium, 20
+    slices, against the oracle (the FP64 and FP32 tolerances) and the
+    radix-16 run (rounding level)."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    tol = FP64_TOL if prec == wos.FP64 else FP32_TOL
+    n, dx, wl, steps = 1024, 0.5e-6, 1.55e-6, 20
+    index = graded_index(n, dx)
+    ex = oracle.gaussian(n, dx, 10e-6)
+    z = steps * 5e-6
+    ref, _, wo = oracle.split_step(ex, None, dx, z, steps, index, oracle.ctx_for(wl))
+    # (tile width, rows per row workgroup, row radix, column radix); (0, ...): built-in shape
+    shapes = [(0, 0, 16, 16), (4, 0, 8, 8), (4, 2, 16, 8), (2, 4, 8, 16), (8, 1, 8, 8)]
+    base = None
+    try:
+        for shape in shapes:
+            wos.split_step_set_shape(prec, n, *shape)
+            out, _, wg = wos.split_step(ex, None, dx, z, steps, index, wos.ctx_for(wl), precision=prec)
+            e = rel_l2(out, ref)
+            print(f"split-step {precision} shape {shape}: rel-L2 vs oracle {e:.3e}", flush=True)
+            assert e <= tol, (shape, e)
+            assert wg.as_dict().keys() == wo.as_dict().keys()
+            if base is None:
+                base = out
+            else:
+                d = rel_l2(out, base)
+                assert d <= (1e-13 if prec == wos.FP64 else 2e-6), (shape, d)
+    finally:
+        wos.split_step_set_shape(prec, n, 0)
+
+
 def test_config3_bpm_full_fp32_vs_fp64(wos, oracle):
     """Config 3 in full: 400 slices of dz = 5um. FP32 bulk path vs the FP64
     path (itself pinned to the oracle above) and the absorbed power against

@@ -40,8 +40,8 @@ int launch_rows8_f64(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
 int launch_rows8_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
 bool rows_valid8_f64(int N, int rows);
 bool rows_valid8_f32(int N, int rows);
-int launch_cols8_f64(int N, int W, const ColArgs& a, hipStream_t s);
-int launch_cols8_f32(int N, int W, const ColArgs& a, hipStream_t s);
+int launch_cols8_f64(int N, int mode, int W, const ColArgs& a, hipStream_t s);
+int launch_cols8_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s);
 bool col_w_valid8_f64(int N, int W);
 bool col_w_valid8_f32(int N, int W);
 int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
@@ -477,11 +477,11 @@ bool col_w_valid(int prec, int N, int W, int radix) {
 int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
   if (W <= 0) W = default_col_w(prec, N, a.cols);
   if (a.radix == 8) {
-    if (mode != 1) {
-      set_error("the radix-8 schedule builds the fused asm column pass only");
+    if (mode != 1 && mode != 3) {
+      set_error("the radix-8 schedule builds the asm and split-step column passes only");
       return WOS_ERR_UNSUPPORTED;
     }
-    return prec == WOS_FP64 ? launch_cols8_f64(N, W, a, s) : launch_cols8_f32(N, W, a, s);
+    return prec == WOS_FP64 ? launch_cols8_f64(N, mode, W, a, s) : launch_cols8_f32(N, mode, W, a, s);
   }
   return prec == WOS_FP64 ? launch_cols_f64(N, mode, dir, W, a, s) : launch_cols_f32(N, mode, dir, W, a, s);
 }
@@ -1694,6 +1694,13 @@ static Tuning split_step_shape(int prec, int n) {
   auto it = g_split_shapes.find((long long)n * 2 + prec);
   if (it != g_split_shapes.end()) return it->second;
   Tuning t;
+  if (prec == WOS_FP64 && n == 1024) {
+    // measured best at 1024^2 FP64 (400 slices): radix-8 rows and columns on
+    // 2-column tiles, 35.3k slices/s against 33.8k for the radix-16 shape
+    t.tw = t.col_w = 2;
+    t.radix = t.col_radix = 8;
+    return t;
+  }
   t.tw = default_col_w(prec, n, n);
   if (n >= 512 && t.tw > 4) t.tw = 4;  // 1024^2: 4-column tiles with 4 (FP32) or 2 (FP64) rows measured best
   if (n >= 512) {
@@ -1755,10 +1762,11 @@ static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, doubl
   col.pw3 = w ? pw3 : nullptr;
   col.htab = dv.ir.p;
   if (prec == WOS_FP32) {  // the slice kernels use compensated twiddles
-    const void* twa;
-    if ((rc = get_twiddles(n, prec, &twa, 1))) return rc;
-    scr.tw = twa;
-    col.tw = twa;
+    const void *twr, *twc;
+    if ((rc = get_twiddles(n, prec, &twr, 1, L.tu.radix))) return rc;
+    if ((rc = get_twiddles(n, prec, &twc, 1, L.tu.col_radix))) return rc;
+    scr.tw = twr;
+    col.tw = twc;
   }
   const bool want = w != nullptr;
   hipEvent_t e0, e1;
@@ -2081,7 +2089,7 @@ int wos_hip_propagate1d(double* a, int n, double dx, double z, double wl, const 
 // PropagateSplitStep (optics/medium.go:22-45): Strang slices
 // [ASM dz/2, medium phase dz, ASM dz/2], each ASM half step a full Propagate
 // with its own diagnostics, as the reference executes them.
-int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows) {
+int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows, int radix, int col_radix) {
   WOS_ENTER();
   if (!valid_prec(precision) || n <= 0 || !pow2(n)) {
     set_error("wos_hip_split_step_set_shape: bad arguments");
@@ -2092,15 +2100,26 @@ int wos_hip_split_step_set_shape(int precision, int n, int tile_w, int rows) {
     g_split_shapes.erase(key);
     return WOS_OK;
   }
+  if (radix == 0) radix = 16;
+  if (col_radix == 0) col_radix = 16;
+  const bool r8 = radix == 8 || col_radix == 8;
+  if (!((radix == 16 || radix == 8) && (col_radix == 16 || col_radix == 8)) || (r8 && !radix8_ok(n))) {
+    set_error("wos_hip_split_step_set_shape: radix-16 or radix-8 (256..8192 points) schedules only");
+    return WOS_ERR_INVALID;
+  }
   const int dw = default_col_w(precision, n, n);
-  if (!(tile_w == dw || (n >= 512 && tile_w >= 2 && tile_w < dw)) || (n % tile_w) != 0 ||
-      !col_w_valid(precision, n, tile_w) || (rows > 0 && !rows_valid(precision, n, rows))) {
+  const bool tile_built = col_radix == 8 ? tile_w >= 2 : (tile_w == dw || (n >= 512 && tile_w >= 2 && tile_w < dw));
+  if (!tile_built || (n % tile_w) != 0 || !col_w_valid(precision, n, tile_w, col_radix) ||
+      (rows > 0 && !rows_valid(precision, n, rows, radix))) {
     set_error("wos_hip_split_step_set_shape: column tile or rows not built for the split-step kernels");
     return WOS_ERR_INVALID;
   }
   Tuning t;
   t.tw = tile_w;
+  t.col_w = tile_w;
   t.rows = rows;
+  t.radix = radix;
+  t.col_radix = col_radix;
   g_split_shapes[key] = t;
   return WOS_OK;
 }

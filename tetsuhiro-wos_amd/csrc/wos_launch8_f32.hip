@@ -6,6 +6,8 @@ int launch_rows8_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s) 
   return launch_rows8_T<float>(N, dir, rows, a, s);
 }
 bool rows_valid8_f32(int N, int rows) { return rows_valid8_T<float>(N, rows); }
-int launch_cols8_f32(int N, int W, const ColArgs& a, hipStream_t s) { return launch_cols8_T<float>(N, W, a, s); }
+int launch_cols8_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s) {
+  return launch_cols8_T<float>(N, mode, W, a, s);
+}
 bool col_w_valid8_f32(int N, int W) { return col_w_valid8_T<float>(N, W); }
 }  // namespace wos

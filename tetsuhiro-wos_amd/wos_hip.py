@@ -146,7 +146,7 @@ def load(path=None):
                                     C.POINTER(PropStats)]
     L.wos_hip_propagate1d.argtypes = [P, I, D, D, D, C.POINTER(Ctx), I, C.POINTER(Warnings)]
     L.wos_hip_split_step.argtypes = [P, P, I, D, D, I, P, I, C.POINTER(Ctx), I, C.POINTER(Warnings)]
-    L.wos_hip_split_step_set_shape.argtypes = [I, I, I, I]
+    L.wos_hip_split_step_set_shape.argtypes = [I, I, I, I, I, I]
     L.wos_hip_polychromatic.argtypes = [P, P, I, D, D, P, P, I, I, C.POINTER(Ctx), I, P,
                                         C.POINTER(Warnings)]
     L.wos_hip_volume.argtypes = [P, P, I, D, P, I, I, C.POINTER(Ctx), I, P, P, C.POINTER(Warnings)]
@@ -454,9 +454,11 @@ def split_step(ex, ey, dx, z, steps, index, ctx, precision=FP64):
     return ex, ey, w
 
 
-def split_step_set_shape(precision, n, tile_w, rows=0):
-    """Launch shape of the split-step slices (tile_w = 0: built-in choice)."""
-    _check(load().wos_hip_split_step_set_shape(precision, n, tile_w, rows), "split_step_set_shape")
+def split_step_set_shape(precision, n, tile_w, rows=0, radix=0, col_radix=None):
+    """Launch shape of the split-step slices (tile_w = 0: built-in choice);
+    col_radix None: the row passes' radix."""
+    _check(load().wos_hip_split_step_set_shape(precision, n, tile_w, rows, radix,
+                                               radix if col_radix is None else col_radix), "split_step_set_shape")
 
 
 def polychromatic(ex, ey, dx, z, wavelengths, weights, method, ctx, precision=FP64):
