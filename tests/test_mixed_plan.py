@@ -1,0 +1,99 @@
+"""CPU check of the mixed-radix Stockham design (wos_mixed.hpp) for the grid
+sides the kernels build: the plan (radix-16 stages, one leftover power of
+two, then radix 3 and 5), the per-stage butterfly maps (loads x[j + r N/RAD],
+stores y[(j - k) RAD + k + r NS], k = j mod NS, each output written once),
+the twiddle table layout ([k][r-1] per stage, conjugated for the inverse)
+and the radix-3/5 butterfly formulas, emulated in numpy and compared with
+numpy's FFT."""
+import numpy as np
+import pytest
+
+SIZES = [100, 200, 384, 400, 768, 1000, 1536, 2000, 3072]
+
+
+def plan(n):
+    a = b = c = 0
+    m = n
+    while m % 2 == 0:
+        m //= 2
+        a += 1
+    while m % 3 == 0:
+        m //= 3
+        b += 1
+    while m % 5 == 0:
+        m //= 5
+        c += 1
+    assert m == 1
+    return [16] * (a // 4) + ([1 << (a % 4)] if a % 4 else []) + [3] * b + [5] * c
+
+
+def twiddles(n):
+    rad, tab, ns = plan(n), [], None
+    ns = rad[0]
+    for s in range(1, len(rad)):
+        L = ns * rad[s]
+        for k in range(ns):
+            for r in range(1, rad[s]):
+                tab.append(np.exp(-2j * np.pi * ((r * k) % L) / L))
+        ns = L
+    return np.array(tab)
+
+
+def mul_i(d, a):
+    return -1j * a if d < 0 else 1j * a
+
+
+def butterfly(R, d, v):
+    if R == 3:
+        s3 = np.sin(2 * np.pi / 3)
+        a, b = v[1] + v[2], v[1] - v[2]
+        t, u = v[0] - 0.5 * a, mul_i(d, s3 * b)
+        return [v[0] + a, t + u, t - u]
+    if R == 5:
+        c1, c2 = np.cos(2 * np.pi / 5), np.cos(4 * np.pi / 5)
+        s1, s2 = np.sin(2 * np.pi / 5), np.sin(4 * np.pi / 5)
+        a1, b1, a2, b2 = v[1] + v[4], v[1] - v[4], v[2] + v[3], v[2] - v[3]
+        t1, t2 = v[0] + c1 * a1 + c2 * a2, v[0] + c2 * a1 + c1 * a2
+        u1, u2 = mul_i(d, s1 * b1 + s2 * b2), mul_i(d, s2 * b1 - s1 * b2)
+        return [v[0] + a1 + a2, t1 + u1, t2 + u2, t2 - u2, t1 - u1]
+    k = np.arange(R)
+    return list(np.exp(d * 2j * np.pi * np.outer(k, k) / R) @ np.array(v))
+
+
+def transform(x, d):
+    n = len(x)
+    rad, tw = plan(n), twiddles(n)
+    data, ns, off = x.astype(complex).copy(), 1, 0
+    for s, R in enumerate(rad):
+        nb = n // R
+        out = np.empty_like(data)
+        written = np.zeros(n, bool)
+        for j in range(nb):
+            k = j % ns
+            v = [data[j + r * nb] for r in range(R)]
+            if ns > 1:
+                for r in range(1, R):
+                    w = tw[off + k * (R - 1) + r - 1]
+                    v[r] = v[r] * (w if d < 0 else np.conj(w))
+            v = butterfly(R, d, v)
+            o = (j - k) * R + k
+            for r in range(R):
+                assert not written[o + r * ns]
+                written[o + r * ns] = True
+                out[o + r * ns] = v[r]
+        assert written.all()
+        data = out
+        if s >= 1:
+            off += ns * (R - 1)
+        ns *= R
+    assert off == len(tw)
+    return data
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_mixed_radix_plan_matches_numpy(n):
+    x = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 1).standard_normal(n)
+    want = np.fft.fft(x)
+    assert np.linalg.norm(transform(x, -1) - want) <= 1e-13 * np.linalg.norm(want)
+    want = np.fft.ifft(x) * n
+    assert np.linalg.norm(transform(x, +1) - want) <= 1e-13 * np.linalg.norm(want)

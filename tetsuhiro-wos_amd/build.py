@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwos_hip.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["wos_hip.hip", "wos_launch_f64.hip", "wos_launch_f32.hip", "wos_launch8_f64.hip", "wos_launch8_f32.hip"]
+SOURCES = ["wos_hip.hip", "wos_launch_f64.hip", "wos_launch_f32.hip", "wos_launch8_f64.hip", "wos_launch8_f32.hip",
+           "wos_mixed_f64.hip", "wos_mixed_f32.hip"]
 ARCH = os.environ.get("WOS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -44,6 +44,12 @@ int launch_cols8_f64(int N, int mode, int W, const ColArgs& a, hipStream_t s);
 int launch_cols8_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s);
 bool col_w_valid8_f64(int N, int W);
 bool col_w_valid8_f32(int N, int W);
+int launch_rows_mixed_f64(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_rows_mixed_f32(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_cols_mixed_f64(int N, int W, const ColArgs& a, hipStream_t s);
+int launch_cols_mixed_f32(int N, int W, const ColArgs& a, hipStream_t s);
+int mixed_col_w_f64(int N);
+int mixed_col_w_f32(int N);
 int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int launch_cols_bs_f32(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int bs_default_w_f64(int M, int cols);
@@ -357,6 +363,72 @@ int get_twiddles(int n, int prec, const void** out, int acc, int radix) {
   d.twiddles[key] = p;
   *out = p;
   return 0;
+}
+
+// Host mirror of MPlan<N> (wos_mixed.hpp): stage radices of the mixed-radix
+// plan of n = 2^a 3^b 5^c points.
+static std::vector<int> mixed_plan_host(int n) {
+  int a = 0, b = 0, c = 0, m = n;
+  while (m % 2 == 0) m /= 2, ++a;
+  while (m % 3 == 0) m /= 3, ++b;
+  while (m % 5 == 0) m /= 5, ++c;
+  std::vector<int> rad;
+  for (int i = 0; i < a / 4; ++i) rad.push_back(16);
+  if (a % 4) rad.push_back(1 << (a % 4));
+  for (int i = 0; i < b; ++i) rad.push_back(3);
+  for (int i = 0; i < c; ++i) rad.push_back(5);
+  return rad;
+}
+
+// Stage twiddle tables of the mixed-radix plan of n points (stages 1..S-1,
+// the layout MPlan<N>::tw_off addresses): entry [k][r-1] of stage (NS, RAD)
+// is exp(-2 pi i r k/(NS RAD)), formed in long double from the reduced
+// fraction (r k mod NS RAD)/(NS RAD) and rounded once; cached.
+static int get_mixed_twiddles(int n, int prec, const void** out) {
+  Device& d = device();
+  const long long key = (1LL << 40) + (long long)n * 2 + prec;
+  auto it = d.twiddles.find(key);
+  if (it != d.twiddles.end()) {
+    *out = it->second;
+    return 0;
+  }
+  const std::vector<int> rad = mixed_plan_host(n);
+  std::vector<double> tab;  // (re, im) pairs
+  long long ns = rad[0];
+  for (size_t s = 1; s < rad.size(); ++s) {
+    const long long L = ns * rad[s];
+    for (long long k = 0; k < ns; ++k)
+      for (int r = 1; r < rad[s]; ++r) {
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)((r * k) % L) / (long double)L;
+        tab.push_back((double)cosl(ang));
+        tab.push_back((double)sinl(ang));
+      }
+    ns = L;
+  }
+  const size_t cnt = std::max<size_t>(tab.size() / 2, 1);
+  void* p = nullptr;
+  WOS_HIP_TRY(hipMalloc(&p, cnt * csize(prec)));
+  if (prec == WOS_FP64) {
+    if (!tab.empty()) WOS_HIP_TRY(hipMemcpy(p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tf(tab.begin(), tab.end());
+    if (!tf.empty()) WOS_HIP_TRY(hipMemcpy(p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  d.twiddles[key] = p;
+  *out = p;
+  return 0;
+}
+
+// widest column tile of the mixed-radix kernels for n (0: n not built)
+static int mixed_col_w(int prec, int n) {
+  if (n <= 1 || pow2(n)) return 0;
+  return prec == WOS_FP64 ? mixed_col_w_f64(n) : mixed_col_w_f32(n);
+}
+static int launch_rows_mixed(int prec, int N, int dir, const RowArgs& a, hipStream_t s) {
+  return prec == WOS_FP64 ? launch_rows_mixed_f64(N, dir, a, s) : launch_rows_mixed_f32(N, dir, a, s);
+}
+static int launch_cols_mixed(int prec, int N, int W, const ColArgs& a, hipStream_t s) {
+  return prec == WOS_FP64 ? launch_cols_mixed_f64(N, W, a, s) : launch_cols_mixed_f32(N, W, a, s);
 }
 
 __global__ void k_convert_scale_f64(const cpx<double>* __restrict__ in, cpx<double>* __restrict__ out, int n, double sc) {
@@ -992,7 +1064,9 @@ struct TransferLaunch {
   Tuning tu;
   RowArgs fwd, inv;
   ColArgs col;
-  // non-power-of-two M: Bluestein kernels with internal length Mb
+  // non-power-of-two M: the mixed-radix kernels (mr) where built, else
+  // Bluestein kernels with internal length Mb
+  bool mr = false;
   bool bs = false;
   int Mb = 0;
   BsRowArgs bfwd, binv;
@@ -1014,11 +1088,17 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     set_error("transform length not supported: " + std::to_string(M));
     return WOS_ERR_UNSUPPORTED;
   }
-  const bool bs = !pow2(M);
+  const bool mr = !pow2(M) && mixed_col_w(prec, M) > 0;
+  const bool bs = !pow2(M) && !mr;
   const int Mk = bs ? bs_len(M) : M;  // kernel FFT length
   if (bs) {
     tu.tw = bs_default_w(prec, Mk, M);
     tu.rows = rows_per_block_host(Mk);
+    tu.radix = tu.col_radix = 16;
+  } else if (mr) {  // one row per row workgroup; the widest column tile built
+    tu = Tuning();
+    tu.tw = tu.col_w = mixed_col_w(prec, M);
+    tu.rows = 1;
     tu.radix = tu.col_radix = 16;
   } else {
     const bool lean = tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM;
@@ -1029,14 +1109,20 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     tu = resolve_tuning(prec, M, tu);
   }
   const void *tw, *tw64, *twc;
-  int rc = get_twiddles(Mk, prec, &tw, 0, tu.radix);
-  if (rc) return rc;
-  if ((rc = get_twiddles(Mk, WOS_FP64, &tw64, 0, tu.radix))) return rc;
-  if ((rc = get_twiddles(Mk, prec, &twc, 0, tu.col_radix))) return rc;
+  int rc;
+  if (mr) {
+    if ((rc = get_mixed_twiddles(M, prec, &tw))) return rc;
+    tw64 = twc = tw;
+  } else {
+    if ((rc = get_twiddles(Mk, prec, &tw, 0, tu.radix))) return rc;
+    if ((rc = get_twiddles(Mk, WOS_FP64, &tw64, 0, tu.radix))) return rc;
+    if ((rc = get_twiddles(Mk, prec, &twc, 0, tu.col_radix))) return rc;
+  }
   L.prec = prec;
   L.M = M;
   L.mode = (tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM) ? 1 : 2;
   L.tu = tu;
+  L.mr = mr;
   L.bs = bs;
   L.Mb = Mk;
   const long long iunit = inter_unit_elems(n, M);
@@ -1143,6 +1229,11 @@ static void set_transfer_power(TransferLaunch& L, double* pw_in, double* pw_out)
 
 // kernel k of a transfer step: 0 forward rows, 1 columns, 2 inverse rows
 static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s) {
+  if (L.mr) {
+    if (k == 0) return launch_rows_mixed(L.prec, L.M, -1, L.fwd, s);
+    if (k == 1) return launch_cols_mixed(L.prec, L.M, L.tu.tw, L.col, s);
+    return launch_rows_mixed(L.prec, L.M, 1, L.inv, s);
+  }
   if (L.bs) {
     if (k == 0) return launch_rows_bs(L.prec, L.Mb, L.bfwd, s);
     if (k == 1) return launch_cols_bs(L.prec, L.Mb, 2, L.tu.tw, L.bcol, s);

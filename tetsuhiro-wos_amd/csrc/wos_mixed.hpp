@@ -1,0 +1,232 @@
+// wos_mixed.hpp — mixed-radix (2, 3, 5) Stockham FFT for grid sides
+// N = 2^a 3^b 5^c that are not powers of two (config grids 384, 1000, 1536
+// and their zero-padded 768, 2000, 3072), in place of Bluestein's three
+// power-of-two transforms of length >= 2N - 1.
+//
+// A transform lives in LDS for its whole duration. Stage s (radix RAD,
+// sub-transform length NS) has N/RAD butterflies; thread t takes butterflies
+// t, t + T, ... (the map changes from stage to stage, so there is no
+// register-resident chain across stages as in the power-of-two kernels):
+//   load:  v[r] = x[j + r N/RAD]                 (j the butterfly, k = j mod NS)
+//   twiddle v[r] *= w^(r k), w = exp(dir 2 pi i/(NS RAD)), from a per-stage table
+//   v = DFT_RAD(v)
+//   store: y[(j - k) RAD + k + r NS] = v[r]
+// with a barrier between the loads and the stores of a stage. The table
+// layout is the power-of-two engine's: entry [k][r-1] of a stage at its
+// offset, conjugated for the inverse direction.
+#pragma once
+#include "wos_kernels.hpp"
+
+namespace wos {
+
+// exp(DIR 2 pi i m/3) and exp(DIR 2 pi i m/5) butterflies
+template <int DIR, typename R>
+__device__ __forceinline__ void dft3(cpx<R>* v) {
+  constexpr double S3 = 0.86602540378443864676;  // sin(2 pi/3)
+  const cpx<R> a = v[1] + v[2], b = v[1] - v[2];
+  const cpx<R> t = {v[0].x - R(0.5) * a.x, v[0].y - R(0.5) * a.y};
+  const cpx<R> u = mul_i<DIR>(cscale(b, (R)S3));
+  v[0] = v[0] + a;
+  v[1] = t + u;
+  v[2] = t - u;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft5(cpx<R>* v) {
+  constexpr double C1 = 0.30901699437494742410;   // cos(2 pi/5)
+  constexpr double C2 = -0.80901699437494742410;  // cos(4 pi/5)
+  constexpr double S1 = 0.95105651629515357212;   // sin(2 pi/5)
+  constexpr double S2 = 0.58778525229247312917;   // sin(4 pi/5)
+  const cpx<R> a1 = v[1] + v[4], b1 = v[1] - v[4], a2 = v[2] + v[3], b2 = v[2] - v[3];
+  const cpx<R> x0 = v[0];
+  const cpx<R> t1 = {x0.x + (R)C1 * a1.x + (R)C2 * a2.x, x0.y + (R)C1 * a1.y + (R)C2 * a2.y};
+  const cpx<R> t2 = {x0.x + (R)C2 * a1.x + (R)C1 * a2.x, x0.y + (R)C2 * a1.y + (R)C1 * a2.y};
+  const cpx<R> u1 = mul_i<DIR>(cpx<R>{(R)S1 * b1.x + (R)S2 * b2.x, (R)S1 * b1.y + (R)S2 * b2.y});
+  const cpx<R> u2 = mul_i<DIR>(cpx<R>{(R)S2 * b1.x - (R)S1 * b2.x, (R)S2 * b1.y - (R)S1 * b2.y});
+  v[0] = x0 + a1 + a2;
+  v[1] = t1 + u1;
+  v[4] = t1 - u1;
+  v[2] = t2 + u2;
+  v[3] = t2 - u2;
+}
+
+template <int RAD, int DIR, typename R>
+__device__ __forceinline__ void dft_mixed(cpx<R>* v) {
+  if constexpr (RAD == 3) dft3<DIR>(v);
+  else if constexpr (RAD == 5) dft5<DIR>(v);
+  else dft<RAD, DIR, R>(v);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-radix plan: radix-16 stages while four factors of two remain, one
+// stage for the leftover power of two (2, 4 or 8), then radix-3 and radix-5
+// stages. T threads per transform.
+// ---------------------------------------------------------------------------
+constexpr int count_factor(int n, int f) { return n % f == 0 ? 1 + count_factor(n / f, f) : 0; }
+constexpr int pow_int(int b, int e) { return e == 0 ? 1 : b * pow_int(b, e - 1); }
+constexpr bool mixed_len(int n) {
+  return n > 1 && pow_int(2, count_factor(n, 2)) * pow_int(3, count_factor(n, 3)) * pow_int(5, count_factor(n, 5)) == n;
+}
+
+template <int N>
+struct MPlan {
+  static_assert(mixed_len(N), "N = 2^a 3^b 5^c");
+  static constexpr int A = count_factor(N, 2), B = count_factor(N, 3), C = count_factor(N, 5);
+  static constexpr int N16 = A / 4, REM2 = 1 << (A % 4);
+  static constexpr int S = N16 + (REM2 > 1 ? 1 : 0) + B + C;
+  static constexpr int T = N >= 1500 ? 256 : 128;  // threads per transform
+  static constexpr int rad(int s) {
+    return s < N16 ? 16 : (REM2 > 1 && s == N16) ? REM2 : (s < N16 + (REM2 > 1 ? 1 : 0) + B ? 3 : 5);
+  }
+  static constexpr int ns(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= rad(i);
+    return p;
+  }
+  static constexpr int tw_off(int s) {  // stages 1 .. S-1 (stage 0 has NS = 1)
+    int o = 0;
+    for (int i = 1; i < s; ++i) o += ns(i) * (rad(i) - 1);
+    return o;
+  }
+  static constexpr int tw_total() { return tw_off(S); }
+  static constexpr int bpt(int s) { return (N / rad(s) + T - 1) / T; }  // butterflies per thread
+};
+
+// One stage on a transform in LDS (element p at lds[lpad(p) * ES]).
+// Every thread of the workgroup calls it (it holds two barriers).
+template <typename R, int N, int DIR, int S, int ES>
+__device__ __forceinline__ void mixed_stage(cpx<R>* lds, int t, const cpx<R>* __restrict__ tw) {
+  using P = MPlan<N>;
+  constexpr int RAD = P::rad(S), NS = P::ns(S), NB = N / RAD, BPT = P::bpt(S), T = P::T;
+  constexpr int OFF = P::tw_off(S);
+  cpx<R> v[BPT][RAD];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * T;
+    if (NB % T == 0 || j < NB) {
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[b][r] = lds[lpad(j + r * NB) * ES];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * T;
+    if (NB % T == 0 || j < NB) {
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+        const cpx<R>* tk = tw + OFF + k * (RAD - 1);
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) v[b][r] = DIR < 0 ? cmul(v[b][r], tk[r - 1]) : cmulc(v[b][r], tk[r - 1]);
+      }
+      dft_mixed<RAD, DIR>(v[b]);
+      const int o = (j - k) * RAD + k;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) lds[lpad(o + r * NS) * ES] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename R, int N, int DIR, int S, int ES>
+__device__ __forceinline__ void mixed_stages(cpx<R>* lds, int t, const cpx<R>* __restrict__ tw) {
+  if constexpr (S < MPlan<N>::S) {
+    mixed_stage<R, N, DIR, S, ES>(lds, t, tw);
+    mixed_stages<R, N, DIR, S + 1, ES>(lds, t, tw);
+  }
+}
+
+// Full N-point transform of the LDS-resident data (natural order in and
+// out); the caller synchronises before (data written) and gets a barrier
+// after the last stage.
+template <typename R, int N, int DIR, int ES>
+__device__ __forceinline__ void mixed_fft(cpx<R>* lds, int t, const cpx<R>* __restrict__ tw) {
+  mixed_stages<R, N, DIR, 0, ES>(lds, t, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: one row per workgroup (T threads), RowArgs as the power-of-two
+// row kernels (embed window, tiled or row-major layouts, output scale,
+// per-workgroup FP64 power sums).
+// ---------------------------------------------------------------------------
+template <typename R, int N, int DIR>
+__global__ __launch_bounds__(MPlan<N>::T) void k_rows_mixed(const RowArgs a) {
+  constexpr int T = MPlan<N>::T;
+  __shared__ cpx<R> lds[lds_len(N)];
+  __shared__ double red[(T + 63) / 64];
+  const int t = threadIdx.x;
+  const int unit = blockIdx.x / a.rows;
+  const int row = blockIdx.x % a.rows;
+  const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
+  cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
+  double pin = 0;
+  for (int p = t; p < N; p += T) {
+    const int q = p - a.in_col0;
+    cpx<R> x = {R(0), R(0)};
+    if (q >= 0 && q < a.in_cols) x = in[tiled_offset(q, row, a.in_ld, a.in_tw, a.in_tstride)];
+    pin += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    lds[lpad(p)] = x;
+  }
+  if (a.pw_in) block_sum_store<T>(pin, red, a.pw_in, blockIdx.x);  // uniform
+  __syncthreads();
+  mixed_fft<R, N, DIR, 1>(lds, t, (const cpx<R>*)a.tw);
+  const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
+  const bool scale = a.scale_im != 0 || a.scale_re != 1;
+  double pout = 0;
+  for (int p = t; p < N; p += T) {
+    const int q = p - a.out_col0;
+    if (q >= 0 && q < a.out_cols) {
+      cpx<R> y = lds[lpad(p)];
+      if (scale) y = cmul(y, sc);
+      pout += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+      out[tiled_offset(q, row, a.out_ld, a.out_tw, a.out_tstride)] = y;
+    }
+  }
+  if (a.pw_out) block_sum_store<T>(pout, red, a.pw_out, blockIdx.x);  // uniform
+}
+
+// ---------------------------------------------------------------------------
+// Column pass over W adjacent columns of the column-tiled intermediate:
+// forward transform -> transfer function (any TF kind, tf_eval) ->
+// inverse transform, in place. Threads are interleaved over the columns
+// (c = tid % W) so a row of the tile is read as one W-element segment.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int W>
+__global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a) {
+  constexpr int T = MPlan<N>::T;
+  __shared__ cpx<R> lds[lds_len(N) * W];
+  const int tid = threadIdx.x;
+  const int c = tid % W, t = tid / W;
+  const int tpu = a.cols / W;
+  const int unit = blockIdx.x / tpu;
+  const int col = (blockIdx.x % tpu) * W + c;
+  const int tl = a.tiled ? 1 << a.tlg : 0;
+  const int ld = a.tiled ? tl : a.ld;
+  cpx<R>* base =
+      (cpx<R>*)a.data + (size_t)unit * a.unit + (a.tiled ? (col >> a.tlg) * a.tstride + (col & (tl - 1)) : col);
+  cpx<R>* lc = lds + c;
+  for (int p = t; p < N; p += T) {
+    const int q = p - a.in_row0;
+    cpx<R> x = {R(0), R(0)};
+    if (q >= 0 && q < a.rows) x = base[(size_t)q * ld];
+    lc[lpad(p) * W] = x;
+  }
+  __syncthreads();
+  const cpx<R>* tw = (const cpx<R>*)a.tw;
+  mixed_fft<R, N, -1, W>(lc, t, tw);
+  const TFUnit& u = a.tf[unit];
+  for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], tf_eval<R>(u, col, p));
+  __syncthreads();
+  mixed_fft<R, N, +1, W>(lc, t, tw);
+  const R sc = (R)a.scale;
+  for (int p = t; p < N; p += T) {
+    const int q = p - a.out_row0;
+    if (q >= 0 && q < a.out_rows) {
+      cpx<R> y = lc[lpad(p) * W];
+      if (sc != R(1)) y = cscale(y, sc);
+      base[(size_t)q * ld] = y;
+    }
+  }
+}
+
+}  // namespace wos
