@@ -46,8 +46,8 @@ bool col_w_valid8_f64(int N, int W);
 bool col_w_valid8_f32(int N, int W);
 int launch_rows_mixed_f64(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_rows_mixed_f32(int N, int dir, const RowArgs& a, hipStream_t s);
-int launch_cols_mixed_f64(int N, int W, const ColArgs& a, hipStream_t s);
-int launch_cols_mixed_f32(int N, int W, const ColArgs& a, hipStream_t s);
+int launch_cols_mixed_f64(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
+int launch_cols_mixed_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int mixed_col_w_f64(int N);
 int mixed_col_w_f32(int N);
 int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
@@ -427,8 +427,9 @@ static int mixed_col_w(int prec, int n) {
 static int launch_rows_mixed(int prec, int N, int dir, const RowArgs& a, hipStream_t s) {
   return prec == WOS_FP64 ? launch_rows_mixed_f64(N, dir, a, s) : launch_rows_mixed_f32(N, dir, a, s);
 }
-static int launch_cols_mixed(int prec, int N, int W, const ColArgs& a, hipStream_t s) {
-  return prec == WOS_FP64 ? launch_cols_mixed_f64(N, W, a, s) : launch_cols_mixed_f32(N, W, a, s);
+// mode 0: one column transform (dir); mode 2: forward -> TF -> inverse
+static int launch_cols_mixed(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
+  return prec == WOS_FP64 ? launch_cols_mixed_f64(N, mode, dir, W, a, s) : launch_cols_mixed_f32(N, mode, dir, W, a, s);
 }
 
 __global__ void k_convert_scale_f64(const cpx<double>* __restrict__ in, cpx<double>* __restrict__ out, int n, double sc) {
@@ -514,6 +515,13 @@ static int rows_any(int prec, void* d, int L, int rows, int ld, long long unit, 
   ra.units = units;
   ra.in_cols = L;
   ra.out_cols = L;
+  if (mixed_col_w(prec, L) > 0) {  // 2^a 3^b 5^c lengths with a built mixed-radix plan
+    const void* tw;
+    if ((rc = get_mixed_twiddles(L, prec, &tw))) return rc;
+    ra.tw = tw;
+    ra.scale_re = scale;
+    return launch_rows_mixed(prec, L, inverse ? 1 : -1, ra, s);
+  }
   if (pow2(L)) {
     const void* tw;
     if ((rc = get_twiddles(L, prec, &tw))) return rc;
@@ -990,6 +998,13 @@ static int fft2d_dev(int prec, void* d, int n, int units, int inverse) {
     ca.tw = tw;
     return launch_cols(prec, n, 0, inverse ? 1 : -1, 0, ca, dv.stream);
   }
+  if (mixed_col_w(prec, n) > 0) {
+    const void* tw;
+    if ((rc = get_mixed_twiddles(n, prec, &tw))) return rc;
+    ca.scale = sc;
+    ca.tw = tw;
+    return launch_cols_mixed(prec, n, 0, inverse ? 1 : -1, mixed_col_w(prec, n), ca, dv.stream);
+  }
   const int M = bs_len(n);
   BsColArgs A;
   A.io = ca;
@@ -1231,7 +1246,7 @@ static void set_transfer_power(TransferLaunch& L, double* pw_in, double* pw_out)
 static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s) {
   if (L.mr) {
     if (k == 0) return launch_rows_mixed(L.prec, L.M, -1, L.fwd, s);
-    if (k == 1) return launch_cols_mixed(L.prec, L.M, L.tu.tw, L.col, s);
+    if (k == 1) return launch_cols_mixed(L.prec, L.M, 2, 1, L.tu.tw, L.col, s);
     return launch_rows_mixed(L.prec, L.M, 1, L.inv, s);
   }
   if (L.bs) {

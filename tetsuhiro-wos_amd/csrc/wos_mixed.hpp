@@ -186,12 +186,14 @@ __global__ __launch_bounds__(MPlan<N>::T) void k_rows_mixed(const RowArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Column pass over W adjacent columns of the column-tiled intermediate:
-// forward transform -> transfer function (any TF kind, tf_eval) ->
-// inverse transform, in place. Threads are interleaved over the columns
-// (c = tid % W) so a row of the tile is read as one W-element segment.
+// Column pass over W adjacent columns (tiled intermediate or row-major).
+// MODE 0: one transform in direction DIR, then the scale; MODE 2: forward
+// transform -> transfer function (any TF kind, tf_eval) -> inverse
+// transform, then the scale. In place. Threads are interleaved over the
+// columns (c = tid % W) so a row of the tile is read as one W-element
+// segment.
 // ---------------------------------------------------------------------------
-template <typename R, int N, int W>
+template <typename R, int N, int W, int MODE, int DIR>
 __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a) {
   constexpr int T = MPlan<N>::T;
   __shared__ cpx<R> lds[lds_len(N) * W];
@@ -213,11 +215,15 @@ __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a)
   }
   __syncthreads();
   const cpx<R>* tw = (const cpx<R>*)a.tw;
-  mixed_fft<R, N, -1, W>(lc, t, tw);
-  const TFUnit& u = a.tf[unit];
-  for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], tf_eval<R>(u, col, p));
-  __syncthreads();
-  mixed_fft<R, N, +1, W>(lc, t, tw);
+  if constexpr (MODE == 2) {
+    mixed_fft<R, N, -1, W>(lc, t, tw);
+    const TFUnit& u = a.tf[unit];
+    for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], tf_eval<R>(u, col, p));
+    __syncthreads();
+    mixed_fft<R, N, +1, W>(lc, t, tw);
+  } else {
+    mixed_fft<R, N, DIR, W>(lc, t, tw);
+  }
   const R sc = (R)a.scale;
   for (int p = t; p < N; p += T) {
     const int q = p - a.out_row0;
