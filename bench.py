@@ -34,13 +34,13 @@ VALU_WAVE_INSTR_PER_S = 614.4e9
 # HBM bytes per full-batch launch (64 x 2048^2), rocprofv3 PMC: 2 x FETCH_SIZE + WRITE_SIZE
 # (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md
 PMC_HBM_BYTES = {
-    "fp32": {"row_fft_fwd": (1048758 * 2 + 2097152) * 1024, "col_fft_tf_ifft": (1049123 * 2 + 2097152) * 1024,
-             "row_fft_inv": (1048762 * 2 + 2097152) * 1024},
-    "fp64": {"row_fft_fwd": (2104635 * 2 + 4194305) * 1024, "col_fft_tf_ifft": (2119539 * 2 + 4194310) * 1024,
-             "row_fft_inv": (2102526 * 2 + 4194304) * 1024},
+    "fp32": {"row_fft_fwd": (1048722 * 2 + 2097169) * 1024, "col_fft_tf_ifft": (1048901 * 2 + 2097262) * 1024,
+             "row_fft_inv": (1048717 * 2 + 2097160) * 1024},
+    "fp64": {"row_fft_fwd": (2097374 * 2 + 4195035) * 1024, "col_fft_tf_ifft": (2097664 * 2 + 4198770) * 1024,
+             "row_fft_inv": (2097382 * 2 + 4194355) * 1024},
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
-PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1774 * 262144, "row_fft_inv": 872 * 262144}}
+PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1532 * 262144, "row_fft_inv": 872 * 262144}}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
 
