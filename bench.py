@@ -55,9 +55,10 @@ def run_gpu(wos, precision, steps, warmup, shape=None):
     b = wos.Batch(N, DX, wls, zs, wos.ctx_for(550e-9), "asm", precision)
     b.fill_schell(W0, SIGMA_C, SEED)
     inp0 = b.download(0)
-    if shape:   # a fixed launch shape: tile_w, col_w, rows, row groups, split[, radix[, col_radix]]
+    if shape:   # a fixed launch shape: tile_w, col_w, rows, row groups, split[, radix[, col_radix[, col_split]]]
         shape = list(shape) + [16] * (6 - len(shape))
         shape = shape + [shape[5]] * (7 - len(shape))
+        shape = shape + [0] * (8 - len(shape))
         b.set_tuning(*shape)
         tune = (dict(zip(wos.TUNING_KEYS, shape)), b.time(1, 5)[0])
     else:
@@ -457,8 +458,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline batch (no other workloads)")
     ap.add_argument("--no-tune", action="store_true",
                     help="extra workloads at their default launch shapes (profiling: no autotuner launches)")
-    ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix]]")
-    ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix]]")
+    ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
+    ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
     args = ap.parse_args()
     shape32 = [int(x) for x in args.shape32.split(",")] if args.shape32 else None
     shape64 = [int(x) for x in args.shape64.split(",")] if args.shape64 else None

@@ -213,19 +213,20 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
  * workgroup, the row groups of the pipelined kernels (0: one group, 2, 4)
  * and, for radix-16 one-group kernels, split LDS exchanges (real and
  * imaginary parts in two rounds through half the LDS); for the column pass
- * its own stage schedule (col_radix) and the columns per workgroup (col_w:
- * tile_w, tile_w/2 or tile_w/4). For each tile_w the row and column shapes
+ * its own stage schedule (col_radix), the columns per workgroup (col_w:
+ * tile_w, tile_w/2 or tile_w/4) and, radix-16, split LDS exchanges
+ * (col_split). For each tile_w the row and column shapes
  * are searched apart, each kernel timed over `iters` launches with HIP
  * events on the batch stream, and the tile_w with the smallest sum wins (a
  * captured graph is dropped). The launches propagate the batch's fields:
- * refill them afterwards. shape (may be NULL, else 7 ints) receives
- * {tile_w, col_w, rows, groups, split, radix, col_radix}; best_ms (may be
- * NULL) the time of one whole run at that shape. */
+ * refill them afterwards. shape (may be NULL, else 8 ints) receives
+ * {tile_w, col_w, rows, groups, split, radix, col_radix, col_split}; best_ms
+ * (may be NULL) the time of one whole run at that shape. */
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
 /* Set the launch shape directly (e.g. from a cached autotune table);
  * radix / col_radix 0 mean 16. */
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
-                             int col_radix);
+                             int col_radix, int col_split);
 /* capture one run into a hipGraph; later runs replay it */
 int wos_hip_batch_capture(wos_batch b);
 /* enqueue `iters` runs on the batch's stream (asynchronous) */

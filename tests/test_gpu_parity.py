@@ -476,13 +476,19 @@ def test_batch_radix8_schedule(wos, oracle, precision, n, method):
         shapes8 += [(2, 1, 1, 2), (4, 2, 4, 4)]
     # the row and column passes take their schedules independently
     runs = [(tw, cw, rows, groups, 0, 8, 8) for tw, cw, rows, groups in shapes8]
-    runs += [(4, 4, 0, 0, 1, 16, 8), (8, 4, 2, 2, 0, 8, 16)]
+    runs += [(4, 4, 0, 0, 1, 16, 8), (8, 4, 2, 2, 0, 8, 16), (4, 4, 0, 0, 1, 16, 16, 0), (4, 4, 0, 0, 1, 16, 16, 1),
+             (8, 4, 2, 2, 0, 8, 16, 1)]
     outs = {}
     for shape in runs:
         out = outs[shape] = run(shape)
         for u in range(2):
             e = rel_l2(out[u], ref16[u])
             assert e <= (1e-14 if prec == wos.FP64 else 1e-6), (n, method, shape, u, e)
+    # split column exchanges move the same values through half the LDS; as another
+    # instantiation the compiler may contract a product's multiply-adds differently
+    for u in range(2):
+        e = rel_l2(outs[(4, 4, 0, 0, 1, 16, 16, 1)][u], outs[(4, 4, 0, 0, 1, 16, 16, 0)][u])
+        assert e <= (1e-15 if prec == wos.FP64 else 1e-6), (u, e)
     if n <= 1024:
         out = outs[runs[0]]   # radix-8 row and column passes
         for u in range(2):

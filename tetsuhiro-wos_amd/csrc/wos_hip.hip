@@ -1038,6 +1038,7 @@ struct Tuning {
   int split = 0;   // one-group row kernels exchange real and imaginary parts separately
   int radix = 0;      // stage schedule of the row passes: 8 radix-8, else radix-16
   int col_radix = 0;  // stage schedule of the column pass: 8 radix-8, else radix-16
+  int col_split = 0;  // radix-16 asm column pass: exchanges through half the LDS
 };
 
 // The radix-8 schedule is built for the asm transfer kernels from 256 points.
@@ -1185,6 +1186,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ca.tf = d_tf;
   ca.tw = twc;
   ca.radix = tu.col_radix;
+  ca.split = !bs && !mr && L.mode == 1 && tu.col_radix == 16 && M >= 256 ? tu.col_split : 0;
   RowArgs& rb = L.inv;
   rb = row_args_init();
   rb.in = inter;
@@ -2636,7 +2638,8 @@ static int time_transfer_kernel(const TransferLaunch& L, int k, int iters, hipSt
 //   row passes (forward + inverse time): stage schedule (radix-16, or
 //     radix-8 from 256 points), rows per workgroup, row groups of the
 //     pipelined kernels, split exchanges (radix-16 one-group kernels);
-//   column pass: stage schedule, columns per workgroup (tw, tw/2 or tw/4).
+//   column pass: stage schedule, columns per workgroup (tw, tw/2 or tw/4),
+//     split exchanges (radix-16).
 // The tw with the smallest sum of its best row and column times wins.
 // build(t, L) builds the three launches of shape t.
 template <typename Build>
@@ -2688,16 +2691,19 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
         if (radix == 8 && !radix8_ok(M)) continue;
         for (int cw = tw; cw >= 1 && cw * 4 >= tw; cw /= 2) {
           if (!col_w_valid(prec, M, cw, radix)) continue;
-          Tuning t = tr;
-          t.col_w = cw;
-          t.col_radix = radix;
-          TransferLaunch L;
-          double c;
-          if ((rc = build(t, L))) return rc;
-          if ((rc = time_transfer_kernel(L, 1, iters, s, e0, e1, &c))) return rc;
-          if (c < bcol) {
-            bcol = c;
-            tc = t;
+          for (int cs = 0; cs <= (radix == 16 && M >= 256 ? 1 : 0); ++cs) {
+            Tuning t = tr;
+            t.col_w = cw;
+            t.col_radix = radix;
+            t.col_split = cs;
+            TransferLaunch L;
+            double c;
+            if ((rc = build(t, L))) return rc;
+            if ((rc = time_transfer_kernel(L, 1, iters, s, e0, e1, &c))) return rc;
+            if (c < bcol) {
+              bcol = c;
+              tc = t;
+            }
           }
         }
       }
@@ -2726,6 +2732,7 @@ static void tuning_out(const Tuning& t, int* shape) {
   shape[4] = t.split;
   shape[5] = t.radix == 8 ? 8 : 16;
   shape[6] = t.col_radix == 8 ? 8 : 16;
+  shape[7] = t.col_split;
 }
 
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) {
@@ -2762,11 +2769,15 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) 
 }
 
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
-                             int col_radix) {
+                             int col_radix, int col_split) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
   if (radix == 0) radix = 16;
   if (col_radix == 0) col_radix = 16;
+  if (col_split && col_radix != 16) {
+    set_error("wos_hip_batch_set_tuning: split column exchanges are radix-16 column passes only");
+    return WOS_ERR_INVALID;
+  }
   if (!(radix == 16 || (radix == 8 && radix8_ok(b->M) && !split)) ||
       !(col_radix == 16 || (col_radix == 8 && radix8_ok(b->M)))) {
     set_error("wos_hip_batch_set_tuning: row radix " + std::to_string(radix) + (split ? " with split exchanges" : "") +
@@ -2792,6 +2803,7 @@ int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int g
   b->tune.split = split ? 1 : 0;
   b->tune.radix = radix;
   b->tune.col_radix = col_radix;
+  b->tune.col_split = col_split ? 1 : 0;
   return WOS_OK;
 }
 

@@ -630,6 +630,7 @@ struct ColArgs {
   int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
   const void* htab;    // MODE 3: FP64 H table in W-column tile order (nullable: evaluate H)
   int radix;           // stage schedule of tw: 0 or 16 radix-16, 8 radix-8 (MODE 1 only)
+  int split;           // MODE 1, radix-16: exchanges through half the LDS (real, then imaginary parts)
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
@@ -639,12 +640,17 @@ __device__ __forceinline__ int remap_xcd(int bid, int nb) {
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-template <typename R, int N, int W, int MODE, int DIR, int VV = 16>
+// CS: the stage exchanges and the mirrored-TF exchange move real and
+// imaginary parts in two rounds through half the LDS, so more workgroups fit
+// a CU.
+template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false>
 __global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) {
   constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
-  __shared__ cpx<R> lds[lds_len(N) * W];
+  __shared__ cpx<R> lds[CS ? (lds_len(N) * W + 1) / 2 : lds_len(N) * W];
   const int tid = threadIdx.x;
   const int c = tid % W, t = tid / W;
+  // this thread's column in LDS (CS: values of R, W apart)
+  cpx<R>* const lc = CS ? (cpx<R>*)((R*)lds + c) : lds + c;
   const int bid = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int tpu = a.cols / W;
   const int unit = bid / tpu;
@@ -687,7 +693,7 @@ __global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) 
       }
   }
   constexpr bool ACC = MODE == 3 && sizeof(R) == 4;  // split-step slices: compensated twiddles
-  fft_block<R, N, DIR0, W, ACC, false, VV>(v, t, tw, lds + c);
+  fft_block<R, N, DIR0, W, ACC, CS, VV>(v, t, tw, lc);
   if constexpr (MODE == 1 || MODE == 2) {
     constexpr int RL = last_rad<N, -1, VV>();
     if (active) {
@@ -715,23 +721,57 @@ __global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) 
             return h;
           }
         };
-        __syncthreads();  // all waves are done reading the forward transform's last exchange
+        if constexpr (CS) {  // the exchange in two rounds: real parts, then imaginary parts
+          R* hr = (R*)lds + c;
+          cpx<R> hv[V2];
 #pragma unroll
-        for (int m = 0; m < V2; ++m) {
-          const cpx<R> h = tf_at(t + T * m);
-          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
-          hx[(m * T + t) * W] = h;
-        }
-        if (t == 0) {  // row N/2 (Nyquist) is its own mirror
-          constexpr int m = V2;
-          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
-        }
-        __syncthreads();
+          for (int m = 0; m < V2; ++m) {
+            hv[m] = tf_at(t + T * m);
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hv[m]);
+          }
+          if (t == 0) {  // row N/2 (Nyquist) is its own mirror
+            constexpr int m = V2;
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
+          }
+          R part[V - V2];
+          __syncthreads();  // all waves are done reading the forward transform's last exchange
 #pragma unroll
-        for (int m = V2; m < V; ++m) {
-          if (t == 0 && m == V2) continue;
-          const int ms = t == 0 ? V - m : V - 1 - m;
-          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
+          for (int m = 0; m < V2; ++m) hr[(m * T + t) * W] = hv[m].x;
+          __syncthreads();
+#pragma unroll
+          for (int m = V2; m < V; ++m) {
+            const int ms = t == 0 ? V - m : V - 1 - m;
+            part[m - V2] = (t == 0 && m == V2) ? R(0) : hr[(ms * T + pt) * W];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int m = 0; m < V2; ++m) hr[(m * T + t) * W] = hv[m].y;
+          __syncthreads();
+#pragma unroll
+          for (int m = V2; m < V; ++m) {
+            if (t == 0 && m == V2) continue;
+            const int ms = t == 0 ? V - m : V - 1 - m;
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], cpx<R>{part[m - V2], hr[(ms * T + pt) * W]});
+          }
+        } else {
+          __syncthreads();  // all waves are done reading the forward transform's last exchange
+#pragma unroll
+          for (int m = 0; m < V2; ++m) {
+            const cpx<R> h = tf_at(t + T * m);
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
+            hx[(m * T + t) * W] = h;
+          }
+          if (t == 0) {  // row N/2 (Nyquist) is its own mirror
+            constexpr int m = V2;
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
+          }
+          __syncthreads();
+#pragma unroll
+          for (int m = V2; m < V; ++m) {
+            if (t == 0 && m == V2) continue;
+            const int ms = t == 0 ? V - m : V - 1 - m;
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
+          }
         }
       } else if constexpr (MODE == 1 && sizeof(R) == 4) {
         const TF32Col cc = tf32_col(u, col);
@@ -835,7 +875,7 @@ __global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) 
     }
   }
   if constexpr (MODE >= 1) {
-    fft_block<R, N, +1, W, ACC, false, VV>(v, t, tw, lds + c);
+    fft_block<R, N, +1, W, ACC, CS, VV>(v, t, tw, lc);
   }
   constexpr int DIRL = MODE >= 1 ? +1 : DIR;
   constexpr int RL = last_rad<N, DIRL, VV>();

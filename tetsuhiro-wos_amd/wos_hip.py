@@ -79,7 +79,7 @@ def make_shape(kind, verts=None, **params):
 
 
 # launch shape fields of a batch (wos_hip_batch_autotune / _set_tuning)
-TUNING_KEYS = ("tile_w", "col_w", "rows", "row_groups", "split_lds", "radix", "col_radix")
+TUNING_KEYS = ("tile_w", "col_w", "rows", "row_groups", "split_lds", "radix", "col_radix", "col_split_lds")
 
 
 class WosError(RuntimeError):
@@ -162,7 +162,7 @@ def load(path=None):
     L.wos_hip_batch_device_ptr.restype = P
     L.wos_hip_batch_time.argtypes = [P, I, I, C.POINTER(D), P]
     L.wos_hip_batch_autotune.argtypes = [P, I, P, C.POINTER(D)]
-    L.wos_hip_batch_set_tuning.argtypes = [P, I, I, I, I, I, I, I]
+    L.wos_hip_batch_set_tuning.argtypes = [P, I, I, I, I, I, I, I, I]
     L.wos_hip_mask_create.argtypes = [C.POINTER(P), I, I, D, D, C.POINTER(Ctx), I, P]
     L.wos_hip_mask_destroy.argtypes = [P]
     L.wos_hip_mask_forward.argtypes = [P, P, P]
@@ -530,16 +530,16 @@ class Batch:
 
     def autotune(self, iters=5):
         """Pick the launch shape; returns ({tile_w, col_w, rows, row_groups,
-        split_lds, radix, col_radix}, ms per run)."""
-        shape = np.zeros(7, dtype=np.int32)
+        split_lds, radix, col_radix, col_split_lds}, ms per run)."""
+        shape = np.zeros(8, dtype=np.int32)
         ms = C.c_double()
         _check(load().wos_hip_batch_autotune(self.h, iters, _p(shape), C.byref(ms)), "batch_autotune")
         return dict(zip(TUNING_KEYS, (int(x) for x in shape))), ms.value
 
-    def set_tuning(self, tile_w, col_w=None, rows=0, groups=0, split=0, radix=0, col_radix=None):
+    def set_tuning(self, tile_w, col_w=None, rows=0, groups=0, split=0, radix=0, col_radix=None, col_split=0):
         """col_radix None: the row passes' radix."""
         _check(load().wos_hip_batch_set_tuning(self.h, tile_w, tile_w if col_w is None else col_w, rows, groups,
-                                               split, radix, radix if col_radix is None else col_radix),
+                                               split, radix, radix if col_radix is None else col_radix, col_split),
                "batch_set_tuning")
 
     def capture(self):
