@@ -94,3 +94,27 @@ def test_invalid_and_unsupported_arguments(wos):
     with pytest.raises(wos.WosError):
         bt.set_tuning(4, 4, 3, 0)
     bt.close()
+
+
+def test_fresnel_ir_spectrum_cache(wos, oracle):
+    """fresnel_ir keeps the impulse-response spectrum of the last (precision,
+    grid, dx, wl, z): repeated calls reuse it, a change of z or precision
+    rebuilds it, and every result equals the first (bit for bit) and the
+    oracle (1e-12)."""
+    n, dx, wl = 128, 4e-6, 633e-9
+    rng = np.random.default_rng(9)
+    ex = np.exp(1j * rng.uniform(0, 0.5, (n, n))) * np.exp(-((np.arange(n) - n / 2) ** 2)[None, :] / 400.0)
+    ctx = wos.ctx_for(wl)
+    first = {}
+    for z in (0.01, 0.02, 0.01, 0.01):
+        for prec in (wos.FP64, wos.FP32, wos.FP64):
+            out, _, _, _ = wos.propagate(ex, None, dx, z, "fresnel_ir", ctx, precision=prec)
+            key = (z, prec)
+            if key in first:
+                assert np.array_equal(out, first[key]), key
+            else:
+                first[key] = out
+    for z in (0.01, 0.02):
+        ref, _, _, _ = oracle.propagate(ex, None, dx, z, "fresnel_ir", oracle.ctx_for(wl))
+        assert rel_l2(first[(z, wos.FP64)], ref) <= 1e-12
+        assert rel_l2(first[(z, wos.FP32)], ref) <= 1e-5

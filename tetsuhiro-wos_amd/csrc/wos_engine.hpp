@@ -51,6 +51,12 @@ struct Device {
   // work: per-unit working copies; aux: spectral-diagnostic copies;
   // scratch: padded intermediates; ir: fresnel_ir spectrum; screens: BPM tables
   DevBuf field, stage, work, aux, scratch, ir, screens, params, partial_in, partial_out, hist, out;
+  // fresnel_ir: the impulse-response spectrum of the last (precision, grid,
+  // dx, wl, z), reused while they repeat
+  DevBuf fir;
+  bool fir_ok = false;
+  int fir_prec = 0, fir_m = 0;
+  double fir_dx = 0, fir_wl = 0, fir_z = 0;
   void* host_pin = nullptr;                // pinned staging for parameter uploads
   size_t host_pin_cap = 0;
   hipEvent_t pin_ev = nullptr;             // recorded after the last copy out of host_pin

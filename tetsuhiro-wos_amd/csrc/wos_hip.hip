@@ -1658,17 +1658,31 @@ static int propagate_dev(int prec, void* d, int n, int ncomp, double dx, double 
         set_error("fresnel_ir transform length not supported");
         return WOS_ERR_UNSUPPORTED;
       }
-      // impulse-response spectrum (optics/propagate.go:543-559) in its own buffer
+      // impulse-response spectrum (optics/propagate.go:543-559) in its own
+      // buffer, computed once per (precision, grid, dx, wl, z)
       const size_t hb = (size_t)m * m * csize(prec);
-      DevBuf& ir = dv.ir;
-      if ((rc = ir.ensure(hb))) return rc;
-      const size_t total = (size_t)m * m;
-      if (prec == WOS_FP64)
-        hipLaunchKernelGGL(k_fresnel_ir_kernel<double>, dim3(grid_for(total)), dim3(256), 0, dv.stream, (cpx<double>*)ir.p, m, dx, wl, z);
-      else
-        hipLaunchKernelGGL(k_fresnel_ir_kernel<float>, dim3(grid_for(total)), dim3(256), 0, dv.stream, (cpx<float>*)ir.p, m, dx, wl, z);
-      WOS_HIP_TRY(hipGetLastError());
-      if ((rc = fft2d_dev(prec, ir.p, m, 1, 0))) return rc;
+      DevBuf& ir = dv.fir;
+      const bool cached = dv.fir_ok && dv.fir_prec == prec && dv.fir_m == m && dv.fir_dx == dx && dv.fir_wl == wl &&
+                          dv.fir_z == z && ir.cap >= hb;
+      if (!cached) {
+        dv.fir_ok = false;
+        if ((rc = ir.ensure(hb))) return rc;
+        const size_t total = (size_t)m * m;
+        if (prec == WOS_FP64)
+          hipLaunchKernelGGL(k_fresnel_ir_kernel<double>, dim3(grid_for(total)), dim3(256), 0, dv.stream,
+                             (cpx<double>*)ir.p, m, dx, wl, z);
+        else
+          hipLaunchKernelGGL(k_fresnel_ir_kernel<float>, dim3(grid_for(total)), dim3(256), 0, dv.stream,
+                             (cpx<float>*)ir.p, m, dx, wl, z);
+        WOS_HIP_TRY(hipGetLastError());
+        if ((rc = fft2d_dev(prec, ir.p, m, 1, 0))) return rc;
+        dv.fir_ok = true;
+        dv.fir_prec = prec;
+        dv.fir_m = m;
+        dv.fir_dx = dx;
+        dv.fir_wl = wl;
+        dv.fir_z = z;
+      }
       TFUnit u = make_asm_tf(wl, z, ctx, m, dx, 0, 0);
       u.kind = TF_TABLE;
       u.table = ir.p;
@@ -2000,8 +2014,9 @@ int wos_hip_release_cache(void) {
   for (auto& kv : d.chirps) (void)hipFree(kv.second);
   d.chirps.clear();
   for (DevBuf* b : {&d.field, &d.stage, &d.work, &d.aux, &d.scratch, &d.ir, &d.screens, &d.params, &d.partial_in,
-                    &d.partial_out, &d.hist, &d.out})
+                    &d.partial_out, &d.hist, &d.out, &d.fir})
     b->release();
+  d.fir_ok = false;
   return WOS_OK;
 }
 
