@@ -417,7 +417,9 @@ typedef struct wos_mask_s* wos_mask;
 int wos_hip_mask_create(wos_mask* out, int n, int units, double dx, double z, const wos_ctx* ctx, int precision,
                         const double* target);
 int wos_hip_mask_destroy(wos_mask m);
-/* theta: units*n*n doubles (unit-major, row-major); loss: units doubles (may be NULL) */
+/* theta: units*n*n doubles (unit-major, row-major) in host memory, or NULL
+ * to use the masks already in device memory (the last upload, or what the
+ * caller wrote at wos_hip_mask_device_theta); loss: units doubles (may be NULL) */
 int wos_hip_mask_forward(wos_mask m, const double* theta, double* loss);
 /* gradient of the last forward's losses: units*n*n doubles (may be NULL to
  * leave it on the device, see wos_hip_mask_device_grad); repeatable */
@@ -426,6 +428,10 @@ int wos_hip_mask_backward(wos_mask m, double* grad);
 int wos_hip_mask_field(wos_mask m, int unit, double* out);
 /* device pointer of the gradient (units*n*n doubles) */
 void* wos_hip_mask_device_grad(wos_mask m);
+/* device pointer of the masks theta (units*n*n doubles): an optimiser that
+ * keeps its step on the GPU writes theta there and calls the forward with
+ * theta = NULL, so no mask crosses PCIe */
+void* wos_hip_mask_device_theta(wos_mask m);
 
 /* ---- multi-plane phase design through a split-step medium (no reference
  * counterpart) ------------------------------------------------------------

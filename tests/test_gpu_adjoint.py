@@ -2,6 +2,7 @@
 adjoint gradient (no reference counterpart). Checked against the oracle's
 FP64 CPU adjoint (itself pinned to central differences in test_oracle.py)
 and against central finite differences of the GPU's own FP64 forward."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -52,6 +53,20 @@ def test_mask_operator_fp64_matches_cpu_adjoint(wos, oracle, n):
         assert eg <= 1e-10
     u_ref, _, _, _ = oracle.propagate(np.exp(1j * theta[1]), None, dx, z, "asm", oracle.ctx_for(wl))
     assert rel_l2(op.field(1), u_ref) <= 1e-12
+    # device-resident masks: forward(None) reuses the uploaded theta; masks
+    # written straight into device memory give the host-upload results
+    assert np.array_equal(op.forward(None), loss)
+    assert op.backward(to_host=False) is None
+    assert np.array_equal(op.backward(), grad)
+    theta2 = masks(2, n, 4)
+    loss2, grad2 = op.forward(theta2), op.backward()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    op.forward(theta)
+    assert hip.hipMemcpy(wos.load().wos_hip_mask_device_theta(op.h), theta2.ctypes.data, theta2.nbytes, 1) == 0
+    assert np.array_equal(op.forward(None), loss2)
+    assert np.array_equal(op.backward(), grad2)
     op.close()
 
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
     "wos_hip_batch_autotune", "wos_hip_batch_set_tuning", "wos_hip_last_device_ms",
     "wos_hip_mask_create", "wos_hip_mask_destroy", "wos_hip_mask_forward", "wos_hip_mask_backward",
-    "wos_hip_mask_field", "wos_hip_mask_device_grad",
+    "wos_hip_mask_field", "wos_hip_mask_device_grad", "wos_hip_mask_device_theta",
     "wos_hip_pipeline_create", "wos_hip_pipeline_destroy", "wos_hip_pipeline_add",
     "wos_hip_pipeline_set_intensity", "wos_hip_pipeline_upload", "wos_hip_pipeline_fill_schell",
     "wos_hip_pipeline_autotune", "wos_hip_pipeline_capture", "wos_hip_pipeline_run", "wos_hip_pipeline_sync",
@@ -170,6 +170,8 @@ def load(path=None):
     L.wos_hip_mask_field.argtypes = [P, I, P]
     L.wos_hip_mask_device_grad.argtypes = [P]
     L.wos_hip_mask_device_grad.restype = P
+    L.wos_hip_mask_device_theta.argtypes = [P]
+    L.wos_hip_mask_device_theta.restype = P
     L.wos_hip_pipeline_create.argtypes = [C.POINTER(P), I]
     L.wos_hip_pipeline_destroy.argtypes = [P]
     L.wos_hip_pipeline_add.argtypes = [P, I, D, I, I, I, P, P, C.POINTER(Ctx), C.POINTER(I)]
@@ -590,15 +592,17 @@ class MaskOp:
             pass
 
     def forward(self, theta):
-        """theta: (units, n, n) radians -> losses (units,)"""
-        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(self.units, self.n, self.n)
+        """theta: (units, n, n) radians -> losses (units,); None: the masks
+        already in device memory"""
+        th = None if theta is None else np.ascontiguousarray(theta, dtype=np.float64).reshape(self.units, self.n, self.n)
         loss = np.zeros(self.units, dtype=np.float64)
         _check(load().wos_hip_mask_forward(self.h, _p(th), _p(loss)), "mask_forward")
         return loss
 
-    def backward(self):
-        """dL/dtheta of the last forward, (units, n, n)"""
-        g = np.zeros((self.units, self.n, self.n), dtype=np.float64)
+    def backward(self, to_host=True):
+        """dL/dtheta of the last forward, (units, n, n); to_host False leaves
+        it in device memory (wos_hip_mask_device_grad) and returns None"""
+        g = np.zeros((self.units, self.n, self.n), dtype=np.float64) if to_host else None
         _check(load().wos_hip_mask_backward(self.h, _p(g)), "mask_backward")
         return g
 
