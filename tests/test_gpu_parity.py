@@ -226,13 +226,15 @@ def test_methods_non_pow2_grid_fp64_parity(wos, oracle, method):
     assert wg.as_dict().keys() == wo.as_dict().keys()
 
 
-@pytest.mark.parametrize("n,method", [(384, "asm"), (384, "asm_pad"), (1536, "asm"), (400, "asm_shift")])
+@pytest.mark.parametrize("n,method", [(384, "asm"), (384, "asm_pad"), (1536, "asm"), (400, "asm_shift"), (600, "asm_pad"),
+                                      (1500, "asm")])
 def test_mixed_radix_grids_match_oracle(wos, oracle, n, method):
     """Grid sides 2^a 3^b 5^c run the mixed-radix kernels (radix 16/8/4/2
     stages, then 3 and 5) in place of Bluestein: 384 (16, 8, 3), its padded
-    768 (16, 16, 3), 1536 (16, 16, 2, 3) and 400 (16, 5, 5) with the shifted
-    transfer function, against the oracle's Bluestein path (the FP64 and
-    FP32 tolerances)."""
+    768 (16, 16, 3), 1536 (16, 16, 2, 3), 400 (16, 5, 5) with the shifted
+    transfer function, 600 padded to 1200 (16, 3, 5, 5) and 1500
+    (4, 3, 5, 5, 5), against the oracle's Bluestein path (the FP64 and FP32
+    tolerances)."""
     wl = 632.8e-9
     ex, dx = gaussian_tilted(oracle, n, n * 4e-6, wl, n * 4e-6 / 8, 0.01)
     z = 0.01

@@ -8,7 +8,7 @@ numpy's FFT."""
 import numpy as np
 import pytest
 
-SIZES = [100, 200, 384, 400, 768, 1000, 1536, 2000, 3072]
+SIZES = [100, 200, 300, 384, 400, 500, 600, 768, 800, 1000, 1200, 1500, 1536, 2000, 2400, 3000, 3072, 4000]
 
 
 def plan(n):
