@@ -218,7 +218,19 @@ __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a)
   if constexpr (MODE == 2) {
     mixed_fft<R, N, -1, W>(lc, t, tw);
     const TFUnit& u = a.tf[unit];
-    for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], tf_eval<R>(u, col, p));
+    // transfer functions even in fy (no spectral shift): row N - p has the
+    // exact negative of row p's frequency, so one evaluation serves both
+    const bool mirror = u.fy_off == 0.0 && (u.kind == TF_ASM || u.kind == TF_CONJ_ASM || u.kind == TF_FRESNEL ||
+                                            u.kind == TF_BANDLIMIT);
+    if (mirror) {
+      for (int p = t; p <= N / 2; p += T) {
+        const cpx<R> h = tf_eval<R>(u, col, p);
+        lc[lpad(p) * W] = cmul(lc[lpad(p) * W], h);
+        if (p != 0 && 2 * p != N) lc[lpad(N - p) * W] = cmul(lc[lpad(N - p) * W], h);
+      }
+    } else {
+      for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], tf_eval<R>(u, col, p));
+    }
     __syncthreads();
     mixed_fft<R, N, +1, W>(lc, t, tw);
   } else {
