@@ -139,7 +139,7 @@ struct BsColArgs {
 template <typename R, int M, int W, int MODE>
 __global__ __launch_bounds__(W* Plan<M>::T) void k_cols_bs(const BsColArgs A) {
   const ColArgs& a = A.io;
-  constexpr int T = Plan<M>::T, V = Plan<M>::V;
+  constexpr int V = Plan<M>::V;
   constexpr int R0 = first_rad<M, -1>();
   __shared__ cpx<R> lds[lds_len(M) * W];
   const int tid = threadIdx.x;
@@ -188,32 +188,6 @@ __global__ __launch_bounds__(W* Plan<M>::T) void k_cols_bs(const BsColArgs A) {
       const int q = p - a.out_row0;
       if (active && p < L && q >= 0 && q < a.out_rows) base[q * ld] = cscale(v[b * R0 + r], sc);
     }
-}
-
-// b[k] of the Bluestein convolution (optics/fft.go:182-187) on M points:
-// b[0] = 1, b[k] = b[M-k] = conj(c[k]) for 0 < k < L, else 0 (FP64).
-static __global__ void k_bs_b0(const cpx<double>* __restrict__ chirp, cpx<double>* __restrict__ b, int L, int M) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  cpx<double> v = {0.0, 0.0};
-  if (i == 0) v = {1.0, 0.0};
-  else if (i < L) v = {chirp[i].x, -chirp[i].y};
-  else if (i > M - L) v = {chirp[M - i].x, -chirp[M - i].y};
-  b[i] = v;
-}
-
-// chirp[k] = cexpI(sign*math.Pi*float64(k*k)/float64(L)) (optics/fft.go:174-176)
-static __global__ void k_bs_chirp(cpx<double>* __restrict__ chirp, int L, double sign) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= L) return;
-  double ph;
-  {
-#pragma clang fp contract(off)
-    ph = sign * M_PI * (double)((long long)k * k) / (double)L;
-  }
-  double s, c;
-  sincos(ph, &s, &c);
-  chirp[k] = {c, s};
 }
 
 }  // namespace wos

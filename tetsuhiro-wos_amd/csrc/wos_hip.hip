@@ -443,6 +443,32 @@ __global__ void k_convert_scale_f32(const cpx<double>* __restrict__ in, cpx<floa
 
 int launch_rows(int prec, int N, int dir, const RowArgs& a, hipStream_t s, int rows);
 
+// b[k] of the Bluestein convolution (optics/fft.go:182-187) on M points:
+// b[0] = 1, b[k] = b[M-k] = conj(c[k]) for 0 < k < L, else 0 (FP64).
+__global__ void k_bs_b0(const cpx<double>* __restrict__ chirp, cpx<double>* __restrict__ b, int L, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  cpx<double> v = {0.0, 0.0};
+  if (i == 0) v = {1.0, 0.0};
+  else if (i < L) v = {chirp[i].x, -chirp[i].y};
+  else if (i > M - L) v = {chirp[M - i].x, -chirp[M - i].y};
+  b[i] = v;
+}
+
+// chirp[k] = cexpI(sign*math.Pi*float64(k*k)/float64(L)) (optics/fft.go:174-176)
+__global__ void k_bs_chirp(cpx<double>* __restrict__ chirp, int L, double sign) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= L) return;
+  double ph;
+  {
+#pragma clang fp contract(off)
+    ph = sign * M_PI * (double)((long long)k * k) / (double)L;
+  }
+  double s, c;
+  sincos(ph, &s, &c);
+  chirp[k] = {c, s};
+}
+
 // Bluestein tables of (L, direction) in precision prec: chirp[L] and
 // FFT_M(b)/M, generated in FP64 on the device (the spectrum by the engine's
 // own M-point FFT) and rounded to prec. Cached.
@@ -1261,15 +1287,6 @@ static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s)
   if (k == 0) return launch_rows(L.prec, L.M, -1, L.fwd, s, L.tu.rows);
   if (k == 1) return launch_cols(L.prec, L.M, L.mode, 0, L.bs ? L.tu.tw : L.tu.col_w, L.col, s);
   return launch_rows(L.prec, L.M, 1, L.inv, s, L.tu.rows);
-}
-
-static int enqueue_transfer(int prec, void* d, void* inter, int n, int units, bool pad, const TFUnit* d_tf,
-                            int tf_kind, double post_re, double post_im, double* pw_in, double* pw_out,
-                            hipStream_t s, Tuning tu) {
-  TransferLaunch L;
-  int rc = build_transfer(L, prec, d, inter, n, units, pad, d_tf, tf_kind, post_re, post_im, pw_in, pw_out, tu);
-  for (int k = 0; k < 3 && !rc; ++k) rc = launch_transfer_kernel(L, k, s);
-  return rc;
 }
 
 // Synchronous wrapper on the device stream with per-field power sums.

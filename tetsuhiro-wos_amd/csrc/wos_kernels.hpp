@@ -553,7 +553,7 @@ __global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows(const RowArgs 
 // one-group kernel alternates between loading and computing.
 template <typename R, int N, int DIR, int ROWS, int G, int VV = 16>
 __global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows_pipe(const RowArgs a) {
-  constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
+  constexpr int V = Plan<N, VV>::V;
   constexpr int R0 = first_rad<N, DIR, VV>(), RL = last_rad<N, DIR, VV>();
   constexpr int LS = row_lds_stride<R>(N, ROWS);
   __shared__ cpx<R> lds[ROWS * LS];
