@@ -642,9 +642,10 @@ __device__ __forceinline__ int remap_xcd(int bid, int nb) {
 
 // CS: the stage exchanges and the mirrored-TF exchange move real and
 // imaginary parts in two rounds through half the LDS, so more workgroups fit
-// a CU.
+// a CU; in FP64 the kernel is also held to 128 VGPRs (4 waves per SIMD), as
+// registers would otherwise keep a second workgroup out.
 template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false>
-__global__ __launch_bounds__((W * Plan<N, VV>::T)) void k_cols(const ColArgs a) {
+__global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 : 1)) void k_cols(const ColArgs a) {
   constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
   __shared__ cpx<R> lds[CS ? (lds_len(N) * W + 1) / 2 : lds_len(N) * W];
   const int tid = threadIdx.x;
