@@ -34,10 +34,10 @@ VALU_WAVE_INSTR_PER_S = 614.4e9
 # HBM bytes per full-batch launch (64 x 2048^2), rocprofv3 PMC: 2 x FETCH_SIZE + WRITE_SIZE
 # (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md
 PMC_HBM_BYTES = {
-    "fp32": {"row_fft_fwd": (1048722 * 2 + 2097169) * 1024, "col_fft_tf_ifft": (1048901 * 2 + 2097262) * 1024,
-             "row_fft_inv": (1048717 * 2 + 2097160) * 1024},
-    "fp64": {"row_fft_fwd": (2097374 * 2 + 4195035) * 1024, "col_fft_tf_ifft": (2097664 * 2 + 4198770) * 1024,
-             "row_fft_inv": (2097382 * 2 + 4194355) * 1024},
+    "fp32": {"row_fft_fwd": (1048722 * 2 + 2097166) * 1024, "col_fft_tf_ifft": (1048909 * 2 + 2097259) * 1024,
+             "row_fft_inv": (1048724 * 2 + 2097155) * 1024},
+    "fp64": {"row_fft_fwd": (2097381 * 2 + 4195062) * 1024, "col_fft_tf_ifft": (2856819 * 2 + 6360144) * 1024,
+             "row_fft_inv": (2097380 * 2 + 4194348) * 1024},
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
 PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1532 * 262144, "row_fft_inv": 872 * 262144}}
