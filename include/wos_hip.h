@@ -365,6 +365,38 @@ int wos_hip_field_diffuser(wos_field f, double sigma, double correlation, double
  * c.re, c.im, d.re, d.im}; polarizer, retarder, rotator and custom_jones
  * (optics/elements.go:1018-1085) are this with their matrices */
 int wos_hip_field_jones(wos_field f, const double* abcd);
+/* More train elements (optics/elements.go:156-178), each applied in the
+ * reference's per-pixel order: cylindrical_lens (elements.go:230-270),
+ * apodizer (427-456), grating (458-531), axicon (534-556), spiral_phase
+ * (560-595), wedge (598-622), zone_plate (625-681), mirror (787-828),
+ * concave/convex mirror (831-899), zernike (902-1008). `sub` is the grating
+ * kind (0 amplitude_sin, 1 amplitude_binary, 2 phase_sin, 3 phase_binary,
+ * 4 blazed), the zone-plate kind (0 phase, 1 amplitude) or the spiral charge;
+ * the other fields carry the constructors' parameters (zernike: Noll
+ * coefficients c1..c21 in waves at [1..21], normalisation radius in radius).
+ * Parameters the reference's constructors reject return WOS_ERR_INVALID. */
+#define WOS_EL_CYLINDRICAL_LENS 0
+#define WOS_EL_APODIZER 1
+#define WOS_EL_GRATING 2
+#define WOS_EL_AXICON 3
+#define WOS_EL_SPIRAL_PHASE 4
+#define WOS_EL_WEDGE 5
+#define WOS_EL_ZONE_PLATE 6
+#define WOS_EL_MIRROR 7
+#define WOS_EL_CONCAVE_MIRROR 8
+#define WOS_EL_CONVEX_MIRROR 9
+#define WOS_EL_ZERNIKE 10
+typedef struct {
+  int32_t kind, sub;
+  double f, aperture, x0, y0, rotation;
+  double waist, amplitude;
+  double alpha, index;
+  double period, modulation, duty, blaze_depth, offset;
+  double radius;
+  double reflectivity, curvature, tilt_x, tilt_y;
+  double zernike[22];
+} wos_element;
+int wos_hip_field_element(wos_field f, const wos_element* element, double wavelength);
 /* Field.ScaleAmplitude (optics/field.go:134-143), the beamsplitter's arms */
 int wos_hip_field_scale(wos_field f, double re, double im);
 /* Field.ApplyBandlimit (optics/field.go:264-294); fraction outside (0, 1): no-op */

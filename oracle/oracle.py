@@ -6,6 +6,7 @@ Arrays are numpy complex128 (the memory layout of a Go []complex128),
 row-major, index y*N + x.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -67,6 +68,52 @@ SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4
                "double_slit": 7, "cross": 8, "polygon": 9}
 
 
+ELEMENT_KINDS = ("cylindrical_lens", "apodizer", "grating", "axicon", "spiral_phase", "wedge", "zone_plate",
+                 "mirror", "concave_mirror", "convex_mirror", "zernike")
+GRATING_KINDS = ("amplitude_sin", "amplitude_binary", "phase_sin", "phase_binary", "blazed")
+
+
+class Element(C.Structure):
+    """or_element: the parsed parameters of one train element
+    (optics/elements.go:156-178)."""
+    _fields_ = [("kind", C.c_int32), ("sub", C.c_int32),
+                ("f", C.c_double), ("aperture", C.c_double), ("x0", C.c_double), ("y0", C.c_double),
+                ("rotation", C.c_double), ("waist", C.c_double), ("amplitude", C.c_double),
+                ("alpha", C.c_double), ("index", C.c_double), ("period", C.c_double),
+                ("modulation", C.c_double), ("duty", C.c_double), ("blaze_depth", C.c_double),
+                ("offset", C.c_double), ("radius", C.c_double), ("reflectivity", C.c_double),
+                ("curvature", C.c_double), ("tilt_x", C.c_double), ("tilt_y", C.c_double),
+                ("zernike", C.c_double * 22)]
+
+
+def make_element(element, cls=Element, **p):
+    """An element descriptor with the reference constructors' defaults
+    (optics/elements.go): grating kind by name, zone-plate kind "phase" or
+    "amplitude", spiral charge, Zernike coefficients c1..c21 in waves."""
+    e = cls()
+    e.kind = ELEMENT_KINDS.index(element)
+    e.f = p.get("f", {"cylindrical_lens": 0.1, "zone_plate": 0.1}.get(element, 0.0))
+    e.aperture, e.x0, e.y0 = p.get("aperture", 0.0), p.get("x", 0.0), p.get("y", 0.0)
+    e.rotation = p.get("rotation", 0.0)
+    e.waist, e.amplitude = p.get("waist", 0.001), p.get("amplitude", 1.0)
+    e.alpha = p.get("alpha", 0.02 if element == "axicon" else 0.01)
+    e.index = p.get("index", 1.5)
+    e.period, e.modulation, e.duty = p.get("period", 1e-4), p.get("modulation", 1.0), p.get("duty", 0.5)
+    e.blaze_depth, e.offset = p.get("blaze_depth", math.pi), p.get("offset", 0.0)
+    e.radius = p.get("radius", {"zone_plate": 0.01, "zernike": 0.01}.get(element, 0.5))
+    e.reflectivity, e.curvature = p.get("reflectivity", 1.0), p.get("curvature", 0.0)
+    e.tilt_x, e.tilt_y = p.get("tilt_x", 0.0), p.get("tilt_y", 0.0)
+    if element == "grating":
+        e.sub = GRATING_KINDS.index(p.get("kind", "amplitude_sin"))
+    elif element == "zone_plate":
+        e.sub = ("phase", "amplitude").index(p.get("kind", "phase"))
+    elif element == "spiral_phase":
+        e.sub = int(p.get("charge", 1))
+    for j in range(1, 22):
+        e.zernike[j] = p.get(f"c{j}", 0.0)
+    return e
+
+
 class Shape(C.Structure):
     """or_shape: a parsed aperture outline (optics/shape.go:28-48)."""
     _fields_ = [("kind", C.c_int32), ("n_verts", C.c_int32)] + \
@@ -125,6 +172,11 @@ class OField:
 
     def aperture(self, g, x0=0.0, y0=0.0):
         lib().or_apply_aperture(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, C.byref(g), x0, y0)
+
+    def element(self, e, wl):
+        rc = lib().or_apply_element(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, wl, C.byref(e))
+        if rc:
+            raise ValueError("element parameters rejected")
 
     def diffuser(self, sigma=np.pi, corr=2e-5, amp=1.0, seed=1, x0=0.0, y0=0.0):
         lib().or_apply_diffuser(_p(self.ex), _p(self.ey), int(self.polarized), self.n, self.dx, sigma, corr, amp,
@@ -208,6 +260,8 @@ def lib():
         L.or_schell_realization.argtypes = [P, P, I, D, D, C.c_uint64, C.c_uint64]
         L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
         L.or_apply_lens_at.argtypes = [P, P, I, I, D, D, D, D, D, D]
+        L.or_apply_element.argtypes = [P, P, I, I, D, D, C.POINTER(Element)]
+        L.or_apply_element.restype = I
         L.or_propagate_vectorial.argtypes = [P, P, P, I, D, D, C.POINTER(Ctx)]
         L.or_propagate_uniaxial.argtypes = [P, P, I, D, D, D, D, D]
         L.or_plane_view.argtypes = [P, P, I, I, D, P]

@@ -1743,3 +1743,179 @@ double or_bpm_mask_loss_grad(const double* theta, const double* phi, const or_cp
     return L;
 }
 
+/* ------------------------------------------------------------------ */
+/* More train elements (optics/elements.go), each in the reference's     */
+/* operation order; the field multiplies by t with Go's complex product. */
+/* ------------------------------------------------------------------ */
+
+/* nollNM, radialZernike, factorial (optics/elements.go:918-962) */
+static const int or_noll[22][2] = {{0, 0}, {0, 0}, {1, 1}, {1, -1}, {2, 0}, {2, -2}, {2, 2}, {3, -1}, {3, 1},
+                                   {3, -3}, {3, 3}, {4, 0}, {4, 2}, {4, -2}, {4, 4}, {4, -4}, {5, 1}, {5, -1},
+                                   {5, 3}, {5, -3}, {5, 5}, {5, -5}};
+static double or_fact(int n) {
+    double f = 1;
+    for (int i = 2; i <= n; i++) f *= (double)i;
+    return f;
+}
+static double or_radial_zernike(int n, int m, double rho) {
+    double s = 0;
+    for (int k = 0; k <= (n - m) / 2; k++) {
+        double num = or_fact(n - k);
+        double den = or_fact(k) * or_fact((n + m) / 2 - k) * or_fact((n - m) / 2 - k);
+        s += pow(-1, (double)k) * num / den * pow(rho, (double)(n - 2 * k));
+    }
+    return s;
+}
+
+int or_apply_element(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double wl, const or_element* e) {
+    const double k = 2 * M_PI / wl;
+    double c = 0, d = 0, cr = 1, sr = 0, thr = 0;
+    switch (e->kind) {
+        case OR_EL_CYLINDRICAL_LENS: /* optics/elements.go:234-270 */
+            if (e->f == 0) return 2;
+            c = -k / (2 * e->f);
+            cr = cos(e->rotation);
+            sr = sin(e->rotation);
+            break;
+        case OR_EL_APODIZER: /* optics/elements.go:431-456 */
+            if (e->waist <= 0) return 2;
+            break;
+        case OR_EL_GRATING: /* optics/elements.go:464-531 */
+            if (e->period <= 0 || e->duty < 0 || e->duty > 1 || e->sub < 0 || e->sub > 4) return 2;
+            cr = cos(e->rotation);
+            sr = sin(e->rotation);
+            thr = cos(M_PI * e->duty);
+            break;
+        case OR_EL_AXICON: /* optics/elements.go:536-556 */
+        case OR_EL_WEDGE:  /* optics/elements.go:600-622 */
+            c = -k * (e->index - 1) * tan(e->alpha);
+            cr = cos(e->rotation);
+            sr = sin(e->rotation);
+            break;
+        case OR_EL_SPIRAL_PHASE: /* optics/elements.go:566-595 */
+            if (e->f != 0) c = -k / (2 * e->f);
+            break;
+        case OR_EL_ZONE_PLATE: /* optics/elements.go:631-681 */
+            if (e->f <= 0 || (e->sub != 0 && e->sub != 1)) return 2;
+            break;
+        case OR_EL_MIRROR: /* optics/elements.go:792-828 */
+            if (e->reflectivity < 0 || e->reflectivity > 1) return 2;
+            c = -k * e->curvature;
+            d = -2 * k;
+            break;
+        case OR_EL_CONCAVE_MIRROR: /* optics/elements.go:839-899 */
+        case OR_EL_CONVEX_MIRROR:
+            if (e->radius <= 0 || e->reflectivity < 0 || e->reflectivity > 1) return 2;
+            c = e->kind == OR_EL_CONVEX_MIRROR ? k / e->radius : -k / e->radius;
+            break;
+        case OR_EL_ZERNIKE: /* optics/elements.go:907-1008 */
+            if (e->radius <= 0) return 2;
+            break;
+        default:
+            return 2;
+    }
+    for (int j = 0; j < n; j++) {
+        const double y = coord(j, n, dx);
+        for (int i = 0; i < n; i++) {
+            const double x = coord(i, n, dx);
+            const size_t idx = (size_t)j * n + i;
+            or_cplx t = {1, 0};
+            int zero = 0, skip = 0;
+            switch (e->kind) {
+                case OR_EL_CYLINDRICAL_LENS: {
+                    const double ddy = y - e->y0, ddx = x - e->x0;
+                    if (e->aperture > 0 && ddx * ddx + ddy * ddy > e->aperture * e->aperture) {
+                        zero = 1;
+                        break;
+                    }
+                    const double u = cr * ddx + sr * ddy;
+                    t = cexpI(c * u * u);
+                    break;
+                }
+                case OR_EL_APODIZER: {
+                    const double ddy = y - e->y0, ddx = x - e->x0;
+                    t = cre(e->amplitude * exp(-(ddx * ddx + ddy * ddy) / (e->waist * e->waist)));
+                    break;
+                }
+                case OR_EL_GRATING: {
+                    const double uu = cr * x + sr * y;
+                    const double ph = 2 * M_PI * uu / e->period;
+                    switch (e->sub) {
+                        case 0: t = cre(0.5 * (1 + e->modulation * cos(ph + e->offset))); break;
+                        case 1: t = cre(cos(ph + e->offset) >= thr ? 1.0 : 0.0); break;
+                        case 2: t = cexpI(e->modulation / 2 * sin(ph + e->offset)); break;
+                        case 3: t = cos(ph + e->offset) >= 0 ? cexpI(e->modulation) : cre(1); break;
+                        default: {
+                            double frac = fmod(uu / e->period, 1);
+                            if (frac < 0) frac += 1;
+                            t = cexpI(e->blaze_depth * frac);
+                        }
+                    }
+                    break;
+                }
+                case OR_EL_AXICON: t = cexpI(c * or_hypot(x, y)); break;
+                case OR_EL_SPIRAL_PHASE: {
+                    const double ddy = y - e->y0, ddx = x - e->x0;
+                    double ph = (double)e->sub * atan2(ddy, ddx);
+                    if (c != 0) ph += c * (ddx * ddx + ddy * ddy);
+                    t = cexpI(ph);
+                    break;
+                }
+                case OR_EL_WEDGE: t = cexpI(c * (cr * x + sr * y)); break;
+                case OR_EL_ZONE_PLATE: {
+                    const double r2 = x * x + y * y;
+                    if (r2 > e->radius * e->radius) {
+                        skip = 1;
+                        break;
+                    }
+                    const double r = sqrt(r2);
+                    const int m = (int)floor(2 * (sqrt(e->f * e->f + r * r) - e->f) / wl + 0.5);
+                    if (m % 2 != 0) t = e->sub == 0 ? cexpI(M_PI) : cre(0);
+                    break;
+                }
+                case OR_EL_MIRROR:
+                    t = cmul(cre(e->reflectivity), cexpI(c * (x * x + y * y) + d * (e->tilt_x * x + e->tilt_y * y)));
+                    break;
+                case OR_EL_CONCAVE_MIRROR:
+                case OR_EL_CONVEX_MIRROR: {
+                    const double ddy = y - e->y0, ddx = x - e->x0;
+                    const double r2 = ddx * ddx + ddy * ddy;
+                    if (e->aperture > 0 && r2 > e->aperture * e->aperture) {
+                        zero = 1;
+                        break;
+                    }
+                    t = cmul(cre(e->reflectivity), cexpI(c * r2));
+                    break;
+                }
+                default: { /* Zernike */
+                    const double rho = or_hypot(x, y) / e->radius;
+                    if (rho > 1) {
+                        skip = 1;
+                        break;
+                    }
+                    const double theta = atan2(y, x);
+                    double opd = 0;
+                    for (int z = 1; z <= 21; z++) {
+                        const double cz = e->zernike[z];
+                        if (cz == 0) continue;
+                        const int nn = or_noll[z][0], mm = or_noll[z][1];
+                        const double rad = or_radial_zernike(nn, mm < 0 ? -mm : mm, rho);
+                        const double ang = mm >= 0 ? cos((double)mm * theta) : sin((double)(-mm) * theta);
+                        if (mm == 0) opd += cz * sqrt((double)(nn + 1)) * rad;
+                        else opd += cz * sqrt(2 * (double)(nn + 1)) * rad * ang;
+                    }
+                    t = cexpI(2 * M_PI * opd);
+                }
+            }
+            if (skip) continue;
+            if (zero) {
+                ex[idx] = cre(0);
+                if (polarized) ey[idx] = cre(0);
+                continue;
+            }
+            ex[idx] = cmul(ex[idx], t);
+            if (polarized) ey[idx] = cmul(ey[idx], t);
+        }
+    }
+    return 0;
+}

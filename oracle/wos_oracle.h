@@ -187,6 +187,35 @@ double or_bpm_mask_loss_grad(const double* theta, const double* phi, const or_cp
                              const double* target, int n, double dx, double z, int steps, const or_ctx* ctx,
                              double* grad_theta, double* grad_phi);
 
+/* More train elements (optics/elements.go:156-178): the parsed parameters of
+ * one element; kind OR_EL_*, sub = grating kind (0 amplitude_sin,
+ * 1 amplitude_binary, 2 phase_sin, 3 phase_binary, 4 blazed), zone-plate
+ * kind (0 phase, 1 amplitude) or spiral charge. */
+#define OR_EL_CYLINDRICAL_LENS 0
+#define OR_EL_APODIZER 1
+#define OR_EL_GRATING 2
+#define OR_EL_AXICON 3
+#define OR_EL_SPIRAL_PHASE 4
+#define OR_EL_WEDGE 5
+#define OR_EL_ZONE_PLATE 6
+#define OR_EL_MIRROR 7
+#define OR_EL_CONCAVE_MIRROR 8
+#define OR_EL_CONVEX_MIRROR 9
+#define OR_EL_ZERNIKE 10
+typedef struct {
+    int32_t kind, sub;
+    double f, aperture, x0, y0, rotation;
+    double waist, amplitude;
+    double alpha, index;
+    double period, modulation, duty, blaze_depth, offset;
+    double radius;
+    double reflectivity, curvature, tilt_x, tilt_y;
+    double zernike[22];
+} or_element;
+/* Apply one element at wavelength wl; returns 0, or 2 for parameters the
+ * reference's constructor rejects. */
+int or_apply_element(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, double wl, const or_element* e);
+
 #ifdef __cplusplus
 }
 #endif

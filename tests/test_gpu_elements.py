@@ -323,3 +323,44 @@ def test_anisotropic_matches_oracle(wos, oracle, polarized, n, eps):
     gx, gy = f.download()
     assert rel_l2(np.concatenate([gx, gy]), np.concatenate([o.ex, o.ey])) <= FP64_TOL
     f.close()
+
+
+ELEMENTS = [
+    ("cylindrical_lens", dict(f=0.03, aperture=100e-6, x=5e-6, y=-3e-6, rotation=0.3)),
+    ("apodizer", dict(waist=80e-6, amplitude=0.8, x=-6e-6)),
+    ("grating", dict(kind="amplitude_sin", period=37.3e-6, modulation=0.7, rotation=0.4, offset=0.2)),
+    ("grating", dict(kind="amplitude_binary", period=37.3e-6, duty=0.37, rotation=0.4, offset=0.2)),
+    ("grating", dict(kind="phase_sin", period=37.3e-6, modulation=1.3, rotation=0.4)),
+    ("grating", dict(kind="phase_binary", period=41.1e-6, modulation=2.2, offset=0.1)),
+    ("grating", dict(kind="blazed", period=29.9e-6, blaze_depth=2.1, rotation=-0.7)),
+    ("axicon", dict(alpha=0.01, index=1.45)),
+    ("spiral_phase", dict(charge=3, f=0.05, x=3e-6)),
+    ("wedge", dict(alpha=0.005, rotation=1.1)),
+    ("zone_plate", dict(f=2.13e-3, radius=110e-6, kind="phase")),
+    ("zone_plate", dict(f=2.13e-3, radius=110e-6, kind="amplitude")),
+    ("mirror", dict(reflectivity=0.9, curvature=3.3, tilt_x=2e-4, tilt_y=-1e-4)),
+    ("concave_mirror", dict(radius=0.07, aperture=110e-6, x=4e-6)),
+    ("convex_mirror", dict(radius=0.09, reflectivity=0.8)),
+    ("zernike", dict(radius=100e-6, c4=0.3, c7=-0.2, c12=0.1, c21=0.05)),
+]
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+@pytest.mark.parametrize("name,params", ELEMENTS, ids=[f"{n}-{p.get('kind', '')}" for n, p in ELEMENTS])
+def test_further_elements_match_oracle(wos, oracle, name, params, polarized):
+    """The further train elements (wos_hip_field_element) on a resident field
+    against the oracle's restatements: FP64 and FP32 (the per-pixel
+    transmission is formed in FP64 in both, so threshold elements open the
+    same pixels)."""
+    e = compare(wos, oracle, 128, 2e-6, polarized, lambda f, m: f.element(m.make_element(name, **params), 633e-9))
+    print(f"{name} {params.get('kind', '')} polarized={polarized}: FP64 {e[wos.FP64]:.2e} FP32 {e[wos.FP32]:.2e}",
+          flush=True)
+
+
+def test_further_elements_reject_bad_parameters(wos):
+    f = wos.Field.from_host(np.ones((16, 16), dtype=complex), dx=1e-6)
+    for name, p in (("apodizer", dict(waist=0)), ("grating", dict(period=-1)), ("zone_plate", dict(f=0)),
+                    ("mirror", dict(reflectivity=2)), ("zernike", dict(radius=0)), ("cylindrical_lens", dict(f=0))):
+        with pytest.raises(RuntimeError):
+            f.element(wos.make_element(name, **p), 633e-9)
+    f.close()

@@ -384,3 +384,137 @@ def test_berreman_reductions(oracle):
                                           np.diag([nx ** 2, ny ** 2, nz ** 2]), wl)
     assert np.max(np.abs(bx - np.exp(1j * k0 * nx * z))) < 1e-12
     assert np.max(np.abs(by - 2 * np.exp(1j * k0 * ny * z))) < 1e-12
+
+
+# ---------------------------------------------------------------- further elements
+
+def _ones(oracle, n=96, dx=2e-6):
+    return oracle.OField(np.ones((n, n), dtype=complex), dx=dx)
+
+
+def test_element_relations_pin_the_restatement(oracle):
+    """Relations the optics fixes: crossed cylindrical lenses (rotation 0 and
+    pi/2) make the spherical lens; a mirror of curvature k is a lens of focal
+    length 1/(2k); a concave mirror of radius R is a lens of f = R/2 and the
+    convex one its conjugate; reflectivity scales the amplitude."""
+    wl, n, dx = 633e-9, 96, 2e-6
+    lens = _ones(oracle, n, dx)
+    lens.lens(wl, 0.04)
+    cyl = _ones(oracle, n, dx)
+    cyl.element(oracle.make_element("cylindrical_lens", f=0.04), wl)
+    cyl.element(oracle.make_element("cylindrical_lens", f=0.04, rotation=math.pi / 2), wl)
+    assert np.max(np.abs(cyl.ex - lens.ex)) <= 1e-12
+    mir = _ones(oracle, n, dx)
+    mir.element(oracle.make_element("mirror", curvature=1 / (2 * 0.04), reflectivity=0.8), wl)
+    assert np.max(np.abs(mir.ex - 0.8 * lens.ex)) <= 1e-12
+    cc = _ones(oracle, n, dx)
+    cc.element(oracle.make_element("concave_mirror", radius=0.08), wl)
+    assert np.max(np.abs(cc.ex - lens.ex)) <= 1e-12
+    cv = _ones(oracle, n, dx)
+    cv.element(oracle.make_element("convex_mirror", radius=0.08, reflectivity=0.5), wl)
+    assert np.max(np.abs(cv.ex - 0.5 * np.conj(lens.ex))) <= 1e-12
+
+
+def test_element_closed_forms(oracle):
+    """Tilted mirror: exp(-2ik(tx x + ty y)); wedge and axicon: linear phase
+    -k (n-1) tan(alpha) along the wedge axis / the radius; spiral phase plate:
+    charge times the azimuth; apodizer: amp exp(-r^2/w^2)."""
+    wl, n, dx = 633e-9, 96, 2e-6
+    k = 2 * math.pi / wl
+    x = (np.arange(n) - n / 2) * dx
+    X, Y = x[None, :], x[:, None]
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("mirror", tilt_x=3e-4, tilt_y=-2e-4), wl)
+    assert np.max(np.abs(f.ex - np.exp(-2j * k * (3e-4 * X - 2e-4 * Y)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("wedge", alpha=0.004, index=1.45, rotation=0.7), wl)
+    c = -k * 0.45 * math.tan(0.004)
+    assert np.max(np.abs(f.ex - np.exp(1j * c * (math.cos(0.7) * X + math.sin(0.7) * Y)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("axicon", alpha=0.01, index=1.5), wl)
+    c = -k * 0.5 * math.tan(0.01)
+    assert np.max(np.abs(f.ex - np.exp(1j * c * np.hypot(X, Y)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("spiral_phase", charge=3, x=4e-6), wl)
+    assert np.max(np.abs(f.ex - np.exp(3j * np.arctan2(Y + 0 * X, X - 4e-6)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("apodizer", waist=50e-6, amplitude=0.7), wl)
+    assert np.max(np.abs(f.ex - 0.7 * np.exp(-(X ** 2 + Y ** 2) / 50e-6 ** 2))) <= 1e-15
+
+
+def test_grating_kinds(oracle):
+    """amplitude_sin: 0.5 (1 + m cos(2 pi u / period + offset)); phase_sin:
+    exp(i m/2 sin(...)); blazed: exp(i depth frac(u / period)); the binary
+    kinds take two values, open on a fraction of the period set by the duty
+    cycle (amplitude) or by the sign of the cosine (phase)."""
+    wl, n, dx = 633e-9, 128, 1e-6
+    x = (np.arange(n) - n / 2) * dx
+    X, Y = x[None, :], x[:, None]
+    per, rot, off = 23.7e-6, 0.35, 0.2
+    u = math.cos(rot) * X + math.sin(rot) * Y
+    ph = 2 * math.pi * u / per + off
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("grating", kind="amplitude_sin", period=per, modulation=0.6, rotation=rot,
+                                  offset=off), wl)
+    assert np.max(np.abs(f.ex - 0.5 * (1 + 0.6 * np.cos(ph)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("grating", kind="phase_sin", period=per, modulation=1.3, rotation=rot,
+                                  offset=off), wl)
+    assert np.max(np.abs(f.ex - np.exp(1j * 0.65 * np.sin(ph)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("grating", kind="blazed", period=per, blaze_depth=2.1, rotation=rot), wl)
+    assert np.max(np.abs(f.ex - np.exp(1j * 2.1 * np.mod(u / per, 1.0)))) <= 1e-12
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("grating", kind="amplitude_binary", period=per, duty=0.3, rotation=rot), wl)
+    assert set(np.unique(f.ex.real)) <= {0.0, 1.0} and abs(np.mean(f.ex.real) - 0.3) < 0.02
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("grating", kind="phase_binary", period=per, modulation=0.9, rotation=rot), wl)
+    vals = np.unique(np.round(f.ex, 12))
+    assert len(vals) == 2 and abs(np.mean(np.abs(f.ex - 1) < 1e-12) - 0.5) < 0.03
+
+
+def test_zone_plate_zone_radii(oracle):
+    """Zones of the exact point-to-point zone plate, r_m^2 = m wl f + (m wl/2)^2:
+    the reference rounds the zone index m = floor(2 (sqrt(f^2 + r^2) - f)/wl
+    + 1/2) (optics/elements.go:662), so zone m spans the radii of m - 1/2 to
+    m + 1/2 in that formula. The amplitude plate is clear in even zones and
+    opaque in odd ones, the phase plate shifts odd zones by pi; nothing
+    changes outside `radius`."""
+    wl, n, dx, fz, R = 633e-9, 128, 1e-6, 2.13e-3, 55e-6
+    x = (np.arange(n) - n / 2) * dx
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    edges = np.array([(m + 0.5) * wl * fz + ((m + 0.5) * wl / 2) ** 2 for m in range(12)])
+    zone = np.searchsorted(edges, r2, side="right")
+    near = np.min(np.abs(r2[..., None] - edges[None, None, :]), axis=-1) < 1e-12   # boundary pixels left out
+    inside = (r2 <= R * R) & ~near
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("zone_plate", f=fz, radius=R, kind="amplitude"), wl)
+    assert np.array_equal(f.ex.real[inside], (zone[inside] % 2 == 0).astype(float))
+    assert np.all(f.ex[r2 > R * R] == 1)
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("zone_plate", f=fz, radius=R, kind="phase"), wl)
+    assert np.max(np.abs(f.ex[inside] - np.where(zone[inside] % 2 == 0, 1, -1))) <= 1e-15
+
+
+def test_zernike_defocus_and_coma(oracle):
+    """Noll Z4 = sqrt(3) (2 rho^2 - 1) and Z7 = sqrt(8) (3 rho^3 - 2 rho) sin(theta)
+    (coefficients in waves); outside the normalisation radius nothing changes."""
+    wl, n, dx, R = 633e-9, 96, 2e-6, 70e-6
+    x = (np.arange(n) - n / 2) * dx
+    X, Y = x[None, :], x[:, None]
+    rho, th = np.hypot(X, Y) / R, np.arctan2(Y, X)
+    f = _ones(oracle, n, dx)
+    f.element(oracle.make_element("zernike", radius=R, c4=0.3, c7=-0.2), wl)
+    opd = 0.3 * math.sqrt(3) * (2 * rho ** 2 - 1) - 0.2 * math.sqrt(8) * (3 * rho ** 3 - 2 * rho) * np.sin(th)
+    inside = rho <= 1
+    assert np.max(np.abs(f.ex[inside] - np.exp(2j * math.pi * opd[inside]))) <= 1e-12
+    assert np.all(f.ex[~inside] == 1)
+
+
+def test_element_rejections(oracle):
+    f = _ones(oracle, 16)
+    for kind, p in (("apodizer", dict(waist=0)), ("grating", dict(period=0)), ("grating", dict(duty=1.5)),
+                    ("zone_plate", dict(f=-1)), ("mirror", dict(reflectivity=1.5)), ("concave_mirror", dict(radius=0)),
+                    ("zernike", dict(radius=0)), ("cylindrical_lens", dict(f=0))):
+        with pytest.raises(ValueError):
+            f.element(oracle.make_element(kind, **p), 633e-9)

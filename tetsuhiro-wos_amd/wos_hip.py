@@ -41,7 +41,7 @@ EXPORTS = [
     "wos_hip_bpm_mask_create", "wos_hip_bpm_mask_destroy", "wos_hip_bpm_mask_forward", "wos_hip_bpm_mask_backward",
     "wos_hip_field_create", "wos_hip_field_destroy", "wos_hip_field_clone", "wos_hip_field_upload",
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
-    "wos_hip_field_propagate_bandlimited",
+    "wos_hip_field_propagate_bandlimited", "wos_hip_field_element",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
@@ -53,6 +53,51 @@ VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity"
 
 SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4, "superellipse": 5, "ring": 6,
                "double_slit": 7, "cross": 8, "polygon": 9}
+
+
+ELEMENT_KINDS = ("cylindrical_lens", "apodizer", "grating", "axicon", "spiral_phase", "wedge", "zone_plate",
+                 "mirror", "concave_mirror", "convex_mirror", "zernike")
+GRATING_KINDS = ("amplitude_sin", "amplitude_binary", "phase_sin", "phase_binary", "blazed")
+
+
+class Element(C.Structure):
+    """wos_element: the parameters of one further train element."""
+    _fields_ = [("kind", C.c_int32), ("sub", C.c_int32),
+                ("f", C.c_double), ("aperture", C.c_double), ("x0", C.c_double), ("y0", C.c_double),
+                ("rotation", C.c_double), ("waist", C.c_double), ("amplitude", C.c_double),
+                ("alpha", C.c_double), ("index", C.c_double), ("period", C.c_double),
+                ("modulation", C.c_double), ("duty", C.c_double), ("blaze_depth", C.c_double),
+                ("offset", C.c_double), ("radius", C.c_double), ("reflectivity", C.c_double),
+                ("curvature", C.c_double), ("tilt_x", C.c_double), ("tilt_y", C.c_double),
+                ("zernike", C.c_double * 22)]
+
+
+def make_element(element, **p):
+    """A wos_element with the reference constructors' defaults
+    (optics/elements.go): grating kind by name, zone-plate kind "phase" or
+    "amplitude", spiral charge, Zernike coefficients c1..c21 in waves."""
+    e = Element()
+    e.kind = ELEMENT_KINDS.index(element)
+    e.f = p.get("f", {"cylindrical_lens": 0.1, "zone_plate": 0.1}.get(element, 0.0))
+    e.aperture, e.x0, e.y0 = p.get("aperture", 0.0), p.get("x", 0.0), p.get("y", 0.0)
+    e.rotation = p.get("rotation", 0.0)
+    e.waist, e.amplitude = p.get("waist", 0.001), p.get("amplitude", 1.0)
+    e.alpha = p.get("alpha", 0.02 if element == "axicon" else 0.01)
+    e.index = p.get("index", 1.5)
+    e.period, e.modulation, e.duty = p.get("period", 1e-4), p.get("modulation", 1.0), p.get("duty", 0.5)
+    e.blaze_depth, e.offset = p.get("blaze_depth", np.pi), p.get("offset", 0.0)
+    e.radius = p.get("radius", {"zone_plate": 0.01, "zernike": 0.01}.get(element, 0.5))
+    e.reflectivity, e.curvature = p.get("reflectivity", 1.0), p.get("curvature", 0.0)
+    e.tilt_x, e.tilt_y = p.get("tilt_x", 0.0), p.get("tilt_y", 0.0)
+    if element == "grating":
+        e.sub = GRATING_KINDS.index(p.get("kind", "amplitude_sin"))
+    elif element == "zone_plate":
+        e.sub = ("phase", "amplitude").index(p.get("kind", "phase"))
+    elif element == "spiral_phase":
+        e.sub = int(p.get("charge", 1))
+    for j in range(1, 22):
+        e.zernike[j] = p.get(f"c{j}", 0.0)
+    return e
 
 
 class Shape(C.Structure):
@@ -204,6 +249,7 @@ def load(path=None):
     L.wos_hip_field_propagate.argtypes = [P, D, I, C.POINTER(Ctx), C.POINTER(Warnings), C.POINTER(PropStats)]
     L.wos_hip_field_propagate_bandlimited.argtypes = [P, D, I, C.POINTER(Ctx), D, D, C.POINTER(Warnings),
                                                       C.POINTER(PropStats)]
+    L.wos_hip_field_element.argtypes = [P, C.POINTER(Element), D]
     L.wos_hip_field_lens.argtypes = [P, D, D, D, D, D]
     L.wos_hip_field_aperture.argtypes = [P, C.POINTER(Shape), D, D]
     L.wos_hip_field_diffuser.argtypes = [P, D, D, D, C.c_int64, D, D]
@@ -364,6 +410,10 @@ class Field:
 
     def aperture(self, g, x0=0.0, y0=0.0):
         _check(load().wos_hip_field_aperture(self.h, C.byref(g), x0, y0), "field_aperture")
+
+    def element(self, e, wl):
+        """One of the further train elements (make_element)."""
+        _check(load().wos_hip_field_element(self.h, C.byref(e), wl), "field_element")
 
     def diffuser(self, sigma=np.pi, corr=2e-5, amp=1.0, seed=1, x0=0.0, y0=0.0):
         _check(load().wos_hip_field_diffuser(self.h, sigma, corr, amp, seed, x0, y0), "field_diffuser")
