@@ -369,7 +369,9 @@ int wos_hip_field_jones(wos_field f, const double* abcd);
  * reference's per-pixel order: cylindrical_lens (elements.go:230-270),
  * apodizer (427-456), grating (458-531), axicon (534-556), spiral_phase
  * (560-595), wedge (598-622), zone_plate (625-681), mirror (787-828),
- * concave/convex mirror (831-899), zernike (902-1008). `sub` is the grating
+ * concave/convex mirror (831-899), zernike (902-1008), kerr (1168-1205) and
+ * saturable_absorber (1211-1242), both on I = |Ex|^2 (+ |Ey|^2 when
+ * polarized). `sub` is the grating
  * kind (0 amplitude_sin, 1 amplitude_binary, 2 phase_sin, 3 phase_binary,
  * 4 blazed), the zone-plate kind (0 phase, 1 amplitude) or the spiral charge;
  * the other fields carry the constructors' parameters (zernike: Noll
@@ -386,6 +388,8 @@ int wos_hip_field_jones(wos_field f, const double* abcd);
 #define WOS_EL_CONCAVE_MIRROR 8
 #define WOS_EL_CONVEX_MIRROR 9
 #define WOS_EL_ZERNIKE 10
+#define WOS_EL_KERR 11
+#define WOS_EL_SATURABLE_ABSORBER 12
 typedef struct {
   int32_t kind, sub;
   double f, aperture, x0, y0, rotation;
@@ -395,6 +399,7 @@ typedef struct {
   double radius;
   double reflectivity, curvature, tilt_x, tilt_y;
   double zernike[22];
+  double n2, tpa, length, alpha0, isat; /* kerr, saturable_absorber */
 } wos_element;
 int wos_hip_field_element(wos_field f, const wos_element* element, double wavelength);
 /* Field.ScaleAmplitude (optics/field.go:134-143), the beamsplitter's arms */

@@ -1811,8 +1811,33 @@ int or_apply_element(or_cplx* ex, or_cplx* ey, int polarized, int n, double dx, 
         case OR_EL_ZERNIKE: /* optics/elements.go:907-1008 */
             if (e->radius <= 0) return 2;
             break;
+        case OR_EL_KERR: /* optics/elements.go:1172-1205 */
+            if (e->length < 0) return 2;
+            break;
+        case OR_EL_SATURABLE_ABSORBER: /* optics/elements.go:1215-1242 */
+            if (e->alpha0 < 0 || e->length < 0 || e->isat <= 0) return 2;
+            break;
         default:
             return 2;
+    }
+    if (e->kind == OR_EL_KERR || e->kind == OR_EL_SATURABLE_ABSORBER) {
+        const size_t nn = (size_t)n * n;
+        for (size_t idx = 0; idx < nn; idx++) {
+            /* Field.Intensity (optics/field.go:65-74) */
+            double I = ex[idx].re * ex[idx].re + ex[idx].im * ex[idx].im;
+            if (polarized) I += ey[idx].re * ey[idx].re + ey[idx].im * ey[idx].im;
+            or_cplx t;
+            if (e->kind == OR_EL_KERR) {
+                if (I == 0) continue;
+                t = cexpI(k * e->n2 * I * e->length);
+                if (e->tpa != 0) t = cmul(t, cre(exp(-0.5 * e->tpa * I * e->length)));
+            } else {
+                t = cre(exp(-e->alpha0 * e->length / (2 * (1 + I / e->isat))));
+            }
+            ex[idx] = cmul(ex[idx], t);
+            if (polarized) ey[idx] = cmul(ey[idx], t);
+        }
+        return 0;
     }
     for (int j = 0; j < n; j++) {
         const double y = coord(j, n, dx);

@@ -202,6 +202,8 @@ double or_bpm_mask_loss_grad(const double* theta, const double* phi, const or_cp
 #define OR_EL_CONCAVE_MIRROR 8
 #define OR_EL_CONVEX_MIRROR 9
 #define OR_EL_ZERNIKE 10
+#define OR_EL_KERR 11
+#define OR_EL_SATURABLE_ABSORBER 12
 typedef struct {
     int32_t kind, sub;
     double f, aperture, x0, y0, rotation;
@@ -211,6 +213,7 @@ typedef struct {
     double radius;
     double reflectivity, curvature, tilt_x, tilt_y;
     double zernike[22];
+    double n2, tpa, length, alpha0, isat;
 } or_element;
 /* Apply one element at wavelength wl; returns 0, or 2 for parameters the
  * reference's constructor rejects. */

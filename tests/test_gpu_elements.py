@@ -342,6 +342,9 @@ ELEMENTS = [
     ("concave_mirror", dict(radius=0.07, aperture=110e-6, x=4e-6)),
     ("convex_mirror", dict(radius=0.09, reflectivity=0.8)),
     ("zernike", dict(radius=100e-6, c4=0.3, c7=-0.2, c12=0.1, c21=0.05)),
+    ("kerr", dict(kind="tpa", n2=1e-7, tpa=300.0, length=2e-3)),
+    ("kerr", dict(n2=-2e-7, length=1e-3)),
+    ("saturable_absorber", dict(alpha0=500.0, isat=0.3, length=2e-3)),
 ]
 
 
@@ -360,7 +363,8 @@ def test_further_elements_match_oracle(wos, oracle, name, params, polarized):
 def test_further_elements_reject_bad_parameters(wos):
     f = wos.Field.from_host(np.ones((16, 16), dtype=complex), dx=1e-6)
     for name, p in (("apodizer", dict(waist=0)), ("grating", dict(period=-1)), ("zone_plate", dict(f=0)),
-                    ("mirror", dict(reflectivity=2)), ("zernike", dict(radius=0)), ("cylindrical_lens", dict(f=0))):
+                    ("mirror", dict(reflectivity=2)), ("zernike", dict(radius=0)), ("cylindrical_lens", dict(f=0)),
+                    ("kerr", dict(length=-1)), ("saturable_absorber", dict(isat=0))):
         with pytest.raises(RuntimeError):
             f.element(wos.make_element(name, **p), 633e-9)
     f.close()

@@ -518,3 +518,36 @@ def test_element_rejections(oracle):
                     ("zernike", dict(radius=0)), ("cylindrical_lens", dict(f=0))):
         with pytest.raises(ValueError):
             f.element(oracle.make_element(kind, **p), 633e-9)
+
+
+def test_kerr_and_saturable_absorber(oracle):
+    """Kerr without absorption only rotates the phase, by k n2 I L, and leaves
+    zero-intensity pixels alone; two-photon absorption scales the amplitude
+    by exp(-tpa I L / 2). The saturable absorber transmits exp(-alpha0 L / 2)
+    in the amplitude at I = 0 and tends to 1 far above I_sat; I sums both
+    components of a polarised field."""
+    wl, n, dx = 633e-9, 32, 1e-6
+    k = 2 * math.pi / wl
+    rng = np.random.default_rng(3)
+    ex = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    ex[0, :4] = 0
+    ey = 0.5 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    ey[0, :4] = 0
+    I = np.abs(ex) ** 2 + np.abs(ey) ** 2
+    f = oracle.OField(ex.copy(), ey.copy(), dx=dx)
+    f.element(oracle.make_element("kerr", n2=1e-7, length=2e-3), wl)
+    assert np.allclose(np.abs(f.ex), np.abs(ex), rtol=1e-14, atol=0)
+    assert np.max(np.abs(f.ex - ex * np.exp(1j * k * 1e-7 * I * 2e-3))) <= 1e-12 * np.max(np.abs(ex))
+    assert np.array_equal(f.ex[0, :4], ex[0, :4])
+    f = oracle.OField(ex.copy(), ey.copy(), dx=dx)
+    f.element(oracle.make_element("kerr", n2=0.0, tpa=300.0, length=2e-3), wl)
+    assert np.allclose(f.ey, ey * np.exp(-0.5 * 300.0 * I * 2e-3), rtol=1e-13, atol=0)
+    f = oracle.OField(ex.copy(), ey.copy(), dx=dx)
+    f.element(oracle.make_element("saturable_absorber", alpha0=500.0, isat=1e-12, length=2e-3), wl)
+    bright = I > 1e-6
+    assert np.allclose(f.ex[bright], ex[bright], rtol=1e-5)        # I >> I_sat: transparent
+    assert np.allclose(f.ex[0, :4], 0)
+    g = oracle.OField(np.zeros((n, n), dtype=complex) + 1e-9, dx=dx)
+    g.element(oracle.make_element("saturable_absorber", alpha0=500.0, isat=1e6, length=2e-3), wl)
+    assert np.allclose(g.ex, 1e-9 * math.exp(-500.0 * 2e-3 / 2), rtol=1e-12)   # I << I_sat
+

@@ -56,7 +56,7 @@ SHAPE_KINDS = {"circle": 0, "slit": 1, "square": 2, "rectangle": 3, "ellipse": 4
 
 
 ELEMENT_KINDS = ("cylindrical_lens", "apodizer", "grating", "axicon", "spiral_phase", "wedge", "zone_plate",
-                 "mirror", "concave_mirror", "convex_mirror", "zernike")
+                 "mirror", "concave_mirror", "convex_mirror", "zernike", "kerr", "saturable_absorber")
 GRATING_KINDS = ("amplitude_sin", "amplitude_binary", "phase_sin", "phase_binary", "blazed")
 
 
@@ -69,7 +69,8 @@ class Element(C.Structure):
                 ("modulation", C.c_double), ("duty", C.c_double), ("blaze_depth", C.c_double),
                 ("offset", C.c_double), ("radius", C.c_double), ("reflectivity", C.c_double),
                 ("curvature", C.c_double), ("tilt_x", C.c_double), ("tilt_y", C.c_double),
-                ("zernike", C.c_double * 22)]
+                ("zernike", C.c_double * 22), ("n2", C.c_double), ("tpa", C.c_double), ("length", C.c_double),
+                ("alpha0", C.c_double), ("isat", C.c_double)]
 
 
 def make_element(element, **p):
@@ -97,6 +98,8 @@ def make_element(element, **p):
         e.sub = int(p.get("charge", 1))
     for j in range(1, 22):
         e.zernike[j] = p.get(f"c{j}", 0.0)
+    e.n2, e.tpa, e.length = p.get("n2", 1e-14), p.get("tpa", 0.0), p.get("length", 1e-3)
+    e.alpha0, e.isat = p.get("alpha0", 1.0), p.get("isat", 1e6)
     return e
 
 
