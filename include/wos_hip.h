@@ -402,6 +402,11 @@ typedef struct {
   double n2, tpa, length, alpha0, isat; /* kerr, saturable_absorber */
 } wos_element;
 int wos_hip_field_element(wos_field f, const wos_element* element, double wavelength);
+/* mediumEl.Apply (optics/elements.go:1105-1126): PropagateSplitStep through a
+ * uniform index `index` + i `absorption` over `distance` in `steps` slices on
+ * the resident field; warnings as wos_hip_split_step */
+int wos_hip_field_medium(wos_field f, double distance, double index, double absorption, int steps,
+                         const wos_ctx* ctx, wos_warnings* w);
 /* Field.ScaleAmplitude (optics/field.go:134-143), the beamsplitter's arms */
 int wos_hip_field_scale(wos_field f, double re, double im);
 /* Field.ApplyBandlimit (optics/field.go:264-294); fraction outside (0, 1): no-op */

@@ -368,3 +368,25 @@ def test_further_elements_reject_bad_parameters(wos):
         with pytest.raises(RuntimeError):
             f.element(wos.make_element(name, **p), 633e-9)
     f.close()
+
+
+@pytest.mark.parametrize("n,polarized", [(128, True), (100, False)])
+def test_medium_element_matches_oracle(wos, oracle, n, polarized):
+    """The medium element (PropagateSplitStep through a uniform index n + i k)
+    on a resident field against the oracle's split-step BPM: the merged device
+    pipeline on a power-of-two grid, the half steps one by one on 100."""
+    dx, wl, dist, steps = 1e-6, 633e-9, 60e-6, 6
+    ex, ey = field(n, 41, polarized)
+    ref_x, ref_y, wo = oracle.split_step(ex, ey, dx, dist, steps, np.full((n, n), 1.45 + 2e-4j), oracle.ctx_for(wl))
+    ref = np.concatenate([ref_x, ref_y]) if polarized else ref_x
+    for prec, tol in ((wos.FP64, FP64_TOL), (wos.FP32, FP32_TOL)):
+        f = wos.Field.from_host(ex, ey, dx=dx, precision=prec)
+        wg = f.medium(dist, 1.45, 2e-4, steps, wos.ctx_for(wl))
+        got = np.concatenate(f.download()) if polarized else f.download()[0]
+        e = rel_l2(got, ref)
+        print(f"medium n={n} polarized={polarized} prec={prec}: rel-L2 {e:.2e}", flush=True)
+        assert e <= tol
+        if prec == wos.FP64:
+            assert wg.as_dict().keys() == wo.as_dict().keys()
+        f.close()
+

@@ -1986,6 +1986,83 @@ static int download_fields(int prec, const void* d, double* const* dst, int ncom
   return 0;
 }
 
+__global__ void k_fill_c128(cpx<double>* __restrict__ a, size_t count, double re, double im) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = {re, im};
+}
+
+// Screen tables t(x, y) of PropagateSplitStep's slices (optics/medium.go:
+// 47-65) in dv.screens: n_planes index planes from host complex128 arrays,
+// or one uniform index nre + i nim when index is NULL; FP64 tables, then a
+// rounded FP32 copy for FP32. *tables gets the precision's copy.
+static int split_step_screens(int prec, int n, int n_planes, double k, double dz, const double* index, double nre,
+                              double nim, const void** tables) {
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  const size_t tb = (size_t)n_planes * nn * 16;
+  int rc;
+  if ((rc = dv.screens.ensure(tb + (prec == WOS_FP32 ? (size_t)n_planes * nn * 8 : 0)))) return rc;
+  if ((rc = dv.work.ensure(nn * 16))) return rc;
+  for (int p = 0; p < n_planes; ++p) {
+    if (index) {
+      if ((rc = host_to_dev(dv.work.p, index + (size_t)p * nn * 2, nn * 16, dv.stream))) return rc;
+    } else {
+      hipLaunchKernelGGL(k_fill_c128, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.work.p, nn, nre,
+                         nim);
+    }
+    hipLaunchKernelGGL(k_screen_table, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.work.p,
+                       (cpx<double>*)dv.screens.p + (size_t)p * nn, nn, k, dz);
+    WOS_HIP_TRY(hipGetLastError());
+    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  }
+  *tables = dv.screens.p;
+  if (prec == WOS_FP32) {
+    *tables = (char*)dv.screens.p + tb;
+    const size_t tot = (size_t)n_planes * nn;
+    hipLaunchKernelGGL(k_z2c, dim3(grid_for(tot)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.screens.p,
+                       (cpx<float>*)*tables, tot);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
+// PropagateSplitStep (optics/medium.go:22-45) on ncomp device components at
+// d with the screens of split_step_screens: the merged device pipeline on
+// power-of-two grids, else the half steps one by one.
+static int split_step_run(int prec, void* d, int n, int ncomp, double dx, double z, int steps, int n_planes,
+                          const void* tables, const wos_ctx& ctx, wos_warnings* w) {
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  const double dz = z / (double)steps;
+  int rc;
+  // merged device pipeline unless a half step needs the wrap diagnostic
+  // (its quick check, optics/propagate.go:251, passes for BPM slices)
+  const bool quick_ok = std::fabs(dz / 2) * ctx.wavelength / (2 * dx) <= (double)n * dx;
+  if (pow2(n) && quick_ok && !getenv("WOS_SPLIT_STEP_REFERENCE_ORDER")) {
+    if ((rc = split_step_fast(prec, d, n, ncomp, dx, z, steps, dv.screens.p, n_planes, ctx, w))) return rc;
+    if (w && dz / 2 < 0) {  // asmWarn's backward diagnostics, one per half step
+      int zero, reg;
+      double alpha, limit;
+      evanescent_policy(ctx, dz / 2, &zero, &reg, &alpha, &limit);
+      for (int i = 0; i < 2 * steps; ++i) asm_warn(w, dz / 2, 0, 0, zero, reg);
+    }
+    return 0;
+  }
+  for (int s = 0; s < steps; ++s) {
+    if ((rc = propagate_dev(prec, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, ctx, w, nullptr))) return rc;
+    const void* t = (const char*)tables + (n_planes == 1 ? 0 : (size_t)s * nn * csize(prec));
+    if (prec == WOS_FP64)
+      hipLaunchKernelGGL(k_mul_table<double>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<double>*)d,
+                         (const cpx<double>*)t, nn, ncomp);
+    else
+      hipLaunchKernelGGL(k_mul_table32, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<float>*)d,
+                         (const cpx<float>*)t, nn, ncomp);
+    WOS_HIP_TRY(hipGetLastError());
+    if ((rc = propagate_dev(prec, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, ctx, w, nullptr))) return rc;
+  }
+  return 0;
+}
+
 static bool valid_prec(int p) { return p == WOS_FP64 || p == WOS_FP32; }
 
 }  // namespace wos
@@ -2279,7 +2356,6 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
     set_error("wos_hip_split_step: n_planes must be 1 or steps");
     return WOS_ERR_INVALID;
   }
-  Device& dv = device();
   const int ncomp = ey ? 2 : 1;
   const size_t nn = (size_t)n * n;
   const double* src[2] = {ex, ey};
@@ -2287,52 +2363,11 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
   void* d;
   int rc;
   if ((rc = upload_fields(precision, src, ncomp, nn, &d))) return rc;
-  const double dz = z / (double)steps;
-  const double k = 2.0 * M_PI / ctx->wavelength;
-  // screen tables t(x, y) per plane: FP64, plus a rounded copy for FP32
-  const size_t tb = (size_t)n_planes * nn * 16;
-  if ((rc = dv.screens.ensure(tb + (precision == WOS_FP32 ? (size_t)n_planes * nn * 8 : 0)))) return rc;
-  if ((rc = dv.work.ensure(nn * 16))) return rc;
-  for (int p = 0; p < n_planes; ++p) {
-    if ((rc = host_to_dev(dv.work.p, index + (size_t)p * nn * 2, nn * 16, dv.stream))) return rc;
-    hipLaunchKernelGGL(k_screen_table, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.work.p,
-                       (cpx<double>*)dv.screens.p + (size_t)p * nn, nn, k, dz);
-    WOS_HIP_TRY(hipGetLastError());
-    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
-  }
-  const void* tables = dv.screens.p;
-  if (precision == WOS_FP32) {
-    tables = (char*)dv.screens.p + tb;
-    const size_t tot = (size_t)n_planes * nn;
-    hipLaunchKernelGGL(k_z2c, dim3(grid_for(tot)), dim3(256), 0, dv.stream, (const cpx<double>*)dv.screens.p,
-                       (cpx<float>*)tables, tot);
-    WOS_HIP_TRY(hipGetLastError());
-  }
-  // merged device pipeline unless a half step needs the wrap diagnostic
-  // (its quick check, optics/propagate.go:251, passes for BPM slices)
-  const bool quick_ok = std::fabs(dz / 2) * ctx->wavelength / (2 * dx) <= (double)n * dx;
-  if (pow2(n) && quick_ok && !getenv("WOS_SPLIT_STEP_REFERENCE_ORDER")) {
-    if ((rc = split_step_fast(precision, d, n, ncomp, dx, z, steps, dv.screens.p, n_planes, *ctx, w))) return rc;
-    if (w && dz / 2 < 0) {  // asmWarn's backward diagnostics, one per half step
-      int zero, reg;
-      double alpha, limit;
-      evanescent_policy(*ctx, dz / 2, &zero, &reg, &alpha, &limit);
-      for (int i = 0; i < 2 * steps; ++i) asm_warn(w, dz / 2, 0, 0, zero, reg);
-    }
-    return download_fields(precision, d, dst, ncomp, nn);
-  }
-  for (int s = 0; s < steps; ++s) {
-    if ((rc = propagate_dev(precision, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, *ctx, w, nullptr))) return rc;
-    const void* t = (const char*)tables + (n_planes == 1 ? 0 : (size_t)s * nn * csize(precision));
-    if (precision == WOS_FP64)
-      hipLaunchKernelGGL(k_mul_table<double>, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<double>*)d,
-                         (const cpx<double>*)t, nn, ncomp);
-    else
-      hipLaunchKernelGGL(k_mul_table32, dim3(grid_for(nn * ncomp)), dim3(256), 0, dv.stream, (cpx<float>*)d,
-                         (const cpx<float>*)t, nn, ncomp);
-    WOS_HIP_TRY(hipGetLastError());
-    if ((rc = propagate_dev(precision, d, n, ncomp, dx, dz / 2, WOS_METHOD_ASM, *ctx, w, nullptr))) return rc;
-  }
+  const void* tables;
+  if ((rc = split_step_screens(precision, n, n_planes, 2.0 * M_PI / ctx->wavelength, z / (double)steps, index, 0, 0,
+                               &tables)))
+    return rc;
+  if ((rc = split_step_run(precision, d, n, ncomp, dx, z, steps, n_planes, tables, *ctx, w))) return rc;
   return download_fields(precision, d, dst, ncomp, nn);
 }
 

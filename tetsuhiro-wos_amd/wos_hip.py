@@ -41,7 +41,7 @@ EXPORTS = [
     "wos_hip_bpm_mask_create", "wos_hip_bpm_mask_destroy", "wos_hip_bpm_mask_forward", "wos_hip_bpm_mask_backward",
     "wos_hip_field_create", "wos_hip_field_destroy", "wos_hip_field_clone", "wos_hip_field_upload",
     "wos_hip_field_download", "wos_hip_field_polarized", "wos_hip_field_device_ptr", "wos_hip_field_propagate",
-    "wos_hip_field_propagate_bandlimited", "wos_hip_field_element",
+    "wos_hip_field_propagate_bandlimited", "wos_hip_field_element", "wos_hip_field_medium",
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
@@ -253,6 +253,7 @@ def load(path=None):
     L.wos_hip_field_propagate_bandlimited.argtypes = [P, D, I, C.POINTER(Ctx), D, D, C.POINTER(Warnings),
                                                       C.POINTER(PropStats)]
     L.wos_hip_field_element.argtypes = [P, C.POINTER(Element), D]
+    L.wos_hip_field_medium.argtypes = [P, D, D, D, I, C.POINTER(Ctx), C.POINTER(Warnings)]
     L.wos_hip_field_lens.argtypes = [P, D, D, D, D, D]
     L.wos_hip_field_aperture.argtypes = [P, C.POINTER(Shape), D, D]
     L.wos_hip_field_diffuser.argtypes = [P, D, D, D, C.c_int64, D, D]
@@ -417,6 +418,13 @@ class Field:
     def element(self, e, wl):
         """One of the further train elements (make_element)."""
         _check(load().wos_hip_field_element(self.h, C.byref(e), wl), "field_element")
+
+    def medium(self, distance, index, absorption, steps, ctx):
+        """mediumEl.Apply: split-step through a uniform index; returns the warnings."""
+        w = Warnings()
+        _check(load().wos_hip_field_medium(self.h, distance, index, absorption, steps, C.byref(ctx), C.byref(w)),
+               "field_medium")
+        return w
 
     def diffuser(self, sigma=np.pi, corr=2e-5, amp=1.0, seed=1, x0=0.0, y0=0.0):
         _check(load().wos_hip_field_diffuser(self.h, sigma, corr, amp, seed, x0, y0), "field_diffuser")
