@@ -118,3 +118,46 @@ def test_fresnel_ir_spectrum_cache(wos, oracle):
         ref, _, _, _ = oracle.propagate(ex, None, dx, z, "fresnel_ir", oracle.ctx_for(wl))
         assert rel_l2(first[(z, wos.FP64)], ref) <= 1e-12
         assert rel_l2(first[(z, wos.FP32)], ref) <= 1e-5
+
+
+def test_concurrent_callers_match_serial_runs(wos):
+    """The C ABI serialises every call on a per-device lock: drop-in
+    propagations, 2-D FFTs and a resident-field element chain issued from
+    six host threads at once give the bit-exact results of serial runs."""
+    import threading
+    rng = np.random.default_rng(12)
+    fields = [rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128)) for _ in range(6)]
+    ctx = wos.ctx_for(633e-9)
+
+    def work(a, prec):
+        out, _, _, _ = wos.propagate(a, None, 2e-6, 3e-3 if prec == wos.FP64 else 5e-3, "asm_pad", ctx, precision=prec)
+        spec = wos.fft2d(a)
+        f = wos.Field.from_host(a, dx=2e-6, precision=prec)
+        f.lens(633e-9, 0.05)
+        f.element(wos.make_element("grating", kind="phase_sin", period=31e-6, modulation=0.8), 633e-9)
+        f.propagate(2e-3, "asm", ctx, bandlimit=(0.9, 0.05))
+        got = f.download()[0]
+        f.close()
+        return out, spec, got
+
+    precs = [wos.FP64, wos.FP32] * 3
+    serial = [work(a, p) for a, p in zip(fields, precs)]
+    results = [None] * 6
+    errors = []
+
+    def run(i):
+        try:
+            for _ in range(3):
+                results[i] = work(fields[i], precs[i])
+        except Exception as exc:  # reported below
+            errors.append(exc)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(6)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(6):
+        for a, b in zip(results[i], serial[i]):
+            assert np.array_equal(a, b), i
