@@ -251,6 +251,52 @@ def test_mixed_radix_grids_match_oracle(wos, oracle, n, method):
             assert abs(sg.p_out - so.p_out) <= 1e-11 * so.p_out
 
 
+@pytest.mark.parametrize("n,method", [(90, "asm"), (720, "asm"), (360, "asm_pad"), (1080, "asm"), (960, "asm_pad"),
+                                      (640, "asm_shift"), (1350, "asm")])
+def test_runtime_mixed_sides_match_oracle(wos, oracle, n, method):
+    """2^a 3^b 5^c sides with no compiled plan run the runtime-length
+    mixed-radix kernels (stage list as an argument, dynamic LDS): 90
+    (2, 3, 3, 5), 720 (16, 3, 3, 5), 360 padded to 720, 1080 (8, 3, 3, 3, 5),
+    960 padded to 1920 (16, 8, 3, 5), 640 (16, 8, 5) with the shifted
+    transfer function and 1350 (2, 3, 3, 3, 5, 5), against the oracle's
+    Bluestein path."""
+    wl = 632.8e-9
+    ex, dx = gaussian_tilted(oracle, n, n * 4e-6, wl, n * 4e-6 / 8, 0.01)
+    z = 0.01
+    ref, _, wo, so = oracle.propagate(ex, None, dx, z, method, oracle.ctx_for(wl))
+    for prec, tol in ((wos.FP64, FP64_TOL), (wos.FP32, FP32_TOL)):
+        out, _, wg, sg = wos.propagate(ex, None, dx, z, method, wos.ctx_for(wl), precision=prec)
+        err = rel_l2(out, ref)
+        print(f"runtime mixed-radix n={n} {method} prec={prec} rel-L2 {err:.3e}", flush=True)
+        assert err <= tol
+        if prec == wos.FP64:
+            assert wg.as_dict().keys() == wo.as_dict().keys()
+            assert abs(sg.p_out - so.p_out) <= 1e-11 * so.p_out
+
+
+@pytest.mark.parametrize("n", [90, 720, 1920, 3600, 4050])
+def test_runtime_mixed_fft_matches_numpy(wos, oracle, n):
+    """The drop-in fft2D / fft1DAny on runtime-plan sides against numpy's FFT
+    (FP64), with the round trip; 4050 (2, 3, 3, 3, 3, 5, 5) takes the
+    largest workgroup, 512 threads."""
+    src = oracle.ref_field(n, 1.3) if n <= 1920 else None
+    if src is not None:
+        want = np.fft.fft2(src)
+        got = wos.fft2d(src)
+        err = rel_l2(got, want)
+        print(f"runtime mixed fft2d n={n} rel-L2 {err:.3e}", flush=True)
+        assert err <= FP64_TOL
+        assert np.max(np.abs(wos.fft2d(got, inverse=True) - src)) <= 1e-11
+    i = np.arange(n)
+    line = np.sin(i * 0.01) + 1j * np.cos(i * 0.017)
+    want = np.fft.fft(line)
+    got = wos.fft1d(line)
+    err = rel_l2(got, want)
+    print(f"runtime mixed fft1d n={n} rel-L2 {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    assert np.max(np.abs(wos.fft1d(got, inverse=True) - line)) <= 1e-12
+
+
 def config2_input(oracle, seed=7):
     """1000^2, dx=1.5um, 532nm: x-polarised plane wave times an i.i.d. uniform
     random phase (diffuser hash, one value per pixel) inside a circular

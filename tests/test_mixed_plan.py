@@ -97,3 +97,54 @@ def test_mixed_radix_plan_matches_numpy(n):
     assert np.linalg.norm(transform(x, -1) - want) <= 1e-13 * np.linalg.norm(want)
     want = np.fft.ifft(x) * n
     assert np.linalg.norm(transform(x, +1) - want) <= 1e-13 * np.linalg.norm(want)
+
+
+# Sides without a compiled plan run the runtime-length kernels
+# (k_rows_mixed_rt / k_cols_mixed_rt): same plan, butterfly maps and tables.
+RT_SIZES = [6, 90, 640, 720, 960, 1080, 1280, 1800, 1920, 2160, 2700, 3240, 3600, 3840, 4050]
+
+
+def smooth_sides(limit):
+    out = []
+    for n in range(2, limit + 1):
+        m = n
+        for f in (2, 3, 5):
+            while m % f == 0:
+                m //= f
+        if m == 1 and n & (n - 1):
+            out.append(n)
+    return out
+
+
+def bmax(r):
+    return 1 if r >= 16 else 4 if r == 2 else 3 if r == 3 else 2
+
+
+def rt_threads(n):
+    """Thread count mixed_rt_plan picks: the smallest multiple of 64 with
+    ceil((n / RAD) / T) <= bmax(RAD) for every stage."""
+    t = 64
+    for r in plan(n):
+        t = max(t, -(-(n // r) // bmax(r)))
+    return -(-t // 64) * 64
+
+
+@pytest.mark.parametrize("n", smooth_sides(4096))
+def test_runtime_plan_gives_every_butterfly_one_thread(n):
+    t = rt_threads(n)
+    assert t <= 512 and len(plan(n)) <= 21
+    assert (n + n // 16 + 1) * 16 <= 160 * 1024  # FP64 transform in LDS
+    for r in plan(n):
+        nb = n // r
+        hits = np.zeros(nb, int)
+        for b in range(bmax(r)):
+            j = np.arange(t) + b * t
+            hits[j[j < nb]] += 1
+        assert (hits == 1).all()
+
+
+@pytest.mark.parametrize("n", RT_SIZES)
+def test_runtime_sides_match_numpy(n):
+    x = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 1).standard_normal(n)
+    want = np.fft.fft(x)
+    assert np.linalg.norm(transform(x, -1) - want) <= 1e-13 * np.linalg.norm(want)

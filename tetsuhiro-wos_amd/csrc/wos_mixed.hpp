@@ -247,4 +247,188 @@ __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Runtime-length variants for the 2^a 3^b 5^c sides without a compiled
+// MPlan<N>: the same plan, butterfly maps and twiddle tables, with the
+// stage list passed as an argument and the transform in dynamic LDS.
+// A thread holds at most mixed_bmax(RAD) butterflies of a radix-RAD stage;
+// the host picks T so that ceil((N / RAD) / T) <= mixed_bmax(RAD) for every
+// stage of the plan, so each butterfly has a thread.
+// ---------------------------------------------------------------------------
+// The stage radices travel packed, three bits a stage (mixed_rad_code), so
+// the kernel indexes no argument array; NS and the table offset of a stage
+// follow from the radices before it.
+constexpr int kMixedMaxStages = 21;
+constexpr int kMixedMaxThreads = 512;
+struct MixedRt {
+  int n, s, t;              // length, stage count, threads per transform
+  unsigned long long code;  // radix code of stage i at bits 3i .. 3i+2
+};
+constexpr int mixed_rad_code(int rad) { return rad == 16 ? 0 : rad == 8 ? 1 : rad == 4 ? 2 : rad == 2 ? 3 : rad == 3 ? 4 : 5; }
+constexpr int mixed_bmax(int rad) { return rad >= 16 ? 1 : rad == 2 ? 4 : rad == 3 ? 3 : 2; }
+
+template <typename R, int RAD, int DIR, int ES>
+__device__ __forceinline__ void mixed_stage_rt(cpx<R>* lds, int t, int T, int N, int NS, const cpx<R>* __restrict__ tw) {
+  constexpr int BM = mixed_bmax(RAD);
+  const int NB = N / RAD;
+  cpx<R> v[BM][RAD];
+#pragma unroll
+  for (int b = 0; b < BM; ++b) {
+    const int j = t + b * T;
+    if (j < NB) {
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[b][r] = lds[lpad(j + r * NB) * ES];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BM; ++b) {
+    const int j = t + b * T;
+    if (j < NB) {
+      const int k = j % NS;
+      if (NS > 1) {
+        const cpx<R>* tk = tw + k * (RAD - 1);
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) v[b][r] = DIR < 0 ? cmul(v[b][r], tk[r - 1]) : cmulc(v[b][r], tk[r - 1]);
+      }
+      dft_mixed<RAD, DIR>(v[b]);
+      const int o = (j - k) * RAD + k;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) lds[lpad(o + r * NS) * ES] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// the plan's stages in order; every thread of the workgroup takes the same
+// branch (the radix is uniform), so the stage barriers are reached by all
+template <typename R, int DIR, int ES>
+__device__ __forceinline__ void mixed_fft_rt(cpx<R>* lds, int t, const MixedRt& p, const cpx<R>* __restrict__ tw) {
+  int NS = 1, off = 0;  // stage 0 (NS == 1) reads no twiddles
+  for (int s = 0; s < p.s; ++s) {
+    const cpx<R>* ts = tw + off;
+    int rad;
+    switch ((int)(p.code >> (3 * s)) & 7) {
+      case 0: rad = 16; mixed_stage_rt<R, 16, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      case 1: rad = 8; mixed_stage_rt<R, 8, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      case 2: rad = 4; mixed_stage_rt<R, 4, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      case 3: rad = 2; mixed_stage_rt<R, 2, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      case 4: rad = 3; mixed_stage_rt<R, 3, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      default: rad = 5; mixed_stage_rt<R, 5, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+    }
+    if (s > 0) off += NS * (rad - 1);
+    NS *= rad;
+  }
+}
+
+// block reduction as block_sum_store with the workgroup size known at run time
+__device__ __forceinline__ void block_sum_store_rt(double v, double* red, double* out, int slot) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < nw; ++i) s += red[i];
+    out[slot] = s;
+  }
+}
+
+// k_rows_mixed with the plan at run time: p.t threads, lds_len(p.n) elements
+// of dynamic LDS
+template <typename R, int DIR>
+__global__ __launch_bounds__(kMixedMaxThreads) void k_rows_mixed_rt(const RowArgs a, const MixedRt p) {
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  cpx<R>* lds = (cpx<R>*)dyn_lds;
+  __shared__ double red[kMixedMaxThreads / 64];
+  const int N = p.n, T = p.t;
+  const int t = threadIdx.x;
+  const int unit = blockIdx.x / a.rows;
+  const int row = blockIdx.x % a.rows;
+  const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
+  cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
+  double pin = 0;
+  for (int q0 = t; q0 < N; q0 += T) {
+    const int q = q0 - a.in_col0;
+    cpx<R> x = {R(0), R(0)};
+    if (q >= 0 && q < a.in_cols) x = in[tiled_offset(q, row, a.in_ld, a.in_tw, a.in_tstride)];
+    pin += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    lds[lpad(q0)] = x;
+  }
+  if (a.pw_in) block_sum_store_rt(pin, red, a.pw_in, blockIdx.x);  // uniform
+  __syncthreads();
+  mixed_fft_rt<R, DIR, 1>(lds, t, p, (const cpx<R>*)a.tw);
+  const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
+  const bool scale = a.scale_im != 0 || a.scale_re != 1;
+  double pout = 0;
+  for (int q0 = t; q0 < N; q0 += T) {
+    const int q = q0 - a.out_col0;
+    if (q >= 0 && q < a.out_cols) {
+      cpx<R> y = lds[lpad(q0)];
+      if (scale) y = cmul(y, sc);
+      pout += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+      out[tiled_offset(q, row, a.out_ld, a.out_tw, a.out_tstride)] = y;
+    }
+  }
+  if (a.pw_out) block_sum_store_rt(pout, red, a.pw_out, blockIdx.x);  // uniform
+}
+
+// k_cols_mixed with the plan at run time: W * p.t threads,
+// lds_len(p.n) * W elements of dynamic LDS
+template <typename R, int W, int MODE, int DIR>
+__global__ __launch_bounds__(kMixedMaxThreads) void k_cols_mixed_rt(const ColArgs a, const MixedRt p) {
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  cpx<R>* lds = (cpx<R>*)dyn_lds;
+  const int N = p.n, T = p.t;
+  const int tid = threadIdx.x;
+  const int c = tid % W, t = tid / W;
+  const int tpu = a.cols / W;
+  const int unit = blockIdx.x / tpu;
+  const int col = (blockIdx.x % tpu) * W + c;
+  const int tl = a.tiled ? 1 << a.tlg : 0;
+  const int ld = a.tiled ? tl : a.ld;
+  cpx<R>* base =
+      (cpx<R>*)a.data + (size_t)unit * a.unit + (a.tiled ? (col >> a.tlg) * a.tstride + (col & (tl - 1)) : col);
+  cpx<R>* lc = lds + c;
+  for (int q0 = t; q0 < N; q0 += T) {
+    const int q = q0 - a.in_row0;
+    cpx<R> x = {R(0), R(0)};
+    if (q >= 0 && q < a.rows) x = base[(size_t)q * ld];
+    lc[lpad(q0) * W] = x;
+  }
+  __syncthreads();
+  const cpx<R>* tw = (const cpx<R>*)a.tw;
+  if constexpr (MODE == 2) {
+    mixed_fft_rt<R, -1, W>(lc, t, p, tw);
+    const TFUnit& u = a.tf[unit];
+    const bool mirror = u.fy_off == 0.0 && (u.kind == TF_ASM || u.kind == TF_CONJ_ASM || u.kind == TF_FRESNEL ||
+                                            u.kind == TF_BANDLIMIT);
+    if (mirror) {
+      for (int q0 = t; q0 <= N / 2; q0 += T) {
+        const cpx<R> h = tf_eval<R>(u, col, q0);
+        lc[lpad(q0) * W] = cmul(lc[lpad(q0) * W], h);
+        if (q0 != 0 && 2 * q0 != N) lc[lpad(N - q0) * W] = cmul(lc[lpad(N - q0) * W], h);
+      }
+    } else {
+      for (int q0 = t; q0 < N; q0 += T) lc[lpad(q0) * W] = cmul(lc[lpad(q0) * W], tf_eval<R>(u, col, q0));
+    }
+    __syncthreads();
+    mixed_fft_rt<R, 1, W>(lc, t, p, tw);
+  } else {
+    mixed_fft_rt<R, DIR, W>(lc, t, p, tw);
+  }
+  const R sc = (R)a.scale;
+  for (int q0 = t; q0 < N; q0 += T) {
+    const int q = q0 - a.out_row0;
+    if (q >= 0 && q < a.out_rows) {
+      cpx<R> y = lc[lpad(q0) * W];
+      if (sc != R(1)) y = cscale(y, sc);
+      base[(size_t)q * ld] = y;
+    }
+  }
+}
+
 }  // namespace wos
