@@ -252,14 +252,16 @@ def test_mixed_radix_grids_match_oracle(wos, oracle, n, method):
 
 
 @pytest.mark.parametrize("n,method", [(90, "asm"), (720, "asm"), (360, "asm_pad"), (1080, "asm"), (960, "asm_pad"),
-                                      (640, "asm_shift"), (1350, "asm")])
+                                      (640, "asm_shift"), (1350, "asm"), (700, "asm"), (350, "asm_pad"),
+                                      (1470, "asm")])
 def test_runtime_mixed_sides_match_oracle(wos, oracle, n, method):
     """2^a 3^b 5^c sides with no compiled plan run the runtime-length
     mixed-radix kernels (stage list as an argument, dynamic LDS): 90
     (2, 3, 3, 5), 720 (16, 3, 3, 5), 360 padded to 720, 1080 (8, 3, 3, 3, 5),
     960 padded to 1920 (16, 8, 3, 5), 640 (16, 8, 5) with the shifted
-    transfer function and 1350 (2, 3, 3, 3, 5, 5), against the oracle's
-    Bluestein path."""
+    transfer function, 1350 (2, 3, 3, 3, 5, 5), and with radix-7 stages 700
+    (4, 5, 5, 7), 350 padded to 700 and 1470 (2, 3, 5, 7, 7), against the
+    oracle's Bluestein path."""
     wl = 632.8e-9
     ex, dx = gaussian_tilted(oracle, n, n * 4e-6, wl, n * 4e-6 / 8, 0.01)
     z = 0.01
@@ -274,11 +276,12 @@ def test_runtime_mixed_sides_match_oracle(wos, oracle, n, method):
             assert abs(sg.p_out - so.p_out) <= 1e-11 * so.p_out
 
 
-@pytest.mark.parametrize("n", [90, 720, 1920, 3600, 4050])
+@pytest.mark.parametrize("n", [90, 720, 1920, 3600, 4050, 49, 1400, 3430])
 def test_runtime_mixed_fft_matches_numpy(wos, oracle, n):
     """The drop-in fft2D / fft1DAny on runtime-plan sides against numpy's FFT
     (FP64), with the round trip; 4050 (2, 3, 3, 3, 3, 5, 5) takes the
-    largest workgroup, 512 threads."""
+    largest workgroup, 512 threads; 49, 1400 and 3430 (2, 5, 7, 7, 7) run
+    radix-7 stages."""
     src = oracle.ref_field(n, 1.3) if n <= 1920 else None
     if src is not None:
         want = np.fft.fft2(src)

@@ -23,8 +23,12 @@ def plan(n):
     while m % 5 == 0:
         m //= 5
         c += 1
+    d = 0
+    while m % 7 == 0:
+        m //= 7
+        d += 1
     assert m == 1
-    return [16] * (a // 4) + ([1 << (a % 4)] if a % 4 else []) + [3] * b + [5] * c
+    return [16] * (a // 4) + ([1 << (a % 4)] if a % 4 else []) + [3] * b + [5] * c + [7] * d
 
 
 def twiddles(n):
@@ -56,6 +60,15 @@ def butterfly(R, d, v):
         t1, t2 = v[0] + c1 * a1 + c2 * a2, v[0] + c2 * a1 + c1 * a2
         u1, u2 = mul_i(d, s1 * b1 + s2 * b2), mul_i(d, s2 * b1 - s1 * b2)
         return [v[0] + a1 + a2, t1 + u1, t2 + u2, t2 - u2, t1 - u1]
+    if R == 7:  # pairs a_k = v_k + v_(7-k), b_k = v_k - v_(7-k), as dft7 in the kernel
+        a = [v[k] + v[7 - k] for k in (1, 2, 3)]
+        b = [v[k] - v[7 - k] for k in (1, 2, 3)]
+        out = [v[0] + sum(a)] + [None] * 6
+        for m in (1, 2, 3):
+            t = v[0] + sum(np.cos(2 * np.pi * m * k / 7) * a[k - 1] for k in (1, 2, 3))
+            u = mul_i(d, sum(np.sin(2 * np.pi * m * k / 7) * b[k - 1] for k in (1, 2, 3)))
+            out[m], out[7 - m] = t + u, t - u
+        return out
     k = np.arange(R)
     return list(np.exp(d * 2j * np.pi * np.outer(k, k) / R) @ np.array(v))
 
@@ -99,16 +112,18 @@ def test_mixed_radix_plan_matches_numpy(n):
     assert np.linalg.norm(transform(x, +1) - want) <= 1e-13 * np.linalg.norm(want)
 
 
-# Sides without a compiled plan run the runtime-length kernels
-# (k_rows_mixed_rt / k_cols_mixed_rt): same plan, butterfly maps and tables.
-RT_SIZES = [6, 90, 640, 720, 960, 1080, 1280, 1800, 1920, 2160, 2700, 3240, 3600, 3840, 4050]
+# Sides without a compiled plan, and sides with factors of 7, run the
+# runtime-length kernels (k_rows_mixed_rt / k_cols_mixed_rt): same plan,
+# radix-7 stages last, same butterfly maps and tables.
+RT_SIZES = [6, 90, 640, 720, 960, 1080, 1280, 1800, 1920, 2160, 2700, 3240, 3600, 3840, 4050,
+            7, 49, 700, 1400, 2100, 2940, 3430]
 
 
 def smooth_sides(limit):
     out = []
     for n in range(2, limit + 1):
         m = n
-        for f in (2, 3, 5):
+        for f in (2, 3, 5, 7):
             while m % f == 0:
                 m //= f
         if m == 1 and n & (n - 1):

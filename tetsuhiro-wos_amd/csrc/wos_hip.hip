@@ -365,18 +365,21 @@ int get_twiddles(int n, int prec, const void** out, int acc, int radix) {
   return 0;
 }
 
-// Host mirror of MPlan<N> (wos_mixed.hpp): stage radices of the mixed-radix
-// plan of n = 2^a 3^b 5^c points.
+// Host mirror of MPlan<N> and mixed_rt_plan (wos_mixed.hpp,
+// wos_launch_mixed.inc): stage radices of the mixed-radix plan of
+// n = 2^a 3^b 5^c 7^d points.
 static std::vector<int> mixed_plan_host(int n) {
-  int a = 0, b = 0, c = 0, m = n;
+  int a = 0, b = 0, c = 0, d = 0, m = n;
   while (m % 2 == 0) m /= 2, ++a;
   while (m % 3 == 0) m /= 3, ++b;
   while (m % 5 == 0) m /= 5, ++c;
+  while (m % 7 == 0) m /= 7, ++d;
   std::vector<int> rad;
   for (int i = 0; i < a / 4; ++i) rad.push_back(16);
   if (a % 4) rad.push_back(1 << (a % 4));
   for (int i = 0; i < b; ++i) rad.push_back(3);
   for (int i = 0; i < c; ++i) rad.push_back(5);
+  for (int i = 0; i < d; ++i) rad.push_back(7);
   return rad;
 }
 

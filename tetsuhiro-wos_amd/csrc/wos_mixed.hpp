@@ -1,6 +1,6 @@
-// wos_mixed.hpp — mixed-radix (2, 3, 5) Stockham FFT for grid sides
-// N = 2^a 3^b 5^c that are not powers of two (config grids 384, 1000, 1536
-// and their zero-padded 768, 2000, 3072), in place of Bluestein's three
+// wos_mixed.hpp — mixed-radix (2, 3, 5, 7) Stockham FFT for grid sides
+// N = 2^a 3^b 5^c 7^d that are not powers of two (config grids 384, 1000,
+// 1536 and their zero-padded 768, 2000, 3072), in place of Bluestein's three
 // power-of-two transforms of length >= 2N - 1.
 //
 // A transform lives in LDS for its whole duration. Stage s (radix RAD,
@@ -50,10 +50,39 @@ __device__ __forceinline__ void dft5(cpx<R>* v) {
   v[3] = t2 - u2;
 }
 
+// 7 points: with a_k = v_k + v_(7-k), b_k = v_k - v_(7-k) (k = 1..3),
+// X_m = v_0 + sum_k cos(2 pi mk/7) a_k + DIR i sum_k sin(2 pi mk/7) b_k and
+// X_(7-m) the same with the sine part negated (m = 1..3)
+template <int DIR, typename R>
+__device__ __forceinline__ void dft7(cpx<R>* v) {
+  constexpr double C1 = 0.62348980185873353053, C2 = -0.22252093395631440429, C3 = -0.90096886790241912624;
+  constexpr double S1 = 0.78183148246802980871, S2 = 0.97492791218182360702, S3 = 0.43388373911755812048;
+  const cpx<R> a1 = v[1] + v[6], b1 = v[1] - v[6], a2 = v[2] + v[5], b2 = v[2] - v[5], a3 = v[3] + v[4],
+               b3 = v[3] - v[4];
+  const cpx<R> x0 = v[0];
+  const cpx<R> t1 = {x0.x + (R)C1 * a1.x + (R)C2 * a2.x + (R)C3 * a3.x, x0.y + (R)C1 * a1.y + (R)C2 * a2.y + (R)C3 * a3.y};
+  const cpx<R> t2 = {x0.x + (R)C2 * a1.x + (R)C3 * a2.x + (R)C1 * a3.x, x0.y + (R)C2 * a1.y + (R)C3 * a2.y + (R)C1 * a3.y};
+  const cpx<R> t3 = {x0.x + (R)C3 * a1.x + (R)C1 * a2.x + (R)C2 * a3.x, x0.y + (R)C3 * a1.y + (R)C1 * a2.y + (R)C2 * a3.y};
+  const cpx<R> u1 = mul_i<DIR>(
+      cpx<R>{(R)S1 * b1.x + (R)S2 * b2.x + (R)S3 * b3.x, (R)S1 * b1.y + (R)S2 * b2.y + (R)S3 * b3.y});
+  const cpx<R> u2 = mul_i<DIR>(
+      cpx<R>{(R)S2 * b1.x - (R)S3 * b2.x - (R)S1 * b3.x, (R)S2 * b1.y - (R)S3 * b2.y - (R)S1 * b3.y});
+  const cpx<R> u3 = mul_i<DIR>(
+      cpx<R>{(R)S3 * b1.x - (R)S1 * b2.x + (R)S2 * b3.x, (R)S3 * b1.y - (R)S1 * b2.y + (R)S2 * b3.y});
+  v[0] = x0 + a1 + a2 + a3;
+  v[1] = t1 + u1;
+  v[6] = t1 - u1;
+  v[2] = t2 + u2;
+  v[5] = t2 - u2;
+  v[3] = t3 + u3;
+  v[4] = t3 - u3;
+}
+
 template <int RAD, int DIR, typename R>
 __device__ __forceinline__ void dft_mixed(cpx<R>* v) {
   if constexpr (RAD == 3) dft3<DIR>(v);
   else if constexpr (RAD == 5) dft5<DIR>(v);
+  else if constexpr (RAD == 7) dft7<DIR>(v);
   else dft<RAD, DIR, R>(v);
 }
 
@@ -249,7 +278,8 @@ __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a)
 
 // ---------------------------------------------------------------------------
 // Runtime-length variants for the 2^a 3^b 5^c sides without a compiled
-// MPlan<N>: the same plan, butterfly maps and twiddle tables, with the
+// MPlan<N>, and for sides with factors of 7 (radix-7 stages after the
+// radix-5 ones): the same plan, butterfly maps and twiddle tables, with the
 // stage list passed as an argument and the transform in dynamic LDS.
 // A thread holds at most mixed_bmax(RAD) butterflies of a radix-RAD stage;
 // the host picks T so that ceil((N / RAD) / T) <= mixed_bmax(RAD) for every
@@ -264,7 +294,9 @@ struct MixedRt {
   int n, s, t;              // length, stage count, threads per transform
   unsigned long long code;  // radix code of stage i at bits 3i .. 3i+2
 };
-constexpr int mixed_rad_code(int rad) { return rad == 16 ? 0 : rad == 8 ? 1 : rad == 4 ? 2 : rad == 2 ? 3 : rad == 3 ? 4 : 5; }
+constexpr int mixed_rad_code(int rad) {
+  return rad == 16 ? 0 : rad == 8 ? 1 : rad == 4 ? 2 : rad == 2 ? 3 : rad == 3 ? 4 : rad == 5 ? 5 : 6;
+}
 constexpr int mixed_bmax(int rad) { return rad >= 16 ? 1 : rad == 2 ? 4 : rad == 3 ? 3 : 2; }
 
 template <typename R, int RAD, int DIR, int ES>
@@ -301,8 +333,10 @@ __device__ __forceinline__ void mixed_stage_rt(cpx<R>* lds, int t, int T, int N,
 }
 
 // the plan's stages in order; every thread of the workgroup takes the same
-// branch (the radix is uniform), so the stage barriers are reached by all
-template <typename R, int DIR, int ES>
+// branch (the radix is uniform), so the stage barriers are reached by all.
+// R7: the plan has radix-7 stages (a separate kernel, so that the sides
+// without them do not carry its registers)
+template <typename R, int DIR, int ES, bool R7>
 __device__ __forceinline__ void mixed_fft_rt(cpx<R>* lds, int t, const MixedRt& p, const cpx<R>* __restrict__ tw) {
   int NS = 1, off = 0;  // stage 0 (NS == 1) reads no twiddles
   for (int s = 0; s < p.s; ++s) {
@@ -314,7 +348,11 @@ __device__ __forceinline__ void mixed_fft_rt(cpx<R>* lds, int t, const MixedRt& 
       case 2: rad = 4; mixed_stage_rt<R, 4, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
       case 3: rad = 2; mixed_stage_rt<R, 2, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
       case 4: rad = 3; mixed_stage_rt<R, 3, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
-      default: rad = 5; mixed_stage_rt<R, 5, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      case 5: rad = 5; mixed_stage_rt<R, 5, DIR, ES>(lds, t, p.t, p.n, NS, ts); break;
+      default:  // code 6, only in plans launched with R7
+        rad = 7;
+        if constexpr (R7) mixed_stage_rt<R, 7, DIR, ES>(lds, t, p.t, p.n, NS, ts);
+        break;
     }
     if (s > 0) off += NS * (rad - 1);
     NS *= rad;
@@ -339,7 +377,7 @@ __device__ __forceinline__ void block_sum_store_rt(double v, double* red, double
 
 // k_rows_mixed with the plan at run time: p.t threads, lds_len(p.n) elements
 // of dynamic LDS
-template <typename R, int DIR>
+template <typename R, int DIR, bool R7>
 __global__ __launch_bounds__(kMixedMaxThreads) void k_rows_mixed_rt(const RowArgs a, const MixedRt p) {
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   cpx<R>* lds = (cpx<R>*)dyn_lds;
@@ -360,7 +398,7 @@ __global__ __launch_bounds__(kMixedMaxThreads) void k_rows_mixed_rt(const RowArg
   }
   if (a.pw_in) block_sum_store_rt(pin, red, a.pw_in, blockIdx.x);  // uniform
   __syncthreads();
-  mixed_fft_rt<R, DIR, 1>(lds, t, p, (const cpx<R>*)a.tw);
+  mixed_fft_rt<R, DIR, 1, R7>(lds, t, p, (const cpx<R>*)a.tw);
   const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
   const bool scale = a.scale_im != 0 || a.scale_re != 1;
   double pout = 0;
@@ -378,7 +416,7 @@ __global__ __launch_bounds__(kMixedMaxThreads) void k_rows_mixed_rt(const RowArg
 
 // k_cols_mixed with the plan at run time: W * p.t threads,
 // lds_len(p.n) * W elements of dynamic LDS
-template <typename R, int W, int MODE, int DIR>
+template <typename R, int W, int MODE, int DIR, bool R7>
 __global__ __launch_bounds__(kMixedMaxThreads) void k_cols_mixed_rt(const ColArgs a, const MixedRt p) {
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   cpx<R>* lds = (cpx<R>*)dyn_lds;
@@ -402,7 +440,7 @@ __global__ __launch_bounds__(kMixedMaxThreads) void k_cols_mixed_rt(const ColArg
   __syncthreads();
   const cpx<R>* tw = (const cpx<R>*)a.tw;
   if constexpr (MODE == 2) {
-    mixed_fft_rt<R, -1, W>(lc, t, p, tw);
+    mixed_fft_rt<R, -1, W, R7>(lc, t, p, tw);
     const TFUnit& u = a.tf[unit];
     const bool mirror = u.fy_off == 0.0 && (u.kind == TF_ASM || u.kind == TF_CONJ_ASM || u.kind == TF_FRESNEL ||
                                             u.kind == TF_BANDLIMIT);
@@ -416,9 +454,9 @@ __global__ __launch_bounds__(kMixedMaxThreads) void k_cols_mixed_rt(const ColArg
       for (int q0 = t; q0 < N; q0 += T) lc[lpad(q0) * W] = cmul(lc[lpad(q0) * W], tf_eval<R>(u, col, q0));
     }
     __syncthreads();
-    mixed_fft_rt<R, 1, W>(lc, t, p, tw);
+    mixed_fft_rt<R, 1, W, R7>(lc, t, p, tw);
   } else {
-    mixed_fft_rt<R, DIR, W>(lc, t, p, tw);
+    mixed_fft_rt<R, DIR, W, R7>(lc, t, p, tw);
   }
   const R sc = (R)a.scale;
   for (int q0 = t; q0 < N; q0 += T) {
