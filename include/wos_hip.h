@@ -109,6 +109,12 @@ double wos_hip_last_device_ms(void);
 int wos_hip_release_cache(void);
 
 /* ---- drop-in transforms: gpuFFT2D / gpuFFT1D (optics/gpu.go:275-304) ---- */
+/* Lengths: powers of two up to 8192 (Stockham radix 16/8); other lengths
+ * whose Bluestein size (next power of two >= 2n - 1) is at most 8192, i.e.
+ * n <= 4096: mixed-radix Stockham when n = 2^a 3^b 5^c 7^d, Bluestein
+ * otherwise, both with the reference's unnormalised forward and 1/n inverse.
+ * Longer non-powers of two return WOS_ERR_UNSUPPORTED (the Go caller then
+ * keeps its CPU path, as optics/fft.go:136-141 does on any GPU error). */
 /* In-place 2-D transform of an n*n complex128 host array (fft2D, optics/fft.go:209-216). */
 int wos_hip_fft2d_z(double* a, int n, int inverse);
 /* In-place 1-D transform of a length-n complex128 host array (fft1DAny, optics/fft.go:135-142). */
