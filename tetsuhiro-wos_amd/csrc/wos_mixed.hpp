@@ -377,8 +377,11 @@ __device__ __forceinline__ void block_sum_store_rt(double v, double* red, double
 
 // k_rows_mixed with the plan at run time: p.t threads, lds_len(p.n) elements
 // of dynamic LDS
+// FP64: capped at 168 VGPRs (3 waves per SIMD), a few bytes of spills,
+// faster than 2 waves per SIMD; the column kernels measured slower under the
+// same cap
 template <typename R, int DIR, bool R7>
-__global__ __launch_bounds__(kMixedMaxThreads) void k_rows_mixed_rt(const RowArgs a, const MixedRt p) {
+__global__ __launch_bounds__(kMixedMaxThreads, (sizeof(R) == 8 ? 3 : 1)) void k_rows_mixed_rt(const RowArgs a, const MixedRt p) {
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   cpx<R>* lds = (cpx<R>*)dyn_lds;
   __shared__ double red[kMixedMaxThreads / 64];
