@@ -81,6 +81,39 @@ def test_pipeline_stages_match_oracle(wos, oracle):
     p.close()
 
 
+def test_pipeline_runtime_mixed_side_large_lds(wos):
+    """A 3888^2 FP64 stage (2^4 3^5: the runtime-length mixed-radix kernels
+    with 66 KB of dynamic LDS per transform and one-column tiles) under graph
+    capture: replays equal the eager run, which matches the single-field
+    operator and an independent numpy angular-spectrum propagation."""
+    n, dx, wl, z = 3888, 1e-6, 532e-9, 2e-3
+    ctx = wos.ctx_for(wl)
+    src = random_field(n, 5)
+    p = wos.Pipeline(wos.FP64)
+    s = p.add(n, dx, np.array([wl]), np.array([z]), ctx, "asm")
+    p.upload(s, 0, src)
+    p.run(1)
+    eager = p.download(s, 0)
+    single, _, _, _ = wos.propagate(src, None, dx, z, "asm", ctx)
+    e = rel_l2(eager, single)
+    print(f"pipeline 3888 FP64 vs single-field operator rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-13
+    f = np.fft.fftfreq(n, dx)
+    rho2 = (wl * f[None, :]) ** 2 + (wl * f[:, None]) ** 2    # < 1 everywhere: 1/wl > 1/(2 dx)
+    h = np.exp(1j * (2 * np.pi / wl) * z * np.sqrt(1 - rho2))
+    ref = np.fft.ifft2(np.fft.fft2(src) * h)
+    e = rel_l2(eager, ref)
+    print(f"pipeline 3888 FP64 vs numpy ASM rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-10  # kz z up to 2.4e4 rad: the two phase formulas differ by ~1e-12 rad
+    p.capture()
+    for rep in range(2):
+        p.run(1)
+        p.sync()
+        assert np.array_equal(p.download(s, 0), eager), rep
+        wos.fft2d(random_field(64, 99))   # eager work between replays
+    p.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_pipeline_graph_replays_match_eager(wos, precision):
     """Capture once, replay three times with eager library work and
