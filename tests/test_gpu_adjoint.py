@@ -32,10 +32,11 @@ def masks(units, n, seed):
     return rng.uniform(0, 2 * math.pi, (units, n, n)).astype(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("n", [64, 128, 100])
+@pytest.mark.parametrize("n", [64, 128, 100, 94])
 def test_mask_operator_fp64_matches_cpu_adjoint(wos, oracle, n):
     """Loss and gradient of two masks vs the oracle's FP64 adjoint; n = 100
-    runs the Bluestein path with the conjugate transfer function."""
+    runs the mixed-radix path and n = 94 = 2 47 the Bluestein path, both with
+    the conjugate transfer function."""
     dx, wl, z = 4e-6, 532e-9, 2e-3
     target = ring_target(n, dx, radius=n * dx / 4, sigma=8e-6)
     theta = masks(2, n, 3)
@@ -141,11 +142,11 @@ def bpm_case(n, dx, steps, seed):
     return theta, phi, index, target / target.sum()
 
 
-@pytest.mark.parametrize("n", [64, 100])
+@pytest.mark.parametrize("n", [64, 100, 94])
 def test_bpm_mask_operator_matches_cpu_adjoint(wos, oracle, n):
     """Loss and gradients (input phase and every design phase plane) vs the
-    oracle's FP64 BPM adjoint; n = 100 runs the Bluestein path. FP32 error
-    against FP64 reported."""
+    oracle's FP64 BPM adjoint; n = 100 runs the mixed-radix path, n = 94 the
+    Bluestein path. FP32 error against FP64 reported."""
     dx, wl, steps, z = 1e-6, 1.0e-6, 4, 80e-6
     theta, phi, index, target = bpm_case(n, dx, steps, 2)
     L, gt, gp = oracle.bpm_mask_loss_grad(theta, phi, index, target, dx, z, steps, oracle.ctx_for(wl))

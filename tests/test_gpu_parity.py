@@ -187,10 +187,12 @@ def test_propagate1d_fp64_parity(wos, oracle, n):
 
 # ---------------------------------------------------------------- non-power-of-two (Bluestein)
 
-@pytest.mark.parametrize("n", [12, 27, 100, 384, 1000])
+@pytest.mark.parametrize("n", [12, 27, 100, 384, 1000, 11, 94, 1018])
 def test_fft2d_bluestein_matches_oracle(wos, oracle, n):
-    """TestGPUFFT2DMatchesCPU's non-power-of-two sizes (optics/gpu_test.go:54),
-    through the chirp-z path of optics/fft.go:159-204."""
+    """TestGPUFFT2DMatchesCPU's non-power-of-two sizes (optics/gpu_test.go:54)
+    against the oracle's chirp-z path (optics/fft.go:159-204): 12, 27, 100,
+    384 and 1000 run the mixed-radix kernels, 11, 94 = 2 47 and 1018 = 2 509
+    the device Bluestein path."""
     src = oracle.ref_field(n, 1.3)
     cpu = oracle.fft2d(src)
     gpu = wos.fft2d(src)
