@@ -229,13 +229,14 @@ def test_methods_non_pow2_grid_fp64_parity(wos, oracle, method):
 
 
 @pytest.mark.parametrize("n,method", [(384, "asm"), (384, "asm_pad"), (1536, "asm"), (400, "asm_shift"), (600, "asm_pad"),
-                                      (1500, "asm")])
+                                      (1500, "asm"), (720, "asm"), (1920, "asm")])
 def test_mixed_radix_grids_match_oracle(wos, oracle, n, method):
     """Grid sides 2^a 3^b 5^c run the mixed-radix kernels (radix 16/8/4/2
     stages, then 3 and 5) in place of Bluestein: 384 (16, 8, 3), its padded
     768 (16, 16, 3), 1536 (16, 16, 2, 3), 400 (16, 5, 5) with the shifted
     transfer function, 600 padded to 1200 (16, 3, 5, 5) and 1500
-    (4, 3, 5, 5, 5), against the oracle's Bluestein path (the FP64 and FP32
+    (4, 3, 5, 5, 5), 720 (16, 3, 3, 5) and 1920 (16, 8, 3, 5), against the
+    oracle's Bluestein path (the FP64 and FP32
     tolerances)."""
     wl = 632.8e-9
     ex, dx = gaussian_tilted(oracle, n, n * 4e-6, wl, n * 4e-6 / 8, 0.01)
@@ -253,14 +254,14 @@ def test_mixed_radix_grids_match_oracle(wos, oracle, n, method):
             assert abs(sg.p_out - so.p_out) <= 1e-11 * so.p_out
 
 
-@pytest.mark.parametrize("n,method", [(90, "asm"), (720, "asm"), (360, "asm_pad"), (1080, "asm"), (960, "asm_pad"),
-                                      (640, "asm_shift"), (1350, "asm"), (700, "asm"), (350, "asm_pad"),
+@pytest.mark.parametrize("n,method", [(90, "asm"), (960, "asm"), (450, "asm_pad"), (1080, "asm"), (1152, "asm"),
+                                      (480, "asm_shift"), (1350, "asm"), (700, "asm"), (350, "asm_pad"),
                                       (1470, "asm")])
 def test_runtime_mixed_sides_match_oracle(wos, oracle, n, method):
     """2^a 3^b 5^c sides with no compiled plan run the runtime-length
     mixed-radix kernels (stage list as an argument, dynamic LDS): 90
-    (2, 3, 3, 5), 720 (16, 3, 3, 5), 360 padded to 720, 1080 (8, 3, 3, 3, 5),
-    960 padded to 1920 (16, 8, 3, 5), 640 (16, 8, 5) with the shifted
+    (2, 3, 3, 5), 960 (16, 4, 3, 5), 450 padded to 900 (4, 3, 3, 5, 5), 1080
+    (8, 3, 3, 3, 5), 1152 (16, 8, 3, 3), 480 (16, 2, 3, 5) with the shifted
     transfer function, 1350 (2, 3, 3, 3, 5, 5), and with radix-7 stages 700
     (4, 5, 5, 7), 350 padded to 700 and 1470 (2, 3, 5, 7, 7), against the
     oracle's Bluestein path."""
@@ -278,7 +279,7 @@ def test_runtime_mixed_sides_match_oracle(wos, oracle, n, method):
             assert abs(sg.p_out - so.p_out) <= 1e-11 * so.p_out
 
 
-@pytest.mark.parametrize("n", [90, 720, 1920, 3600, 4050, 49, 1400, 3430])
+@pytest.mark.parametrize("n", [90, 960, 1800, 3600, 4050, 49, 1400, 3430])
 def test_runtime_mixed_fft_matches_numpy(wos, oracle, n):
     """The drop-in fft2D / fft1DAny on runtime-plan sides against numpy's FFT
     (FP64), with the round trip; 4050 (2, 3, 3, 3, 3, 5, 5) takes the

@@ -8,7 +8,8 @@ numpy's FFT."""
 import numpy as np
 import pytest
 
-SIZES = [100, 200, 300, 384, 400, 500, 600, 768, 800, 1000, 1200, 1500, 1536, 2000, 2400, 3000, 3072, 4000]
+SIZES = [100, 200, 300, 384, 400, 500, 600, 640, 720, 768, 800, 1000, 1200, 1280, 1500, 1536, 1920, 2000, 2160, 2400,
+         2560, 3000, 3072, 3840, 4000]
 
 
 def plan(n):
@@ -115,7 +116,7 @@ def test_mixed_radix_plan_matches_numpy(n):
 # Sides without a compiled plan, and sides with factors of 7, run the
 # runtime-length kernels (k_rows_mixed_rt / k_cols_mixed_rt): same plan,
 # radix-7 stages last, same butterfly maps and tables.
-RT_SIZES = [6, 90, 640, 720, 960, 1080, 1280, 1800, 1920, 2160, 2700, 3240, 3600, 3840, 4050,
+RT_SIZES = [6, 90, 450, 480, 900, 960, 1080, 1152, 1800, 2700, 3240, 3600, 3888, 4050,
             7, 49, 700, 1400, 2100, 2940, 3430]
 
 
