@@ -227,10 +227,15 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
  * captured graph is dropped). The launches propagate the batch's fields:
  * refill them afterwards. shape (may be NULL, else 8 ints) receives
  * {tile_w, col_w, rows, groups, split, radix, col_radix, col_split}; best_ms
- * (may be NULL) the time of one whole run at that shape. */
+ * (may be NULL) the time of one whole run at that shape. On mixed-radix
+ * grids (2^a 3^b 5^c 7^d sides) the choices are the layout tile width
+ * (tile_w: 16, 8, 4, 2, 1 dividing the side) and the columns per column
+ * workgroup (col_w: a built width dividing tile_w), each pair timed as a
+ * whole run; Bluestein grids keep their fixed shape. */
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
 /* Set the launch shape directly (e.g. from a cached autotune table);
- * radix / col_radix 0 mean 16. */
+ * radix / col_radix 0 mean 16. Mixed-radix grids take tile_w and col_w
+ * only (as autotune searches them) and ignore the other fields. */
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
                              int col_radix, int col_split);
 /* capture one run into a hipGraph; later runs replay it */

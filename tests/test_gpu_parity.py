@@ -558,3 +558,48 @@ def test_batch_radix8_schedule(wos, oracle, precision, n, method):
     with pytest.raises(RuntimeError):
         b.set_tuning(4, 4, 0, 0, 1, 8, 16)   # split exchanges are radix-16 row kernels only
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("n,method", [(1000, "asm"), (1080, "asm"), (500, "asm_pad")])
+def test_batch_mixed_radix_tile_widths(wos, oracle, precision, n, method):
+    """On mixed-radix grids (1000 = 8 5 5 5 and 500 padded to 1000 on the
+    compiled plan, 1080 = 8 27 5 on the runtime plan) every layout tile and
+    column workgroup width the side builds gives each unit of a batch run to
+    rounding of the oracle's Propagate, shapes the side does not build are
+    refused, and autotune settles on a built shape."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    tol = FP64_TOL if prec == wos.FP64 else FP32_TOL
+    dx, wls, zs = 2e-6, np.array([450e-9, 633e-9]), np.array([5e-3, 9e-3])
+    rng = np.random.default_rng(n)
+    srcs = [oracle.gaussian(n, dx, n * dx / 6) * np.exp(1j * rng.uniform(-0.3, 0.3, (n, n))) for _ in range(2)]
+    refs = [oracle.propagate(f, None, dx, z, method, oracle.ctx_for(w))[0] for f, w, z in zip(srcs, wls, zs)]
+    b = wos.Batch(n, dx, wls, zs, wos.ctx_for(550e-9), method, prec)
+    built = []
+    for tw in (16, 8, 4, 2, 1):
+        for cw in (tw, tw // 2, tw // 4):
+            if cw < 1:
+                continue
+            try:
+                b.set_tuning(tw, cw)
+            except wos.WosError:
+                continue
+            built.append((tw, cw))
+            for u, f in enumerate(srcs):
+                b.upload(u, f)
+            b.run(1)
+            b.sync()
+            for u in range(2):
+                e = rel_l2(b.download(u), refs[u])
+                print(f"mixed tile n={n} {method} prec={prec} tile {tw} columns {cw} unit {u} rel-L2 {e:.2e}",
+                      flush=True)
+                assert e <= tol
+    assert len(built) >= 4 and any(tw != cw for tw, cw in built), built
+    for bad in ((4, 8), (3, 3), (32, 4)):   # columns wider than the tile, a tile not a power of two, too wide
+        with pytest.raises(wos.WosError):
+            b.set_tuning(*bad)
+    shape, ms = b.autotune(2)
+    print(f"mixed tile n={n} {method} prec={prec}: built {built}, autotune {shape['tile_w']}/{shape['col_w']} "
+          f"({ms:.3f} ms)", flush=True)
+    assert (shape["tile_w"], shape["col_w"]) in built
+    b.close()

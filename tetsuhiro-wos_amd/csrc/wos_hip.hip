@@ -50,6 +50,8 @@ int launch_cols_mixed_f64(int N, int mode, int dir, int W, const ColArgs& a, hip
 int launch_cols_mixed_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int mixed_col_w_f64(int N);
 int mixed_col_w_f32(int N);
+bool mixed_col_valid_f64(int N, int W);
+bool mixed_col_valid_f32(int N, int W);
 int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int launch_cols_bs_f32(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int bs_default_w_f64(int M, int cols);
@@ -426,6 +428,16 @@ static int get_mixed_twiddles(int n, int prec, const void** out) {
 static int mixed_col_w(int prec, int n) {
   if (n <= 1 || pow2(n)) return 0;
   return prec == WOS_FP64 ? mixed_col_w_f64(n) : mixed_col_w_f32(n);
+}
+// whether the mixed-radix column pass of n is built with tile width W
+static bool mixed_col_valid(int prec, int n, int W) {
+  if (mixed_col_w(prec, n) <= 0) return false;
+  return prec == WOS_FP64 ? mixed_col_valid_f64(n, W) : mixed_col_valid_f32(n, W);
+}
+// a mixed-radix shape: layout tiles of tw columns (a power of two up to 16
+// dividing n) and column workgroups of cw columns (a built width dividing tw)
+static bool mixed_shape_valid(int prec, int n, int tw, int cw) {
+  return tw > 0 && tw <= 16 && pow2(tw) && n % tw == 0 && cw > 0 && tw % cw == 0 && mixed_col_valid(prec, n, cw);
 }
 static int launch_rows_mixed(int prec, int N, int dir, const RowArgs& a, hipStream_t s) {
   return prec == WOS_FP64 ? launch_rows_mixed_f64(N, dir, a, s) : launch_rows_mixed_f32(N, dir, a, s);
@@ -1140,9 +1152,14 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     tu.tw = bs_default_w(prec, Mk, M);
     tu.rows = rows_per_block_host(Mk);
     tu.radix = tu.col_radix = 16;
-  } else if (mr) {  // one row per row workgroup; the widest column tile built
+  } else if (mr) {  // one row per row workgroup; the requested shape if built, else the widest column tile for both
+    const int cw = tu.col_w > 0 ? tu.col_w : tu.tw;
+    const bool ok = mixed_shape_valid(prec, M, tu.tw, cw);
+    const int w = mixed_col_w(prec, M);
+    const int t_w = ok ? tu.tw : w, c_w = ok ? cw : w;
     tu = Tuning();
-    tu.tw = tu.col_w = mixed_col_w(prec, M);
+    tu.tw = t_w;
+    tu.col_w = c_w;
     tu.rows = 1;
     tu.radix = tu.col_radix = 16;
   } else {
@@ -1279,7 +1296,7 @@ static void set_transfer_power(TransferLaunch& L, double* pw_in, double* pw_out)
 static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s) {
   if (L.mr) {
     if (k == 0) return launch_rows_mixed(L.prec, L.M, -1, L.fwd, s);
-    if (k == 1) return launch_cols_mixed(L.prec, L.M, 2, 1, L.tu.tw, L.col, s);
+    if (k == 1) return launch_cols_mixed(L.prec, L.M, 2, 1, L.tu.col_w, L.col, s);
     return launch_rows_mixed(L.prec, L.M, 1, L.inv, s);
   }
   if (L.bs) {
@@ -2818,7 +2835,26 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) 
   WOS_HIP_TRY(hipEventCreate(&e1));
   double best = 1e300;
   Tuning bt = b->tune;
-  if (!pow2(b->M)) {  // Bluestein grids run their fixed launch shape
+  if (!pow2(b->M) && mixed_col_w(b->prec, b->M) > 0) {  // mixed radix: layout tile and column tile widths
+    rc = 0;
+    for (int tw : {16, 8, 4, 2, 1}) {
+      for (int cw = tw; cw >= 1 && !rc; cw /= 2) {
+        if (!mixed_shape_valid(b->prec, b->M, tw, cw)) continue;
+        Tuning t;
+        t.tw = tw;
+        t.col_w = cw;
+        b->tune = t;
+        double ms;
+        if ((rc = batch_time_runs(b, iters, e0, e1, &ms))) break;
+        if (ms < best) {
+          best = ms;
+          bt = t;
+        }
+      }
+      if (rc) break;
+    }
+    b->tune = bt;
+  } else if (!pow2(b->M)) {  // Bluestein grids run their fixed launch shape
     rc = batch_time_runs(b, iters, e0, e1, &best);
     bt = Tuning();
   } else {
@@ -2846,6 +2882,19 @@ int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int g
   if (!b) return WOS_ERR_INVALID;
   if (radix == 0) radix = 16;
   if (col_radix == 0) col_radix = 16;
+  if (!pow2(b->M) && mixed_col_w(b->prec, b->M) > 0) {  // mixed radix: the tile widths only
+    if (!mixed_shape_valid(b->prec, b->M, tile_w, col_w)) {
+      set_error("wos_hip_batch_set_tuning: tile width " + std::to_string(tile_w) + " / column width " +
+                std::to_string(col_w) + " not built for the mixed-radix grid of " + std::to_string(b->M));
+      return WOS_ERR_INVALID;
+    }
+    WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+    batch_drop_graph(b);
+    b->tune = Tuning();
+    b->tune.tw = tile_w;
+    b->tune.col_w = col_w;
+    return WOS_OK;
+  }
   if (col_split && col_radix != 16) {
     set_error("wos_hip_batch_set_tuning: split column exchanges are radix-16 column passes only");
     return WOS_ERR_INVALID;

@@ -9,4 +9,5 @@ int launch_cols_mixed_f32(int N, int mode, int dir, int W, const ColArgs& a, hip
   return launch_cols_mixed_T<float>(N, mode, dir, W, a, s);
 }
 int mixed_col_w_f32(int N) { return mixed_col_w_T<float>(N); }
+bool mixed_col_valid_f32(int N, int W) { return mixed_col_valid_T<float>(N, W); }
 }  // namespace wos

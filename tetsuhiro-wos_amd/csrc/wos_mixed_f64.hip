@@ -9,4 +9,5 @@ int launch_cols_mixed_f64(int N, int mode, int dir, int W, const ColArgs& a, hip
   return launch_cols_mixed_T<double>(N, mode, dir, W, a, s);
 }
 int mixed_col_w_f64(int N) { return mixed_col_w_T<double>(N); }
+bool mixed_col_valid_f64(int N, int W) { return mixed_col_valid_T<double>(N, W); }
 }  // namespace wos
