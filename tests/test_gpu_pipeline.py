@@ -241,3 +241,39 @@ def test_capture_before_any_run(wos, precision):
         p.close()
     for a, e in zip(*outs):
         assert np.array_equal(a, e)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pipeline_autotuned_mixed_stages_match_oracle(wos, oracle, precision):
+    """Pipeline autotune on mixed-radix stages (384 = 16 8 3 compiled, 1080 =
+    8 27 5 runtime, 300 padded to 600) searches their layout and column tile
+    widths; the graph replay at the chosen shapes matches the oracle's
+    Propagate (the FP64 and FP32 tolerances)."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    tol = FP64_TOL if prec == wos.FP64 else FP32_TOL
+    ctx = wos.ctx_for(550e-9)
+    p = wos.Pipeline(prec)
+    cases = [(384, 2e-6, "asm", np.array([450e-9, 633e-9]), np.array([3e-3, 6e-3])),
+             (1080, 1.5e-6, "asm", np.array([532e-9]), np.array([4e-3])),
+             (300, 2e-6, "asm_pad", np.array([600e-9, 700e-9]), np.array([2e-3, 5e-3]))]
+    stages = []
+    for k, (n, dx, method, wls, zs) in enumerate(cases):
+        s = p.add(n, dx, wls, zs, ctx, method)
+        for u in range(len(wls)):
+            p.upload(s, u, random_field(n, 70 + 10 * k + u))
+        stages.append(s)
+    ms = p.autotune(2)
+    print(f"pipeline mixed stages autotuned ({precision}): stage ms {np.round(ms, 4).tolist()}", flush=True)
+    for k, (n, dx, method, wls, zs) in enumerate(cases):   # autotune ran the stages: upload again
+        for u in range(len(wls)):
+            p.upload(stages[k], u, random_field(n, 70 + 10 * k + u))
+    p.capture()
+    p.run(1)
+    p.sync()
+    for k, (n, dx, method, wls, zs) in enumerate(cases):
+        for u, (w, z) in enumerate(zip(wls, zs)):
+            r, _, _, _ = oracle.propagate(random_field(n, 70 + 10 * k + u), None, dx, z, method, oracle.ctx_for(w))
+            e = rel_l2(p.download(stages[k], u), r)
+            print(f"pipeline mixed n={n} {method} unit {u} ({precision}) rel-L2 {e:.2e}", flush=True)
+            assert e <= tol
+    p.close()
