@@ -332,6 +332,7 @@ def bench_train(wos, n=2048, dx=2e-6, reps=5):
 
 
 def cpu_timed(fn, reps, warm=1):
+    """Median wall time of `reps` calls after `warm` untimed calls."""
     for _ in range(warm):
         fn()
     ts = []
@@ -343,6 +344,7 @@ def cpu_timed(fn, reps, warm=1):
 
 
 def cpu_model():
+    """The host CPU's model string as /proc/cpuinfo gives it."""
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -353,22 +355,40 @@ def cpu_model():
     return "unknown"
 
 
+def host_threads():
+    """Host threads the CPU baseline runs on, and what they are taken from:
+    every CPU this process may run on (sched_getaffinity), capped by
+    OMP_NUM_THREADS when the environment sets it (a job's CPU share on a
+    shared host); nproc is reported beside it."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = nproc
+    threads, source = usable, "sched_getaffinity"
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and 0 < int(omp) < threads:
+        threads, source = int(omp), "OMP_NUM_THREADS (the job's CPU share)"
+    return {"nproc": nproc, "affinity_cpus": usable, "threads": threads, "threads_from": source,
+            "cpu_model": cpu_model()}
+
+
 def cpu_configs(cfg12, fields, bpm, bpm_check20, adj):
     """Restated reference CPU path (the FP64 oracle) on each configuration:
     the checker for the GPU outputs and the CPU timing beside them (median
-    of the runs after one warm-up; all host threads up to 16, and 8 threads
-    for configuration 1 as in the reference's published 8-core figures)."""
+    of the runs after one warm-up; the host threads of host_threads(), and 8
+    threads for configuration 1 as in the reference's published 8-core
+    figures)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    out = {"cpu_model": cpu_model(), "threads": threads}
+    out = host_threads()
+    threads = out["threads"]
     for name, (ex, ey, n, dx, wl, z, method) in config12_inputs().items():
         O.set_threads(threads)
         ctx = O.ctx_for(wl)
         ref = O.propagate(ex, ey, dx, z, method, ctx)
         refc = np.concatenate([ref[0]] if ey is None else [ref[0], ref[1]])
-        reps = 5 if name == "config1" else 2
-        t = cpu_timed(lambda: O.propagate(ex, ey, dx, z, method, ctx), reps)
+        t = cpu_timed(lambda: O.propagate(ex, ey, dx, z, method, ctx), 5)
         r = {"cpu_ms_per_propagate": round(t * 1e3, 1),
              "rel_l2_fp64_vs_oracle": rel_l2(np.concatenate(fields[(name, "fp64")]), refc),
              "rel_l2_fp32_vs_oracle": rel_l2(np.concatenate(fields[(name, "fp32")]), refc)}
@@ -385,7 +405,7 @@ def cpu_configs(cfg12, fields, bpm, bpm_check20, adj):
     index = np.where(rr < 100e-6, 1.45 - 0.01 * (rr / 100e-6) ** 2, 1.44) + 1e-5j
     ex = np.exp(-(rr / 10e-6) ** 2).astype(np.complex128)
     ctx = O.ctx_for(wl)
-    t = cpu_timed(lambda: O.split_step(ex, None, dx, 5 * 5e-6, 5, index, ctx), 2)
+    t = cpu_timed(lambda: O.split_step(ex, None, dx, 5 * 5e-6, 5, index, ctx), 5)
     ref, _, _ = O.split_step(ex, None, dx, 20 * 5e-6, 20, index, ctx)
     out["config3"] = {"cpu_steps_per_s": round(5 / t, 2), "sample": "5 slices of 5 um",
                       "gpu_speedup_fp64": round(bpm["fp64"]["steps_per_s"] * t / 5, 1),
@@ -397,10 +417,11 @@ def cpu_configs(cfg12, fields, bpm, bpm_check20, adj):
     for n, dx in ((2048, 2e-6), (1024, 2e-6), (384, 4.096e-3 / 384), (1000, 4.096e-3 / 1000), (1536, 4.096e-3 / 1536)):
         f = np.exp(-((np.arange(n) - n / 2) * dx / W0) ** 2)
         f = (f[None, :] * f[:, None]).astype(np.complex128)
-        per[n] = cpu_timed(lambda: O.propagate(f, None, dx, 20e-3, "asm", ctx), 1)
+        per[n] = cpu_timed(lambda: O.propagate(f, None, dx, 20e-3, "asm", ctx), 5 if n <= 1024 else 3)
     total = 64 * per[2048] + 128 * per[1024] + 11 * per[384] + 11 * per[1000] + 10 * per[1536]
     out["config4"] = {"cpu_s_per_224_props": round(total, 2), "cpu_props_per_s": round(224 / total, 2),
-                      "sample": "one propagation per grid size, weighted 64/128/11/11/10"}
+                      "sample": "one propagation per grid size (median of 5, 3 at 1536^2 and 2048^2), "
+                                "weighted 64/128/11/11/10"}
     # configuration 5: one mask's forward + adjoint, times 8
     n, dx, wl, z = 1024, 4e-6, 532e-9, 20e-3
     xx = (np.arange(n) - n / 2) * dx
@@ -411,7 +432,7 @@ def cpu_configs(cfg12, fields, bpm, bpm_check20, adj):
     tg /= tg.sum()
     ctx = O.ctx_for(wl)
     loss0, grad0 = O.mask_loss_grad(adj["theta0"], tg, dx, z, ctx)
-    t = cpu_timed(lambda: O.mask_loss_grad(adj["theta0"], tg, dx, z, ctx), 2)
+    t = cpu_timed(lambda: O.mask_loss_grad(adj["theta0"], tg, dx, z, ctx), 5)
     out["config5"] = {"cpu_ms_forward_backward_8_masks": round(8 * t * 1e3, 1), "sample": "one mask, times 8",
                       "grad_rel_l2_fp64_vs_oracle": rel_l2(adj["grad0"]["fp64"], grad0),
                       "grad_rel_l2_fp32_vs_oracle": rel_l2(adj["grad0"]["fp32"], grad0),
@@ -424,7 +445,7 @@ def cpu_train(g, out64, dx=2e-6):
     the device-resident train)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    O.set_threads(min(16, os.cpu_count() or 1))
+    O.set_threads(host_threads()["threads"])
     shape = O.make_shape("circle", radius=1.5e-3, edge_sigma=5e-6)
     t0 = time.perf_counter()
     f = O.OField(g, dx=dx)
@@ -434,28 +455,38 @@ def cpu_train(g, out64, dx=2e-6):
             "rel_l2_fp64_vs_oracle": rel_l2(np.concatenate(out64), np.concatenate([f.ex, f.ey]))}
 
 
-def cpu_baseline(inp, wl, budget_s=12.0):
-    """Restated reference CPU path (the FP64 oracle: radix-2 FFT over all host
-    threads, serial TF loop and Power sums) on one 2048^2 unit of the same batch."""
+def cpu_baseline(inp, wl, reps=5):
+    """Restated reference CPU path (the FP64 oracle: radix-2 FFT rows and
+    columns over the host threads, serial TF loop and Power sums, as
+    optics/fft.go:91-242 and optics/propagate.go:132-148) on one 2048^2 unit
+    of the same batch: the median of `reps` propagations after one warm-up,
+    on the host threads of host_threads() and on 8 threads (the reference's
+    published figures are for 8 cores, README.md:83-89)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    O.set_threads(threads)
+    ht = host_threads()
+    threads = ht["threads"]
     ctx = O.ctx_for(wl)
-    t = time.perf_counter()
-    out, _, _, _ = O.propagate(inp, None, DX, Z, "asm", ctx)
-    first = time.perf_counter() - t
-    count, t0 = 0, time.perf_counter()
-    while True:
+    O.set_threads(threads)
+    out, _, _, _ = O.propagate(inp, None, DX, Z, "asm", ctx)   # the warm-up, kept as the checker's output
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
         O.propagate(inp, None, DX, Z, "asm", ctx)
-        count += 1
-        if time.perf_counter() - t0 >= budget_s or count >= 50:
-            break
-    el = time.perf_counter() - t0
-    return out, {"value": count / el, "unit": "props/s", "cores": threads, "kind": "port",
-                 "sample": f"{count} FP64 ASM propagations of one 2048x2048 complex128 unit "
-                           f"(unit 0 of the batch) after 1 warm-up ({first:.2f} s), "
-                           f"{threads} threads, {el:.1f} s"}
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    O.set_threads(8)
+    med8 = cpu_timed(lambda: O.propagate(inp, None, DX, Z, "asm", ctx), reps)
+    O.set_threads(threads)
+    return out, dict({"value": round(1 / med, 3), "unit": "props/s", "cores": threads, "kind": "port",
+                      "sample": f"median of {reps} FP64 ASM propagations of one 2048x2048 complex128 unit "
+                                f"(unit 0 of the batch) after 1 warm-up, {threads} threads "
+                                f"(runs {', '.join(f'{x:.3f}' for x in ts)} s)",
+                      "s_per_prop": round(med, 4),
+                      "eight_threads": {"s_per_prop": round(med8, 4), "props_per_s": round(1 / med8, 3),
+                                        "reference_8_cores_s_per_prop": 0.7,
+                                        "reference_source": "README.md:83-89 (~0.7 s at 2048^2, 8 cores)"}},
+                     **ht)
 
 
 def main():
@@ -553,7 +584,6 @@ def main():
     line["element_train_2048"], train_in, train_out64 = bench_train(wos)
     if not args.no_cpu:
         ref, cb = cpu_baseline(r64["unit0_in"], r64["wl0"])
-        cb["cpu_model"] = cpu_model()
         line["cpu_baseline"] = cb
         line["rel_l2_fp64_vs_oracle"] = rel_l2(r64["unit0"], ref)
         line["rel_l2_fp32_vs_oracle"] = rel_l2(r32["unit0"], ref)

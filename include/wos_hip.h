@@ -194,6 +194,36 @@ int wos_hip_volume(const double* ex, const double* ey, int n, double dx, const d
                    int method, const wos_ctx* ctx, int precision, double* out_ex, double* out_ey,
                    wos_warnings* w);
 
+/* ---- partially coherent light (optics/coherence.go) ----
+ * Go's math/rand NormFloat64 stream cannot be reproduced outside Go, so the
+ * Gaussian-Schell realisations take their complex Gaussian noise from a
+ * counter-based generator (SplitMix64 + Box-Muller; realisation r, pixel i
+ * draws counters 2(r*n*n + i) and 2(r*n*n + i) + 1). Everything else follows
+ * the reference: sqrt(max(0, Re FFT(mu))) in FP64, the inverse FFT, the
+ * envelope sqrt(exp(-2 r^2 / w0^2)). A width or coherence <= 0 returns
+ * WOS_ERR_INVALID ("schell: width and coherence must be > 0"); m <= 0 is
+ * taken as 1, as the reference does. */
+
+/* GenerateSchellRealizations (optics/coherence.go:22-81): m realisations of
+ * n x n written to out as m contiguous complex128 planes (FP64 throughout). */
+int wos_hip_schell_realizations(int n, double dx, double w0, double sigma_c, int64_t seed, int m, double* out);
+
+/* AverageIntensity (optics/coherence.go:85-100) over m fields given as m
+ * contiguous complex128 planes of n x n (ey NULL: scalar fields, else the Ey
+ * planes in the same layout): out (n*n float64) sums Intensity(i) in field
+ * order, then multiplies by the rounded reciprocal 1/m. m == 0 writes
+ * nothing (the reference returns nil). */
+int wos_hip_average_intensity(const double* ex, const double* ey, int m, int n, double* out);
+
+/* PropagatePartiallyCoherent (optics/coherence.go:105-116): m Schell
+ * realisations generated on the device, each propagated by `method`
+ * (Propagate, any method, warnings accumulated as the reference's ctx
+ * collects them), and their AverageIntensity in out (n*n float64). The
+ * realisations never leave the device. */
+int wos_hip_propagate_partially_coherent(int n, double dx, double w0, double sigma_c, int64_t seed, int m, double z,
+                                         int method, const wos_ctx* ctx, int precision, double* out,
+                                         wos_warnings* w);
+
 /* ---- device-resident batches (no reference counterpart: batching is new) ----
  * A batch holds n_units independent fields of n x n on the device in
  * `precision`, each propagated by `method` with its own wavelength and
@@ -272,6 +302,11 @@ int wos_hip_pipeline_add(wos_pipeline p, int n, double dx, int method, int units
 /* the stage also writes sum_u weights[u] |U_u|^2 (n*n float64) in unit order,
  * the incoherent sum of PropagatePolychromatic */
 int wos_hip_pipeline_set_intensity(wos_pipeline p, int stage, const double* weights);
+/* the stage writes AverageIntensity of its outputs (optics/coherence.go:85-100):
+ * sum_u |U_u|^2 in unit order times 1/units; with units Schell realisations
+ * (wos_hip_pipeline_fill_schell) of one wavelength and distance the stage is
+ * PropagatePartiallyCoherent (optics/coherence.go:105-116) for asm / asm_pad */
+int wos_hip_pipeline_set_average(wos_pipeline p, int stage);
 /* copy source src_index of a stage from a complex128 host array of n*n */
 int wos_hip_pipeline_upload(wos_pipeline p, int stage, int src_index, const double* ex);
 /* fill a stage's sources with Gaussian-Schell realisations first_realisation,

@@ -262,6 +262,10 @@ def lib():
         L.or_schell_sqrt_spectrum.argtypes = [P, I, D, D]
         L.or_schell_realization.argtypes = [P, P, I, D, D, C.c_uint64, C.c_uint64]
         L.or_compute_stats.argtypes = [P, P, I, D, D, D, D, C.POINTER(PlaneStats)]
+        L.or_average_intensity.argtypes = [P, P, I, C.c_size_t, P]
+        L.or_propagate_partially_coherent.argtypes = [I, D, D, D, C.c_uint64, I, D, I, C.POINTER(Ctx), P,
+                                                      C.POINTER(Warnings)]
+        L.or_propagate_partially_coherent.restype = I
         L.or_apply_lens_at.argtypes = [P, P, I, I, D, D, D, D, D, D]
         L.or_apply_element.argtypes = [P, P, I, I, D, D, C.POINTER(Element)]
         L.or_apply_element.restype = I
@@ -481,6 +485,29 @@ def schell_realization(sqrt_s, dx, w0, seed, r):
     a = np.zeros((n, n), dtype=np.complex128)
     lib().or_schell_realization(_p(a), _p(_c128(sqrt_s)), n, dx, w0, seed, r)
     return a
+
+
+def average_intensity(fields_x, fields_y=None):
+    """AverageIntensity over m fields given as (m, n, n) arrays."""
+    fx = _c128(fields_x)
+    fy = None if fields_y is None else _c128(fields_y)
+    m, n = fx.shape[0], fx.shape[1]
+    out = np.zeros((n, n), dtype=np.float64)
+    lib().or_average_intensity(_p(fx), _p(fy), m, n * n, _p(out))
+    return out
+
+
+def propagate_partially_coherent(n, dx, w0, sigma_c, seed, m, z, method, ctx):
+    """PropagatePartiallyCoherent with the counter-based Schell generator;
+    returns (average intensity, warnings)."""
+    mcode = METHODS[method] if isinstance(method, str) else int(method)
+    out = np.zeros((n, n), dtype=np.float64)
+    w = Warnings()
+    rc = lib().or_propagate_partially_coherent(n, dx, w0, sigma_c, seed, m, z, mcode, C.byref(ctx), _p(out),
+                                               C.byref(w))
+    if rc:
+        raise ValueError(f"or_propagate_partially_coherent: rc {rc}")
+    return out, w
 
 
 def compute_stats(ex, ey, dx, wl, ap=0.0, dist=0.0):

@@ -1500,7 +1500,7 @@ double or_normal(uint64_t seed, uint64_t counter) {
     return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
 }
 
-/* GenerateSchellRealizations, spectrum part (optics/coherence.go:156-179) */
+/* GenerateSchellRealizations, spectrum part (optics/coherence.go:29-53) */
 void or_schell_sqrt_spectrum(or_cplx* s, int n, double dx, double sigma_c) {
     for (int j = 0; j < n; j++) {
         double dy = (j <= n / 2) ? (double)j * dx : (double)(j - n) * dx;
@@ -1519,7 +1519,8 @@ void or_schell_sqrt_spectrum(or_cplx* s, int n, double dx, double sigma_c) {
     }
 }
 
-/* GenerateSchellRealizations, one realisation (optics/coherence.go:181-205);
+/* GenerateSchellRealizations, one realisation: the envelope and the noise
+ * (optics/coherence.go:54-78);
  * noise counter for pixel i of realisation r: (r*n*n + i)*2 (+1 for imag). */
 void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx, double w0,
                            uint64_t seed, uint64_t realization) {
@@ -1540,6 +1541,50 @@ void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx,
             ex[idx] = cmul(ex[idx], cre(sqrt(I)));
         }
     }
+}
+
+/* AverageIntensity (optics/coherence.go:85-100): out[i] accumulates
+ * Intensity(i) of fields 0, 1, ..., m-1 in that order (ey NULL: scalar
+ * fields), then is multiplied by the rounded reciprocal 1/m. Fields are m
+ * contiguous n*n planes. m == 0 writes nothing (the reference returns nil). */
+void or_average_intensity(const or_cplx* ex, const or_cplx* ey, int m, size_t nn, double* out) {
+    if (m <= 0) return;
+    for (size_t i = 0; i < nn; i++) out[i] = 0;
+    for (int k = 0; k < m; k++) {
+        const or_cplx* fx = ex + (size_t)k * nn;
+        const or_cplx* fy = ey ? ey + (size_t)k * nn : NULL;
+        for (size_t i = 0; i < nn; i++) out[i] += intensity_at(fx, fy, i);
+    }
+    double inv = 1 / (double)m;
+    for (size_t i = 0; i < nn; i++) out[i] *= inv;
+}
+
+/* PropagatePartiallyCoherent (optics/coherence.go:105-116): m Schell
+ * realisations (GenerateSchellRealizations, optics/coherence.go:22-28 for
+ * the argument checks and m <= 0 -> 1), each propagated by Propagate, then
+ * AverageIntensity. The realisations are generated and propagated one at a
+ * time and accumulated in realisation order, which is the order
+ * AverageIntensity sums them in. Returns 2 on invalid arguments. */
+int or_propagate_partially_coherent(int n, double dx, double w0, double sigma_c, uint64_t seed, int m, double z,
+                                    int method, const or_ctx* ctx, double* out, or_warnings* w) {
+    if (!(w0 > 0) || !(sigma_c > 0) || n <= 0) return 2;
+    if (m <= 0) m = 1;
+    size_t nn = (size_t)n * n;
+    or_cplx* sq = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    or_cplx* f = (or_cplx*)malloc(sizeof(or_cplx) * nn);
+    or_schell_sqrt_spectrum(sq, n, dx, sigma_c);
+    for (size_t i = 0; i < nn; i++) out[i] = 0;
+    int rc = 0;
+    for (int k = 0; k < m; k++) {
+        or_schell_realization(f, sq, n, dx, w0, seed, (uint64_t)k);
+        if ((rc = or_propagate(f, NULL, n, dx, z, method, ctx, w, NULL))) break;
+        for (size_t i = 0; i < nn; i++) out[i] += intensity_at(f, NULL, i);
+    }
+    double inv = 1 / (double)m;
+    for (size_t i = 0; i < nn; i++) out[i] *= inv;
+    free(sq);
+    free(f);
+    return rc;
 }
 
 /* ComputeStats (optics/metrics.go:31-91). */

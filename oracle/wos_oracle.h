@@ -16,6 +16,7 @@
 #ifndef WOS_ORACLE_H
 #define WOS_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -162,6 +163,10 @@ double or_normal(uint64_t seed, uint64_t counter);
 void or_schell_sqrt_spectrum(or_cplx* sqrt_s, int n, double dx, double sigma_c);
 void or_schell_realization(or_cplx* ex, const or_cplx* sqrt_s, int n, double dx,
                            double w0, uint64_t seed, uint64_t realization);
+void or_average_intensity(const or_cplx* ex, const or_cplx* ey, int m, size_t nn, double* out); /* AverageIntensity */
+int or_propagate_partially_coherent(int n, double dx, double w0, double sigma_c, uint64_t seed, int m, double z,
+                                    int method, const or_ctx* ctx, double* out,
+                                    or_warnings* w);                    /* PropagatePartiallyCoherent */
 
 /* optics/metrics.go:31-91: PlaneStats (field order of the Go struct). */
 typedef struct {

@@ -561,13 +561,16 @@ def test_batch_radix8_schedule(wos, oracle, precision, n, method):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("n,method", [(1000, "asm"), (1080, "asm"), (500, "asm_pad")])
+@pytest.mark.parametrize("n,method", [(1000, "asm"), (1080, "asm"), (500, "asm_pad"), (1000, "asm_pad"),
+                                      (1536, "asm")])
 def test_batch_mixed_radix_tile_widths(wos, oracle, precision, n, method):
     """On mixed-radix grids (1000 = 8 5 5 5 and 500 padded to 1000 on the
-    compiled plan, 1080 = 8 27 5 on the runtime plan) every layout tile and
-    column workgroup width the side builds gives each unit of a batch run to
-    rounding of the oracle's Propagate, shapes the side does not build are
-    refused, and autotune settles on a built shape."""
+    compiled plan, 1080 = 8 27 5 on the runtime plan, and the sides with a
+    factor 16, where 16-column layout tiles build: 1000 padded to 2000 =
+    16 125 and 1536 = 16 96) every layout tile and column workgroup width
+    the side builds (every power of two up to the tile) gives each unit of a
+    batch run to rounding of the oracle's Propagate, shapes the side does
+    not build are refused, and autotune settles on a built shape."""
     prec = wos.FP64 if precision == "fp64" else wos.FP32
     tol = FP64_TOL if prec == wos.FP64 else FP32_TOL
     dx, wls, zs = 2e-6, np.array([450e-9, 633e-9]), np.array([5e-3, 9e-3])
@@ -577,9 +580,7 @@ def test_batch_mixed_radix_tile_widths(wos, oracle, precision, n, method):
     b = wos.Batch(n, dx, wls, zs, wos.ctx_for(550e-9), method, prec)
     built = []
     for tw in (16, 8, 4, 2, 1):
-        for cw in (tw, tw // 2, tw // 4):
-            if cw < 1:
-                continue
+        for cw in (c for c in (16, 8, 4, 2, 1) if c <= tw):
             try:
                 b.set_tuning(tw, cw)
             except wos.WosError:
@@ -595,6 +596,8 @@ def test_batch_mixed_radix_tile_widths(wos, oracle, precision, n, method):
                       flush=True)
                 assert e <= tol
     assert len(built) >= 4 and any(tw != cw for tw, cw in built), built
+    if (n if method == "asm" else 2 * n) % 16 == 0:
+        assert any(tw == 16 for tw, _ in built), built
     for bad in ((4, 8), (3, 3), (32, 4)):   # columns wider than the tile, a tile not a power of two, too wide
         with pytest.raises(wos.WosError):
             b.set_tuning(*bad)

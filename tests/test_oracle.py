@@ -320,6 +320,81 @@ def test_schell_coherence(oracle):
         assert abs(got - math.exp(-d * d / (2 * sigc * sigc))) <= 0.03
 
 
+def far_field_radius(I, n, dx_out):
+    """farFieldRadius (optics/coherence_test.go:67-128): 1/e^2 radius from a
+    weighted log-linear fit of the radial profile above 2% of its peak."""
+    nb = 128
+    dr = dx_out * n / 2 / nb
+    c = (np.arange(n) - n / 2) * dx_out
+    r = np.hypot(c[None, :], c[:, None])
+    b = (r / dr).astype(int)
+    keep = b < nb
+    s = np.bincount(b[keep], weights=I[keep], minlength=nb)
+    cnt = np.bincount(b[keep], minlength=nb)
+    ok = cnt > 0
+    v = np.where(ok, s / np.maximum(cnt, 1), 0.0)
+    use = ok & (v >= 0.02 * v[ok].max())
+    r2 = ((np.arange(nb) + 0.5) * dr) ** 2
+    w, lv = cnt[use].astype(float), np.log(v[use])
+    sw, sx, sy, sxx, sxy = w.sum(), (w * r2[use]).sum(), (w * lv).sum(), (w * r2[use] ** 2).sum(), \
+        (w * r2[use] * lv).sum()
+    slope = (sw * sxy - sx * sy) / (sw * sxx - sx * sx)
+    return math.sqrt(-2 / slope)
+
+
+def test_schell_far_field(oracle):
+    """TestSchellFarField (optics/coherence_test.go:47-65):
+    PropagatePartiallyCoherent of 160 realisations with the Fraunhofer
+    method; the far-field 1/e^2 radius is (wl z / pi) sqrt(1/w0^2 +
+    1/sigma_c^2) within 10%."""
+    n, dx, w0, sigc, wl, z = 512, 1.25e-5, 1e-3, 3.75e-5, 632.8e-9, 1.0
+    I, _ = oracle.propagate_partially_coherent(n, dx, w0, sigc, 2, 160, z, "fraunhofer", oracle.ctx_for(wl))
+    got = far_field_radius(I, n, wl * z / (n * dx))
+    want = (wl * z / math.pi) * math.sqrt(1 / w0 ** 2 + 1 / sigc ** 2)
+    assert abs(got - want) / want <= 0.1, (got, want)
+
+
+def test_average_intensity_closed_forms(oracle):
+    """AverageIntensity (optics/coherence.go:85-100): m copies of one field
+    average to its intensity (to the rounding of 1/m); two fields to
+    (I1 + I2) / 2 exactly; doubling every field quadruples the result
+    exactly; the Ey component counts for polarised fields."""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+    b = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+    ia, ib = a.real ** 2 + a.imag ** 2, b.real ** 2 + b.imag ** 2
+    for m in (1, 3, 7):
+        got = oracle.average_intensity(np.stack([a] * m))
+        assert np.max(np.abs(got - ia) / ia) <= 4e-16 * m
+    assert np.array_equal(oracle.average_intensity(np.stack([a, b])), (ia + ib) / 2)
+    assert np.array_equal(oracle.average_intensity(np.stack([2 * a, 2 * b, a])),
+                          4 * oracle.average_intensity(np.stack([a, b, a / 2])))
+    got = oracle.average_intensity(np.stack([a, a]), np.stack([b, b]))
+    assert np.max(np.abs(got - (ia + ib)) / (ia + ib)) <= 4e-16
+
+
+def test_partially_coherent_reduces_to_its_parts(oracle):
+    """PropagatePartiallyCoherent (optics/coherence.go:105-116) with one
+    realisation is the intensity of Propagate applied to that realisation;
+    with z = 0 it is AverageIntensity of the realisations themselves; m <= 0
+    is taken as 1; a width or coherence <= 0 is refused."""
+    n, dx, w0, sc, seed, wl = 64, 4e-6, 60e-6, 15e-6, 9, 532e-9
+    ctx = oracle.ctx_for(wl)
+    sq = oracle.schell_sqrt_spectrum(n, dx, sc)
+    r0 = oracle.schell_realization(sq, dx, w0, seed, 0)
+    u, _, _, _ = oracle.propagate(r0, None, dx, 2e-3, "asm", ctx)
+    got, _ = oracle.propagate_partially_coherent(n, dx, w0, sc, seed, 1, 2e-3, "asm", ctx)
+    assert np.array_equal(got, u.real ** 2 + u.imag ** 2)
+    got0, _ = oracle.propagate_partially_coherent(n, dx, w0, sc, seed, 0, 2e-3, "asm", ctx)
+    assert np.array_equal(got0, got)
+    fields = np.stack([oracle.schell_realization(sq, dx, w0, seed, k) for k in range(5)])
+    got, _ = oracle.propagate_partially_coherent(n, dx, w0, sc, seed, 5, 0.0, "asm", ctx)
+    assert np.array_equal(got, oracle.average_intensity(fields))
+    for bad in ((0.0, sc), (w0, -1.0)):
+        with pytest.raises(ValueError):
+            oracle.propagate_partially_coherent(n, dx, bad[0], bad[1], seed, 2, 1e-3, "asm", ctx)
+
+
 def test_normal_generator_moments(oracle):
     L = oracle.lib()
     v = np.array([L.or_normal(7, c) for c in range(20000)])

@@ -104,6 +104,45 @@ def test_aperture_outlines_cross_checks(oracle):
     assert np.max(np.abs(t - want)) < 1e-14
 
 
+# SplitMix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number
+# generators", OOPSLA 2014; Vigna's splitmix64.c): published outputs 1..5 of
+# the generator started from state 0 and from state 1234567.
+SPLITMIX64_FROM_0 = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC,
+                     0x1B39896A51A8749B]
+SPLITMIX64_FROM_1234567 = [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+                           16408922859458223821]
+
+
+def test_hash_noise01_is_splitmix64(oracle):
+    """hashNoise01 (optics/elements.go:710-721) hashes
+    h = seed*G ^ x*P2 ^ y*P5 (mod 2^64) with SplitMix64's output function,
+    G = 0x9E3779B97F4A7C15 being SplitMix64's state increment. Output k of
+    SplitMix64 from state s0 is that function of s0 + k*G, so inputs that
+    make h equal s0 + k*G must reproduce the published outputs in the top 53
+    bits (the value is (h >> 11) / 2^53, exact in FP64). The x and y terms
+    are reached through the inverses of their odd multipliers, and the
+    exclusive-or of the terms through seed and x together."""
+    M = 1 << 64
+    G, P2, P5 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9
+    inv = lambda a: pow(a, -1, M)
+    s64 = lambda v: v - M if v >= 1 << 63 else v
+
+    def top53(x, y, seed):
+        v = oracle.hash_noise01(s64(x % M), s64(y % M), s64(seed % M))
+        assert 0.0 <= v < 1.0
+        return int(v * 2.0 ** 53)
+
+    for k, want in enumerate(SPLITMIX64_FROM_0, start=1):
+        assert top53(0, 0, k) == want >> 11, k
+        assert top53(k * G * inv(P2), 0, 0) == want >> 11, k
+        assert top53(0, k * G * inv(P5), 0) == want >> 11, k
+    for k, want in enumerate(SPLITMIX64_FROM_1234567, start=1):
+        state = (1234567 + k * G) % M
+        assert top53(0, 0, state * inv(G)) == want >> 11, k
+        assert top53((state ^ (k * G % M)) * inv(P2), 0, k) == want >> 11, k
+        assert top53(0, (state ^ (k * G % M)) * inv(P5), k) == want >> 11, k
+
+
 def test_diffuser_statistics_anchoring_and_correlation(oracle):
     n, dx = 128, 1e-6
     rng = np.random.default_rng(2)

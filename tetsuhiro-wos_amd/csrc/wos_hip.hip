@@ -880,7 +880,7 @@ __device__ __forceinline__ double normal_at(uint64_t seed, uint64_t counter) {
 }
 
 // Degree of coherence mu on the wrapped difference grid
-// (optics/coherence.go:156-172), FP64.
+// (optics/coherence.go:29-45), FP64.
 __global__ void k_schell_mu(cpx<double>* __restrict__ s, int n, double dx, double sigma_c) {
   const size_t nn = (size_t)n * n;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
@@ -898,7 +898,7 @@ __global__ void k_schell_sqrt(cpx<double>* __restrict__ s, size_t nn) {
     s[i] = {sqrt(v), 0.0};
   }
 }
-// complex Gaussian noise / sqrt(2) times sqrt(S) (optics/coherence.go:195-200)
+// complex Gaussian noise / sqrt(2) times sqrt(S) (optics/coherence.go:63-73)
 __global__ void k_schell_noise(cpx<double>* __restrict__ out, const cpx<double>* __restrict__ sq, size_t nn,
                                uint64_t seed, uint64_t realization) {
   const double inv_sqrt2 = 0.70710678118654752440;
@@ -909,7 +909,7 @@ __global__ void k_schell_noise(cpx<double>* __restrict__ out, const cpx<double>*
     out[i] = cmul(v, sq[i]);
   }
 }
-// times sqrt(I), I = exp(-2 r^2 / w0^2) (optics/coherence.go:182-189, 202-204), stored in R
+// times sqrt(I), I = exp(-2 r^2 / w0^2) (optics/coherence.go:54-62, 74-77), stored in R
 template <typename R>
 __global__ void k_schell_envelope(const cpx<double>* __restrict__ in, cpx<R>* __restrict__ out, int n, double dx,
                                   double w0) {
@@ -2378,6 +2378,13 @@ int wos_hip_split_step(double* ex, double* ey, int n, double dx, double z, int s
   }
   const int ncomp = ey ? 2 : 1;
   const size_t nn = (size_t)n * n;
+  // a z-invariant medium sampled once per slice: every plane repeats the
+  // first bit for bit, so one screen table serves every slice
+  if (n_planes > 1) {
+    int p = 1;
+    while (p < n_planes && !memcmp(index, index + (size_t)p * nn * 2, nn * 16)) ++p;
+    if (p == n_planes) n_planes = 1;
+  }
   const double* src[2] = {ex, ey};
   double* dst[2] = {ex, ey};
   void* d;
@@ -2470,6 +2477,43 @@ int wos_hip_volume(const double* ex, const double* ey, int n, double dx, const d
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
+// sqrt(max(0, Re FFT(mu))) of GenerateSchellRealizations (optics/coherence.go:29-53)
+// into sq (n x n complex128), in FP64 on the device stream.
+static int schell_spectrum(int n, double dx, double sigma_c, void* sq) {
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  hipLaunchKernelGGL(k_schell_mu, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)sq, n, dx, sigma_c);
+  WOS_HIP_TRY(hipGetLastError());
+  int rc;
+  if ((rc = fft2d_dev(WOS_FP64, sq, n, 1, 0))) return rc;
+  hipLaunchKernelGGL(k_schell_sqrt, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)sq, nn);
+  WOS_HIP_TRY(hipGetLastError());
+  return WOS_OK;
+}
+
+// Realisation r of `seed` (optics/coherence.go:54-78 with the counter-based
+// normal generator): noise times sq, inverse FFT, envelope, generated in FP64
+// in `work` (n x n complex128) and stored in precision prec at dst; stream
+// ordered.
+static int schell_unit(int prec, int n, double dx, double w0, uint64_t seed, uint64_t r, const void* sq, void* work,
+                       void* dst) {
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  hipLaunchKernelGGL(k_schell_noise, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)work,
+                     (const cpx<double>*)sq, nn, seed, r);
+  WOS_HIP_TRY(hipGetLastError());
+  int rc;
+  if ((rc = fft2d_dev(WOS_FP64, work, n, 1, 1))) return rc;
+  if (prec == WOS_FP64)
+    hipLaunchKernelGGL(k_schell_envelope<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                       (const cpx<double>*)work, (cpx<double>*)dst, n, dx, w0);
+  else
+    hipLaunchKernelGGL(k_schell_envelope<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                       (const cpx<double>*)work, (cpx<float>*)dst, n, dx, w0);
+  WOS_HIP_TRY(hipGetLastError());
+  return WOS_OK;
+}
+
 // Gaussian-Schell realisations u0 .. u0+count-1 of `seed` (optics/coherence.go:22-80
 // with the counter-based normal generator) into dst (count fields of n x n,
 // precision prec), generated in FP64 on the device stream; synchronous.
@@ -2480,27 +2524,134 @@ static int schell_fill(int prec, int n, double dx, double w0, double sigma_c, ui
   int rc;
   if ((rc = dv.aux.ensure(nn * 16))) return rc;    // sqrt(S)
   if ((rc = dv.work.ensure(nn * 16))) return rc;   // one realisation
-  hipLaunchKernelGGL(k_schell_mu, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, n, dx, sigma_c);
-  WOS_HIP_TRY(hipGetLastError());
-  if ((rc = fft2d_dev(WOS_FP64, dv.aux.p, n, 1, 0))) return rc;
-  hipLaunchKernelGGL(k_schell_sqrt, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.aux.p, nn);
-  WOS_HIP_TRY(hipGetLastError());
-  for (int u = 0; u < count; ++u) {
-    hipLaunchKernelGGL(k_schell_noise, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (cpx<double>*)dv.work.p,
-                       (const cpx<double>*)dv.aux.p, nn, seed, (uint64_t)(u0 + u));
-    WOS_HIP_TRY(hipGetLastError());
-    if ((rc = fft2d_dev(WOS_FP64, dv.work.p, n, 1, 1))) return rc;
-    if (prec == WOS_FP64)
-      hipLaunchKernelGGL(k_schell_envelope<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
-                         (const cpx<double>*)dv.work.p, (cpx<double>*)dst + u * nn, n, dx, w0);
-    else
-      hipLaunchKernelGGL(k_schell_envelope<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
-                         (const cpx<double>*)dv.work.p, (cpx<float>*)dst + u * nn, n, dx, w0);
-    WOS_HIP_TRY(hipGetLastError());
-  }
+  if ((rc = schell_spectrum(n, dx, sigma_c, dv.aux.p))) return rc;
+  for (int u = 0; u < count; ++u)
+    if ((rc = schell_unit(prec, n, dx, w0, seed, (uint64_t)(u0 + u), dv.aux.p,
+                          dv.work.p, (char*)dst + (size_t)u * nn * csize(prec))))
+      return rc;
   WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
   return WOS_OK;
 }
+
+// Device memory owned by one call, freed when the call returns.
+struct CallBuf {
+  void* p = nullptr;
+  ~CallBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? WOS_ERR_NOMEM : WOS_ERR_HIP;
+    }
+    return WOS_OK;
+  }
+};
+
+__global__ void k_scale_f64(double* __restrict__ a, size_t count, double s) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    a[i] *= s;
+}
+
+extern "C" {
+
+// GenerateSchellRealizations (optics/coherence.go:22-81)
+int wos_hip_schell_realizations(int n, double dx, double w0, double sigma_c, int64_t seed, int m, double* out) {
+  WOS_ENTER();
+  if (n <= 0 || !(dx > 0) || !out) {
+    set_error("wos_hip_schell_realizations: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (!(w0 > 0) || !(sigma_c > 0)) {
+    set_error("schell: width and coherence must be > 0");
+    return WOS_ERR_INVALID;
+  }
+  if (m <= 0) m = 1;
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  CallBuf sq, work, f;
+  int rc;
+  if ((rc = sq.alloc(nn * 16)) || (rc = work.alloc(nn * 16)) || (rc = f.alloc(nn * 16))) return rc;
+  if ((rc = schell_spectrum(n, dx, sigma_c, sq.p))) return rc;
+  for (int k = 0; k < m; ++k) {
+    if ((rc = schell_unit(WOS_FP64, n, dx, w0, (uint64_t)seed, (uint64_t)k, sq.p, work.p, f.p))) return rc;
+    if ((rc = dev_to_host(out + (size_t)k * nn * 2, f.p, nn * 16, dv.stream))) return rc;
+  }
+  return WOS_OK;
+}
+
+// AverageIntensity (optics/coherence.go:85-100)
+int wos_hip_average_intensity(const double* ex, const double* ey, int m, int n, double* out) {
+  WOS_ENTER();
+  if (n <= 0 || m < 0 || (m > 0 && (!ex || !out))) {
+    set_error("wos_hip_average_intensity: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (m == 0) return WOS_OK;   // the reference returns nil: nothing is written
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  const int ncomp = ey ? 2 : 1;
+  CallBuf f, acc;
+  int rc;
+  if ((rc = f.alloc((size_t)ncomp * nn * 16)) || (rc = acc.alloc(nn * sizeof(double)))) return rc;
+  WOS_HIP_TRY(hipMemsetAsync(acc.p, 0, nn * sizeof(double), dv.stream));
+  for (int k = 0; k < m; ++k) {
+    if ((rc = host_to_dev(f.p, ex + (size_t)k * nn * 2, nn * 16, dv.stream))) return rc;
+    if (ey && (rc = host_to_dev((char*)f.p + nn * 16, ey + (size_t)k * nn * 2, nn * 16, dv.stream))) return rc;
+    hipLaunchKernelGGL(k_accum_intensity<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                       (const cpx<double>*)f.p, ncomp, nn, 1.0, (double*)acc.p);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_scale_f64, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (double*)acc.p, nn,
+                     1 / (double)m);
+  WOS_HIP_TRY(hipGetLastError());
+  return dev_to_host(out, acc.p, nn * sizeof(double), dv.stream);
+}
+
+// PropagatePartiallyCoherent (optics/coherence.go:105-116)
+int wos_hip_propagate_partially_coherent(int n, double dx, double w0, double sigma_c, int64_t seed, int m, double z,
+                                         int method, const wos_ctx* ctx, int precision, double* out,
+                                         wos_warnings* w) {
+  WOS_ENTER();
+  if (n <= 0 || !(dx > 0) || !ctx || !out || !valid_prec(precision) || method < WOS_METHOD_ASM ||
+      method > WOS_METHOD_FRAUNHOFER) {
+    set_error("wos_hip_propagate_partially_coherent: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  if (!(w0 > 0) || !(sigma_c > 0)) {
+    set_error("schell: width and coherence must be > 0");
+    return WOS_ERR_INVALID;
+  }
+  if (m <= 0) m = 1;
+  Device& dv = device();
+  const size_t nn = (size_t)n * n;
+  CallBuf sq, work, f, acc;
+  int rc;
+  if ((rc = sq.alloc(nn * 16)) || (rc = work.alloc(nn * 16)) || (rc = f.alloc(nn * csize(precision))) ||
+      (rc = acc.alloc(nn * sizeof(double))))
+    return rc;
+  if ((rc = schell_spectrum(n, dx, sigma_c, sq.p))) return rc;
+  WOS_HIP_TRY(hipMemsetAsync(acc.p, 0, nn * sizeof(double), dv.stream));
+  for (int k = 0; k < m; ++k) {
+    if ((rc = schell_unit(precision, n, dx, w0, (uint64_t)seed, (uint64_t)k, sq.p, work.p, f.p))) return rc;
+    if ((rc = propagate_dev(precision, f.p, n, 1, dx, z, method, *ctx, w, nullptr))) return rc;
+    if (precision == WOS_FP64)
+      hipLaunchKernelGGL(k_accum_intensity<double>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<double>*)f.p, 1, nn, 1.0, (double*)acc.p);
+    else
+      hipLaunchKernelGGL(k_accum_intensity<float>, dim3(grid_for(nn)), dim3(256), 0, dv.stream,
+                         (const cpx<float>*)f.p, 1, nn, 1.0, (double*)acc.p);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_scale_f64, dim3(grid_for(nn)), dim3(256), 0, dv.stream, (double*)acc.p, nn,
+                     1 / (double)m);
+  WOS_HIP_TRY(hipGetLastError());
+  return dev_to_host(out, acc.p, nn * sizeof(double), dv.stream);
+}
+
+}  // extern "C"
 
 struct wos_batch_s {
   int n = 0, M = 0, method = 0, prec = 0, units = 0;

@@ -3,7 +3,8 @@
 This is the host-side mirror of the reference's optics operators for the
 propagation path: fft2d/fft1d stand for fft2D/fft1DAny (optics/fft.go:135,
 209), propagate for Propagate (optics/propagate.go:65), split_step for
-PropagateSplitStep (optics/medium.go:22), polychromatic for
+PropagateSplitStep (optics/medium.go:22; propagate_split_step takes its
+IndexFunc), polychromatic for
 PropagatePolychromatic (optics/polychromatic.go:18), volume for Propagate3D
 (optics/volume.go:8) and propagate1d for Propagate1D
 (optics/propagate.go:718). Errors raise WosError; there is no CPU fallback:
@@ -45,7 +46,8 @@ EXPORTS = [
     "wos_hip_field_lens", "wos_hip_field_aperture", "wos_hip_field_diffuser", "wos_hip_field_jones",
     "wos_hip_field_scale", "wos_hip_field_bandlimit", "wos_hip_field_stats", "wos_hip_field_dx",
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
-    "wos_hip_propagate_anisotropic", "wos_hip_field_anisotropic",
+    "wos_hip_propagate_anisotropic", "wos_hip_field_anisotropic", "wos_hip_schell_realizations",
+    "wos_hip_average_intensity", "wos_hip_propagate_partially_coherent", "wos_hip_pipeline_set_average",
 ]
 
 VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity", "pol_azimuth", "pol_ellip",
@@ -269,6 +271,11 @@ def load(path=None):
     L.wos_hip_field_anisotropic.argtypes = [P, D, P, D]
     L.wos_hip_field_dx.argtypes = [P]
     L.wos_hip_field_dx.restype = D
+    L.wos_hip_schell_realizations.argtypes = [I, D, D, D, C.c_int64, I, P]
+    L.wos_hip_average_intensity.argtypes = [P, P, I, I, P]
+    L.wos_hip_propagate_partially_coherent.argtypes = [I, D, D, D, C.c_int64, I, D, I, C.POINTER(Ctx), I, P,
+                                                       C.POINTER(Warnings)]
+    L.wos_hip_pipeline_set_average.argtypes = [P, I]
     _lib = L
     return L
 
@@ -517,6 +524,49 @@ def split_step(ex, ey, dx, z, steps, index, ctx, precision=FP64):
     return ex, ey, w
 
 
+def sample_index_planes(index_fn, n, dx, z, steps):
+    """The IndexFunc of PropagateSplitStep (optics/medium.go:22-45) sampled
+    as applyMediumPhase reads it (optics/medium.go:50-65): slice s at
+    zc = z*s/steps + dz/2, pixel (i, j) at (X(i), Y(j)) with
+    X(i) = (i - N/2) dx. index_fn(x, y, zc) takes broadcast x (1, n) and
+    y (n, 1) arrays and returns the complex index. Planes are compared with
+    the first as they are sampled; a z-invariant medium (every plane equal
+    bit for bit) returns the one plane. Returns a (planes, n, n) complex128
+    array with planes 1 or steps."""
+    if steps <= 0:
+        steps = 1
+    dz = z / steps
+    c = (np.arange(n) - n / 2) * dx
+    x, y = c[None, :], c[:, None]
+
+    def plane(s):
+        zc = z * s / steps + dz / 2
+        return np.ascontiguousarray(np.broadcast_to(index_fn(x, y, zc), (n, n)), dtype=np.complex128)
+
+    first = plane(0)
+    planes = None
+    for s in range(1, steps):
+        p = plane(s)
+        if planes is None and not np.array_equal(p.view(np.float64), first.view(np.float64)):
+            planes = np.empty((steps, n, n), dtype=np.complex128)
+            planes[:s] = first
+        if planes is not None:
+            planes[s] = p
+    return first[None] if planes is None else planes
+
+
+def propagate_split_step(ex, ey, dx, z, index_fn, steps, ctx, precision=FP64):
+    """PropagateSplitStep(f, z, n, steps, ctx) (optics/medium.go:22) with the
+    reference's arguments: the IndexFunc is sampled on the host
+    (sample_index_planes) and the planes go to wos_hip_split_step. Returns
+    (ex, ey, warnings) on copies."""
+    if z == 0:
+        return _c128(ex).copy(), None if ey is None else _c128(ey).copy(), Warnings()
+    steps = steps if steps > 0 else 1
+    planes = sample_index_planes(index_fn, _c128(ex).shape[0], dx, z, steps)
+    return split_step(ex, ey, dx, z, steps, planes[0] if planes.shape[0] == 1 else planes, ctx, precision)
+
+
 def split_step_set_shape(precision, n, tile_w, rows=0, radix=0, col_radix=None):
     """Launch shape of the split-step slices (tile_w = 0: built-in choice);
     col_radix None: the row passes' radix."""
@@ -550,6 +600,35 @@ def volume(ex, ey, dx, zs, method, ctx, precision=FP64):
     _check(load().wos_hip_volume(_p(ex), _p(ey), n, dx, _p(zs), zs.size, m, C.byref(ctx), precision,
                                  _p(ox), _p(oy), C.byref(w)), "volume")
     return ox, oy, w
+
+
+def schell_realizations(n, dx, w0, sigma_c, seed, m):
+    """GenerateSchellRealizations on the device: (m, n, n) complex128."""
+    out = np.zeros((max(m, 1), n, n), dtype=np.complex128)
+    _check(load().wos_hip_schell_realizations(n, dx, w0, sigma_c, seed, m, _p(out)), "schell_realizations")
+    return out
+
+
+def average_intensity(fields_x, fields_y=None):
+    """AverageIntensity of m fields given as (m, n, n) arrays (None for m = 0)."""
+    fx = _c128(fields_x)
+    if fx.shape[0] == 0:
+        return None
+    fy = None if fields_y is None else _c128(fields_y)
+    out = np.zeros(fx.shape[1:], dtype=np.float64)
+    _check(load().wos_hip_average_intensity(_p(fx), _p(fy), fx.shape[0], fx.shape[1], _p(out)), "average_intensity")
+    return out
+
+
+def propagate_partially_coherent(n, dx, w0, sigma_c, seed, m, z, method, ctx, precision=FP64):
+    """PropagatePartiallyCoherent on the device; returns (intensity, warnings)."""
+    mcode = METHODS[method] if isinstance(method, str) else int(method)
+    out = np.zeros((n, n), dtype=np.float64)
+    w = Warnings()
+    _check(load().wos_hip_propagate_partially_coherent(n, dx, w0, sigma_c, seed, m, z, mcode, C.byref(ctx),
+                                                       precision, _p(out), C.byref(w)),
+           "propagate_partially_coherent")
+    return out, w
 
 
 class Batch:
@@ -711,6 +790,10 @@ class Pipeline:
     def set_intensity(self, stage, weights):
         w = np.ascontiguousarray(weights, dtype=np.float64)
         _check(load().wos_hip_pipeline_set_intensity(self.h, stage, _p(w)), "pipeline_set_intensity")
+
+    def set_average(self, stage):
+        """AverageIntensity of the stage's outputs (sum of |U_u|^2, times 1/units)."""
+        _check(load().wos_hip_pipeline_set_average(self.h, stage), "pipeline_set_average")
 
     def upload(self, stage, src_index, ex):
         ex = _c128(ex)
