@@ -224,6 +224,22 @@ int wos_hip_propagate_partially_coherent(int n, double dx, double w0, double sig
                                          int method, const wos_ctx* ctx, int precision, double* out,
                                          wos_warnings* w);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): Propagate by asm or asm_pad (M = n or 2n a power of two
+ * up to 8192) of a device-resident field, in place: `field` holds ncomp
+ * (1 or 2) components of n x n, contiguous, in complex128 (WOS_FP64) or
+ * complex64 (WOS_FP32). The kernels are those of the resident-field
+ * Propagate (wos_hip_field_propagate gives the same values); every buffer
+ * besides the field lives in the library's static per-device scratch of
+ * 72 MiB (enough for one 2048^2 FP64 or two 2048^2 FP32 components); larger
+ * calls return WOS_ERR_UNSUPPORTED with the size needed. No warnings or
+ * power sums are produced. `stream` is a hipStream_t (NULL: the library's
+ * stream); the call only enqueues work. Call wos_hip_init once beforehand.
+ * Calls that overlap on one GPU share its scratch and race. */
+int wos_hip_propagate_device_noalloc(void* field, int n, int ncomp, double dx, double z, int method,
+                                     const wos_ctx* ctx, int precision, void* stream);
+
 /* ---- device-resident batches (no reference counterpart: batching is new) ----
  * A batch holds n_units independent fields of n x n on the device in
  * `precision`, each propagated by `method` with its own wavelength and

@@ -57,6 +57,8 @@ struct Device {
   bool fir_ok = false;
   int fir_prec = 0, fir_m = 0;
   double fir_dx = 0, fir_wl = 0, fir_z = 0;
+  void* noalloc = nullptr;                 // this device's static scratch (wos_hip_propagate_device_noalloc)
+  size_t noalloc_bytes = 0;
   void* host_pin = nullptr;                // pinned staging for parameter uploads
   size_t host_pin_cap = 0;
   hipEvent_t pin_ev = nullptr;             // recorded after the last copy out of host_pin

@@ -22,6 +22,14 @@
 #include "wos_engine.hpp"
 #include "wos_bluestein.hpp"
 
+// Device memory of wos_hip_propagate_device_noalloc (callers that cannot
+// allocate): the TF records, twiddle tables and column-tiled intermediate of
+// one call live here. Each GPU holds its own copy for the life of the
+// process; host code uses the address hipGetSymbolAddress returns for the
+// current device (Device::noalloc), never the array's name.
+constexpr size_t kNoAllocScratchBytes = (size_t)72 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace wos {
 
 int launch_rows_f64(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
@@ -296,6 +304,15 @@ int ensure_init() {
   WOS_HIP_TRY(hipGetDeviceProperties(&prop, 0));
   snprintf(d.name, sizeof(d.name), "%s (%s)", prop.name, prop.gcnArchName);
   WOS_HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+  // this device's copy of the no-allocation scratch array
+  void* na = nullptr;
+  hipError_t e = hipGetSymbolAddress(&na, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !na) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return WOS_ERR_HIP;
+  }
+  d.noalloc = na;
+  d.noalloc_bytes = kNoAllocScratchBytes;
   d.ready = true;
   return 0;
 }
@@ -322,10 +339,52 @@ static void plan_stages(int n, int radix, std::vector<int>& fwd, std::vector<int
   for (int i = 0; i < nf; ++i) inv.push_back(radix);
 }
 
-// Stage twiddle tables of an n-point transform (forward stages 1..S-1, then
-// inverse stages 1..S-1, the layout Plan<N>::tw_off_* addresses), cached.
-// acc: [hi, lo] pairs for the compensated split-step kernels; radix: the
-// stage schedule (16 or 8) whose stages the table holds.
+// Stages of the twiddle table of an n-point transform under the radix-16 or
+// radix-8 schedule: forward stages 1..S-1, then inverse stages 1..S-1, the
+// layout Plan<N>::tw_off_* addresses. Returns the stage count (at most
+// kMaxTwStages) and the table's entry count in *entries.
+struct TwStage {
+  int off, ns, rad;
+};
+constexpr int kMaxTwStages = 32;
+static int twiddle_stages(int n, int radix, TwStage* st, int* entries) {
+  std::vector<int> fwd, inv;
+  plan_stages(n, radix == 8 ? 8 : 16, fwd, inv);
+  int off = 0, cnt = 0;
+  for (const std::vector<int>* plan : {&fwd, &inv}) {
+    int ns = (*plan)[0];
+    for (size_t i = 1; i < plan->size() && cnt < kMaxTwStages; ++i) {
+      const int rad = (*plan)[i];
+      st[cnt++] = {off, ns, rad};
+      off += ns * (rad - 1);
+      ns *= rad;
+    }
+  }
+  *entries = off;
+  return cnt;
+}
+
+// Generate the twiddle table of an n-point transform into p (entries of
+// csize(prec) bytes, twice that for acc) in the order of stream s.
+static int launch_twiddles(void* p, int n, int prec, int acc, int radix, hipStream_t s) {
+  TwStage st[kMaxTwStages];
+  int entries;
+  const int ns = twiddle_stages(n, radix, st, &entries);
+  for (int i = 0; i < ns; ++i) {
+    const TwStage& q = st[i];
+    const int cnt = q.ns * (q.rad - 1), bs = 256, gs = (cnt + bs - 1) / bs;
+    if (prec == WOS_FP64)
+      hipLaunchKernelGGL(k_stage_twiddles<double>, dim3(gs), dim3(bs), 0, s, (cpx<double>*)p, q.off, q.ns, q.rad, acc);
+    else
+      hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(gs), dim3(bs), 0, s, (cpx<float>*)p, q.off, q.ns, q.rad, acc);
+    WOS_HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
+// Stage twiddle tables of an n-point transform, cached per (n, precision,
+// acc, radix). acc: [hi, lo] pairs for the compensated split-step kernels;
+// radix: the stage schedule (16 or 8) whose stages the table holds.
 int get_twiddles(int n, int prec, const void** out, int acc, int radix) {
   Device& d = device();
   if (radix != 8) radix = 16;
@@ -335,29 +394,15 @@ int get_twiddles(int n, int prec, const void** out, int acc, int radix) {
     *out = it->second;
     return 0;
   }
-  std::vector<int> fwd, inv;
-  plan_stages(n, radix, fwd, inv);
-  struct St { int off, ns, rad; };
-  std::vector<St> st;
-  int off = 0;
-  for (const std::vector<int>* plan : {&fwd, &inv}) {
-    int ns = (*plan)[0];
-    for (size_t i = 1; i < plan->size(); ++i) {
-      const int rad = (*plan)[i];
-      st.push_back({off, ns, rad});
-      off += ns * (rad - 1);
-      ns *= rad;
-    }
-  }
+  TwStage st[kMaxTwStages];
+  int entries;
+  twiddle_stages(n, radix, st, &entries);
   void* p = nullptr;
-  WOS_HIP_TRY(hipMalloc(&p, (size_t)std::max(off, 1) * csize(prec) * (acc ? 2 : 1)));
-  for (const St& q : st) {
-    const int cnt = q.ns * (q.rad - 1), bs = 256, gs = (cnt + bs - 1) / bs;
-    if (prec == WOS_FP64)
-      hipLaunchKernelGGL(k_stage_twiddles<double>, dim3(gs), dim3(bs), 0, d.stream, (cpx<double>*)p, q.off, q.ns, q.rad, acc);
-    else
-      hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(gs), dim3(bs), 0, d.stream, (cpx<float>*)p, q.off, q.ns, q.rad, acc);
-    WOS_HIP_TRY(hipGetLastError());
+  WOS_HIP_TRY(hipMalloc(&p, (size_t)std::max(entries, 1) * csize(prec) * (acc ? 2 : 1)));
+  int rc = launch_twiddles(p, n, prec, acc, radix, d.stream);
+  if (rc) {
+    (void)hipFree(p);
+    return rc;
   }
   // consumers run on other streams (batches, pipelines): the table is
   // complete before its first use
@@ -1138,7 +1183,7 @@ static inline long long inter_unit_elems(int n, int M) { return (long long)n * M
 
 static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int n, int units, bool pad,
                           const TFUnit* d_tf, int tf_kind, double post_re, double post_im, double* pw_in,
-                          double* pw_out, Tuning tu) {
+                          double* pw_out, Tuning tu, const void* const* tabs = nullptr) {
   const int M = pad ? 2 * n : n;
   const int off = pad ? n / 2 : 0;
   if (!len_ok(M)) {
@@ -1175,6 +1220,10 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   if (mr) {
     if ((rc = get_mixed_twiddles(M, prec, &tw))) return rc;
     tw64 = twc = tw;
+  } else if (tabs) {  // tables the caller generated (row schedule, its FP64 twin, column schedule)
+    tw = tabs[0];
+    tw64 = tabs[1];
+    twc = tabs[2];
   } else {
     if ((rc = get_twiddles(Mk, prec, &tw, 0, tu.radix))) return rc;
     if ((rc = get_twiddles(Mk, WOS_FP64, &tw64, 0, tu.radix))) return rc;
@@ -3168,3 +3217,4 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 #include "wos_stats.inc"
 #include "wos_field.inc"
 #include "wos_aniso.inc"
+#include "wos_noalloc.inc"

@@ -48,6 +48,7 @@ EXPORTS = [
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
     "wos_hip_propagate_anisotropic", "wos_hip_field_anisotropic", "wos_hip_schell_realizations",
     "wos_hip_average_intensity", "wos_hip_propagate_partially_coherent", "wos_hip_pipeline_set_average",
+    "wos_hip_propagate_device_noalloc",
 ]
 
 VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity", "pol_azimuth", "pol_ellip",
@@ -276,6 +277,7 @@ def load(path=None):
     L.wos_hip_propagate_partially_coherent.argtypes = [I, D, D, D, C.c_int64, I, D, I, C.POINTER(Ctx), I, P,
                                                        C.POINTER(Warnings)]
     L.wos_hip_pipeline_set_average.argtypes = [P, I]
+    L.wos_hip_propagate_device_noalloc.argtypes = [P, I, I, D, D, I, C.POINTER(Ctx), I, P]
     _lib = L
     return L
 
@@ -600,6 +602,14 @@ def volume(ex, ey, dx, zs, method, ctx, precision=FP64):
     _check(load().wos_hip_volume(_p(ex), _p(ey), n, dx, _p(zs), zs.size, m, C.byref(ctx), precision,
                                  _p(ox), _p(oy), C.byref(w)), "volume")
     return ox, oy, w
+
+
+def propagate_device_noalloc(dev_ptr, n, ncomp, dx, z, method, ctx, precision=FP64, stream=None):
+    """wos_hip_propagate_device_noalloc on device memory at dev_ptr (in place,
+    enqueued on `stream`, None: the library's stream)."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    _check(load().wos_hip_propagate_device_noalloc(dev_ptr, n, ncomp, dx, z, m, C.byref(ctx), precision, stream),
+           "propagate_device_noalloc")
 
 
 def schell_realizations(n, dx, w0, sigma_c, seed, m):
