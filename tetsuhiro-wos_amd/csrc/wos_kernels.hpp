@@ -236,6 +236,31 @@ __device__ __forceinline__ cpx<float> fresnel_tf_fp32(const TFUnit& u, double fx
   return {pr * c - pi * s, pr * s + pi * c};
 }
 
+// Evanescent branch of the exact FP64 asmTF (rho > 1), out of line.
+__device__ __noinline__ cpx<double> asm_tf_fp64_evan(const TFUnit& u, double rho) {
+#pragma clang fp contract(off)
+  const double decay = u.k * u.az * sqrt((rho - 1) * (rho + 1));
+  return {asm_evanescent(u, decay), 0.0};
+}
+
+// Exact FP64 asmTF at transform position (ix, iy) with the propagating
+// branch inline: a kernel whose registers are full of transform values keeps
+// them across it (a call would have to save the values the calling
+// convention leaves unprotected); only the rare evanescent rows call out.
+__device__ __forceinline__ cpx<double> asm_tf_fp64_inl(const TFUnit& u, int ix, int iy) {
+#pragma clang fp contract(off)
+  const double fx = freq_of(ix, u.M, u.den) + u.fx_off;
+  const double fy = freq_of(iy, u.M, u.den) + u.fy_off;
+  const double rho = go_hypot(u.wl * fx, u.wl * fy);
+  if (rho <= 1) {
+    const double ph = u.k * u.z * sqrt((1 - rho) * (1 + rho));
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    return {cs, sn};
+  }
+  return asm_tf_fp64_evan(u, rho);
+}
+
 // Exact FP64 asmTF at transform position (ix, iy), out of line so the
 // sixteen call sites of a column thread share one copy of the code.
 __device__ __noinline__ cpx<double> asm_tf_fp64_at(const TFUnit& u, int ix, int iy) {
@@ -640,12 +665,13 @@ __device__ __forceinline__ int remap_xcd(int bid, int nb) {
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-// CS: the stage exchanges and the mirrored-TF exchange move real and
-// imaginary parts in two rounds through half the LDS, so more workgroups fit
-// a CU; in FP64 the kernel is also held to 128 VGPRs (4 waves per SIMD), as
-// registers would otherwise keep a second workgroup out.
+// CS: the stage exchanges move real and imaginary parts in two rounds
+// through half the LDS, so more workgroups fit a CU; in FP64 the kernel is
+// also held to 128 VGPRs (4 waves per SIMD), as registers would otherwise
+// keep a second workgroup out.
 template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false>
-__global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 : 1)) void k_cols(const ColArgs a) {
+__global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 : 1))
+void k_cols(const ColArgs a) {
   constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
   __shared__ cpx<R> lds[CS ? (lds_len(N) * W + 1) / 2 : lds_len(N) * W];
   const int tid = threadIdx.x;
@@ -717,62 +743,31 @@ __global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 :
           if constexpr (sizeof(R) == 4) {
             return asm_tf_fp32_full(u, cc, p);
           } else {
-            cpx<double> h = asm_tf_fp64_at(u, col, p);
+            cpx<double> h = asm_tf_fp64_inl(u, col, p);
             if (conj) h.y = -h.y;
             return h;
           }
         };
-        if constexpr (CS) {  // the exchange in two rounds: real parts, then imaginary parts
-          R* hr = (R*)lds + c;
-          cpx<R> hv[V2];
+        // One round through LDS as complex values in both layouts: the V/2
+        // values a thread passes on fill (N/2) W complex slots, which the
+        // split layout's half-size array holds too.
+        __syncthreads();  // all waves are done reading the forward transform's last exchange
 #pragma unroll
-          for (int m = 0; m < V2; ++m) {
-            hv[m] = tf_at(t + T * m);
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hv[m]);
-          }
-          if (t == 0) {  // row N/2 (Nyquist) is its own mirror
-            constexpr int m = V2;
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
-          }
-          R part[V - V2];
-          __syncthreads();  // all waves are done reading the forward transform's last exchange
+        for (int m = 0; m < V2; ++m) {
+          const cpx<R> h = tf_at(t + T * m);
+          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
+          hx[(m * T + t) * W] = h;
+        }
+        if (t == 0) {  // row N/2 (Nyquist) is its own mirror
+          constexpr int m = V2;
+          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
+        }
+        __syncthreads();
 #pragma unroll
-          for (int m = 0; m < V2; ++m) hr[(m * T + t) * W] = hv[m].x;
-          __syncthreads();
-#pragma unroll
-          for (int m = V2; m < V; ++m) {
-            const int ms = t == 0 ? V - m : V - 1 - m;
-            part[m - V2] = (t == 0 && m == V2) ? R(0) : hr[(ms * T + pt) * W];
-          }
-          __syncthreads();
-#pragma unroll
-          for (int m = 0; m < V2; ++m) hr[(m * T + t) * W] = hv[m].y;
-          __syncthreads();
-#pragma unroll
-          for (int m = V2; m < V; ++m) {
-            if (t == 0 && m == V2) continue;
-            const int ms = t == 0 ? V - m : V - 1 - m;
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], cpx<R>{part[m - V2], hr[(ms * T + pt) * W]});
-          }
-        } else {
-          __syncthreads();  // all waves are done reading the forward transform's last exchange
-#pragma unroll
-          for (int m = 0; m < V2; ++m) {
-            const cpx<R> h = tf_at(t + T * m);
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
-            hx[(m * T + t) * W] = h;
-          }
-          if (t == 0) {  // row N/2 (Nyquist) is its own mirror
-            constexpr int m = V2;
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
-          }
-          __syncthreads();
-#pragma unroll
-          for (int m = V2; m < V; ++m) {
-            if (t == 0 && m == V2) continue;
-            const int ms = t == 0 ? V - m : V - 1 - m;
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
-          }
+        for (int m = V2; m < V; ++m) {
+          if (t == 0 && m == V2) continue;
+          const int ms = t == 0 ? V - m : V - 1 - m;
+          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
         }
       } else if constexpr (MODE == 1 && sizeof(R) == 4) {
         const TF32Col cc = tf32_col(u, col);
