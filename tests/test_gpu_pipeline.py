@@ -22,16 +22,18 @@ def random_field(n, seed):
 
 
 def small_pipeline(wos, precision):
-    """polychromatic (128^2, 4 wavelengths, one source), volume (64^2, 5
-    planes from z = 0, one source), independent Bluestein fields (94^2,
-    94 = 2 47, 3 sources), asm_pad (64^2, 2 sources)."""
+    """polychromatic (256^2, 4 wavelengths, one source: the forward spectrum
+    is computed once for the stage), volume (64^2, 5 planes from z = 0, one
+    source, below the shared-spectrum size: the forward FFT per plane),
+    independent Bluestein fields (94^2, 94 = 2 47, 3 sources), asm_pad
+    (64^2, 2 sources)."""
     ctx = wos.ctx_for(550e-9)
     p = wos.Pipeline(precision)
     spec = {}
     wl = np.array([450e-9, 520e-9, 600e-9, 680e-9])
-    s0 = p.add(128, 2e-6, wl, np.full(4, 3e-3), ctx, "asm", src_units=1)
+    s0 = p.add(256, 2e-6, wl, np.full(4, 3e-3), ctx, "asm", src_units=1)
     p.set_intensity(s0, [0.5, 1.0, 1.5, 0.25])
-    p.upload(s0, 0, random_field(128, 1))
+    p.upload(s0, 0, random_field(256, 1))
     spec["poly"] = (s0, wl, [0.5, 1.0, 1.5, 0.25])
     zs = np.linspace(0, 2e-3, 5)
     s1 = p.add(64, 2e-6, np.full(5, 532e-9), zs, ctx, "asm", src_units=1)
@@ -62,7 +64,7 @@ def test_pipeline_stages_match_oracle(wos, oracle):
     p.run(1)
     got = outputs(p, spec)
     s0, wl, wt = spec["poly"]
-    ref, _ = oracle.polychromatic(random_field(128, 1), None, 2e-6, 3e-3, wl, wt, "asm", oracle.ctx_for(550e-9))
+    ref, _ = oracle.polychromatic(random_field(256, 1), None, 2e-6, 3e-3, wl, wt, "asm", oracle.ctx_for(550e-9))
     e = rel_l2(got["poly"], ref)
     print(f"pipeline polychromatic FP64 rel-L2 {e:.2e}", flush=True)
     assert e <= FP64_TOL

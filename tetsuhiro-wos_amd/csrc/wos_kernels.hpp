@@ -631,7 +631,9 @@ __global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows_pipe(const Row
 // Column pass over a tile of W adjacent columns. MODE 0: plain transform
 // (direction DIR, output scale); MODE 1: forward -> asm TF -> inverse (the
 // ASM step); MODE 2: forward -> any TF kind -> inverse; MODE 3: the
-// split-step variant of MODE 1 (see below). With `tiled` the
+// split-step variant of MODE 1 (see below); MODE 4: asm TF -> inverse on a
+// spectrum already transformed forward (src, src_unit elements between
+// units: 0 when every unit reads one shared spectrum), written to data. With `tiled` the
 // tile is one contiguous block (row stride W, tiles tstride apart); otherwise
 // rows are ld apart in a row-major array.
 // ---------------------------------------------------------------------------
@@ -655,7 +657,9 @@ struct ColArgs {
   int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
   const void* htab;    // MODE 3: FP64 H table in W-column tile order (nullable: evaluate H)
   int radix;           // stage schedule of tw: 0 or 16 radix-16, 8 radix-8 (MODE 1 only)
-  int split;           // MODE 1, radix-16: exchanges through half the LDS (real, then imaginary parts)
+  int split;           // MODE 1 and 4, radix-16: exchanges through half the LDS (real, then imaginary parts)
+  const void* src;     // MODE 4: the forward spectrum, same layout as data
+  long long src_unit;  // MODE 4: elements between the units of src (0: one spectrum for every unit)
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
@@ -685,8 +689,10 @@ void k_cols(const ColArgs a) {
   const int col = tile * W + c;
   // layout tile of 2^tlg columns; the workgroup's W columns are one part of it
   const int tl = a.tiled ? 1 << a.tlg : 0;
-  cpx<R>* base =
-      (cpx<R>*)a.data + (size_t)unit * a.unit + (a.tiled ? (col >> a.tlg) * a.tstride + (col & (tl - 1)) : col);
+  const size_t toff = a.tiled ? (size_t)(col >> a.tlg) * a.tstride + (col & (tl - 1)) : (size_t)col;
+  cpx<R>* base = (cpx<R>*)a.data + (size_t)unit * a.unit + toff;
+  // where the loads read: in place, or (MODE 4) the spectrum
+  const cpx<R>* ibase = MODE == 4 ? (const cpx<R>*)a.src + (size_t)unit * a.src_unit + toff : base;
   const cpx<R>* tw = (const cpx<R>*)a.tw;
   const bool active = unit < a.units;
   // uniform fast path: contiguous tile, every transform position present;
@@ -695,18 +701,20 @@ void k_cols(const ColArgs a) {
   const bool fastw = fast && tl == W;
 
   constexpr int DIR0 = MODE >= 1 ? -1 : DIR;
-  constexpr int R0 = first_rad<N, DIR0, VV>();
+  // MODE 4 starts where the forward transform ends: its loads take the
+  // forward plan's output map (natural order), the inverse plan's input map
+  constexpr int R0 = MODE == 4 ? last_rad<N, -1, VV>() : first_rad<N, DIR0, VV>();
   cpx<R> v[V];
   if (fast && !fastw) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0, VV>(t, b, r) * tl);
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(ibase + slot_pos<N, R0, VV>(t, b, r) * tl);
   } else if (fastw) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(base + slot_pos<N, R0, VV>(t, b, r) * W);
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(ibase + slot_pos<N, R0, VV>(t, b, r) * W);
   } else {
     const int ld = a.tiled ? tl : a.ld;
 #pragma unroll
@@ -715,13 +723,14 @@ void k_cols(const ColArgs a) {
       for (int r = 0; r < R0; ++r) {
         const int q = slot_pos<N, R0, VV>(t, b, r) - a.in_row0;
         cpx<R> x = {R(0), R(0)};
-        if (active && q >= 0 && q < a.rows) x = base[q * ld];
+        if (active && q >= 0 && q < a.rows) x = ibase[q * ld];
         v[b * R0 + r] = x;
       }
   }
   constexpr bool ACC = MODE == 3 && sizeof(R) == 4;  // split-step slices: compensated twiddles
-  fft_block<R, N, DIR0, W, ACC, CS, VV>(v, t, tw, lc);
-  if constexpr (MODE == 1 || MODE == 2) {
+  if constexpr (MODE != 4) fft_block<R, N, DIR0, W, ACC, CS, VV>(v, t, tw, lc);
+  constexpr bool ASM = MODE == 1 || MODE == 4;   // the lean asm transfer function
+  if constexpr (MODE == 1 || MODE == 2 || MODE == 4) {
     constexpr int RL = last_rad<N, -1, VV>();
     if (active) {
       const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
@@ -732,7 +741,7 @@ void k_cols(const ColArgs a) {
       // thread evaluates H for its rows m < V/2, passes them on through LDS
       // (free between the two transforms) and takes the rest from its
       // partner: each wave evaluates half as many transfer-function values.
-      const bool mirror = MODE == 1 && T >= 2 && V >= 2 && u.fy_off == 0.0;
+      const bool mirror = ASM && T >= 2 && V >= 2 && u.fy_off == 0.0;
       if (mirror) {
         constexpr int NB = V / RL, V2 = V / 2;
         cpx<R>* hx = lds + c;
@@ -769,7 +778,7 @@ void k_cols(const ColArgs a) {
           const int ms = t == 0 ? V - m : V - 1 - m;
           v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
         }
-      } else if constexpr (MODE == 1 && sizeof(R) == 4) {
+      } else if constexpr (ASM && sizeof(R) == 4) {
         const TF32Col cc = tf32_col(u, col);
         bool evan = false;
 #pragma unroll
@@ -802,14 +811,14 @@ void k_cols(const ColArgs a) {
           block_sum_store<W * Plan<N, VV>::T>(s1, redc, a.pw3, blockIdx.x);
         }
       } else {
-        const bool conj = MODE == 1 && u.kind == TF_CONJ_ASM;
+        const bool conj = ASM && u.kind == TF_CONJ_ASM;
 #pragma unroll
         for (int b = 0; b < V / RL; ++b)
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
             const int p = slot_pos<N, RL, VV>(t, b, r);
             cpx<R> h;
-            if constexpr (MODE == 1) {
+            if constexpr (ASM) {
               h = asm_tf_fp64_at(u, col, p);
               if (conj) h.y = -h.y;
             } else {
