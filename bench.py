@@ -32,7 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # MI355X VALU issue ceiling: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
 VALU_WAVE_INSTR_PER_S = 614.4e9
 # HBM bytes per full-batch launch (64 x 2048^2), rocprofv3 PMC: 2 x FETCH_SIZE + WRITE_SIZE
-# (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md
+# (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md, measured at the launch shapes
+# PMC_SHAPE (tile_w, col_w, rows, row_groups, split_lds, radix, col_radix, col_split_lds); a run whose
+# autotuned shape differs reports no traffic figure
 PMC_HBM_BYTES = {
     "fp32": {"row_fft_fwd": (1048722 * 2 + 2097166) * 1024, "col_fft_tf_ifft": (1048909 * 2 + 2097259) * 1024,
              "row_fft_inv": (1048724 * 2 + 2097155) * 1024},
@@ -41,6 +43,7 @@ PMC_HBM_BYTES = {
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
 PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1532 * 262144, "row_fft_inv": 872 * 262144}}
+PMC_SHAPE = {"fp32": (4, 4, 4, 0, 1, 16, 16, 0), "fp64": (4, 4, 2, 0, 1, 16, 16, 1)}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
 
@@ -141,7 +144,10 @@ def bench_bpm(wos):
     out["power_ratio_fp64"] = float(np.sum(np.abs(res["fp64"]) ** 2) / np.sum(np.abs(ex) ** 2))
     # a Strang step is two ASM half steps: 2 x 8 n^2 S algorithmic bytes
     for name, S in (("fp64", 16), ("fp32", 8)):
-        out[name]["hbm_frac"] = round(out[name]["steps_per_s"] * 2 * 8 * n * n * S / (HBM_PEAK_GBS * 1e9), 4)
+        out[name]["frac_algorithmic"] = round(out[name]["steps_per_s"] * 2 * 8 * n * n * S / (HBM_PEAK_GBS * 1e9), 4)
+    out["frac_note"] = ("algorithmic bytes (two ASM half steps per slice) over time against 8 TB/s; the merged "
+                        "half steps move a quarter of them and the 8/16 MiB field stays in the last-level cache, "
+                        "so this is not an HBM measurement and can exceed 1")
     check20 = {}
     for name, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
         check20[name], _, _ = wos.split_step(ex, None, dx, 20 * 5e-6, 20, index, wos.ctx_for(wl), precision=prec)
@@ -175,8 +181,10 @@ def bench_pipeline(wos, steps, tune=True):
         B = 8 * S * (64 * 2048 ** 2 + 128 * 1024 ** 2 + 11 * 384 ** 2 + 11 * 1000 ** 2 + 10 * 1536 ** 2)
         out[name] = {"ms_per_graph_run": round(ms, 4), "props_per_s": round(224 / (ms * 1e-3), 1),
                      "stage_ms": [round(x, 4) for x in st],
-                     "hbm_frac": round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+                     "hbm_frac_algorithmic": round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
         p.close()
+    out["hbm_note"] = ("algorithmic bytes 8 M^2 S per propagation; the shared-source stages compute the "
+                       "source's forward spectrum once and move about half of them")
     out["rel_l2_fp32_vs_fp64_intensity"] = rel_l2(res["fp32"][0], res["fp64"][0])
     out["rel_l2_fp32_vs_fp64_1000sq_field"] = rel_l2(res["fp32"][1], res["fp64"][1])
     return out
@@ -267,11 +275,19 @@ def bench_config12(wos, tune=True):
             reps = 3
             for _ in range(reps):
                 wos.propagate(ex, ey, dx, z, method, wos.ctx_for(wl), precision=prec)
+            # bytes the three passes move by construction: asm 6 n^2 S per component; asm_pad
+            # never stores the zero rows of the 2n grid: n^2 in + 2n^2 out (rows), 2n^2 each way
+            # (columns), 2n^2 in + n^2 out (inverse rows) = 10 n^2 S
+            moved = len(comps) * (10 if method == "asm_pad" else 6) * n * n * S
             res[pname] = {"props_per_s": round(len(comps) / (ms * 1e-3), 1), "device_ms": round(ms, 4),
-                          "hbm_frac": round(len(comps) * 8 * M * M * S / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                          "hbm_frac_algorithmic": round(len(comps) * 8 * M * M * S / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                          "hbm_frac_moved": round(moved / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                           "dropin_ms_per_call": round((time.perf_counter() - t0) * 1e3 / reps, 2)}
         a32, a64 = np.concatenate(fields[(name, "fp32")]), np.concatenate(fields[(name, "fp64")])
         res["rel_l2_fp32_vs_fp64"] = rel_l2(a32, a64)
+        res["hbm_note"] = ("hbm_frac_algorithmic prices 8 M^2 S per component (M the transform side); "
+                           "hbm_frac_moved the bytes the kernels move (6 n^2 S asm, 10 n^2 S for the pruned asm_pad); "
+                           "a 512^2 field stays in the last-level cache, so neither is an HBM measurement there")
         out[name] = res
     return out, fields
 
@@ -500,6 +516,8 @@ def main():
                     help="extra workloads at their default launch shapes (profiling: no autotuner launches)")
     ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
     ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
+    ap.add_argument("--only", default=None, choices=["config12", "bpm", "pipeline", "adjoint", "train"],
+                    help="run one of the other workloads alone and print its figures (profiling)")
     args = ap.parse_args()
     shape32 = [int(x) for x in args.shape32.split(",")] if args.shape32 else None
     shape64 = [int(x) for x in args.shape64.split(",")] if args.shape64 else None
@@ -510,6 +528,13 @@ def main():
 
     import wos_hip as wos
     wos.init()
+    if args.only:
+        tune = not args.no_tune
+        run = {"config12": lambda: bench_config12(wos, tune)[0], "bpm": lambda: bench_bpm(wos)[0],
+               "pipeline": lambda: bench_pipeline(wos, args.steps, tune), "adjoint": lambda: bench_adjoint(wos)[0],
+               "train": lambda: bench_train(wos)[0]}[args.only]
+        print(json.dumps({"only": args.only, "result": run()}), flush=True)
+        return
     r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup, shape32)
     r64 = run_gpu(wos, wos.FP64, args.steps, args.warmup, shape64)
 
@@ -532,6 +557,11 @@ def main():
                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": PMC_HBM_BYTES["fp32"][nm],
                            "valu_issue_frac": round(PMC_VALU["fp32"][nm] / VALU_WAVE_INSTR_PER_S / (k[i] * 1e-3), 3)})
     achieved = per_kernel[dom]["achieved_GBs"]
+    tuned32 = tuple(r32["tune"][k] for k in wos.TUNING_KEYS)
+    traffic_ok = tuned32 == PMC_SHAPE["fp32"]
+    if not traffic_ok:   # the PMC constants belong to another launch shape
+        for pk in per_kernel:
+            pk["traffic"] = None
     # whole step: algorithmic bytes 8*M^2*S per propagation (two 2-D FFTs x
     # two out-of-cache passes x read + write)
     step_gbs = props32 * 8.0 * N * N * 8 / 1e9
@@ -559,11 +589,20 @@ def main():
         "rel_l2_fp32_vs_fp64": err32,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_HBM_BYTES["fp32"][names[dom]],
+                     "traffic": PMC_HBM_BYTES["fp32"][names[dom]] if traffic_ok else None,
+                     "traffic_shape": list(PMC_SHAPE["fp32"]),
+                     "traffic_note": "PMC FETCH_SIZE x 2 + WRITE_SIZE per launch at traffic_shape"
+                                     + ("" if traffic_ok else f"; this run's autotuned shape {list(tuned32)} differs, "
+                                                              "so no traffic figure is given"),
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "kernels": per_kernel,
                      "step_algorithmic_GBs": round(step_gbs, 1),
-                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                     # the three executed passes each read and write the batch once
+                     "step_moved_GBs": round(props32 * 6.0 * N * N * 8 / 1e9, 1),
+                     "step_moved_frac": round(props32 * 6.0 * N * N * 8 / 1e9 / HBM_PEAK_GBS, 4),
+                     "step_note": "step_frac prices 8 N^2 S per propagation (four passes of an unfused forward + "
+                                  "inverse 2-D FFT); step_moved_frac the 6 N^2 S the three executed passes move"},
     }
     if args.no_extra:
         print(json.dumps(line), flush=True)

@@ -125,6 +125,24 @@ def test_methods_fp64_parity(wos, oracle, method, jones):
         assert abs(wg.as_dict()[code][1] - val) <= 1e-9 * max(abs(val), 1e-300)
 
 
+@pytest.mark.parametrize("method", ["asm_shift", "asm_shift_pad"])
+@pytest.mark.parametrize("n", [256, 2048])
+def test_shifted_methods_fp32_centroid(wos, oracle, method, n):
+    """The FP32 path's spectral centroid is a parallel reduction (the FP64
+    path keeps the reference's serial order): the carrier matches the
+    oracle's to FP32 rounding and the shifted propagation to the FP32
+    tolerance, on a grid where the centroid covers 4M (2048^2) pixels."""
+    wl = 632.8e-9
+    ex, dx = gaussian_tilted(oracle, n, n * 1.6e-5, wl, n * 1.6e-5 / 10, 0.004)
+    ref, _, _, so = oracle.propagate(ex, None, dx, 0.05, method, oracle.ctx_for(wl))
+    out, _, _, sg = wos.propagate(ex, None, dx, 0.05, method, wos.ctx_for(wl), precision=wos.FP32)
+    err = rel_l2(out, ref)
+    print(f"{method} n={n} FP32 rel-L2 {err:.3e}; carrier fx {sg.fx_c:.6e} (oracle {so.fx_c:.6e})", flush=True)
+    assert sg.method_used == so.method_used
+    assert abs(sg.fx_c - so.fx_c) <= 1e-6 * abs(so.fx_c) and abs(sg.fy_c - so.fy_c) <= 1e-6 * abs(so.fx_c)
+    assert err <= FP32_TOL
+
+
 @pytest.mark.parametrize("z", [-0.02, 0.02])
 @pytest.mark.parametrize("reg", [False, True])
 def test_evanescent_policies_parity(wos, oracle, z, reg):

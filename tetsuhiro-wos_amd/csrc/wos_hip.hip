@@ -712,8 +712,44 @@ __global__ void k_spec_hist(const cpx<R>* __restrict__ spec, int n, size_t total
   for (int i = threadIdx.x; i < 97; i += blockDim.x) partial[(size_t)blockIdx.x * 97 + i] = h[i];
 }
 
+// Spectral centroid sums of the FP32 path (spectralCentroid,
+// optics/propagate.go:391-416): per-workgroup FP64 partial sums of |U|^2,
+// fx |U|^2 and fy |U|^2 over a fixed grid, summed by the host in workgroup
+// order, so the result does not depend on scheduling. The FP32 field's own
+// rounding (~1e-7) dominates any change of summation order.
+constexpr int kCentroidBlocks = 1024;
+template <typename R>
+__global__ __launch_bounds__(256) void k_spec_centroid_partial(const cpx<R>* __restrict__ spec, int n, int ncomp,
+                                                               double den, double* __restrict__ partial) {
+  __shared__ double red[3][4];
+  const size_t nn = (size_t)n * n, total = (size_t)ncomp * nn;
+  double sw = 0, sx = 0, sy = 0;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const size_t rem = idx % nn;
+    const int j = (int)(rem / n), i = (int)(rem % n);
+    const double re = (double)spec[idx].x, im = (double)spec[idx].y;
+    const double w = re * re + im * im;
+    sw += w;
+    sx += w * freq_of(i, n, den);
+    sy += w * freq_of(j, n, den);
+  }
+  double v[3] = {sw, sx, sy};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+    if (lane == 0) red[q][wave] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    partial[(size_t)blockIdx.x * 3 + q] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+  }
+}
+
 // Spectral centroid sums of spectralCentroid (optics/propagate.go:391-416)
-// in the reference's serial order (component, row, column),
+// for the FP64 path, in the reference's serial order (component, row, column),
 // one lane accumulating while the other waves prefetch the next chunk into
 // LDS. The carrier feeds a phase ramp whose result moves by ~4e-12 rel-L2
 // per 1e-15 relative change of the centroid, so the reference's naive serial
@@ -1479,18 +1515,28 @@ static int spectral_centroid_dev(int prec, const void* d, int n, int ncomp, doub
   if ((rc = dv.aux.ensure(bytes))) return rc;
   WOS_HIP_TRY(hipMemcpyAsync(dv.aux.p, d, bytes, hipMemcpyDeviceToDevice, dv.stream));
   if ((rc = fft2d_dev(prec, dv.aux.p, n, ncomp, 0))) return rc;
-  if ((rc = dv.hist.ensure(sizeof(double) * 3))) return rc;
+  if ((rc = dv.hist.ensure(sizeof(double) * 3 * kCentroidBlocks))) return rc;
   const double den = (double)n * dx;
-  if (prec == WOS_FP64)
+  double h[3] = {0, 0, 0};
+  if (prec == WOS_FP64) {
     hipLaunchKernelGGL(k_spec_centroid_serial<double>, dim3(1), dim3(256), 0, dv.stream, (const cpx<double>*)dv.aux.p, n,
                        ncomp, den, (double*)dv.hist.p);
-  else
-    hipLaunchKernelGGL(k_spec_centroid_serial<float>, dim3(1), dim3(256), 0, dv.stream, (const cpx<float>*)dv.aux.p, n,
-                       ncomp, den, (double*)dv.hist.p);
-  WOS_HIP_TRY(hipGetLastError());
-  double h[3];
-  WOS_HIP_TRY(hipMemcpyAsync(h, dv.hist.p, sizeof(h), hipMemcpyDeviceToHost, dv.stream));
-  WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+    WOS_HIP_TRY(hipGetLastError());
+    WOS_HIP_TRY(hipMemcpyAsync(h, dv.hist.p, sizeof(h), hipMemcpyDeviceToHost, dv.stream));
+    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+  } else {
+    hipLaunchKernelGGL(k_spec_centroid_partial<float>, dim3(kCentroidBlocks), dim3(256), 0, dv.stream,
+                       (const cpx<float>*)dv.aux.p, n, ncomp, den, (double*)dv.hist.p);
+    WOS_HIP_TRY(hipGetLastError());
+    std::vector<double> part(3 * kCentroidBlocks);
+    WOS_HIP_TRY(hipMemcpyAsync(part.data(), dv.hist.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, dv.stream));
+    WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
+    for (int q = 0; q < 3; ++q) {
+      Neumaier acc;
+      for (int b = 0; b < kCentroidBlocks; ++b) acc.add(part[(size_t)b * 3 + q]);
+      h[q] = acc.value();
+    }
+  }
   if (h[0] <= 0) {
     *fx = 0;
     *fy = 0;

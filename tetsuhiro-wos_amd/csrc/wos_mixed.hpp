@@ -214,6 +214,24 @@ __global__ __launch_bounds__(MPlan<N>::T) void k_rows_mixed(const RowArgs a) {
   if (a.pw_out) block_sum_store<T>(pout, red, a.pw_out, blockIdx.x);  // uniform
 }
 
+// The transfer function at transform row p of column col for the mixed-radix
+// column passes: the lean asm forms (FP32: per-column constants and the
+// hardware sine/cosine of a phase reduced in FP64 turns; FP64: the exact
+// form inline) for asm and conjugate asm, tf_eval for the other kinds.
+template <typename R>
+__device__ __forceinline__ cpx<R> mixed_tf(const TFUnit& u, const TF32Col& cc, int col, int p) {
+  if (u.kind == TF_ASM || u.kind == TF_CONJ_ASM) {
+    if constexpr (sizeof(R) == 4) {
+      return asm_tf_fp32_full(u, cc, p);   // cc carries the conjugate's sign
+    } else {
+      cpx<double> h = asm_tf_fp64_inl(u, col, p);
+      if (u.kind == TF_CONJ_ASM) h.y = -h.y;
+      return h;
+    }
+  }
+  return tf_eval<R>(u, col, p);
+}
+
 // ---------------------------------------------------------------------------
 // Column pass over W adjacent columns (tiled intermediate or row-major).
 // MODE 0: one transform in direction DIR, then the scale; MODE 2: forward
@@ -247,18 +265,19 @@ __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a)
   if constexpr (MODE == 2) {
     mixed_fft<R, N, -1, W>(lc, t, tw);
     const TFUnit& u = a.tf[unit];
+    const TF32Col cc = tf32_col(u, col);   // FP32 asm only
     // transfer functions even in fy (no spectral shift): row N - p has the
     // exact negative of row p's frequency, so one evaluation serves both
     const bool mirror = u.fy_off == 0.0 && (u.kind == TF_ASM || u.kind == TF_CONJ_ASM || u.kind == TF_FRESNEL ||
                                             u.kind == TF_BANDLIMIT);
     if (mirror) {
       for (int p = t; p <= N / 2; p += T) {
-        const cpx<R> h = tf_eval<R>(u, col, p);
+        const cpx<R> h = mixed_tf<R>(u, cc, col, p);
         lc[lpad(p) * W] = cmul(lc[lpad(p) * W], h);
         if (p != 0 && 2 * p != N) lc[lpad(N - p) * W] = cmul(lc[lpad(N - p) * W], h);
       }
     } else {
-      for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], tf_eval<R>(u, col, p));
+      for (int p = t; p < N; p += T) lc[lpad(p) * W] = cmul(lc[lpad(p) * W], mixed_tf<R>(u, cc, col, p));
     }
     __syncthreads();
     mixed_fft<R, N, +1, W>(lc, t, tw);
@@ -445,16 +464,17 @@ __global__ __launch_bounds__(kMixedMaxThreads) void k_cols_mixed_rt(const ColArg
   if constexpr (MODE == 2) {
     mixed_fft_rt<R, -1, W, R7>(lc, t, p, tw);
     const TFUnit& u = a.tf[unit];
+    const TF32Col cc = tf32_col(u, col);   // FP32 asm only
     const bool mirror = u.fy_off == 0.0 && (u.kind == TF_ASM || u.kind == TF_CONJ_ASM || u.kind == TF_FRESNEL ||
                                             u.kind == TF_BANDLIMIT);
     if (mirror) {
       for (int q0 = t; q0 <= N / 2; q0 += T) {
-        const cpx<R> h = tf_eval<R>(u, col, q0);
+        const cpx<R> h = mixed_tf<R>(u, cc, col, q0);
         lc[lpad(q0) * W] = cmul(lc[lpad(q0) * W], h);
         if (q0 != 0 && 2 * q0 != N) lc[lpad(N - q0) * W] = cmul(lc[lpad(N - q0) * W], h);
       }
     } else {
-      for (int q0 = t; q0 < N; q0 += T) lc[lpad(q0) * W] = cmul(lc[lpad(q0) * W], tf_eval<R>(u, col, q0));
+      for (int q0 = t; q0 < N; q0 += T) lc[lpad(q0) * W] = cmul(lc[lpad(q0) * W], mixed_tf<R>(u, cc, col, q0));
     }
     __syncthreads();
     mixed_fft_rt<R, 1, W, R7>(lc, t, p, tw);
