@@ -1315,7 +1315,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ca.scale = 1.0;
   // column workgroups narrower than the layout tile share its rows' lines:
   // the XCD-aware order puts the workgroups of one tile on one XCD (one L2)
-  ca.xcd_remap = tu.xcd_remap || (!bs && !mr && tu.col_w < tu.tw);
+  ca.xcd_remap = tu.xcd_remap || (!bs && tu.col_w < tu.tw);
   ca.tf = d_tf;
   ca.tw = twc;
   ca.radix = tu.col_radix;

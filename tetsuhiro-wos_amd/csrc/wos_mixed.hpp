@@ -247,8 +247,11 @@ __global__ __launch_bounds__(W * MPlan<N>::T) void k_cols_mixed(const ColArgs a)
   const int tid = threadIdx.x;
   const int c = tid % W, t = tid / W;
   const int tpu = a.cols / W;
-  const int unit = blockIdx.x / tpu;
-  const int col = (blockIdx.x % tpu) * W + c;
+  // workgroups narrower than the layout tile share its lines: the XCD-aware
+  // order keeps the workgroups of one tile on one XCD (one L2)
+  const int bid = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int unit = bid / tpu;
+  const int col = (bid % tpu) * W + c;
   const int tl = a.tiled ? 1 << a.tlg : 0;
   const int ld = a.tiled ? tl : a.ld;
   cpx<R>* base =
@@ -446,8 +449,11 @@ __global__ __launch_bounds__(kMixedMaxThreads) void k_cols_mixed_rt(const ColArg
   const int tid = threadIdx.x;
   const int c = tid % W, t = tid / W;
   const int tpu = a.cols / W;
-  const int unit = blockIdx.x / tpu;
-  const int col = (blockIdx.x % tpu) * W + c;
+  // workgroups narrower than the layout tile share its lines: the XCD-aware
+  // order keeps the workgroups of one tile on one XCD (one L2)
+  const int bid = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int unit = bid / tpu;
+  const int col = (bid % tpu) * W + c;
   const int tl = a.tiled ? 1 << a.tlg : 0;
   const int ld = a.tiled ? tl : a.ld;
   cpx<R>* base =
