@@ -52,7 +52,20 @@ def rel_l2(a, b):
     return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / den))
 
 
-def run_gpu(wos, precision, steps, warmup, shape=None):
+def tune_batch(b, tune, retune, iters=5):
+    """The launch shape of a batch: the committed tuning table's entry unless
+    retune (or no entry), else the autotuner. Returns (shape, ms, source)."""
+    if not tune:
+        return None, None, "default"
+    if not retune:
+        shape = b.load_tuning()
+        if shape is not None:
+            return shape, b.time(1, 5)[0], "table"
+    shape, ms = b.autotune(iters)
+    return shape, ms, "autotune"
+
+
+def run_gpu(wos, precision, steps, warmup, shape=None, retune=False):
     wls = np.linspace(400e-9, 700e-9, UNITS)
     zs = np.full(UNITS, Z)
     b = wos.Batch(N, DX, wls, zs, wos.ctx_for(550e-9), "asm", precision)
@@ -64,14 +77,16 @@ def run_gpu(wos, precision, steps, warmup, shape=None):
         shape = shape + [0] * (8 - len(shape))
         b.set_tuning(*shape)
         tune = (dict(zip(wos.TUNING_KEYS, shape)), b.time(1, 5)[0])
+        source = "fixed"
     else:
-        tune = b.autotune(5)
+        s_, ms_, source = tune_batch(b, True, retune)
+        tune = (s_, ms_)
     b.fill_schell(W0, SIGMA_C, SEED)   # the tuning launches ran on the data
     b.capture()
     b.run(1)
     b.sync()
     out = {"unit0_in": inp0, "unit0": b.download(0), "unit_last": b.download(UNITS - 1),
-           "wl0": wls[0], "tune": dict(tune[0], ms_per_run=round(tune[1], 4))}
+           "wl0": wls[0], "tune": dict(tune[0], ms_per_run=round(tune[1], 4), source=source)}
     b.run(warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -154,24 +169,36 @@ def bench_bpm(wos):
     return out, check20
 
 
-def bench_pipeline(wos, steps, tune=True):
+def config4_pipeline(wos, prec):
+    """The batched configuration in one pipeline: 64 x 2048^2 polychromatic
+    intensity, 128 z-planes at 1024^2, 32 mixed fields (384^2, 1000^2,
+    1536^2); Gaussian-Schell inputs."""
+    p = wos.Pipeline(prec)
+    ctx = wos.ctx_for(550e-9)
+    s = p.add(2048, 2e-6, np.linspace(400e-9, 700e-9, 64), np.full(64, 20e-3), ctx, "asm", src_units=1)
+    p.set_intensity(s, np.ones(64))
+    p.fill_schell(s, W0, SIGMA_C, SEED, 0)
+    s = p.add(1024, 2e-6, np.full(128, 532e-9), np.linspace(0, 20e-3, 128), ctx, "asm", src_units=1)
+    p.fill_schell(s, W0, SIGMA_C, SEED, 1)
+    for k, n in enumerate((384, 1000, 1536)):
+        cnt = len(range(k, 32, 3))
+        s = p.add(n, 4.096e-3 / n, np.full(cnt, 532e-9), np.full(cnt, 20e-3), ctx, "asm")
+        p.fill_schell(s, W0, SIGMA_C, SEED, 2 + 100 * k)
+    return p
+
+
+def bench_pipeline(wos, steps, tune=True, retune=False):
     """One hipGraph: 64 x 2048^2 polychromatic intensity, 128 z-planes at
     1024^2, 32 mixed fields (384^2, 1000^2, 1536^2); Gaussian-Schell inputs."""
     out, res = {}, {}
     for name, prec in (("fp32", wos.FP32), ("fp64", wos.FP64)):
-        p = wos.Pipeline(prec)
-        ctx = wos.ctx_for(550e-9)
-        s = p.add(2048, 2e-6, np.linspace(400e-9, 700e-9, 64), np.full(64, 20e-3), ctx, "asm", src_units=1)
-        p.set_intensity(s, np.ones(64))
-        p.fill_schell(s, W0, SIGMA_C, SEED, 0)
-        s = p.add(1024, 2e-6, np.full(128, 532e-9), np.linspace(0, 20e-3, 128), ctx, "asm", src_units=1)
-        p.fill_schell(s, W0, SIGMA_C, SEED, 1)
-        for k, n in enumerate((384, 1000, 1536)):
-            cnt = len(range(k, 32, 3))
-            s = p.add(n, 4.096e-3 / n, np.full(cnt, 532e-9), np.full(cnt, 20e-3), ctx, "asm")
-            p.fill_schell(s, W0, SIGMA_C, SEED, 2 + 100 * k)
+        p = config4_pipeline(wos, prec)
+        source = "default"
         if tune:
-            p.autotune(3)
+            source = "table"
+            if retune or p.load_tuning() < 5:
+                p.autotune(3)
+                source = "autotune"
         p.capture()
         ms, st = p.time(2, max(steps, 10), stages=True)
         p.run(1)
@@ -181,7 +208,8 @@ def bench_pipeline(wos, steps, tune=True):
         B = 8 * S * (64 * 2048 ** 2 + 128 * 1024 ** 2 + 11 * 384 ** 2 + 11 * 1000 ** 2 + 10 * 1536 ** 2)
         out[name] = {"ms_per_graph_run": round(ms, 4), "props_per_s": round(224 / (ms * 1e-3), 1),
                      "stage_ms": [round(x, 4) for x in st],
-                     "hbm_frac_algorithmic": round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+                     "hbm_frac_algorithmic": round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                     "tuning": source}
         p.close()
     out["hbm_note"] = ("algorithmic bytes 8 M^2 S per propagation; the shared-source stages compute the "
                        "source's forward spectrum once and move about half of them")
@@ -248,7 +276,7 @@ def config12_inputs():
             "config2": (c2, np.zeros_like(c2), n2, dx2, 532e-9, 10e-3, "asm_pad")}
 
 
-def bench_config12(wos, tune=True):
+def bench_config12(wos, tune=True, retune=False):
     """Configurations 1 and 2 as device-resident batches (one unit per field
     component, replayed from a hipGraph) and as drop-in Propagate calls on
     host buffers. Algorithmic bytes per component: 8 M^2 S (M = 2n for
@@ -260,8 +288,7 @@ def bench_config12(wos, tune=True):
         res = {"grid": n, "method": method, "components": len(comps)}
         for pname, prec in (("fp64", wos.FP64), ("fp32", wos.FP32)):
             b = wos.Batch(n, dx, [wl] * len(comps), [z] * len(comps), wos.ctx_for(wl), method, prec)
-            if tune:
-                b.autotune(3)
+            _, _, source = tune_batch(b, tune, retune, 3)
             for u, f in enumerate(comps):
                 b.upload(u, f)
             b.capture()
@@ -282,7 +309,8 @@ def bench_config12(wos, tune=True):
             res[pname] = {"props_per_s": round(len(comps) / (ms * 1e-3), 1), "device_ms": round(ms, 4),
                           "hbm_frac_algorithmic": round(len(comps) * 8 * M * M * S / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                           "hbm_frac_moved": round(moved / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
-                          "dropin_ms_per_call": round((time.perf_counter() - t0) * 1e3 / reps, 2)}
+                          "dropin_ms_per_call": round((time.perf_counter() - t0) * 1e3 / reps, 2),
+                          "tuning": source}
         a32, a64 = np.concatenate(fields[(name, "fp32")]), np.concatenate(fields[(name, "fp64")])
         res["rel_l2_fp32_vs_fp64"] = rel_l2(a32, a64)
         res["hbm_note"] = ("hbm_frac_algorithmic prices 8 M^2 S per component (M the transform side); "
@@ -505,6 +533,37 @@ def cpu_baseline(inp, wl, reps=5):
                      **ht)
 
 
+def write_tuning(wos, path):
+    """Autotune the batches and pipeline stages the benchmark runs and write
+    their shapes as a tuning table (the format wos_hip_batch_load_tuning
+    reads)."""
+    lines = [f"# launch shapes of libwos_hip on {wos.device_name()}, written by bench.py --write-tuning",
+             "# n precision method units tile_w col_w rows groups split radix col_radix col_split ms"]
+    pname = {wos.FP32: "fp32", wos.FP64: "fp64"}
+
+    def add(n, prec, method, units, shape, ms):
+        vals = " ".join(str(int(shape[k])) for k in wos.TUNING_KEYS)
+        lines.append(f"{n} {pname[prec]} {method} {units} {vals} {ms:.4f}")
+
+    for n, dx, method, units in ((N, DX, "asm", UNITS), (512, 2e-6, "asm", 1), (1000, 1.5e-6, "asm_pad", 2)):
+        for prec in (wos.FP32, wos.FP64):
+            b = wos.Batch(n, dx, np.linspace(500e-9, 600e-9, units), np.full(units, Z), wos.ctx_for(550e-9),
+                          method, prec)
+            b.fill_schell(W0, SIGMA_C, SEED)
+            shape, ms = b.autotune(5)
+            add(n, prec, method, units, shape, ms)
+            b.close()
+    for prec in (wos.FP32, wos.FP64):
+        p = config4_pipeline(wos, prec)
+        ms = p.autotune(3)
+        for i, (n, units) in enumerate(p.stages):
+            add(n, prec, "asm", units, p.get_tuning(i), ms[i])
+        p.close()
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -518,6 +577,9 @@ def main():
     ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
     ap.add_argument("--only", default=None, choices=["config12", "bpm", "pipeline", "adjoint", "train"],
                     help="run one of the other workloads alone and print its figures (profiling)")
+    ap.add_argument("--retune", action="store_true", help="autotune even where the tuning table has a shape")
+    ap.add_argument("--write-tuning", default=None, metavar="PATH",
+                    help="autotune the benchmark's batches and pipeline stages, write the tuning table, exit")
     args = ap.parse_args()
     shape32 = [int(x) for x in args.shape32.split(",")] if args.shape32 else None
     shape64 = [int(x) for x in args.shape64.split(",")] if args.shape64 else None
@@ -528,15 +590,19 @@ def main():
 
     import wos_hip as wos
     wos.init()
+    if args.write_tuning:
+        lines = write_tuning(wos, args.write_tuning)
+        print(json.dumps({"tuning_table": args.write_tuning, "entries": len(lines) - 2}), flush=True)
+        return
     if args.only:
         tune = not args.no_tune
-        run = {"config12": lambda: bench_config12(wos, tune)[0], "bpm": lambda: bench_bpm(wos)[0],
-               "pipeline": lambda: bench_pipeline(wos, args.steps, tune), "adjoint": lambda: bench_adjoint(wos)[0],
-               "train": lambda: bench_train(wos)[0]}[args.only]
+        run = {"config12": lambda: bench_config12(wos, tune, args.retune)[0], "bpm": lambda: bench_bpm(wos)[0],
+               "pipeline": lambda: bench_pipeline(wos, args.steps, tune, args.retune),
+               "adjoint": lambda: bench_adjoint(wos)[0], "train": lambda: bench_train(wos)[0]}[args.only]
         print(json.dumps({"only": args.only, "result": run()}), flush=True)
         return
-    r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup, shape32)
-    r64 = run_gpu(wos, wos.FP64, args.steps, args.warmup, shape64)
+    r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup, shape32, args.retune)
+    r64 = run_gpu(wos, wos.FP64, args.steps, args.warmup, shape64, args.retune)
 
     props32 = UNITS / (r32["ms_per_step"] * 1e-3)
     props64 = UNITS / (r64["ms_per_step"] * 1e-3)
@@ -614,11 +680,11 @@ def main():
         "fft2d_z_2048_ms": round(fft2d_dropin_ms(wos, r64["unit0_in"]), 3),
         "fft2d_note": "one complex128 2-D FFT of a host buffer per call, transfers included; the reference "
                       "publishes 45 ms for this call through cuFFT on a Tesla T4 (README.md:113)"}
-    cfg12, fields = bench_config12(wos, not args.no_tune)
+    cfg12, fields = bench_config12(wos, not args.no_tune, args.retune)
     line["config1_512_asm"] = cfg12["config1"]
     line["config2_1000_asm_pad_jones"] = cfg12["config2"]
     line["bpm_1024_400_slices"], bpm20 = bench_bpm(wos)
-    line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps, not args.no_tune)
+    line["pipeline_one_graph_224_props"] = bench_pipeline(wos, args.steps, not args.no_tune, args.retune)
     line["mask_adjoint"], adj = bench_adjoint(wos)
     line["element_train_2048"], train_in, train_out64 = bench_train(wos)
     if not args.no_cpu:

@@ -107,6 +107,21 @@ const char* wos_hip_version(void);
 double wos_hip_last_device_ms(void);
 /* frees every cached plan/buffer (device memory returns to the runtime) */
 int wos_hip_release_cache(void);
+/* CheckGridMemory (optics/memory.go:74-81) for device memory: WOS_OK when
+ * `units` resident propagations of n x n by `method` in `precision` fit the
+ * budget (75% of the device's memory, memoryBudgetBytes' share above 8 GiB),
+ * else WOS_ERR_NOMEM with a message like the reference's ("grid NxN would
+ * need X of device memory but only Y is available ..."); need and budget
+ * (nullable) receive the estimate and the budget in bytes. Batch and
+ * pipeline creation apply the same check before they allocate.
+ *
+ * Failure injection (all entry points): the environment variable
+ * WOS_HIP_FAIL, a comma-separated list of entry-point names or "all", makes
+ * each listed entry point return WOS_ERR_HIP ("injected failure") before it
+ * reads its arguments or touches the device, so a caller's fallback path can
+ * be exercised without a broken GPU; with a listed wos_hip_device_count
+ * (or "all") the library reports no device. */
+int wos_hip_check_grid_memory(int n, int method, int precision, int units, uint64_t* need, uint64_t* budget);
 
 /* ---- drop-in transforms: gpuFFT2D / gpuFFT1D (optics/gpu.go:275-304) ---- */
 /* Lengths: powers of two up to 8192 (Stockham radix 16/8); other lengths
@@ -505,6 +520,21 @@ int wos_hip_field_view(wos_field f, int view, double mask_cut, float* out);
 /* ComputeStats (optics/metrics.go:31-91) of the resident field */
 int wos_hip_field_stats(wos_field f, double wl, double strehl_aperture, double strehl_distance,
                         wos_plane_stats* out);
+
+/* ---- tuning tables (no reference counterpart) ----
+ * A text file of launch shapes found by an earlier autotune, one entry a
+ * line ('#' comments):
+ *   n precision method units tile_w col_w rows groups split radix col_radix col_split [ms]
+ * precision "fp64" or "fp32", method "asm" or "asm_pad", the eight shape
+ * fields as wos_hip_batch_autotune reports them. A lookup takes the entries
+ * of the grid's n, precision and method and the nearest unit count.
+ * Loading applies the entry's shape (WOS_ERR_UNSUPPORTED: no entry). */
+int wos_hip_batch_load_tuning(wos_batch b, const char* path, int* shape);
+/* the launch shape of one pipeline stage (8 ints, as above), read or set */
+int wos_hip_pipeline_get_tuning(wos_pipeline p, int stage, int* shape);
+int wos_hip_pipeline_set_tuning(wos_pipeline p, int stage, const int* shape);
+/* every stage with an entry takes its shape; *applied = how many did */
+int wos_hip_pipeline_load_tuning(wos_pipeline p, const char* path, int* applied);
 
 /* ---- phase-mask design operator (no reference counterpart) --------------
  * Forward/Backward pair behind a Go operator for phase-mask optimisation. For

@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -110,3 +111,42 @@ def test_no_device_reports_error_without_computing():
         wos_hip.propagate(a, None, 1e-6, 1e-3, "asm", wos_hip.ctx_for(632.8e-9))
     with pytest.raises(wos_hip.WosError):
         wos_hip.Field(8, 1e-6)
+
+
+def fft2d_with_fallback(wos_hip, oracle, a):
+    """fft2D's dispatch (optics/fft.go:209-216): the GPU call, and the CPU
+    path on any error, run on the caller's untouched array."""
+    b = np.array(a, dtype=np.complex128, order="C")
+    rc = wos_hip.load().wos_hip_fft2d_z(b.ctypes.data, b.shape[0], 0)
+    if rc != 0:
+        assert np.array_equal(b, a)   # a failed call leaves the slice as it was
+        return oracle.fft2d(b), rc
+    return b, rc
+
+
+def test_failure_injection_drives_the_fallback(monkeypatch):
+    """WOS_HIP_FAIL makes the listed entry points fail before they touch
+    their arguments, on any machine: the caller's CPU path then runs on the
+    unmodified data; unlisted entry points behave as without it; "all" (or a
+    listed wos_hip_device_count) reports no device (GPUAvailable false)."""
+    import oracle as O
+    import wos_hip
+    lib = wos_hip.load()
+    a = O.ref_field(16, 1.3)
+    monkeypatch.setenv("WOS_HIP_FAIL", "wos_hip_propagate,wos_hip_fft2d_z")
+    got, rc = fft2d_with_fallback(wos_hip, O, a)
+    assert rc == 3 and b"injected failure" in lib.wos_hip_last_error()
+    assert np.allclose(got, np.fft.fft2(a), rtol=0, atol=1e-11)
+    x = a.copy()
+    with pytest.raises(wos_hip.WosError, match="injected"):
+        wos_hip.propagate(x, None, 1e-6, 1e-3, "asm", wos_hip.ctx_for(632.8e-9))
+    assert np.array_equal(x, a)
+    count = lib.wos_hip_device_count()
+    rc1 = lib.wos_hip_fft1d_z(x.ctypes.data, 16, 0)   # not listed
+    assert rc1 != 3 or b"injected" not in lib.wos_hip_last_error()
+    assert rc1 == (0 if count > 0 else 1)
+    monkeypatch.setenv("WOS_HIP_FAIL", "all")
+    assert lib.wos_hip_device_count() == 0
+    assert lib.wos_hip_fft1d_z(x.ctypes.data, 16, 0) == 3
+    monkeypatch.setenv("WOS_HIP_FAIL", "wos_hip_fft2d")   # a prefix names no entry point
+    assert lib.wos_hip_fft2d_z(a.copy().ctypes.data, 16, 0) != 3 or b"injected" not in lib.wos_hip_last_error()

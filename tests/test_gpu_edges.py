@@ -161,3 +161,46 @@ def test_concurrent_callers_match_serial_runs(wos):
     for i in range(6):
         for a, b in zip(results[i], serial[i]):
             assert np.array_equal(a, b), i
+
+
+def test_grid_memory_budget_refuses_before_allocating(wos):
+    """CheckGridMemory on the device (optics/memory.go:74-81): the headline
+    batch fits the budget (75% of the device's memory); a batch or pipeline
+    stage far beyond it is refused with the reference's kind of message
+    before anything is allocated, and the library keeps working."""
+    need, budget = wos.check_grid_memory(2048, "asm", wos.FP64, 64)
+    print(f"64 x 2048^2 FP64 needs {need / 2**30:.1f} GiB of a {budget / 2**30:.1f} GiB budget", flush=True)
+    assert 0 < need < budget
+    with pytest.raises(wos.WosError, match="would need .* of device memory"):
+        wos.check_grid_memory(2048, "asm", wos.FP64, 5000)
+    with pytest.raises(wos.WosError, match="would need"):
+        wos.Batch(2048, 2e-6, np.full(5000, 532e-9), np.full(5000, 1e-3), wos.ctx_for(532e-9), "asm", wos.FP64)
+    p = wos.Pipeline(wos.FP32)
+    with pytest.raises(wos.WosError, match="would need"):
+        p.add(4096, 1e-6, np.full(3000, 532e-9), np.full(3000, 1e-3), wos.ctx_for(532e-9), "asm")
+    s = p.add(64, 1e-6, np.full(2, 532e-9), np.full(2, 1e-3), wos.ctx_for(532e-9), "asm")
+    p.run(1)
+    p.sync()
+    p.close()
+    b = wos.Batch(64, 1e-6, [532e-9], [1e-3], wos.ctx_for(532e-9), "asm", wos.FP64)
+    b.run(1)
+    b.sync()
+    b.close()
+
+
+def test_failure_injection_on_the_device(wos, oracle, monkeypatch):
+    """With the GPU present, a listed entry point fails before it touches the
+    caller's array (the fallback runs on the original data), and without the
+    variable the same call runs on the device again."""
+    from test_abi import fft2d_with_fallback
+    a = oracle.ref_field(64, 0.9)
+    monkeypatch.setenv("WOS_HIP_FAIL", "wos_hip_fft2d_z")
+    got, rc = fft2d_with_fallback(wos, oracle, a)
+    assert rc == 3
+    want = oracle.fft2d(a)
+    assert np.array_equal(got, want)
+    assert wos.device_count() > 0
+    monkeypatch.delenv("WOS_HIP_FAIL")
+    got, rc = fft2d_with_fallback(wos, oracle, a)
+    assert rc == 0
+    assert float(np.sqrt(np.sum(np.abs(got - want) ** 2) / np.sum(np.abs(want) ** 2))) <= 1e-12

@@ -2184,15 +2184,93 @@ static bool valid_prec(int p) { return p == WOS_FP64 || p == WOS_FP32; }
 
 using namespace wos;
 
-#define WOS_ENTER()                                       \
-  int rc_ = ensure_init();                                \
-  if (rc_) return rc_;                                    \
-  std::lock_guard<std::recursive_mutex> lk_(device().mu); \
+// Failure injection for the callers' fallback paths: WOS_HIP_FAIL holds a
+// comma-separated list of entry-point names, or "all"; a listed entry point
+// returns WOS_ERR_HIP at once, before it reads its arguments or touches the
+// device, so the caller's buffers are exactly as it passed them (the Go
+// caller then runs its CPU path, optics/fft.go:136-141, 210-215).
+static bool fail_injected(const char* fn) {
+  const char* e = getenv("WOS_HIP_FAIL");
+  if (!e || !*e) return false;
+  const std::string spec(e), name(fn);
+  if (spec == "all") return true;
+  size_t pos = 0;
+  while (true) {
+    const size_t q = spec.find(',', pos);
+    if (spec.compare(pos, q == std::string::npos ? std::string::npos : q - pos, name) == 0) return true;
+    if (q == std::string::npos) return false;
+    pos = q + 1;
+  }
+}
+
+#define WOS_ENTER()                                                                   \
+  if (fail_injected(__func__)) {                                                      \
+    set_error(std::string(__func__) + ": injected failure (WOS_HIP_FAIL)");           \
+    return WOS_ERR_HIP;                                                               \
+  }                                                                                   \
+  int rc_ = ensure_init();                                                            \
+  if (rc_) return rc_;                                                                \
+  std::lock_guard<std::recursive_mutex> lk_(device().mu);                             \
   (void)hipSetDevice(device().id)
+
+namespace wos {
+// humanBytes (optics/memory.go:57-70)
+static std::string human_bytes(uint64_t b) {
+  char buf[64];
+  if (b >= (1ull << 40)) snprintf(buf, sizeof(buf), "%.1f TiB", (double)b / (double)(1ull << 40));
+  else if (b >= (1ull << 30)) snprintf(buf, sizeof(buf), "%.1f GiB", (double)b / (double)(1ull << 30));
+  else if (b >= (1ull << 20)) snprintf(buf, sizeof(buf), "%.1f MiB", (double)b / (double)(1ull << 20));
+  else if (b >= (1ull << 10)) snprintf(buf, sizeof(buf), "%.1f KiB", (double)b / (double)(1ull << 10));
+  else snprintf(buf, sizeof(buf), "%llu B", (unsigned long long)b);
+  return buf;
+}
+
+// Device bytes of `units` resident propagations of n x n (fields, the
+// column-tiled intermediate of the M-point transforms, the complex128
+// staging an FP32 upload passes through).
+static uint64_t grid_bytes(int n, int method, int prec, int units) {
+  const int M = method == WOS_METHOD_ASM_PAD || method == WOS_METHOD_ASM_SHIFT_PAD ? 2 * n : n;
+  const uint64_t S = csize(prec), nn = (uint64_t)n * n;
+  return (uint64_t)units * (nn * S + (uint64_t)inter_unit_elems(n, M) * S) + (prec == WOS_FP32 ? nn * 16 : 0);
+}
+
+// CheckGridMemory (optics/memory.go:74-81) for device memory: the budget is
+// 75% of the device's memory (memoryBudgetBytes' share on machines above
+// 8 GiB); need is the caller's estimate for the whole job.
+static int check_grid_budget(const char* what, int n, uint64_t need, uint64_t* budget_out = nullptr) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) return WOS_OK;  // unknown: hipMalloc decides
+  const uint64_t budget = (uint64_t)total_b - (uint64_t)total_b / 4;
+  if (budget_out) *budget_out = budget;
+  if (need > budget) {
+    set_error(std::string(what) + ": grid " + std::to_string(n) + "x" + std::to_string(n) + " would need " +
+              human_bytes(need) + " of device memory but only " + human_bytes(budget) +
+              " is available; reduce the grid size or the unit count");
+    return WOS_ERR_NOMEM;
+  }
+  return WOS_OK;
+}
+}  // namespace wos
 
 extern "C" {
 
+// CheckGridMemory (optics/memory.go:74-81) on the device: WOS_OK when `units`
+// resident propagations of n x n by `method` in `precision` fit the budget,
+// else WOS_ERR_NOMEM with the reference's kind of message; need / budget
+// (nullable) receive the estimate and the budget in bytes.
+int wos_hip_check_grid_memory(int n, int method, int precision, int units, uint64_t* need, uint64_t* budget) {
+  WOS_ENTER();
+  if (n <= 0 || units <= 0 || !valid_prec(precision)) {
+    set_error("wos_hip_check_grid_memory: bad arguments");
+    return WOS_ERR_INVALID;
+  }
+  const uint64_t b = grid_bytes(n, method, precision, units);
+  if (need) *need = b;
+  return check_grid_budget("check_grid_memory", n, b, budget);
+}
+
 int wos_hip_device_count(void) {
+  if (fail_injected(__func__)) return 0;   // GPUAvailable() then reports no GPU
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
@@ -2803,6 +2881,10 @@ int wos_hip_batch_create(wos_batch* out, int n, double dx, int method, int preci
     set_error("wos_hip_batch_create: transform length not supported");
     return WOS_ERR_UNSUPPORTED;
   }
+  {
+    const int brc = check_grid_budget("wos_hip_batch_create", n, grid_bytes(n, method, precision, n_units));
+    if (brc) return brc;
+  }
   wos_batch_s* b = new wos_batch_s();
   b->n = n;
   b->M = M;
@@ -3122,55 +3204,72 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) 
   return WOS_OK;
 }
 
+}  // extern "C"
+
+// A launch shape {tile_w, col_w, rows, groups, split, radix, col_radix,
+// col_split} checked against what the kernels build for an M-point grid in
+// precision prec (mixed-radix grids take the two tile widths only).
+static int shape_to_tuning(int prec, int M, const int* sh, Tuning* out, const char* what) {
+  int tile_w = sh[0], col_w = sh[1], rows = sh[2], groups = sh[3], split = sh[4], radix = sh[5], col_radix = sh[6],
+      col_split = sh[7];
+  if (radix == 0) radix = 16;
+  if (col_radix == 0) col_radix = 16;
+  Tuning t;
+  if (!pow2(M) && mixed_col_w(prec, M) > 0) {  // mixed radix: the tile widths only
+    if (!mixed_shape_valid(prec, M, tile_w, col_w)) {
+      set_error(std::string(what) + ": tile width " + std::to_string(tile_w) + " / column width " +
+                std::to_string(col_w) + " not built for the mixed-radix grid of " + std::to_string(M));
+      return WOS_ERR_INVALID;
+    }
+    t.tw = tile_w;
+    t.col_w = col_w;
+    *out = t;
+    return WOS_OK;
+  }
+  if (col_split && col_radix != 16) {
+    set_error(std::string(what) + ": split column exchanges are radix-16 column passes only");
+    return WOS_ERR_INVALID;
+  }
+  if (!(radix == 16 || (radix == 8 && radix8_ok(M) && !split)) ||
+      !(col_radix == 16 || (col_radix == 8 && radix8_ok(M)))) {
+    set_error(std::string(what) + ": row radix " + std::to_string(radix) + (split ? " with split exchanges" : "") +
+              " or column radix " + std::to_string(col_radix) + " not built for a grid of " + std::to_string(M));
+    return WOS_ERR_INVALID;
+  }
+  if (!pow2(M) || tile_w <= 0 || (M % tile_w) != 0 || col_w <= 0 || (tile_w % col_w) != 0 ||
+      !col_w_valid(prec, M, col_w, col_radix) || (rows > 0 && !rows_valid(prec, M, rows, radix)) ||
+      !(groups == 0 || groups == 1 || groups == 2 || groups == 4)) {
+    set_error(std::string(what) + ": tile width " + std::to_string(tile_w) + ", column width " +
+              std::to_string(col_w) + ", rows " + std::to_string(rows) + " or row groups " + std::to_string(groups) +
+              " not built for a grid of " + std::to_string(M) + " under the radix-" + std::to_string(radix) +
+              " row / radix-" + std::to_string(col_radix) + " column schedules");
+    return WOS_ERR_INVALID;
+  }
+  t.tw = tile_w;
+  t.col_w = col_w;
+  t.rows = rows;
+  t.groups = groups;
+  t.split = split ? 1 : 0;
+  t.radix = radix;
+  t.col_radix = col_radix;
+  t.col_split = col_split ? 1 : 0;
+  *out = t;
+  return WOS_OK;
+}
+
+extern "C" {
+
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
                              int col_radix, int col_split) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
-  if (radix == 0) radix = 16;
-  if (col_radix == 0) col_radix = 16;
-  if (!pow2(b->M) && mixed_col_w(b->prec, b->M) > 0) {  // mixed radix: the tile widths only
-    if (!mixed_shape_valid(b->prec, b->M, tile_w, col_w)) {
-      set_error("wos_hip_batch_set_tuning: tile width " + std::to_string(tile_w) + " / column width " +
-                std::to_string(col_w) + " not built for the mixed-radix grid of " + std::to_string(b->M));
-      return WOS_ERR_INVALID;
-    }
-    WOS_HIP_TRY(hipStreamSynchronize(b->stream));
-    batch_drop_graph(b);
-    b->tune = Tuning();
-    b->tune.tw = tile_w;
-    b->tune.col_w = col_w;
-    return WOS_OK;
-  }
-  if (col_split && col_radix != 16) {
-    set_error("wos_hip_batch_set_tuning: split column exchanges are radix-16 column passes only");
-    return WOS_ERR_INVALID;
-  }
-  if (!(radix == 16 || (radix == 8 && radix8_ok(b->M) && !split)) ||
-      !(col_radix == 16 || (col_radix == 8 && radix8_ok(b->M)))) {
-    set_error("wos_hip_batch_set_tuning: row radix " + std::to_string(radix) + (split ? " with split exchanges" : "") +
-              " or column radix " + std::to_string(col_radix) + " not built for a grid of " + std::to_string(b->M));
-    return WOS_ERR_INVALID;
-  }
-  if (!pow2(b->M) || tile_w <= 0 || (b->M % tile_w) != 0 || col_w <= 0 || (tile_w % col_w) != 0 ||
-      !col_w_valid(b->prec, b->M, col_w, col_radix) || (rows > 0 && !rows_valid(b->prec, b->M, rows, radix)) ||
-      !(groups == 0 || groups == 1 || groups == 2 || groups == 4)) {
-    set_error("wos_hip_batch_set_tuning: tile width " + std::to_string(tile_w) + ", column width " +
-              std::to_string(col_w) + ", rows " + std::to_string(rows) + " or row groups " + std::to_string(groups) +
-              " not built for a grid of " + std::to_string(b->M) + " under the radix-" + std::to_string(radix) +
-              " row / radix-" + std::to_string(col_radix) + " column schedules");
-    return WOS_ERR_INVALID;
-  }
+  const int sh[8] = {tile_w, col_w, rows, groups, split, radix, col_radix, col_split};
+  Tuning t;
+  int rc = shape_to_tuning(b->prec, b->M, sh, &t, "wos_hip_batch_set_tuning");
+  if (rc) return rc;
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
   batch_drop_graph(b);
-  b->tune = Tuning();
-  b->tune.tw = tile_w;
-  b->tune.col_w = col_w;
-  b->tune.rows = rows;
-  b->tune.groups = groups;
-  b->tune.split = split ? 1 : 0;
-  b->tune.radix = radix;
-  b->tune.col_radix = col_radix;
-  b->tune.col_split = col_split ? 1 : 0;
+  b->tune = t;
   return WOS_OK;
 }
 
@@ -3264,3 +3363,4 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 #include "wos_field.inc"
 #include "wos_aniso.inc"
 #include "wos_noalloc.inc"
+#include "wos_tuning.inc"

@@ -48,8 +48,11 @@ EXPORTS = [
     "wos_hip_propagate_vectorial", "wos_hip_propagate_uniaxial", "wos_hip_field_uniaxial", "wos_hip_field_view",
     "wos_hip_propagate_anisotropic", "wos_hip_field_anisotropic", "wos_hip_schell_realizations",
     "wos_hip_average_intensity", "wos_hip_propagate_partially_coherent", "wos_hip_pipeline_set_average",
-    "wos_hip_propagate_device_noalloc",
+    "wos_hip_propagate_device_noalloc", "wos_hip_check_grid_memory", "wos_hip_batch_load_tuning",
+    "wos_hip_pipeline_get_tuning", "wos_hip_pipeline_set_tuning", "wos_hip_pipeline_load_tuning",
 ]
+# the tuning table committed beside the library (written by bench.py --write-tuning on an MI355X)
+TUNING_TABLE = os.path.join(_HERE, "tuning_mi355x.txt")
 
 VIEWS = ["total", "amplitude", "ex", "ey", "phase_x", "phase_y", "pol_intensity", "pol_azimuth", "pol_ellip",
          "pol_s1", "pol_s2", "pol_s3", "pol_degree"]
@@ -278,6 +281,11 @@ def load(path=None):
                                                        C.POINTER(Warnings)]
     L.wos_hip_pipeline_set_average.argtypes = [P, I]
     L.wos_hip_propagate_device_noalloc.argtypes = [P, I, I, D, D, I, C.POINTER(Ctx), I, P]
+    L.wos_hip_check_grid_memory.argtypes = [I, I, I, I, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.wos_hip_batch_load_tuning.argtypes = [P, C.c_char_p, P]
+    L.wos_hip_pipeline_get_tuning.argtypes = [P, I, P]
+    L.wos_hip_pipeline_set_tuning.argtypes = [P, I, P]
+    L.wos_hip_pipeline_load_tuning.argtypes = [P, C.c_char_p, C.POINTER(I)]
     _lib = L
     return L
 
@@ -604,6 +612,16 @@ def volume(ex, ey, dx, zs, method, ctx, precision=FP64):
     return ox, oy, w
 
 
+def check_grid_memory(n, method="asm", precision=FP64, units=1):
+    """CheckGridMemory on the device: (need, budget) in bytes; raises
+    WosError (WOS_ERR_NOMEM) when the estimate exceeds the budget."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    need, budget = C.c_uint64(), C.c_uint64()
+    _check(load().wos_hip_check_grid_memory(n, m, precision, units, C.byref(need), C.byref(budget)),
+           "check_grid_memory")
+    return need.value, budget.value
+
+
 def propagate_device_noalloc(dev_ptr, n, ncomp, dx, z, method, ctx, precision=FP64, stream=None):
     """wos_hip_propagate_device_noalloc on device memory at dev_ptr (in place,
     enqueued on `stream`, None: the library's stream)."""
@@ -693,6 +711,16 @@ class Batch:
         _check(load().wos_hip_batch_set_tuning(self.h, tile_w, tile_w if col_w is None else col_w, rows, groups,
                                                split, radix, radix if col_radix is None else col_radix, col_split),
                "batch_set_tuning")
+
+    def load_tuning(self, path=None):
+        """Apply the tuning table's shape for this batch; returns the shape
+        dict, or None when the table has no entry for the grid."""
+        shape = np.zeros(8, dtype=np.int32)
+        rc = load().wos_hip_batch_load_tuning(self.h, (path or TUNING_TABLE).encode(), _p(shape))
+        if rc == 4:
+            return None
+        _check(rc, "batch_load_tuning")
+        return dict(zip(TUNING_KEYS, (int(x) for x in shape)))
 
     def capture(self):
         _check(load().wos_hip_batch_capture(self.h), "batch_capture")
@@ -816,6 +844,23 @@ class Pipeline:
         ms = np.zeros(len(self.stages), dtype=np.float64)
         _check(load().wos_hip_pipeline_autotune(self.h, iters, _p(ms)), "pipeline_autotune")
         return ms
+
+    def get_tuning(self, stage):
+        shape = np.zeros(8, dtype=np.int32)
+        _check(load().wos_hip_pipeline_get_tuning(self.h, stage, _p(shape)), "pipeline_get_tuning")
+        return dict(zip(TUNING_KEYS, (int(x) for x in shape)))
+
+    def set_tuning(self, stage, shape):
+        sh = np.ascontiguousarray([shape[k] for k in TUNING_KEYS] if isinstance(shape, dict) else shape,
+                                  dtype=np.int32)
+        _check(load().wos_hip_pipeline_set_tuning(self.h, stage, _p(sh)), "pipeline_set_tuning")
+
+    def load_tuning(self, path=None):
+        """Stages with an entry in the tuning table take its shape; returns how many did."""
+        n = C.c_int()
+        _check(load().wos_hip_pipeline_load_tuning(self.h, (path or TUNING_TABLE).encode(), C.byref(n)),
+               "pipeline_load_tuning")
+        return n.value
 
     def capture(self):
         _check(load().wos_hip_pipeline_capture(self.h), "pipeline_capture")
