@@ -124,12 +124,20 @@ int wos_hip_release_cache(void);
 int wos_hip_check_grid_memory(int n, int method, int precision, int units, uint64_t* need, uint64_t* budget);
 
 /* ---- drop-in transforms: gpuFFT2D / gpuFFT1D (optics/gpu.go:275-304) ---- */
-/* Lengths: powers of two up to 8192 (Stockham radix 16/8); other lengths
- * whose Bluestein size (next power of two >= 2n - 1) is at most 8192, i.e.
- * n <= 4096: mixed-radix Stockham when n = 2^a 3^b 5^c 7^d, Bluestein
- * otherwise, both with the reference's unnormalised forward and 1/n inverse.
- * Longer non-powers of two return WOS_ERR_UNSUPPORTED (the Go caller then
- * keeps its CPU path, as optics/fft.go:136-141 does on any GPU error). */
+/* Lengths: powers of two up to 8192 run one workgroup per transform
+ * (Stockham radix 16/8); other lengths whose Bluestein size (next power of
+ * two >= 2n - 1) is at most 8192, i.e. n <= 4096, run mixed-radix Stockham
+ * when n = 2^a 3^b 5^c 7^d and Bluestein otherwise. Beyond that, powers of
+ * two up to 2^26 run a four-step factorisation N = N1 N2 over the same
+ * kernels (column transforms, an exact twiddle, row transforms, a
+ * transpose), and other lengths up to 2^25 run the reference's chirp-z
+ * chain (optics/fft.go:159-204) on four-step transforms; fft2d runs such
+ * lengths as row transforms, a transpose, row transforms, a transpose
+ * back. All with the reference's unnormalised forward and 1/n inverse
+ * (fft2d: 1/n per pass). Longer lengths, and grids beyond the device
+ * memory budget, return WOS_ERR_UNSUPPORTED before reading the array (the
+ * Go caller then keeps its CPU path, as optics/fft.go:136-141 does on any
+ * GPU error). */
 /* In-place 2-D transform of an n*n complex128 host array (fft2D, optics/fft.go:209-216). */
 int wos_hip_fft2d_z(double* a, int n, int inverse);
 /* In-place 1-D transform of a length-n complex128 host array (fft1DAny, optics/fft.go:135-142). */

@@ -82,10 +82,13 @@ def test_invalid_and_unsupported_arguments(wos):
     assert lib.wos_hip_propagate(b.ctypes.data, None, 4, 1e-6, 1e-3, 0, C.byref(ctx), 7, None, None) == 2
     assert lib.wos_hip_propagate(b.ctypes.data, None, 4, 0.0, 1e-3, 0, C.byref(ctx), 0, None, None) == 2
     assert np.array_equal(b, np.ones((4, 4)))                    # nothing written
-    # unsupported: beyond the largest transform
+    # unsupported: beyond the largest transform (powers of two up to 2^26,
+    # other lengths whose Bluestein length is at most 2^26); refused before
+    # the buffer is read
     big = np.zeros(1, dtype=np.complex128)
-    assert lib.wos_hip_fft2d_z(big.ctypes.data, 16384, 0) == 4
-    assert lib.wos_hip_fft1d_z(big.ctypes.data, 8193, 0) == 4
+    assert lib.wos_hip_fft2d_z(big.ctypes.data, 1 << 27, 0) == 4
+    assert lib.wos_hip_fft1d_z(big.ctypes.data, 1 << 27, 0) == 4
+    assert lib.wos_hip_fft1d_z(big.ctypes.data, (1 << 25) + 1, 0) == 4
     assert b"not supported" in lib.wos_hip_last_error()
     # a handle-based call with a bad shape reports instead of launching
     bt = wos.Batch(256, 1e-6, [633e-9], [1e-3], ctx, "asm", wos.FP32)

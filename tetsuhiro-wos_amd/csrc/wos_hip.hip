@@ -80,6 +80,14 @@ static int bs_len(int L) {
 // lengths the engine transforms: powers of two up to 8192, others through
 // Bluestein with an internal length up to 8192
 static bool len_ok(int L) { return L >= 1 && (pow2(L) ? L <= max_pow2_len() : bs_len(L) <= max_pow2_len()); }
+// the drop-in transforms on the other lengths (wos_long.inc); the working
+// buffers of one chunk of rows stay under kLongChunkBytes
+constexpr size_t kLongChunkBytes = (size_t)2 << 30;
+static bool long_len_ok(long long L);
+static int long_pow2_dev(int prec, void* d, void* tmp, long long N, int units, int inverse, double scale,
+                         hipStream_t s);
+static int long_fft_dev(int prec, void* d, long long L, long long units, int inverse, double scale, hipStream_t s);
+static int long_fft2d_dev(int prec, void* d, int n, int inverse);
 
 static size_t csize(int prec) { return prec == WOS_FP64 ? 16 : 8; }
 
@@ -542,31 +550,38 @@ static int get_bs_tables(int L, int inverse, int prec, BsTab* out) {
     return 0;
   }
   const int M = bs_len(L);
-  if (M > max_pow2_len()) {
+  if (!long_len_ok(L)) {
     set_error("Bluestein length not supported: " + std::to_string(L));
     return WOS_ERR_UNSUPPORTED;
   }
   int rc;
-  void* tmp = nullptr;  // FP64 chirp[L] then b[M]
-  WOS_HIP_TRY(hipMalloc(&tmp, (size_t)(L + M) * 16));
+  void* tmp = nullptr;  // FP64 chirp[L] then b[M] (then M of four-step workspace for M > 8192)
+  WOS_HIP_TRY(hipMalloc(&tmp, (size_t)(L + M + (M > max_pow2_len() ? M : 0)) * 16));
   cpx<double>* c64 = (cpx<double>*)tmp;
   cpx<double>* b64 = c64 + L;
   hipLaunchKernelGGL(k_bs_chirp, dim3((L + 255) / 256), dim3(256), 0, d.stream, c64, L, inverse ? 1.0 : -1.0);
   hipLaunchKernelGGL(k_bs_b0, dim3((M + 255) / 256), dim3(256), 0, d.stream, (const cpx<double>*)c64, b64, L, M);
   WOS_HIP_TRY(hipGetLastError());
-  const void* tw;
-  if ((rc = get_twiddles(M, WOS_FP64, &tw))) return rc;
-  RowArgs ra = row_args_init();
-  ra.in = b64;
-  ra.out = b64;
-  ra.in_unit = ra.out_unit = M;
-  ra.in_ld = ra.out_ld = M;
-  ra.rows = 1;
-  ra.units = 1;
-  ra.in_cols = M;
-  ra.out_cols = M;
-  ra.tw = tw;
-  if ((rc = launch_rows(WOS_FP64, M, -1, ra, d.stream, 0))) return rc;
+  if (M > max_pow2_len()) {  // the four-step M-point transform
+    if ((rc = long_pow2_dev(WOS_FP64, b64, b64 + M, M, 1, 0, 1.0, d.stream))) {
+      (void)hipFree(tmp);
+      return rc;
+    }
+  } else {
+    const void* tw;
+    if ((rc = get_twiddles(M, WOS_FP64, &tw))) return rc;
+    RowArgs ra = row_args_init();
+    ra.in = b64;
+    ra.out = b64;
+    ra.in_unit = ra.out_unit = M;
+    ra.in_ld = ra.out_ld = M;
+    ra.rows = 1;
+    ra.units = 1;
+    ra.in_cols = M;
+    ra.out_cols = M;
+    ra.tw = tw;
+    if ((rc = launch_rows(WOS_FP64, M, -1, ra, d.stream, 0))) return rc;
+  }
   void* p = nullptr;
   WOS_HIP_TRY(hipMalloc(&p, (size_t)(L + M) * csize(prec)));
   if (prec == WOS_FP64) {
@@ -2315,7 +2330,8 @@ static int fft2d_host(int prec, void* a, int n, int inverse) {
     return WOS_ERR_INVALID;
   }
   if (n == 1) return WOS_OK;  // the 1-point transform is the identity in both directions
-  if (!len_ok(n)) {
+  const bool lng = !len_ok(n);
+  if (lng && !long_len_ok(n)) {
     set_error("2-D FFT size not supported: " + std::to_string(n));
     return WOS_ERR_UNSUPPORTED;
   }
@@ -2323,9 +2339,11 @@ static int fft2d_host(int prec, void* a, int n, int inverse) {
   const size_t nn = (size_t)n * n;
   const size_t bytes = nn * csize(prec);
   int rc;
+  if (lng && (rc = check_grid_budget("fft2d", n, 2 * bytes + (prec == WOS_FP32 ? nn * 16 : 0) + kLongChunkBytes * 2)))
+    return rc;
   if ((rc = dv.field.ensure(bytes))) return rc;
   if ((rc = host_to_dev(dv.field.p, a, bytes, dv.stream))) return rc;
-  if ((rc = fft2d_dev(prec, dv.field.p, n, 1, inverse))) return rc;
+  if ((rc = lng ? long_fft2d_dev(prec, dv.field.p, n, inverse) : fft2d_dev(prec, dv.field.p, n, 1, inverse))) return rc;
   if ((rc = dev_to_host(a, dv.field.p, bytes, dv.stream))) return rc;
   WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
   return WOS_OK;
@@ -2338,7 +2356,8 @@ static int fft1d_host(int prec, void* a, int n, int inverse) {
   }
   if (n == 1) return WOS_OK;
   Device& dv = device();
-  if (!len_ok(n)) {
+  const bool lng = !len_ok(n);
+  if (lng && !long_len_ok(n)) {
     set_error("1-D FFT length not supported: " + std::to_string(n));
     return WOS_ERR_UNSUPPORTED;
   }
@@ -2346,7 +2365,10 @@ static int fft1d_host(int prec, void* a, int n, int inverse) {
   int rc;
   if ((rc = dv.field.ensure(bytes))) return rc;
   if ((rc = host_to_dev(dv.field.p, a, bytes, dv.stream))) return rc;
-  if ((rc = rows_any(prec, dv.field.p, n, 1, n, n, 1, inverse, inverse ? 1.0 / (double)n : 1.0, dv.stream))) return rc;
+  const double sc = inverse ? 1.0 / (double)n : 1.0;
+  if ((rc = lng ? long_fft_dev(prec, dv.field.p, n, 1, inverse, sc, dv.stream)
+                : rows_any(prec, dv.field.p, n, 1, n, n, 1, inverse, sc, dv.stream)))
+    return rc;
   if ((rc = dev_to_host(a, dv.field.p, bytes, dv.stream))) return rc;
   WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
   return WOS_OK;
@@ -3364,3 +3386,4 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
 #include "wos_aniso.inc"
 #include "wos_noalloc.inc"
 #include "wos_tuning.inc"
+#include "wos_long.inc"
