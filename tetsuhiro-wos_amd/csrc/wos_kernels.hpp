@@ -452,6 +452,51 @@ __device__ __forceinline__ void gst(cpx<R>* p, cpx<R> v) {
   __builtin_nontemporal_store(x, (v2*)p);
 }
 
+// Nontemporal raw-buffer accesses: the descriptor (base, size) is uniform
+// and lives in SGPRs, each thread keeps one 32-bit VGPR offset, and the
+// offset of each of its slots is a compile-time byte count passed as the
+// scalar offset, where per-slot 64-bit addresses would stay live from the
+// loads to the stores of an in-place pass. `bytes` bounds the per-thread
+// offset only (the scalar offsets are added after the range check), so
+// callers keep every access inside the extent themselves.
+template <typename R>
+struct BufIO {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ BufIO(const void* base, uint32_t bytes) {
+    // word 3 0x00020000: raw 32-bit data format, as gfx9 buffer loads take it
+    rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+  }
+  __device__ __forceinline__ cpx<R> ld(uint32_t voff, uint32_t soff) const {
+    if constexpr (sizeof(R) == 4) {
+      typedef float f2 __attribute__((ext_vector_type(2)));
+      // the b64 builtin's result is an 8-byte value: reinterpret, never convert
+      const auto x = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, 2);
+      static_assert(sizeof(x) == 8, "64-bit buffer load");
+      const f2 f = __builtin_bit_cast(f2, x);
+      return {f.x, f.y};
+    } else {
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)soff, 2);
+      static_assert(sizeof(x) == 16, "128-bit buffer load");
+      const d2 d = __builtin_bit_cast(d2, x);
+      return {(R)d.x, (R)d.y};
+    }
+  }
+  __device__ __forceinline__ void st(uint32_t voff, uint32_t soff, cpx<R> v) const {
+    if constexpr (sizeof(R) == 4) {
+      typedef float f2 __attribute__((ext_vector_type(2)));
+      typedef unsigned v2u __attribute__((__vector_size__(2 * sizeof(unsigned))));
+      const f2 f = {(float)v.x, (float)v.y};
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, f), rs, (int)voff, (int)soff, 2);
+    } else {
+      typedef unsigned v4u __attribute__((__vector_size__(4 * sizeof(unsigned))));
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const d2 d = {(double)v.x, (double)v.y};
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, d), rs, (int)voff, (int)soff, 2);
+    }
+  }
+};
+
 __device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int tstride) {
   if (lg < 0) return row * ld + q;
   return (q >> lg) * tstride + (row << lg) + (q & ((1 << lg) - 1));
@@ -710,6 +755,20 @@ void k_cols(const ColArgs a) {
     for (int b = 0; b < V / R0; ++b)
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[b * R0 + r] = gld(ibase + slot_pos<N, R0, VV>(t, b, r) * tl);
+  } else if (fastw && sizeof(R) == 8) {
+    // FP64: the workgroup's tile through one buffer descriptor (uniform
+    // base), one VGPR offset per thread (2.71 -> 2.61 ms at 2048^2 x 64;
+    // the FP32 kernel is faster with plain addresses)
+    const cpx<R>* ib = (MODE == 4 ? (const cpx<R>*)a.src + (size_t)unit * a.src_unit
+                                  : (const cpx<R>*)a.data + (size_t)unit * a.unit) +
+                       (size_t)tile * a.tstride;
+    const BufIO<R> io(ib, (uint32_t)(N * W * sizeof(cpx<R>)));
+    const uint32_t vo = (uint32_t)(c + t * W) * sizeof(cpx<R>);
+#pragma unroll
+    for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+      for (int r = 0; r < R0; ++r)
+        v[b * R0 + r] = io.ld(vo, (uint32_t)((b * T + r * (N / R0)) * W * sizeof(cpx<R>)));
   } else if (fastw) {
 #pragma unroll
     for (int b = 0; b < V / R0; ++b)
@@ -889,7 +948,15 @@ void k_cols(const ColArgs a) {
 #pragma unroll
     for (int i = 0; i < V; ++i) v[i] = cscale(v[i], sc);
   }
-  if (fastw) {
+  if (fastw && sizeof(R) == 8) {
+    const BufIO<R> io((cpx<R>*)a.data + (size_t)unit * a.unit + (size_t)tile * a.tstride,
+                      (uint32_t)(N * W * sizeof(cpx<R>)));
+    const uint32_t vo = (uint32_t)(c + t * W) * sizeof(cpx<R>);
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) io.st(vo, (uint32_t)((b * T + r * (N / RL)) * W * sizeof(cpx<R>)), v[b * RL + r]);
+  } else if (fastw) {
 #pragma unroll
     for (int b = 0; b < V / RL; ++b)
 #pragma unroll
