@@ -1422,8 +1422,14 @@ static int transfer_dev(int prec, void* d, int n, int units, bool pad, const std
   if ((rc = dv.scratch.ensure((size_t)units * inter_unit_elems(n, M) * csize(prec)))) return rc;
   if ((rc = dv.params.ensure(sizeof(TFUnit) * units))) return rc;
   if ((rc = dv.upload_params(dv.params.p, tf.data(), sizeof(TFUnit) * units))) return rc;
+  // the asm column modes take no spectral shift (their transfer function
+  // is mirrored in fy): shifted transfers (asm_shift*) run the generic
+  // column mode, which evaluates every unit's own TF kind per element
+  bool shifted = false;
+  for (const TFUnit& u : tf) shifted = shifted || u.fy_off != 0.0;
+  const int mode_kind = shifted ? (int)TF_NONE : tf[0].kind;
   TransferLaunch L;
-  if ((rc = build_transfer(L, prec, d, dv.scratch.p, n, units, pad, (const TFUnit*)dv.params.p, tf[0].kind, post_re,
+  if ((rc = build_transfer(L, prec, d, dv.scratch.p, n, units, pad, (const TFUnit*)dv.params.p, mode_kind, post_re,
                            post_im, nullptr, nullptr, Tuning())))
     return rc;
   const int rpb = L.tu.rows;

@@ -138,6 +138,16 @@ __device__ __noinline__ double asm_evanescent(const TFUnit& u, double decay) {
 }
 
 
+// asm_evanescent inline, for kernels that must not call out.
+__device__ __forceinline__ double asm_evanescent_inl(const TFUnit& u, double decay) {
+  if (u.zero_ev || (u.limit > 0 && decay > u.limit)) return 0.0;
+  if (u.backward) {
+    const double a = exp(decay);
+    return u.reg ? a / (1 + u.alpha * u.alpha * a * a) : a;
+  }
+  return exp(-decay);
+}
+
 // sqrt for the FP32 path's phase: hardware reciprocal square root refined by
 // one Newton step (relative error ~1e-16, enough for a phase kz*s with
 // kz < 1e7 to stay far below the FP32 rounding of the field). 0 for x <= 0.
@@ -199,12 +209,24 @@ __device__ __noinline__ float asm_tf_fp32_evan(const TFUnit& u, TF32Col c, int i
   return (float)asm_evanescent(u, u.k * u.az * sqrt(-x));
 }
 
+// Evanescent factor of the FP32 path at row iy, inline (see asm_tf_fp64_inl
+// for why the column passes make no calls).
+__device__ __forceinline__ float asm_tf_fp32_evan_inl(const TFUnit& u, const TF32Col& c, int iy) {
+  const double ay = fma((double)(iy <= u.M / 2 ? iy : iy - u.M), c.cwl, c.coff);
+  const double x = fma(-ay, ay, c.one_m_ax2);
+  if (x >= 0) return 1.f;
+  return (float)asm_evanescent_inl(u, u.k * u.az * sqrt(-x));
+}
+
 // The FP32-path asmTF at row iy, evanescent rows included (their factor is
-// real: the propagating part returns 1 there).
+// real: the propagating part returns 1 there). INL: the evanescent branch
+// inline, for the register-resident column passes; the LDS-resident
+// mixed-radix passes keep it out of line (a smaller loop body there).
+template <bool INL = false>
 __device__ __forceinline__ cpx<float> asm_tf_fp32_full(const TFUnit& u, const TF32Col& c, int iy) {
   bool evan = false;
   cpx<float> h = asm_tf_fp32_prop(c, iy, u.M, &evan);
-  if (evan) h = {asm_tf_fp32_evan(u, c, iy), 0.f};
+  if (evan) h = {INL ? asm_tf_fp32_evan_inl(u, c, iy) : asm_tf_fp32_evan(u, c, iy), 0.f};
   return h;
 }
 
@@ -236,29 +258,28 @@ __device__ __forceinline__ cpx<float> fresnel_tf_fp32(const TFUnit& u, double fx
   return {pr * c - pi * s, pr * s + pi * c};
 }
 
-// Evanescent branch of the exact FP64 asmTF (rho > 1), out of line.
-__device__ __noinline__ cpx<double> asm_tf_fp64_evan(const TFUnit& u, double rho) {
-#pragma clang fp contract(off)
-  const double decay = u.k * u.az * sqrt((rho - 1) * (rho + 1));
-  return {asm_evanescent(u, decay), 0.0};
-}
-
-// Exact FP64 asmTF at transform position (ix, iy) with the propagating
-// branch inline: a kernel whose registers are full of transform values keeps
-// them across it (a call would have to save the values the calling
-// convention leaves unprotected); only the rare evanescent rows call out.
+// Exact FP64 asmTF at transform position (ix, iy), entirely inline: a
+// kernel whose registers are full of transform values keeps them across it.
+// A call, even on the rare evanescent rows, makes the register allocator
+// keep the live transform values in callee-saved registers or spill them
+// around the whole transfer-function section: the FP64 2048-point column
+// pass spilled 144-160 B per lane with calls here, 20 B without, and runs
+// 2.62 -> 2.24 ms at 2048^2 x 64. Both branches take one square root of
+// the reference's factored forms (optics/propagate.go:166,169).
 __device__ __forceinline__ cpx<double> asm_tf_fp64_inl(const TFUnit& u, int ix, int iy) {
 #pragma clang fp contract(off)
   const double fx = freq_of(ix, u.M, u.den) + u.fx_off;
   const double fy = freq_of(iy, u.M, u.den) + u.fy_off;
   const double rho = go_hypot(u.wl * fx, u.wl * fy);
-  if (rho <= 1) {
-    const double ph = u.k * u.z * sqrt((1 - rho) * (1 + rho));
+  const bool prop = rho <= 1;
+  const double sq = sqrt(prop ? (1 - rho) * (1 + rho) : (rho - 1) * (rho + 1));
+  if (prop) {
+    const double ph = u.k * u.z * sq;
     double sn, cs;
     sincos(ph, &sn, &cs);
     return {cs, sn};
   }
-  return asm_tf_fp64_evan(u, rho);
+  return {asm_evanescent_inl(u, u.k * u.az * sq), 0.0};
 }
 
 // Exact FP64 asmTF at transform position (ix, iy), out of line so the
@@ -800,8 +821,12 @@ void k_cols(const ColArgs a) {
       // thread evaluates H for its rows m < V/2, passes them on through LDS
       // (free between the two transforms) and takes the rest from its
       // partner: each wave evaluates half as many transfer-function values.
-      const bool mirror = ASM && T >= 2 && V >= 2 && u.fy_off == 0.0;
-      if (mirror) {
+      // The asm modes take no spectral shift (fy_off == 0): the host sends
+      // shifted transfers (asm_shift) to MODE 2, so these kernels carry no
+      // second transfer-function path, whose registers would spill the
+      // transform values of the first.
+      constexpr bool MIRROR = ASM && T >= 2 && V >= 2;
+      if constexpr (MIRROR) {
         constexpr int NB = V / RL, V2 = V / 2;
         cpx<R>* hx = lds + c;
         const int pt = (T - t) & (T - 1);
@@ -809,7 +834,7 @@ void k_cols(const ColArgs a) {
         const bool conj = u.kind == TF_CONJ_ASM;
         auto tf_at = [&](int p) -> cpx<R> {
           if constexpr (sizeof(R) == 4) {
-            return asm_tf_fp32_full(u, cc, p);
+            return asm_tf_fp32_full<true>(u, cc, p);
           } else {
             cpx<double> h = asm_tf_fp64_inl(u, col, p);
             if (conj) h.y = -h.y;
@@ -837,7 +862,8 @@ void k_cols(const ColArgs a) {
           const int ms = t == 0 ? V - m : V - 1 - m;
           v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], hx[(ms * T + pt) * W]);
         }
-      } else if constexpr (ASM && sizeof(R) == 4) {
+      } else {
+      if constexpr (ASM && sizeof(R) == 4) {
         const TF32Col cc = tf32_col(u, col);
         bool evan = false;
 #pragma unroll
@@ -850,7 +876,7 @@ void k_cols(const ColArgs a) {
           for (int b = 0; b < V / RL; ++b)
 #pragma unroll
             for (int r = 0; r < RL; ++r)
-              v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan(u, cc, slot_pos<N, RL, VV>(t, b, r)));
+              v[b * RL + r] = cscale(v[b * RL + r], asm_tf_fp32_evan_inl(u, cc, slot_pos<N, RL, VV>(t, b, r)));
         }
       } else if (MODE == 2 && u.kind == TF_ASM_BL) {
         // propagation then band limit: U <- B (H U), with the per-workgroup
@@ -878,13 +904,14 @@ void k_cols(const ColArgs a) {
             const int p = slot_pos<N, RL, VV>(t, b, r);
             cpx<R> h;
             if constexpr (ASM) {
-              h = asm_tf_fp64_at(u, col, p);
+              h = asm_tf_fp64_inl(u, col, p);
               if (conj) h.y = -h.y;
             } else {
               h = tf_eval<R>(u, col, p);
             }
             v[b * RL + r] = cmul(v[b * RL + r], h);
           }
+      }
       }
     }
   }
