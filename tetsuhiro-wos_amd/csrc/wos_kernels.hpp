@@ -721,7 +721,7 @@ struct ColArgs {
   double* pw3;         // MODE 3: per-workgroup sums |U|^2, |H U|^2, |H^2 U|^2; MODE 2 with
                        // TF_ASM_BL: per-workgroup sum |H U|^2 (nullable)
   int tlg;             // log2 of the layout's tile width (tiled); a workgroup covers W of its columns
-  const void* htab;    // MODE 3: FP64 H table in W-column tile order (nullable: evaluate H)
+  const void* htab;    // MODE 3: FP64 H table in W-column tile order (required)
   int radix;           // stage schedule of tw: 0 or 16 radix-16, 8 radix-8 (MODE 1 only)
   int split;           // MODE 1 and 4, radix-16: exchanges through half the LDS (real, then imaginary parts)
   const void* src;     // MODE 4: the forward spectrum, same layout as data
@@ -813,7 +813,7 @@ void k_cols(const ColArgs a) {
   if constexpr (MODE == 1 || MODE == 2 || MODE == 4) {
     constexpr int RL = last_rad<N, -1, VV>();
     if (active) {
-      const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
+      const TFUnit& u = a.tf[unit];  // by reference to global memory: a local copy would take registers
       // Without a spectral shift H(fx, -fy) = H(fx, fy), bit for bit (the
       // frequency of row N-p is the exact negative of row p's). Thread t
       // holds rows p = t + T m (slot (b, r): m = b + r V/RL); row N - p sits
@@ -925,15 +925,15 @@ void k_cols(const ColArgs a) {
     double s0 = 0, s1 = 0, s2 = 0;
     const bool want_pw = a.pw3 != nullptr;
     if (active) {
-      const TFUnit& u = a.tf[unit];  // global: a local copy would be spilled for the out-of-line calls
-      // H of this column from the table (same values as evaluating it here)
-      const cpx<double>* ht = a.htab ? (const cpx<double>*)a.htab + (size_t)(col / W) * N * W + (col % W) : nullptr;
+      // H of this column from the per-run table (htab is required: evaluating
+      // H here would hold its registers across the whole pass)
+      const cpx<double>* ht = (const cpx<double>*)a.htab + (size_t)(col / W) * N * W + (col % W);
 #pragma unroll
       for (int b = 0; b < V / RL; ++b)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
           const int p = slot_pos<N, RL, VV>(t, b, r);
-          const cpx<double> h = ht ? ht[(size_t)p * W] : asm_tf_fp64_at(u, col, p);
+          const cpx<double> h = ht[(size_t)p * W];
           const cpx<double> x = {(double)v[b * RL + r].x, (double)v[b * RL + r].y};
           const cpx<double> y = cmul(x, h);
           cpx<double> out = y;
