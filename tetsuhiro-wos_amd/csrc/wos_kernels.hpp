@@ -230,7 +230,9 @@ __device__ __forceinline__ cpx<float> asm_tf_fp32_full(const TFUnit& u, const TF
   return h;
 }
 
-__device__ __noinline__ cpx<float> asm_tf_fp32_generic(const TFUnit& u, double fx, double fy) {
+// FP32-path asmTF at any (fx, fy); INL keeps the evanescent branch inline
+template <bool INL>
+__device__ __forceinline__ cpx<float> asm_tf_fp32_generic_t(const TFUnit& u, double fx, double fy) {
   double ax = u.wl * fx, ay = u.wl * fy;
   double rho2 = ax * ax + ay * ay;
   if (rho2 <= 1) {
@@ -239,7 +241,10 @@ __device__ __noinline__ cpx<float> asm_tf_fp32_generic(const TFUnit& u, double f
     return {c, s};
   }
   const double decay = u.k * u.az * sqrt(rho2 - 1);
-  return {(float)asm_evanescent(u, decay), 0.f};
+  return {(float)(INL ? asm_evanescent_inl(u, decay) : asm_evanescent(u, decay)), 0.f};
+}
+__device__ __noinline__ cpx<float> asm_tf_fp32_generic(const TFUnit& u, double fx, double fy) {
+  return asm_tf_fp32_generic_t<false>(u, fx, fy);
 }
 
 // Fresnel TF (optics/propagate.go:511-513): cexpI(k z) * cexpI(-pi wl z f^2)
@@ -302,66 +307,66 @@ __device__ __forceinline__ double bandlimit_factor(const TFUnit& u, double fx, d
 }
 
 // Any transfer-function kind (asm, conjugate asm, fresnel, table, band
-// limit), out of line.
-template <typename R>
-__device__ cpx<R> tf_eval(const TFUnit& u, int ix, int iy);
-
-template <>
-__device__ __noinline__ cpx<double> tf_eval<double>(const TFUnit& u, int ix, int iy) {
+// limit). tf_eval is out of line (one copy of the switch for the LDS-resident
+// kernels); tf_eval_t<R, true> is the same code inline, for the
+// register-resident column passes, where a call would cost the registers of
+// every live transform value.
+template <typename R, bool INL>
+__device__ __forceinline__ cpx<R> tf_eval_t(const TFUnit& u, int ix, int iy) {
 #pragma clang fp contract(off)
-  double fx = freq_of(ix, u.M, u.den) + u.fx_off;
-  double fy = freq_of(iy, u.M, u.den) + u.fy_off;
-  switch (u.kind) {
-    case TF_ASM: return asm_tf_fp64(u, fx, fy);
-    case TF_CONJ_ASM: {
-      cpx<double> h = asm_tf_fp64(u, fx, fy);
-      return {h.x, -h.y};
+  const double fx = freq_of(ix, u.M, u.den) + u.fx_off;
+  const double fy = freq_of(iy, u.M, u.den) + u.fy_off;
+  if constexpr (sizeof(R) == 8) {
+    switch (u.kind) {
+      case TF_ASM: return asm_tf_fp64(u, fx, fy);
+      case TF_CONJ_ASM: {
+        cpx<double> h = asm_tf_fp64(u, fx, fy);
+        return {h.x, -h.y};
+      }
+      case TF_FRESNEL: return fresnel_tf_fp64(u, fx, fy);
+      case TF_TABLE: return ((const cpx<double>*)u.table)[(size_t)iy * u.M + ix];
+      case TF_BANDLIMIT: return {bandlimit_factor(u, fx, fy), 0.0};
+      default: return {1.0, 0.0};
     }
-    case TF_FRESNEL: return fresnel_tf_fp64(u, fx, fy);
-    case TF_TABLE: return ((const cpx<double>*)u.table)[(size_t)iy * u.M + ix];
-    case TF_BANDLIMIT: return {bandlimit_factor(u, fx, fy), 0.0};
-    default: return {1.0, 0.0};
+  } else {
+    switch (u.kind) {
+      case TF_ASM: return asm_tf_fp32_generic_t<INL>(u, fx, fy);
+      case TF_CONJ_ASM: {
+        cpx<float> h = asm_tf_fp32_generic_t<INL>(u, fx, fy);
+        return {h.x, -h.y};
+      }
+      case TF_FRESNEL: return fresnel_tf_fp32(u, fx, fy);
+      case TF_TABLE: return ((const cpx<float>*)u.table)[(size_t)iy * u.M + ix];
+      case TF_BANDLIMIT: return {(float)bandlimit_factor(u, fx, fy), 0.f};
+      default: return {1.f, 0.f};
+    }
   }
 }
 
+template <typename R>
+__device__ cpx<R> tf_eval(const TFUnit& u, int ix, int iy);
+template <>
+__device__ __noinline__ cpx<double> tf_eval<double>(const TFUnit& u, int ix, int iy) {
+  return tf_eval_t<double, false>(u, ix, iy);
+}
 template <>
 __device__ __noinline__ cpx<float> tf_eval<float>(const TFUnit& u, int ix, int iy) {
-  double fx = freq_of(ix, u.M, u.den) + u.fx_off;
-  double fy = freq_of(iy, u.M, u.den) + u.fy_off;
-  switch (u.kind) {
-    case TF_ASM: return asm_tf_fp32_generic(u, fx, fy);
-    case TF_CONJ_ASM: {
-      cpx<float> h = asm_tf_fp32_generic(u, fx, fy);
-      return {h.x, -h.y};
-    }
-    case TF_FRESNEL: return fresnel_tf_fp32(u, fx, fy);
-    case TF_TABLE: return ((const cpx<float>*)u.table)[(size_t)iy * u.M + ix];
-    case TF_BANDLIMIT: return {(float)bandlimit_factor(u, fx, fy), 0.f};
-    default: return {1.f, 0.f};
-  }
+  return tf_eval_t<float, false>(u, ix, iy);
 }
 
 // TF_ASM_BL at transform position (ix, iy): the asm transfer function h and
-// ApplyBandlimit's roll-off factor, out of line.
+// ApplyBandlimit's roll-off factor (inline; used by the column passes).
 template <typename R>
 struct TfBand {
   cpx<R> h;
   R band;
 };
 template <typename R>
-__device__ TfBand<R> asm_bl_eval(const TFUnit& u, int ix, int iy);
-
-template <>
-__device__ __noinline__ TfBand<double> asm_bl_eval<double>(const TFUnit& u, int ix, int iy) {
+__device__ __forceinline__ TfBand<R> asm_bl_eval(const TFUnit& u, int ix, int iy) {
 #pragma clang fp contract(off)
   const double fx = freq_of(ix, u.M, u.den), fy = freq_of(iy, u.M, u.den);
-  return {asm_tf_fp64(u, fx, fy), bandlimit_factor(u, fx, fy)};
-}
-
-template <>
-__device__ __noinline__ TfBand<float> asm_bl_eval<float>(const TFUnit& u, int ix, int iy) {
-  const double fx = freq_of(ix, u.M, u.den), fy = freq_of(iy, u.M, u.den);
-  return {asm_tf_fp32_generic(u, fx, fy), (float)bandlimit_factor(u, fx, fy)};
+  if constexpr (sizeof(R) == 8) return {asm_tf_fp64(u, fx, fy), bandlimit_factor(u, fx, fy)};
+  else return {asm_tf_fp32_generic_t<true>(u, fx, fy), (float)bandlimit_factor(u, fx, fy)};
 }
 
 // ---------------------------------------------------------------------------
@@ -907,7 +912,7 @@ void k_cols(const ColArgs a) {
               h = asm_tf_fp64_inl(u, col, p);
               if (conj) h.y = -h.y;
             } else {
-              h = tf_eval<R>(u, col, p);
+              h = tf_eval_t<R, true>(u, col, p);
             }
             v[b * RL + r] = cmul(v[b * RL + r], h);
           }
