@@ -101,6 +101,45 @@ def run_gpu(wos, precision, steps, warmup, shape=None, retune=False):
     return out
 
 
+def bench_persistent(wos, steps):
+    """The headline batch with the persistent transfer (one launch per step,
+    per-XCD work queues; the queued passes one after another, and
+    interleaved task by task), captured and replayed like the headline run;
+    units 0 and 63 compared with the three-launch run at the persistent
+    kernel's shape (bit-identical by construction)."""
+    out = {}
+    wls = np.linspace(400e-9, 700e-9, UNITS)
+    for prec, name in ((wos.FP32, "fp32"), (wos.FP64, "fp64")):
+        b = wos.Batch(N, DX, wls, np.full(UNITS, Z), wos.ctx_for(550e-9), "asm", prec)
+        r = 2 if prec == wos.FP64 else 4
+        b.set_tuning(r, r, r, 0, 1, 16, 16, 1)
+        b.fill_schell(W0, SIGMA_C, SEED)
+        b.run(1)
+        b.sync()
+        three = [b.download(0), b.download(UNITS - 1)]
+        res = {}
+        for il in (False, True):
+            b.fill_schell(W0, SIGMA_C, SEED)
+            b.set_persistent(True, il)
+            b.capture()
+            b.run(1)
+            b.sync()
+            same = bool(np.array_equal(b.download(0), three[0]) and np.array_equal(b.download(UNITS - 1), three[1]))
+            b.run(2)
+            b.sync()
+            t0 = time.perf_counter()
+            b.run(steps)
+            b.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            res["interleaved" if il else "pass_after_pass"] = {
+                "ms_per_step": round(ms, 4), "value": round(UNITS / (ms * 1e-3), 1),
+                "equals_three_launch": same}
+        b.close()
+        out[name] = res
+    out["unit"] = "props/s"
+    return out
+
+
 def pcie_inclusive(wos, inp, wl, precision, reps=5):
     """Drop-in rate: one 2048^2 Propagate (asm) per call from and to host
     complex128 memory (upload, device pipeline with power sums, download)."""
@@ -577,7 +616,7 @@ def main():
                     help="extra workloads at their default launch shapes (profiling: no autotuner launches)")
     ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
     ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
-    ap.add_argument("--only", default=None, choices=["config12", "bpm", "pipeline", "adjoint", "train"],
+    ap.add_argument("--only", default=None, choices=["config12", "bpm", "pipeline", "adjoint", "train", "persistent"],
                     help="run one of the other workloads alone and print its figures (profiling)")
     ap.add_argument("--retune", action="store_true", help="autotune even where the tuning table has a shape")
     ap.add_argument("--write-tuning", default=None, metavar="PATH",
@@ -600,7 +639,8 @@ def main():
         tune = not args.no_tune
         run = {"config12": lambda: bench_config12(wos, tune, args.retune)[0], "bpm": lambda: bench_bpm(wos)[0],
                "pipeline": lambda: bench_pipeline(wos, args.steps, tune, args.retune),
-               "adjoint": lambda: bench_adjoint(wos)[0], "train": lambda: bench_train(wos)[0]}[args.only]
+               "adjoint": lambda: bench_adjoint(wos)[0], "train": lambda: bench_train(wos)[0],
+               "persistent": lambda: bench_persistent(wos, args.steps)}[args.only]
         print(json.dumps({"only": args.only, "result": run()}), flush=True)
         return
     r32 = run_gpu(wos, wos.FP32, args.steps, args.warmup, shape32, args.retune)
@@ -682,6 +722,7 @@ def main():
         "fft2d_z_2048_ms": round(fft2d_dropin_ms(wos, r64["unit0_in"]), 3),
         "fft2d_note": "one complex128 2-D FFT of a host buffer per call, transfers included; the reference "
                       "publishes 45 ms for this call through cuFFT on a Tesla T4 (README.md:113)"}
+    line["persistent_one_launch"] = bench_persistent(wos, args.steps)
     cfg12, fields = bench_config12(wos, not args.no_tune, args.retune)
     line["config1_512_asm"] = cfg12["config1"]
     line["config2_1000_asm_pad_jones"] = cfg12["config2"]

@@ -307,6 +307,20 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
  * only (as autotune searches them) and ignore the other fields. */
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
                              int col_radix, int col_split);
+/* Persistent transfer (on != 0): each run is one launch of a persistent
+ * kernel that takes the forward rows, columns and inverse rows of every unit
+ * from per-XCD device work queues (plus a counter reset and a completion
+ * check), instead of three launches. on == 1 queues each XCD's passes one
+ * after another, on == 2 interleaves a round's three passes task by task. Built for asm batches of side 512,
+ * 1024, 2048 (FP32) and 1024, 2048 (FP64) with its own fixed launch shape
+ * (the tuning of wos_hip_batch_set_tuning / autotune applies to the
+ * three-launch form only); requires workgroups placed on the 8 XCDs in turn
+ * (checked once per process), else WOS_ERR_UNSUPPORTED. A captured graph is
+ * dropped. wos_hip_batch_sync and wos_hip_batch_time report WOS_ERR_HIP if a
+ * persistent run since enabling left work undone (a dependency wait gave
+ * up), and the error stays until the mode is set again. on == 0 returns to
+ * three launches. */
+int wos_hip_batch_set_persistent(wos_batch b, int on);
 /* capture one run into a hipGraph; later runs replay it */
 int wos_hip_batch_capture(wos_batch b);
 /* enqueue `iters` runs on the batch's stream (asynchronous) */

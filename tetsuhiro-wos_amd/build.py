@@ -16,7 +16,11 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwos_hip.so")
 OBJ = os.path.join(HERE, "_build")
 SOURCES = ["wos_hip.hip", "wos_launch_f64.hip", "wos_launch_f32.hip", "wos_launch8_f64.hip", "wos_launch8_f32.hip",
-           "wos_mixed_f64.hip", "wos_mixed_f32.hip"]
+           "wos_mixed_f64.hip", "wos_mixed_f32.hip", "wos_persist.hip"]
+# The persistent transfer kernel runs three task bodies in one loop: without
+# machine-code loop-invariant motion the constants they materialise stay
+# inside the tasks instead of occupying registers across the whole loop.
+EXTRA = {"wos_persist.hip": ["-mllvm", "-disable-machine-licm"]}
 ARCH = os.environ.get("WOS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -30,7 +34,7 @@ FLAGS = [
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

@@ -60,6 +60,12 @@ int mixed_col_w_f64(int N);
 int mixed_col_w_f32(int N);
 bool mixed_col_valid_f64(int N, int W);
 bool mixed_col_valid_f32(int N, int W);
+// persistent transfer (wos_persist.hip)
+bool persist_built(int prec, int N);
+int persist_rows(int prec);
+int persist_grid(int prec, int N, int* grid);
+int launch_transfer_persist(int prec, int N, int grid, const PersistArgs& a, hipStream_t s);
+int persist_xcc_map_ok(hipStream_t s, bool* ok);
 int launch_cols_bs_f64(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int launch_cols_bs_f32(int M, int mode, int W, const BsColArgs& A, hipStream_t s);
 int bs_default_w_f64(int M, int cols);
@@ -2865,18 +2871,73 @@ struct wos_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   Tuning tune;              // zero fields: defaults
+  // persistent transfer: one launch per step (wos_hip_batch_set_persistent)
+  bool persist = false;
+  int persist_grid = 0;
+  int persist_interleave = 0;
+  unsigned* persist_ctr = nullptr;  // persist_ctr_words(units) words
 };
+
+// Sleeps (about 60 ns each, plus a counter read) before a persistent task's
+// dependency wait gives up: far beyond any legitimate wait, which lasts
+// about one task (tens of microseconds).
+constexpr int kPersistSpinLimit = 1 << 22;
+
+// The persistent kernel's shape: row and column workgroups of one size
+// (ROWS rows, ROWS-column tiles), split LDS exchanges, radix-16 schedules.
+static Tuning persist_tuning(int prec) {
+  Tuning t;
+  t.tw = t.col_w = t.rows = persist_rows(prec);
+  t.groups = 0;
+  t.split = 1;
+  t.radix = t.col_radix = 16;
+  t.col_split = 1;
+  return t;
+}
 
 static int batch_build(wos_batch_s* b, TransferLaunch& L) {
   return build_transfer(L, b->prec, b->fields, b->inter, b->n, b->units, b->pad, b->d_tf, TF_ASM, 1.0, 0.0,
-                        nullptr, nullptr, b->tune);
+                        nullptr, nullptr, b->persist ? persist_tuning(b->prec) : b->tune);
+}
+
+static PersistArgs batch_persist_args(const wos_batch_s* b, const TransferLaunch& L) {
+  PersistArgs a;
+  memset(&a, 0, sizeof(a));
+  a.fwd = L.fwd;
+  a.col = L.col;
+  a.col.xcd_remap = 0;  // the queues place the work on XCDs themselves
+  a.inv = L.inv;
+  a.units = b->units;
+  a.rb = (b->n + L.tu.rows - 1) / L.tu.rows;
+  a.cb = L.M / L.tu.col_w;
+  a.interleave = b->persist_interleave && a.rb == a.cb;
+  a.ctr = b->persist_ctr;
+  a.spin_limit = kPersistSpinLimit;
+  return a;
 }
 
 static int batch_enqueue(wos_batch_s* b) {
   TransferLaunch L;
   int rc = batch_build(b, L);
+  if (!rc && b->persist) return launch_transfer_persist(b->prec, b->M, b->persist_grid, batch_persist_args(b, L), b->stream);
   for (int k = 0; k < 3 && !rc; ++k) rc = launch_transfer_kernel(L, k, b->stream);
   return rc;
+}
+
+// The sticky status of the persistent launches since the last check
+// (after the stream is idle): a wait that gave up or a queue left with
+// unclaimed tasks means some unit was not propagated.
+static int batch_persist_status(wos_batch_s* b) {
+  if (!b->persist_ctr) return 0;
+  unsigned st = 0;
+  WOS_HIP_TRY(hipMemcpy(&st, b->persist_ctr + kPersistStatus, sizeof(st), hipMemcpyDeviceToHost));
+  if (st) {
+    set_error(std::string("persistent transfer: ") +
+              ((st & 1) ? "a dependency wait gave up" : "a work queue was not drained") +
+              "; the batch's fields are not valid");
+    return WOS_ERR_HIP;
+  }
+  return 0;
 }
 
 static void batch_drop_graph(wos_batch_s* b) {
@@ -2974,6 +3035,7 @@ int wos_hip_batch_destroy(wos_batch b) {
   (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->persist_ctr) (void)hipFree(b->persist_ctr);
   if (b->inter) (void)hipFree(b->inter);
   if (b->fields) (void)hipFree(b->fields);
   if (b->d_tf) (void)hipFree(b->d_tf);
@@ -3345,6 +3407,56 @@ int wos_hip_batch_run(wos_batch b, int iters) {
 int wos_hip_batch_sync(wos_batch b) {
   if (!b) return WOS_ERR_INVALID;
   WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  return batch_persist_status(b);
+}
+
+int wos_hip_batch_set_persistent(wos_batch b, int on) {
+  WOS_ENTER();
+  if (!b) return WOS_ERR_INVALID;
+  WOS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!on) {
+    if (b->persist) batch_drop_graph(b);
+    b->persist = false;
+    if (b->persist_ctr) (void)hipFree(b->persist_ctr);
+    b->persist_ctr = nullptr;
+    return WOS_OK;
+  }
+  if (b->pad || !persist_built(b->prec, b->M)) {
+    set_error("wos_hip_batch_set_persistent: built for asm batches of side 512, 1024, 2048 (FP32) and 1024, 2048 (FP64)");
+    return WOS_ERR_UNSUPPORTED;
+  }
+  static int xcc_checked = 0;  // 1: workgroups go to the 8 XCDs in turn; -1: they do not
+  static std::string xcc_map;
+  if (!xcc_checked) {
+    bool ok = false;
+    int rc = persist_xcc_map_ok(b->stream, &ok);
+    if (rc) return rc;
+    xcc_checked = ok ? 1 : -1;
+    if (!ok) xcc_map = g_err;
+  }
+  if (xcc_checked < 0) {
+    set_error("wos_hip_batch_set_persistent: workgroups are not placed on the XCDs in turn on this GPU (" + xcc_map +
+              ")");
+    return WOS_ERR_UNSUPPORTED;
+  }
+  int grid = 0;
+  int rc = persist_grid(b->prec, b->M, &grid);
+  if (rc) return rc;
+  const size_t bytes = sizeof(unsigned) * persist_ctr_words(b->units);
+  if (!b->persist_ctr) {
+    hipError_t e = hipMalloc((void**)&b->persist_ctr, bytes);
+    if (e != hipSuccess) {
+      b->persist_ctr = nullptr;
+      set_error(std::string("hipMalloc(persistent counters): ") + hipGetErrorString(e));
+      return WOS_ERR_NOMEM;
+    }
+  }
+  WOS_HIP_TRY(hipMemset(b->persist_ctr, 0, bytes));
+  WOS_HIP_TRY(hipDeviceSynchronize());
+  batch_drop_graph(b);
+  b->persist = true;
+  b->persist_grid = grid;
+  b->persist_interleave = on == 2;
   return WOS_OK;
 }
 
@@ -3363,7 +3475,19 @@ int wos_hip_batch_time(wos_batch b, int warmup, int iters, double* ms_run, doubl
   float ms = 0;
   WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
   *ms_run = ms / iters;
-  if (kernel_ms) {
+  if ((rc = batch_persist_status(b))) return rc;
+  if (kernel_ms && b->persist) {
+    // one launch per run: its time, and no separate passes
+    WOS_HIP_TRY(hipEventRecord(e0, b->stream));
+    for (int i = 0; i < iters; ++i)
+      if ((rc = batch_enqueue(b))) return rc;
+    WOS_HIP_TRY(hipEventRecord(e1, b->stream));
+    WOS_HIP_TRY(hipEventSynchronize(e1));
+    WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    kernel_ms[0] = ms / iters;
+    kernel_ms[1] = kernel_ms[2] = 0;
+    if ((rc = batch_persist_status(b))) return rc;
+  } else if (kernel_ms) {
     // each kernel of the pipeline back to back on the batch stream
     TransferLaunch L;
     if ((rc = batch_build(b, L))) return rc;

@@ -536,23 +536,44 @@ __device__ __forceinline__ int tiled_offset(int q, int row, int ld, int lg, int 
 // the data stays in registers).
 // SPLIT: the stage exchanges move real and imaginary parts in two rounds
 // through half the LDS, so more workgroups fit a CU.
-template <typename R, int N, int DIR, int ROWS, int SCR = 0, bool SPLIT = false, int VV = 16>
-__global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows(const RowArgs a) {
+// Per-row LDS stride of the row pass in complex values (SPLIT: half of a
+// stride in values of R, rounded up so rows start complex-aligned; the
+// multi-row strides are even already).
+template <typename R, int N, int ROWS, bool SPLIT, int VV = 16>
+constexpr int rows_lds_stride() {
+  return SPLIT ? (lds_stride_elems<R>(N, ROWS) + 1) / 2 : row_lds_stride<R>(N, ROWS);
+}
+template <typename R, int N, int ROWS, bool SPLIT, int VV = 16>
+constexpr int rows_lds_elems() { return ROWS * rows_lds_stride<R, N, ROWS, SPLIT, VV>(); }
+
+// threadIdx.x; OPQ: hidden from the optimiser, so that a body run in a loop
+// recomputes what it derives from the thread index (addresses, twiddle
+// indices) instead of hoisting it and holding it live across iterations.
+template <bool OPQ, int NT>
+__device__ __forceinline__ int thread_index() {
+  int tid = threadIdx.x;
+  if constexpr (OPQ) {
+    asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);
+  }
+  return tid;
+}
+
+// The row pass of workgroup `blk` (k_rows is one launch of it; the
+// persistent transfer kernel runs it as a task). lds: rows_lds_elems
+// complex values; red: one double per wave.
+template <typename R, int N, int DIR, int ROWS, int SCR = 0, bool SPLIT = false, int VV = 16, bool OPQ = false>
+__device__ __forceinline__ void rows_body(const RowArgs& a, int blk, cpx<R>* lds, double* red) {
   constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
   static_assert(SCR == 0 || DIR > 0, "the screen follows an inverse transform");
   constexpr int R0 = first_rad<N, DIR, VV>(), RL = SCR ? last_rad<N, -1, VV>() : last_rad<N, DIR, VV>();
   constexpr int NT = ROWS * T;
-  // per-row LDS stride in complex values (SPLIT: half of a stride in values
-  // of R, rounded up so rows start complex-aligned; the multi-row strides
-  // are even already)
-  constexpr int LS = SPLIT ? (lds_stride_elems<R>(N, ROWS) + 1) / 2 : row_lds_stride<R>(N, ROWS);
-  __shared__ cpx<R> lds[ROWS * LS];
-  __shared__ double red[(NT + 63) / 64];
-  const int tid = threadIdx.x;
+  constexpr int LS = rows_lds_stride<R, N, ROWS, SPLIT, VV>();
+  const int tid = thread_index<OPQ, NT>();
   const int lr = tid % ROWS, t = tid / ROWS;
   const int bpu = (a.rows + ROWS - 1) / ROWS;
-  const int unit = blockIdx.x / bpu;
-  const int row = (blockIdx.x % bpu) * ROWS + lr;
+  const int unit = blk / bpu;
+  const int row = (blk % bpu) * ROWS + lr;
   const bool active = row < a.rows && unit < a.units;
   const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
   cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
@@ -583,7 +604,7 @@ __global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows(const RowArgs 
     double pin = 0;
 #pragma unroll
     for (int i = 0; i < V; ++i) pin += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
-    block_sum_store<NT>(pin, red, a.pw_in, blockIdx.x);
+    block_sum_store<NT>(pin, red, a.pw_in, blk);
   }
   constexpr bool ACC = SCR == 1 && sizeof(R) == 4;  // split-step slices: compensated twiddles
   fft_block<R, N, DIR, 1, ACC, SPLIT, VV>(v, t, tw, lds + lr * LS);
@@ -637,8 +658,15 @@ __global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows(const RowArgs 
         const cpx<R> y = v[b * RL + r];
         if (active && q >= 0 && q < a.out_cols) pout += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
       }
-    block_sum_store<NT>(pout, red, a.pw_out, blockIdx.x);
+    block_sum_store<NT>(pout, red, a.pw_out, blk);
   }
+}
+
+template <typename R, int N, int DIR, int ROWS, int SCR = 0, bool SPLIT = false, int VV = 16>
+__global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows(const RowArgs a) {
+  __shared__ cpx<R> lds[rows_lds_elems<R, N, ROWS, SPLIT, VV>()];
+  __shared__ double red[(ROWS * Plan<N, VV>::T + 63) / 64];
+  rows_body<R, N, DIR, ROWS, SCR, SPLIT, VV>(a, blockIdx.x, lds, red);
 }
 
 // Pipelined row pass for whole rows (no embed window, no power sums, no
@@ -734,8 +762,10 @@ struct ColArgs {
 };
 
 __device__ __forceinline__ int remap_xcd(int bid, int nb) {
-  // The dispatcher assigns workgroup i to XCD i % 8; give each XCD a
-  // contiguous range of tiles so neighbouring column tiles share its L2.
+  // The dispatcher assigns workgroup i to XCD (x0 + i) % 8 (x0 carries over
+  // from the previous launch), so workgroups of equal i % 8 share an XCD;
+  // give each such class a contiguous range of tiles so neighbouring column
+  // tiles share its L2.
   if ((nb & 7) != 0) return bid;
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
@@ -744,16 +774,21 @@ __device__ __forceinline__ int remap_xcd(int bid, int nb) {
 // through half the LDS, so more workgroups fit a CU; in FP64 the kernel is
 // also held to 128 VGPRs (4 waves per SIMD), as registers would otherwise
 // keep a second workgroup out.
-template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false>
-__global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 : 1))
-void k_cols(const ColArgs a) {
+// LDS of the column pass in complex values (CS: split exchanges, half).
+template <typename R, int N, int W, bool CS>
+constexpr int cols_lds_elems() { return CS ? (lds_len(N) * W + 1) / 2 : lds_len(N) * W; }
+
+// The column pass of workgroup `blk` of a launch of `nblk` workgroups
+// (k_cols is one launch of it; the persistent transfer kernel runs it as a
+// task). lds: cols_lds_elems complex values.
+template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false, bool OPQ = false>
+__device__ __forceinline__ void cols_body(const ColArgs& a, int blk, int nblk, cpx<R>* lds) {
   constexpr int T = Plan<N, VV>::T, V = Plan<N, VV>::V;
-  __shared__ cpx<R> lds[CS ? (lds_len(N) * W + 1) / 2 : lds_len(N) * W];
-  const int tid = threadIdx.x;
+  const int tid = thread_index<OPQ, W * T>();
   const int c = tid % W, t = tid / W;
   // this thread's column in LDS (CS: values of R, W apart)
   cpx<R>* const lc = CS ? (cpx<R>*)((R*)lds + c) : lds + c;
-  const int bid = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int bid = a.xcd_remap ? remap_xcd(blk, nblk) : blk;
   const int tpu = a.cols / W;
   const int unit = bid / tpu;
   const int tile = bid % tpu;
@@ -898,7 +933,7 @@ void k_cols(const ColArgs a) {
           }
         if (a.pw3) {  // uniform
           __shared__ double redc[(W * Plan<N, VV>::T + 63) / 64];
-          block_sum_store<W * Plan<N, VV>::T>(s1, redc, a.pw3, blockIdx.x);
+          block_sum_store<W * Plan<N, VV>::T>(s1, redc, a.pw3, blk);
         }
       } else {
         const bool conj = ASM && u.kind == TF_CONJ_ASM;
@@ -966,7 +1001,7 @@ void k_cols(const ColArgs a) {
       if (tid < 3) {
         double x = 0;
         for (int w = 0; w < (W * Plan<N, VV>::T + 63) / 64; ++w) x += red3[tid][w];
-        a.pw3[(size_t)blockIdx.x * 3 + tid] = x;
+        a.pw3[(size_t)blk * 3 + tid] = x;
       }
     }
   }
@@ -1009,5 +1044,32 @@ void k_cols(const ColArgs a) {
       }
   }
 }
+
+template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false>
+__global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 : 1))
+void k_cols(const ColArgs a) {
+  __shared__ cpx<R> lds[cols_lds_elems<R, N, W, CS>()];
+  cols_body<R, N, W, MODE, DIR, VV, CS>(a, blockIdx.x, gridDim.x, lds);
+}
+
+// Arguments of the persistent transfer kernel (wos_persist.hip): the three
+// passes of one transfer step on every unit of a batch, run from per-XCD
+// work queues in one launch.
+struct PersistArgs {
+  RowArgs fwd;
+  ColArgs col;
+  RowArgs inv;
+  int units;       // units of the batch
+  int rb, cb;      // row-pass and column-pass workgroups per unit
+  int interleave;  // 1: a round's passes task by task (needs rb == cb); 0: pass after pass
+  unsigned* ctr;   // counters, persist_ctr_words(units) words (layout in wos_persist.hip)
+  int spin_limit;  // sleeps before a dependency wait gives up
+};
+// Counter words: one 128-byte line per XCD (ticket, abort flag), a line for
+// the sticky status, one line per unit (forward rows done, columns done).
+constexpr int kPersistLine = 32;
+constexpr int kPersistStatus = 8 * kPersistLine;
+constexpr int kPersistUnits = kPersistStatus + kPersistLine;
+inline size_t persist_ctr_words(int units) { return (size_t)kPersistUnits + (size_t)kPersistLine * units; }
 
 }  // namespace wos

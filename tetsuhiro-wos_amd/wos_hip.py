@@ -31,7 +31,7 @@ EXPORTS = [
     "wos_hip_batch_create", "wos_hip_batch_destroy", "wos_hip_batch_upload",
     "wos_hip_batch_download", "wos_hip_batch_fill_schell", "wos_hip_batch_capture",
     "wos_hip_batch_run", "wos_hip_batch_sync", "wos_hip_batch_device_ptr", "wos_hip_batch_time",
-    "wos_hip_batch_autotune", "wos_hip_batch_set_tuning", "wos_hip_last_device_ms",
+    "wos_hip_batch_autotune", "wos_hip_batch_set_tuning", "wos_hip_batch_set_persistent", "wos_hip_last_device_ms",
     "wos_hip_mask_create", "wos_hip_mask_destroy", "wos_hip_mask_forward", "wos_hip_mask_backward",
     "wos_hip_mask_field", "wos_hip_mask_device_grad", "wos_hip_mask_device_theta",
     "wos_hip_pipeline_create", "wos_hip_pipeline_destroy", "wos_hip_pipeline_add",
@@ -217,6 +217,7 @@ def load(path=None):
     L.wos_hip_batch_time.argtypes = [P, I, I, C.POINTER(D), P]
     L.wos_hip_batch_autotune.argtypes = [P, I, P, C.POINTER(D)]
     L.wos_hip_batch_set_tuning.argtypes = [P, I, I, I, I, I, I, I, I]
+    L.wos_hip_batch_set_persistent.argtypes = [P, I]
     L.wos_hip_mask_create.argtypes = [C.POINTER(P), I, I, D, D, C.POINTER(Ctx), I, P]
     L.wos_hip_mask_destroy.argtypes = [P]
     L.wos_hip_mask_forward.argtypes = [P, P, P]
@@ -721,6 +722,13 @@ class Batch:
             return None
         _check(rc, "batch_load_tuning")
         return dict(zip(TUNING_KEYS, (int(x) for x in shape)))
+
+    def set_persistent(self, on=True, interleave=False):
+        """One persistent launch per run (per-XCD work queues) instead of
+        three passes, the queued passes one after another or (interleave)
+        task by task; raises WosError where it is not built."""
+        _check(load().wos_hip_batch_set_persistent(self.h, (2 if interleave else 1) if on else 0),
+               "batch_set_persistent")
 
     def capture(self):
         _check(load().wos_hip_batch_capture(self.h), "batch_capture")
