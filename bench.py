@@ -33,8 +33,8 @@ HBM_PEAK_GBS = 8000.0
 VALU_WAVE_INSTR_PER_S = 614.4e9
 # HBM bytes per full-batch launch (64 x 2048^2), rocprofv3 PMC: 2 x FETCH_SIZE + WRITE_SIZE
 # (calibrated on a 1 GiB copy), from profiles/asm_2048x64_hbm_traffic.md, measured at the launch shapes
-# PMC_SHAPE (tile_w, col_w, rows, row_groups, split_lds, radix, col_radix, col_split_lds); a run whose
-# autotuned shape differs reports no traffic figure
+# PMC_SHAPE (tile_w, col_w, rows, row_groups, split_lds, radix, col_radix, col_split_lds, inv_rows,
+# inv_row_groups, inv_split_lds); a run whose autotuned shape differs reports no traffic figure
 PMC_HBM_BYTES = {
     "fp32": {"row_fft_fwd": (1048722 * 2 + 2097160) * 1024, "col_fft_tf_ifft": (1048960 * 2 + 2097160) * 1024,
              "row_fft_inv": (1048724 * 2 + 2097153) * 1024},
@@ -43,7 +43,7 @@ PMC_HBM_BYTES = {
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
 PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1519 * 262144, "row_fft_inv": 872 * 262144}}
-PMC_SHAPE = {"fp32": (4, 4, 4, 0, 1, 16, 16, 1), "fp64": (4, 4, 2, 0, 1, 16, 16, 1)}
+PMC_SHAPE = {"fp32": (4, 4, 4, 2, 1, 16, 16, 1, 2, 2, 1), "fp64": (4, 4, 2, 0, 1, 16, 16, 1, 2, 2, 1)}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
 
@@ -71,10 +71,13 @@ def run_gpu(wos, precision, steps, warmup, shape=None, retune=False):
     b = wos.Batch(N, DX, wls, zs, wos.ctx_for(550e-9), "asm", precision)
     b.fill_schell(W0, SIGMA_C, SEED)
     inp0 = b.download(0)
-    if shape:   # a fixed launch shape: tile_w, col_w, rows, row groups, split[, radix[, col_radix[, col_split]]]
+    if shape:   # a fixed launch shape: tile_w, col_w, rows, row groups, split[, radix[, col_radix[, col_split
+        #         [, inv_rows, inv_groups, inv_split]]]] (no inverse fields: as the forward row pass)
         shape = list(shape) + [16] * (6 - len(shape))
         shape = shape + [shape[5]] * (7 - len(shape))
         shape = shape + [0] * (8 - len(shape))
+        if len(shape) < 11:
+            shape = shape[:8] + shape[2:5]
         b.set_tuning(*shape)
         tune = (dict(zip(wos.TUNING_KEYS, shape)), b.time(1, 5)[0])
         source = "fixed"
@@ -579,7 +582,8 @@ def write_tuning(wos, path):
     their shapes as a tuning table (the format wos_hip_batch_load_tuning
     reads)."""
     lines = [f"# launch shapes of libwos_hip on {wos.device_name()}, written by bench.py --write-tuning",
-             "# n precision method units tile_w col_w rows groups split radix col_radix col_split ms"]
+             "# n precision method units tile_w col_w rows groups split radix col_radix col_split inv_rows inv_groups"
+             " inv_split ms"]
     pname = {wos.FP32: "fp32", wos.FP64: "fp64"}
 
     def add(n, prec, method, units, shape, ms):
@@ -614,8 +618,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline batch (no other workloads)")
     ap.add_argument("--no-tune", action="store_true",
                     help="extra workloads at their default launch shapes (profiling: no autotuner launches)")
-    ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
-    ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape tile_w,col_w,rows,groups,split[,radix[,col_radix[,col_split]]]")
+    ap.add_argument("--shape32", default=None, help="fixed FP32 launch shape tile_w,col_w,rows,groups,split"
+                    "[,radix[,col_radix[,col_split[,inv_rows,inv_groups,inv_split]]]]")
+    ap.add_argument("--shape64", default=None, help="fixed FP64 launch shape (as --shape32)")
     ap.add_argument("--only", default=None, choices=["config12", "bpm", "pipeline", "adjoint", "train", "persistent"],
                     help="run one of the other workloads alone and print its figures (profiling)")
     ap.add_argument("--retune", action="store_true", help="autotune even where the tuning table has a shape")

@@ -40,6 +40,9 @@ extern "C" {
 #define WOS_FP64 0 /* complex128 everywhere (parity path) */
 #define WOS_FP32 1 /* complex64 bulk passes; FP64 twiddles, chirps, kz*z, sums */
 
+/* ---- ints in a launch shape (autotune, set / get / load tuning) ---- */
+#define WOS_SHAPE_LEN 11
+
 /* ---- propagation methods (optics/propagate.go:40-50) ---- */
 #define WOS_METHOD_ASM 0
 #define WOS_METHOD_ASM_PAD 1
@@ -286,16 +289,20 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
  * thread; 8 = radix-8 stages with 8 values per thread and twice the
  * threads, built for power-of-two grids of 256..8192), the rows per
  * workgroup, the row groups of the pipelined kernels (0: one group, 2, 4)
- * and, for radix-16 one-group kernels, split LDS exchanges (real and
- * imaginary parts in two rounds through half the LDS); for the column pass
+ * and, radix-16, split LDS exchanges (real and imaginary parts in two
+ * rounds through half the LDS; pipelined kernels of 2 or 4 rows then also
+ * stage their twiddles in LDS), the forward and the inverse row pass each
+ * taking its own rows, groups and split; for the column pass
  * its own stage schedule (col_radix), the columns per workgroup (col_w:
  * tile_w, tile_w/2 or tile_w/4) and, radix-16, split LDS exchanges
  * (col_split). For each tile_w the row and column shapes
  * are searched apart, each kernel timed over `iters` launches with HIP
  * events on the batch stream, and the tile_w with the smallest sum wins (a
  * captured graph is dropped). The launches propagate the batch's fields:
- * refill them afterwards. shape (may be NULL, else 8 ints) receives
- * {tile_w, col_w, rows, groups, split, radix, col_radix, col_split}; best_ms
+ * refill them afterwards. shape (may be NULL, else WOS_SHAPE_LEN ints)
+ * receives {tile_w, col_w, rows, groups, split, radix, col_radix, col_split,
+ * inv_rows, inv_groups, inv_split} (rows, groups, split: the forward row
+ * pass; inv_*: the inverse row pass); best_ms
  * (may be NULL) the time of one whole run at that shape. On mixed-radix
  * grids (2^a 3^b 5^c 7^d sides) the choices are the layout tile width
  * (tile_w: 16, 8, 4, 2, 1 dividing the side) and the columns per column
@@ -303,10 +310,12 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
  * whole run; Bluestein grids keep their fixed shape. */
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms);
 /* Set the launch shape directly (e.g. from a cached autotune table);
- * radix / col_radix 0 mean 16. Mixed-radix grids take tile_w and col_w
- * only (as autotune searches them) and ignore the other fields. */
+ * radix / col_radix 0 mean 16; inv_rows < 0: the inverse row pass takes
+ * rows, groups and split (inv_groups and inv_split are then ignored).
+ * Mixed-radix grids take tile_w and col_w only (as autotune searches them)
+ * and ignore the other fields. */
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
-                             int col_radix, int col_split);
+                             int col_radix, int col_split, int inv_rows, int inv_groups, int inv_split);
 /* Persistent transfer (on != 0): each run is one launch of a persistent
  * kernel that takes the forward rows, columns and inverse rows of every unit
  * from per-XCD device work queues (plus a counter reset and a completion
@@ -546,13 +555,16 @@ int wos_hip_field_stats(wos_field f, double wl, double strehl_aperture, double s
 /* ---- tuning tables (no reference counterpart) ----
  * A text file of launch shapes found by an earlier autotune, one entry a
  * line ('#' comments):
- *   n precision method units tile_w col_w rows groups split radix col_radix col_split [ms]
- * precision "fp64" or "fp32", method "asm" or "asm_pad", the eight shape
- * fields as wos_hip_batch_autotune reports them. A lookup takes the entries
+ *   n precision method units tile_w col_w rows groups split radix col_radix col_split
+ *     [inv_rows inv_groups inv_split] [ms]
+ * precision "fp64" or "fp32", method "asm" or "asm_pad", the shape fields as
+ * wos_hip_batch_autotune reports them (an entry of eight shape fields and
+ * a time has no inverse row shape of its own). A lookup takes the entries
  * of the grid's n, precision and method and the nearest unit count.
  * Loading applies the entry's shape (WOS_ERR_UNSUPPORTED: no entry). */
 int wos_hip_batch_load_tuning(wos_batch b, const char* path, int* shape);
-/* the launch shape of one pipeline stage (8 ints, as above), read or set */
+/* the launch shape of one pipeline stage (WOS_SHAPE_LEN ints, as above),
+ * read or set */
 int wos_hip_pipeline_get_tuning(wos_pipeline p, int stage, int* shape);
 int wos_hip_pipeline_set_tuning(wos_pipeline p, int stage, const int* shape);
 /* every stage with an entry takes its shape; *applied = how many did */

@@ -482,24 +482,27 @@ def test_config3_bpm_full_fp32_vs_fp64(wos, oracle):
 def test_batch_launch_shapes_bit_identical(wos, precision):
     """Layout tile width, column and row workgroup shapes change only the
     data movement: the one-group kernels give bit-identical results. The
-    pipelined row kernel (row groups > 0) and the split-exchange row kernel
-    are other instantiations of the same arithmetic, where the compiler may
+    pipelined row kernels (row groups > 0, twiddles in registers or, with
+    split exchanges, in LDS) and the split-exchange row kernel are other
+    instantiations of the same arithmetic, where the compiler may
     fuse a complex product's multiply-adds the other way round: they must
     agree to rounding level."""
     prec = wos.FP64 if precision == "fp64" else wos.FP32
     n = 1024
     b = wos.Batch(n, 2e-6, np.array([450e-9, 633e-9]), np.array([5e-3, 9e-3]), wos.ctx_for(550e-9), "asm", prec)
-    # (layout tile width, columns per column workgroup, rows per row workgroup, row groups)
-    # (layout tile width, columns per column workgroup, rows per row workgroup, row groups, split LDS)
+    # (layout tile width, columns per column workgroup, rows per row workgroup, row groups, split LDS
+    #  [, the inverse row pass's rows, groups, split LDS])
     shapes = [(0, 0, 0, 0, 0), (1, 1, 1, 0, 0), (2, 2, 8, 0, 0), (4, 4, 4, 2, 0), (4, 2, 2, 4, 0), (8, 8, 1, 2, 0),
               (2, 1, 1, 4, 0), (8, 8, 8, 4, 0), (8, 4, 2, 2, 0), (8, 2, 4, 0, 0), (16, 4, 1, 4, 0), (4, 4, 4, 0, 1),
-              (8, 4, 2, 0, 1), (2, 2, 8, 0, 1)]
+              (8, 4, 2, 0, 1), (2, 2, 8, 0, 1),
+              (4, 4, 4, 4, 1), (4, 4, 4, 2, 1), (8, 4, 2, 4, 1), (8, 8, 2, 2, 1),   # pipelined, twiddles in LDS
+              (4, 4, 4, 2, 1, 2, 2, 1), (4, 4, 4, 0, 1, 2, 0, 0), (8, 8, 1, 0, 0, 8, 4, 0)]   # own inverse shapes
     if prec == wos.FP32:
         shapes.append((16, 16, 4, 4, 0))   # 16-column workgroups of 1024-point columns fit LDS in FP32 only
     ref = None
-    for tw, cw, rows, groups, split in shapes:
+    for tw, cw, rows, groups, split, *inv in shapes:
         if tw:
-            b.set_tuning(tw, cw, rows, groups, split, 16)
+            b.set_tuning(tw, cw, rows, groups, split, 16, None, 0, *(inv or (-1, 0, 0)))
         b.fill_schell(100e-6, 30e-6, 3)
         b.run(1)
         b.sync()
@@ -508,11 +511,11 @@ def test_batch_launch_shapes_bit_identical(wos, precision):
             ref = out
         else:
             for u in range(2):
-                if groups == 0 and split == 0:
+                if groups == 0 and split == 0 and (not inv or inv[1:] == [0, 0]):
                     assert np.array_equal(out[u], ref[u]), (tw, cw, rows, groups, split, u)
                 else:
                     e = rel_l2(out[u], ref[u])
-                    assert e <= (1e-15 if prec == wos.FP64 else 1e-6), (tw, cw, rows, groups, split, u, e)
+                    assert e <= (1e-15 if prec == wos.FP64 else 1e-6), (tw, cw, rows, groups, split, inv, u, e)
     b.close()
 
 

@@ -13,12 +13,19 @@ def test_batch_tuning_table_lookup(wos, tmp_path):
     t.write_text("# test table\n"
                  "1024 fp32 asm 4 8 4 2 0 0 16 16 0 0.1\n"
                  "1024 fp32 asm 64 4 4 4 0 1 16 16 0\n"
+                 "1024 fp32 asm 16 4 4 4 2 1 16 16 1 2 2 1 0.5   # its own inverse row shape\n"
                  "1024 fp64 asm 4 4 4 2 0 1 16 16 1   # FP64 entry\n"
                  "1000 fp32 asm_pad 2 8 2 1 0 0 16 16 0\n")
     ctx = wos.ctx_for(532e-9)
     b = wos.Batch(1024, 2e-6, np.full(8, 532e-9), np.full(8, 1e-3), ctx, "asm", wos.FP32)
     assert b.load_tuning(str(t)) == dict(tile_w=8, col_w=4, rows=2, row_groups=0, split_lds=0, radix=16,
-                                         col_radix=16, col_split_lds=0)   # 8 units: nearer to 4 than to 64
+                                         col_radix=16, col_split_lds=0, inv_rows=2, inv_row_groups=0,
+                                         inv_split_lds=0)   # 8 units: nearer to 4 than to 64; inverse as forward
+    b.close()
+    b = wos.Batch(1024, 2e-6, np.full(16, 532e-9), np.full(16, 1e-3), ctx, "asm", wos.FP32)
+    assert b.load_tuning(str(t)) == dict(tile_w=4, col_w=4, rows=4, row_groups=2, split_lds=1, radix=16,
+                                         col_radix=16, col_split_lds=1, inv_rows=2, inv_row_groups=2,
+                                         inv_split_lds=1)
     b.close()
     b = wos.Batch(1024, 2e-6, np.full(40, 532e-9), np.full(40, 1e-3), ctx, "asm", wos.FP32)
     assert b.load_tuning(str(t))["rows"] == 4                                # 40 units: the 64-unit entry
@@ -39,6 +46,12 @@ def test_batch_tuning_table_lookup(wos, tmp_path):
         b.load_tuning(str(bad))
     bad.write_text("1024 fp32 asm 4 3 3 2 0 0 16 16 0\n")                   # a shape the kernels do not build
     with pytest.raises(wos.WosError, match="not built"):
+        b.load_tuning(str(bad))
+    bad.write_text("1024 fp32 asm 4 4 4 4 0 1 16 16 0 3 0 0\n")             # an inverse row shape not built
+    with pytest.raises(wos.WosError, match="not built"):
+        b.load_tuning(str(bad))
+    bad.write_text("1024 fp32 asm 4 4 4 4 0 1 16 16 0 4 0\n")               # two of the three inverse fields
+    with pytest.raises(wos.WosError, match="bad entry on line 1"):
         b.load_tuning(str(bad))
     with pytest.raises(wos.WosError, match="cannot open"):
         b.load_tuning(str(tmp_path / "missing.txt"))

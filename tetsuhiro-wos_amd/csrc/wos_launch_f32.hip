@@ -6,6 +6,7 @@ int launch_rows_f32(int N, int dir, int rows, const RowArgs& a, hipStream_t s) {
   return launch_rows_T<float>(N, dir, rows, a, s);
 }
 bool rows_valid_f32(int N, int rows) { return rows_valid_T<float>(N, rows); }
+bool rows_pipe_lds_valid_f32(int N, int rows) { return rows_pipe_lds_valid_T<float>(N, rows); }
 int launch_cols_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
   return launch_cols_T<float>(N, mode, dir, W, a, s);
 }

@@ -6,6 +6,7 @@ int launch_rows_f64(int N, int dir, int rows, const RowArgs& a, hipStream_t s) {
   return launch_rows_T<double>(N, dir, rows, a, s);
 }
 bool rows_valid_f64(int N, int rows) { return rows_valid_T<double>(N, rows); }
+bool rows_pipe_lds_valid_f64(int N, int rows) { return rows_pipe_lds_valid_T<double>(N, rows); }
 int launch_cols_f64(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
   return launch_cols_T<double>(N, mode, dir, W, a, s);
 }

@@ -132,8 +132,22 @@ def make_shape(kind, verts=None, **params):
     return g
 
 
-# launch shape fields of a batch (wos_hip_batch_autotune / _set_tuning)
-TUNING_KEYS = ("tile_w", "col_w", "rows", "row_groups", "split_lds", "radix", "col_radix", "col_split_lds")
+# launch shape fields of a batch (wos_hip_batch_autotune / _set_tuning): rows,
+# row_groups and split_lds are the forward row pass's, inv_* the inverse row
+# pass's (inv_rows < 0 when set: as the forward pass)
+TUNING_KEYS = ("tile_w", "col_w", "rows", "row_groups", "split_lds", "radix", "col_radix", "col_split_lds",
+               "inv_rows", "inv_row_groups", "inv_split_lds")
+SHAPE_LEN = len(TUNING_KEYS)
+
+
+def _shape_array(shape):
+    """A shape dict or sequence as the C ABI's WOS_SHAPE_LEN ints; missing
+    inverse row fields mean "as the forward pass"."""
+    if isinstance(shape, dict):
+        vals = [shape.get(k, -1 if k == "inv_rows" else 0) for k in TUNING_KEYS]
+    else:
+        vals = list(shape) + [-1, 0, 0][len(shape) - 8:] if len(shape) < SHAPE_LEN else list(shape)
+    return np.ascontiguousarray(vals[:SHAPE_LEN], dtype=np.int32)
 
 
 class WosError(RuntimeError):
@@ -216,7 +230,7 @@ def load(path=None):
     L.wos_hip_batch_device_ptr.restype = P
     L.wos_hip_batch_time.argtypes = [P, I, I, C.POINTER(D), P]
     L.wos_hip_batch_autotune.argtypes = [P, I, P, C.POINTER(D)]
-    L.wos_hip_batch_set_tuning.argtypes = [P, I, I, I, I, I, I, I, I]
+    L.wos_hip_batch_set_tuning.argtypes = [P, I, I, I, I, I, I, I, I, I, I, I]
     L.wos_hip_batch_set_persistent.argtypes = [P, I]
     L.wos_hip_mask_create.argtypes = [C.POINTER(P), I, I, D, D, C.POINTER(Ctx), I, P]
     L.wos_hip_mask_destroy.argtypes = [P]
@@ -701,22 +715,26 @@ class Batch:
 
     def autotune(self, iters=5):
         """Pick the launch shape; returns ({tile_w, col_w, rows, row_groups,
-        split_lds, radix, col_radix, col_split_lds}, ms per run)."""
-        shape = np.zeros(8, dtype=np.int32)
+        split_lds, radix, col_radix, col_split_lds, inv_rows, inv_row_groups,
+        inv_split_lds}, ms per run)."""
+        shape = np.zeros(SHAPE_LEN, dtype=np.int32)
         ms = C.c_double()
         _check(load().wos_hip_batch_autotune(self.h, iters, _p(shape), C.byref(ms)), "batch_autotune")
         return dict(zip(TUNING_KEYS, (int(x) for x in shape))), ms.value
 
-    def set_tuning(self, tile_w, col_w=None, rows=0, groups=0, split=0, radix=0, col_radix=None, col_split=0):
-        """col_radix None: the row passes' radix."""
+    def set_tuning(self, tile_w, col_w=None, rows=0, groups=0, split=0, radix=0, col_radix=None, col_split=0,
+                   inv_rows=-1, inv_groups=0, inv_split=0):
+        """col_radix None: the row passes' radix; inv_rows < 0: the inverse
+        row pass takes rows, groups and split."""
         _check(load().wos_hip_batch_set_tuning(self.h, tile_w, tile_w if col_w is None else col_w, rows, groups,
-                                               split, radix, radix if col_radix is None else col_radix, col_split),
+                                               split, radix, radix if col_radix is None else col_radix, col_split,
+                                               inv_rows, inv_groups, inv_split),
                "batch_set_tuning")
 
     def load_tuning(self, path=None):
         """Apply the tuning table's shape for this batch; returns the shape
         dict, or None when the table has no entry for the grid."""
-        shape = np.zeros(8, dtype=np.int32)
+        shape = np.zeros(SHAPE_LEN, dtype=np.int32)
         rc = load().wos_hip_batch_load_tuning(self.h, (path or TUNING_TABLE).encode(), _p(shape))
         if rc == 4:
             return None
@@ -854,13 +872,12 @@ class Pipeline:
         return ms
 
     def get_tuning(self, stage):
-        shape = np.zeros(8, dtype=np.int32)
+        shape = np.zeros(SHAPE_LEN, dtype=np.int32)
         _check(load().wos_hip_pipeline_get_tuning(self.h, stage, _p(shape)), "pipeline_get_tuning")
         return dict(zip(TUNING_KEYS, (int(x) for x in shape)))
 
     def set_tuning(self, stage, shape):
-        sh = np.ascontiguousarray([shape[k] for k in TUNING_KEYS] if isinstance(shape, dict) else shape,
-                                  dtype=np.int32)
+        sh = _shape_array(shape)
         _check(load().wos_hip_pipeline_set_tuning(self.h, stage, _p(sh)), "pipeline_set_tuning")
 
     def load_tuning(self, path=None):
