@@ -36,13 +36,13 @@ VALU_WAVE_INSTR_PER_S = 614.4e9
 # PMC_SHAPE (tile_w, col_w, rows, row_groups, split_lds, radix, col_radix, col_split_lds, inv_rows,
 # inv_row_groups, inv_split_lds); a run whose autotuned shape differs reports no traffic figure
 PMC_HBM_BYTES = {
-    "fp32": {"row_fft_fwd": (1048722 * 2 + 2097160) * 1024, "col_fft_tf_ifft": (1048960 * 2 + 2097160) * 1024,
-             "row_fft_inv": (1048724 * 2 + 2097153) * 1024},
-    "fp64": {"row_fft_fwd": (2097383 * 2 + 4194979) * 1024, "col_fft_tf_ifft": (2100068 * 2 + 4279494) * 1024,
-             "row_fft_inv": (2097384 * 2 + 4194343) * 1024},
+    "fp32": {"row_fft_fwd": (1049551 * 2 + 2097432) * 1024, "col_fft_tf_ifft": (1048962 * 2 + 2097162) * 1024,
+             "row_fft_inv": (1136779 * 2 + 2097156) * 1024},
+    "fp64": {"row_fft_fwd": (2097380 * 2 + 4195090) * 1024, "col_fft_tf_ifft": (2100070 * 2 + 4279157) * 1024,
+             "row_fft_inv": (2097807 * 2 + 4195341) * 1024},
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
-PMC_VALU = {"fp32": {"row_fft_fwd": 820 * 262144, "col_fft_tf_ifft": 1519 * 262144, "row_fft_inv": 872 * 262144}}
+PMC_VALU = {"fp32": {"row_fft_fwd": 1607 * 131072, "col_fft_tf_ifft": 1527 * 262144, "row_fft_inv": 1783 * 131072}}
 PMC_SHAPE = {"fp32": (4, 4, 4, 2, 1, 16, 16, 1, 2, 2, 1), "fp64": (4, 4, 2, 0, 1, 16, 16, 1, 2, 2, 1)}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
