@@ -164,3 +164,129 @@ def test_runtime_sides_match_numpy(n):
     x = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 1).standard_normal(n)
     want = np.fft.fft(x)
     assert np.linalg.norm(transform(x, -1) - want) <= 1e-13 * np.linalg.norm(want)
+
+
+# ------------------------------------------------- register-resident plan (wos_mixreg.hpp)
+
+def _dft5(v, d):
+    c1, c2, s1, s2 = np.cos(2 * np.pi / 5), np.cos(4 * np.pi / 5), np.sin(2 * np.pi / 5), np.sin(4 * np.pi / 5)
+    a1, b1, a2, b2 = v[1] + v[4], v[1] - v[4], v[2] + v[3], v[2] - v[3]
+    t1, t2 = v[0] + c1 * a1 + c2 * a2, v[0] + c2 * a1 + c1 * a2
+    u1, u2 = d * 1j * (s1 * b1 + s2 * b2), d * 1j * (s2 * b1 - s1 * b2)
+    return np.array([v[0] + a1 + a2, t1 + u1, t2 + u2, t2 - u2, t1 - u1])
+
+
+def _dft10(v, d):
+    """2 x 5: DFT-5 over n1 of x[2 n1 + n2], twiddle W10^(n2 k1), DFT-2; X[k1 + 5 k2]."""
+    a, b = _dft5(v[0::2], d), _dft5(v[1::2], d) * np.exp(d * 2j * np.pi * np.arange(5) / 10)
+    return np.concatenate([a + b, a - b])
+
+
+def _dft20(v, d):
+    """4 x 5: DFT-5 over n1 of x[4 n1 + n2], twiddle W20^(n2 k1), DFT-4 over n2; X[k1 + 5 k2]."""
+    a = np.array([_dft5(v[n2::4], d) for n2 in range(4)])
+    a *= np.exp(d * 2j * np.pi * np.outer(np.arange(4), np.arange(5)) / 20)
+    return np.concatenate([sum(a[n2] * np.exp(d * 2j * np.pi * n2 * k2 / 4) for n2 in range(4)) for k2 in range(4)])
+
+
+def _dft_any(v, d):
+    n = len(v)
+    if n == 10:
+        return _dft10(v, d)
+    if n == 20:
+        return _dft20(v, d)
+    if n == 5:
+        return _dft5(v, d)
+    return np.array([np.sum(v * np.exp(d * 2j * np.pi * np.arange(n) * k / n)) for k in range(n)])
+
+
+def _reg_tables(fwd):
+    """Forward stages 1..S-1, then the inverse (reversed) plan's stages 1..S-1:
+    entry [k][r-1] = exp(-2 pi i r k / (NS RAD)) (MRPlan<N>::tw_off)."""
+    tab, offs = [], {}
+    for dname, rads in (("f", fwd), ("i", fwd[::-1])):
+        ns = rads[0]
+        for s in range(1, len(rads)):
+            offs[(dname, s)] = len(tab)
+            L = ns * rads[s]
+            tab += [np.exp(-2j * np.pi * ((r * k) % L) / L) for k in range(ns) for r in range(1, rads[s])]
+            ns = L
+    return np.array(tab), offs
+
+
+def _reg_run(regs, n, fwd, v, d, tab, offs):
+    """Run the stages on per-thread registers regs[t] (slot b RAD + r = input
+    x[j + r n/RAD], j = t + b T), exchanging through an LDS array checked to
+    be written exactly once per exchange."""
+    T = n // v
+    rads = fwd if d < 0 else fwd[::-1]
+    ns = 1
+    for s, rad in enumerate(rads):
+        assert v % rad == 0
+        for t in range(T):
+            for b in range(v // rad):
+                j = t + b * T
+                k = j % ns
+                blk = regs[t][b * rad:(b + 1) * rad].copy()
+                if ns > 1:
+                    w = tab[offs[("f" if d < 0 else "i", s)] + k * (rad - 1) + np.arange(rad - 1)]
+                    blk[1:] *= w if d < 0 else np.conj(w)
+                regs[t][b * rad:(b + 1) * rad] = _dft_any(blk, d)
+        if s + 1 < len(rads):
+            lds = np.full(n, np.nan + 0j)
+            for t in range(T):
+                for b in range(v // rad):
+                    j = t + b * T
+                    k = j % ns
+                    for r in range(rad):
+                        p = (j - k) * rad + k + r * ns
+                        assert np.isnan(lds[p])
+                        lds[p] = regs[t][b * rad + r]
+            assert not np.isnan(lds).any()
+            rn = rads[s + 1]
+            for t in range(T):
+                regs[t] = np.array([lds[t + b * T + r * (n // rn)] for b in range(v // rn) for r in range(rn)])
+        ns *= rad
+    return regs
+
+
+@pytest.mark.parametrize("fwd,v", [([10, 10, 10, 2], 10), ([20, 20, 5], 20)])
+def test_register_resident_plan_2000(fwd, v):
+    """The register-resident 2000-point design: forward stages [10, 10, 10, 2]
+    on 200 threads of 10 values (and the [20, 20, 5] variant with the 4 x 5
+    radix-20 butterfly), the inverse plan reversed so the forward output map
+    is the inverse input map (forward -> transfer function -> inverse stays in
+    registers), the twiddle layout, and the mirror map of the transfer
+    function (row N - p of thread t's row p = t + T m lies in thread T - t at
+    m' = V - 1 - m, thread 0 at m' = V - m)."""
+    n = 2000
+    T = n // v
+    tab, offs = _reg_tables(fwd)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    r0 = fwd[0]
+    regs = [np.array([x[t + b * T + r * (n // r0)] for b in range(v // r0) for r in range(r0)]) for t in range(T)]
+    regs = _reg_run(regs, n, fwd, v, -1, tab, offs)
+    rl = fwd[-1]
+    y = np.zeros(n, complex)
+    for t in range(T):
+        for b in range(v // rl):
+            for r in range(rl):
+                y[t + b * T + r * (n // rl)] = regs[t][b * rl + r]
+    want = np.fft.fft(x)
+    assert np.abs(y - want).max() <= 1e-12 * np.abs(want).max()
+    # slot (m % NB) rl + m // NB of thread t holds row t + T m
+    nb = v // rl
+    for t in range(T):
+        for m in range(v):
+            assert t + (m % nb) * T + (m // nb) * (n // rl) == t + T * m
+            pm = (n - (t + T * m)) % n
+            assert pm == (T * ((v - m) % v) if t == 0 else (T - t) + T * (v - 1 - m))
+    regs = _reg_run(regs, n, fwd, v, +1, tab, offs)   # fused: the inverse starts from the forward's registers
+    r0i = fwd[::-1][-1]
+    z = np.zeros(n, complex)
+    for t in range(T):
+        for b in range(v // r0i):
+            for r in range(r0i):
+                z[t + b * T + r * (n // r0i)] = regs[t][b * r0i + r]
+    assert np.abs(z / n - x).max() <= 1e-13

@@ -60,6 +60,8 @@ int launch_cols_mixed_f64(int N, int mode, int dir, int W, const ColArgs& a, hip
 int launch_cols_mixed_f32(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
 int mixed_col_w_f64(int N);
 int mixed_col_w_f32(int N);
+int mixed_rows_wg_f64(int N);
+int mixed_rows_wg_f32(int N);
 bool mixed_col_valid_f64(int N, int W);
 bool mixed_col_valid_f32(int N, int W);
 // persistent transfer (wos_persist.hip)
@@ -446,10 +448,19 @@ static std::vector<int> mixed_plan_host(int n) {
   return rad;
 }
 
+// Host mirror of MRDef<N> (wos_mixreg.hpp): forward stage radices of the
+// register-resident column pass where one is built, else empty.
+static std::vector<int> mixed_reg_plan_host(int n) {
+  if (n == 2000) return {10, 10, 10, 2};
+  return {};
+}
+
 // Stage twiddle tables of the mixed-radix plan of n points (stages 1..S-1,
-// the layout MPlan<N>::tw_off addresses): entry [k][r-1] of stage (NS, RAD)
-// is exp(-2 pi i r k/(NS RAD)), formed in long double from the reduced
-// fraction (r k mod NS RAD)/(NS RAD) and rounded once; cached.
+// the layout MPlan<N>::tw_off addresses), followed, where a register-resident
+// column pass is built, by its forward stages 1..S-1 and inverse stages
+// 1..S-1 (MRPlan<N>::tw_off): entry [k][r-1] of stage (NS, RAD) is
+// exp(-2 pi i r k/(NS RAD)), formed in long double from the reduced fraction
+// (r k mod NS RAD)/(NS RAD) and rounded once; cached.
 static int get_mixed_twiddles(int n, int prec, const void** out) {
   Device& d = device();
   const long long key = (1LL << 40) + (long long)n * 2 + prec;
@@ -458,18 +469,26 @@ static int get_mixed_twiddles(int n, int prec, const void** out) {
     *out = it->second;
     return 0;
   }
-  const std::vector<int> rad = mixed_plan_host(n);
   std::vector<double> tab;  // (re, im) pairs
-  long long ns = rad[0];
-  for (size_t s = 1; s < rad.size(); ++s) {
-    const long long L = ns * rad[s];
-    for (long long k = 0; k < ns; ++k)
-      for (int r = 1; r < rad[s]; ++r) {
-        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)((r * k) % L) / (long double)L;
-        tab.push_back((double)cosl(ang));
-        tab.push_back((double)sinl(ang));
-      }
-    ns = L;
+  auto add_stages = [&tab](const std::vector<int>& rad) {
+    long long ns = rad[0];
+    for (size_t s = 1; s < rad.size(); ++s) {
+      const long long L = ns * rad[s];
+      for (long long k = 0; k < ns; ++k)
+        for (int r = 1; r < rad[s]; ++r) {
+          const long double ang =
+              -2.0L * 3.141592653589793238462643383279502884L * (long double)((r * k) % L) / (long double)L;
+          tab.push_back((double)cosl(ang));
+          tab.push_back((double)sinl(ang));
+        }
+      ns = L;
+    }
+  };
+  add_stages(mixed_plan_host(n));
+  const std::vector<int> reg = mixed_reg_plan_host(n);
+  if (!reg.empty()) {
+    add_stages(reg);
+    add_stages(std::vector<int>(reg.rbegin(), reg.rend()));
   }
   const size_t cnt = std::max<size_t>(tab.size() / 2, 1);
   void* p = nullptr;
@@ -1289,7 +1308,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
     tu = Tuning();
     tu.tw = t_w;
     tu.col_w = c_w;
-    tu.rows = 1;
+    tu.rows = prec == WOS_FP64 ? mixed_rows_wg_f64(M) : mixed_rows_wg_f32(M);
     tu.radix = tu.col_radix = 16;
     tu = inv_from_fwd(tu);
   } else {

@@ -10,4 +10,5 @@ int launch_cols_mixed_f32(int N, int mode, int dir, int W, const ColArgs& a, hip
 }
 int mixed_col_w_f32(int N) { return mixed_col_w_T<float>(N); }
 bool mixed_col_valid_f32(int N, int W) { return mixed_col_valid_T<float>(N, W); }
+int mixed_rows_wg_f32(int N) { return mixed_rows_wg_T<float>(N); }
 }  // namespace wos
