@@ -250,16 +250,18 @@ def _reg_run(regs, n, fwd, v, d, tab, offs):
     return regs
 
 
-@pytest.mark.parametrize("fwd,v", [([10, 10, 10, 2], 10), ([20, 20, 5], 20)])
-def test_register_resident_plan_2000(fwd, v):
-    """The register-resident 2000-point design: forward stages [10, 10, 10, 2]
-    on 200 threads of 10 values (and the [20, 20, 5] variant with the 4 x 5
-    radix-20 butterfly), the inverse plan reversed so the forward output map
-    is the inverse input map (forward -> transfer function -> inverse stays in
-    registers), the twiddle layout, and the mirror map of the transfer
-    function (row N - p of thread t's row p = t + T m lies in thread T - t at
+@pytest.mark.parametrize("n,fwd,v", [(2000, [10, 10, 10, 2], 10), (2000, [20, 20, 5], 20),
+                                    (1536, [8, 8, 8, 3], 24), (1000, [10, 10, 10], 10)])
+def test_register_resident_plans(n, fwd, v):
+    """The register-resident mixed-radix designs (MRDef in wos_mixreg.hpp):
+    2000 points as [10, 10, 10, 2] on 200 threads of 10 values (and the
+    [20, 20, 5] variant with the 4 x 5 radix-20 butterfly), 1536 as
+    [8, 8, 8, 3] on 64 threads of 24, 1000 as [10, 10, 10] on 100 threads of
+    10; the inverse plan reversed so the forward output map is the inverse
+    input map (forward -> transfer function -> inverse stays in registers),
+    the twiddle layout, and the mirror map of the transfer function (row
+    N - p of thread t's row p = t + T m lies in thread T - t at
     m' = V - 1 - m, thread 0 at m' = V - m)."""
-    n = 2000
     T = n // v
     tab, offs = _reg_tables(fwd)
     rng = np.random.default_rng(7)

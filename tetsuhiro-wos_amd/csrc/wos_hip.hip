@@ -452,6 +452,8 @@ static std::vector<int> mixed_plan_host(int n) {
 // register-resident column pass where one is built, else empty.
 static std::vector<int> mixed_reg_plan_host(int n) {
   if (n == 2000) return {10, 10, 10, 2};
+  if (n == 1536) return {8, 8, 8, 3};
+  if (n == 1000) return {10, 10, 10};
   return {};
 }
 

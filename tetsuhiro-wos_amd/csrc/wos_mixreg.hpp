@@ -107,7 +107,20 @@ struct MRDef {
 template <>
 struct MRDef<2000> {
   static constexpr int S = 4, V = 10;
+  static constexpr bool F32 = false, F64 = true;  // FP32 keeps the LDS-resident kernels
   static constexpr int rad(int s) { return s == 3 ? 2 : 10; }
+};
+template <>
+struct MRDef<1536> {
+  static constexpr int S = 4, V = 24;
+  static constexpr bool F32 = true, F64 = false;
+  static constexpr int rad(int s) { return s == 3 ? 3 : 8; }
+};
+template <>
+struct MRDef<1000> {
+  static constexpr int S = 3, V = 10;
+  static constexpr bool F32 = true, F64 = true;
+  static constexpr int rad(int s) { return 10; }
 };
 
 template <int N>
@@ -118,7 +131,10 @@ constexpr bool mr_built() { return MRDef<N>::S > 0; }
 // are no faster (0.085 against 0.081-0.084 ms at the best tiles; the
 // four-row row pass alone 23 against 17 us), so FP32 keeps those.
 template <typename R, int N>
-constexpr bool mr_use() { return mr_built<N>() && sizeof(R) == 8; }
+constexpr bool mr_use() {
+  if constexpr (mr_built<N>()) return sizeof(R) == 8 ? MRDef<N>::F64 : MRDef<N>::F32;
+  else return false;
+}
 
 template <int N>
 struct MRPlan {
