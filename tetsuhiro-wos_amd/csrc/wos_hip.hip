@@ -3186,6 +3186,25 @@ static int time_transfer_kernel(const TransferLaunch& L, int k, int iters, hipSt
   return 0;
 }
 
+// Time the whole step (the three kernels in order) of a built transfer: one
+// warm-up step, then `iters` steps back to back between two events.
+static int time_transfer_step(const TransferLaunch& L, int iters, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                              double* ms_out) {
+  int rc;
+  for (int k = 0; k < 3; ++k)
+    if ((rc = launch_transfer_kernel(L, k, s))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e0, s));
+  for (int i = 0; i < iters; ++i)
+    for (int k = 0; k < 3; ++k)
+      if ((rc = launch_transfer_kernel(L, k, s))) return rc;
+  WOS_HIP_TRY(hipEventRecord(e1, s));
+  WOS_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  WOS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms / iters;
+  return 0;
+}
+
 // Launch-shape search of one transfer step on a power-of-two grid of side M
 // (n rows per unit). The row passes and the column pass share only the
 // intermediate layout's tile width tw, so for each tw they are searched
@@ -3196,7 +3215,11 @@ static int time_transfer_kernel(const TransferLaunch& L, int k, int iters, hipSt
 //     split exchanges (radix-16);
 //   column pass: stage schedule, columns per workgroup (tw, tw/2 or tw/4),
 //     split exchanges (radix-16).
-// The tw with the smallest sum of its best row and column times wins.
+// The tw with the smallest sum of its best row and column times wins. A
+// pass timed alone runs on caches and a memory system the other passes do
+// not leave it in the step, so the three fastest forward and the three
+// fastest inverse row shapes of the winning tw (and its row schedule) are
+// then timed in whole steps, nine pairings, and the fastest step is kept.
 // build(t, L) builds the three launches of shape t.
 template <typename Build>
 static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const Build& build, Tuning* out) {
@@ -3206,17 +3229,28 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
   WOS_HIP_TRY(hipEventCreate(&e1));
   double best = 1e300;
   Tuning bt;
+  // the fastest row shapes (time, shape) of one tw and row schedule
+  typedef std::vector<std::pair<double, Tuning>> Ranked;
+  auto keep3 = [](Ranked& r, double ms, const Tuning& t) {
+    r.emplace_back(ms, t);
+    std::stable_sort(r.begin(), r.end(),
+                     [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) { return a.first < b.first; });
+    if (r.size() > 3) r.resize(3);
+  };
+  Ranked best_f, best_i;  // of the winning tw
   auto search = [&]() -> int {
     int rc;
     for (int tw = 1; tw <= 16; tw *= 2) {
       if ((M % tw) != 0) continue;
       double brow = 1e300;
       Tuning tr;
+      Ranked rf_tw, ri_tw;
       for (int radix : {16, 8}) {
         if (radix == 8 && !radix8_ok(M)) continue;
         // the forward and the inverse pass each take their fastest shape
         double bf = 1e300, bi = 1e300;
         Tuning tf, ti;
+        Ranked rf, ri;
         for (int rows = 1; rows <= 16; rows *= 2) {
           if (!rows_valid(prec, M, rows, radix)) continue;
           if (rows * tw * S < 64) continue;  // chunks under half a line: never competitive
@@ -3238,6 +3272,8 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
               if ((rc = build(t, L))) return rc;
               if ((rc = time_transfer_kernel(L, 0, iters, s, e0, e1, &f))) return rc;
               if ((rc = time_transfer_kernel(L, 2, iters, s, e0, e1, &i))) return rc;
+              keep3(rf, f, t);
+              keep3(ri, i, t);
               if (f < bf) {
                 bf = f;
                 tf = t;
@@ -3255,6 +3291,8 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
           tr.inv_rows = ti.rows;
           tr.inv_groups = ti.groups;
           tr.inv_split = ti.split;
+          rf_tw = rf;
+          ri_tw = ri;
         }
       }
       if (brow >= 1e300) continue;
@@ -3283,8 +3321,32 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
       if (bcol < 1e300 && brow + bcol < best) {
         best = brow + bcol;
         bt = tc;
+        best_f = rf_tw;
+        best_i = ri_tw;
       }
     }
+    // whole steps: the winning tw's fastest forward x inverse row shapes
+    double bstep = 1e300;
+    Tuning bs = bt;
+    for (const auto& f : best_f)
+      for (const auto& i : best_i) {
+        Tuning t = bt;
+        t.rows = f.second.rows;
+        t.groups = f.second.groups;
+        t.split = f.second.split;
+        t.inv_rows = i.second.rows;
+        t.inv_groups = i.second.groups;
+        t.inv_split = i.second.split;
+        TransferLaunch L;
+        double ms;
+        if ((rc = build(t, L))) return rc;
+        if ((rc = time_transfer_step(L, iters, s, e0, e1, &ms))) return rc;
+        if (ms < bstep) {
+          bstep = ms;
+          bs = t;
+        }
+      }
+    bt = bs;
     return 0;
   };
   const int rc = search();
