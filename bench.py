@@ -322,8 +322,8 @@ def config12_inputs():
 
 def bench_config12(wos, tune=True, retune=False):
     """Configurations 1 and 2 as device-resident batches (one unit per field
-    component, replayed from a hipGraph) and as drop-in Propagate calls on
-    host buffers. Algorithmic bytes per component: 8 M^2 S (M = 2n for
+    component; timed as stream launches, and as hipGraph replays) and as
+    drop-in Propagate calls on host buffers. Algorithmic bytes per component: 8 M^2 S (M = 2n for
     asm_pad)."""
     out, fields = {}, {}
     for name, (ex, ey, n, dx, wl, z, method) in config12_inputs().items():
@@ -335,11 +335,14 @@ def bench_config12(wos, tune=True, retune=False):
             _, _, source = tune_batch(b, tune, retune, 3)
             for u, f in enumerate(comps):
                 b.upload(u, f)
+            # stream launches back to back (a 512^2 propagation takes less GPU
+            # time than one graph launch takes to submit), then graph replays
+            ms = sorted(b.time(3, 30)[0] for _ in range(3))[1]   # median of three timings
             b.capture()
             b.run(1)
             b.sync()
             fields[(name, pname)] = [b.download(u) for u in range(len(comps))]
-            ms, _ = b.time(3, 30)
+            ms_graph = sorted(b.time(3, 30)[0] for _ in range(3))[1]
             b.close()
             S = 16 if prec == wos.FP64 else 8
             t0 = time.perf_counter()
@@ -351,6 +354,7 @@ def bench_config12(wos, tune=True, retune=False):
             # (columns), 2n^2 in + n^2 out (inverse rows) = 10 n^2 S
             moved = len(comps) * (10 if method == "asm_pad" else 6) * n * n * S
             res[pname] = {"props_per_s": round(len(comps) / (ms * 1e-3), 1), "device_ms": round(ms, 4),
+                          "graph_replay_ms": round(ms_graph, 4),
                           "hbm_frac_algorithmic": round(len(comps) * 8 * M * M * S / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                           "hbm_frac_moved": round(moved / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                           "dropin_ms_per_call": round((time.perf_counter() - t0) * 1e3 / reps, 2),
