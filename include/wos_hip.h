@@ -293,7 +293,9 @@ int wos_hip_batch_fill_schell(wos_batch b, double w0, double sigma_c, uint64_t s
  * rounds through half the LDS; pipelined kernels of 2 or 4 rows then also
  * stage their twiddles in LDS), the forward and the inverse row pass each
  * taking its own rows, groups and split; for the column pass
- * its own stage schedule (col_radix), the columns per workgroup (col_w:
+ * its own stage schedule (col_radix: 16, 8, or 32 = radix-16 stages with 32
+ * values per thread and split exchanges, 1024..4096 points, 4 or 8 columns
+ * per workgroup), the columns per workgroup (col_w:
  * tile_w, tile_w/2 or tile_w/4) and, radix-16, split LDS exchanges
  * (col_split). For each tile_w the row and column shapes
  * are searched apart, each kernel timed over `iters` launches with HIP

@@ -496,13 +496,16 @@ def test_batch_launch_shapes_bit_identical(wos, precision):
               (2, 1, 1, 4, 0), (8, 8, 8, 4, 0), (8, 4, 2, 2, 0), (8, 2, 4, 0, 0), (16, 4, 1, 4, 0), (4, 4, 4, 0, 1),
               (8, 4, 2, 0, 1), (2, 2, 8, 0, 1),
               (4, 4, 4, 4, 1), (4, 4, 4, 2, 1), (8, 4, 2, 4, 1), (8, 8, 2, 2, 1),   # pipelined, twiddles in LDS
-              (4, 4, 4, 2, 1, 2, 2, 1), (4, 4, 4, 0, 1, 2, 0, 0), (8, 8, 1, 0, 0, 8, 4, 0)]   # own inverse shapes
+              (4, 4, 4, 2, 1, 2, 2, 1), (4, 4, 4, 0, 1, 2, 0, 0), (8, 8, 1, 0, 0, 8, 4, 0),   # own inverse shapes
+              (8, 8, 4, 2, 1, 2, 2, 1, 32), (4, 4, 4, 0, 1, -1, 0, 0, 32)]   # 32-value column schedule
     if prec == wos.FP32:
         shapes.append((16, 16, 4, 4, 0))   # 16-column workgroups of 1024-point columns fit LDS in FP32 only
     ref = None
     for tw, cw, rows, groups, split, *inv in shapes:
+        col_radix, col_split = (inv[3], 1) if len(inv) > 3 else (None, 0)
+        inv = inv[:3]
         if tw:
-            b.set_tuning(tw, cw, rows, groups, split, 16, None, 0, *(inv or (-1, 0, 0)))
+            b.set_tuning(tw, cw, rows, groups, split, 16, col_radix, col_split, *(inv or (-1, 0, 0)))
         b.fill_schell(100e-6, 30e-6, 3)
         b.run(1)
         b.sync()
@@ -511,7 +514,7 @@ def test_batch_launch_shapes_bit_identical(wos, precision):
             ref = out
         else:
             for u in range(2):
-                if groups == 0 and split == 0 and (not inv or inv[1:] == [0, 0]):
+                if groups == 0 and split == 0 and (not inv or inv[1:] == [0, 0]) and col_radix is None:
                     assert np.array_equal(out[u], ref[u]), (tw, cw, rows, groups, split, u)
                 else:
                     e = rel_l2(out[u], ref[u])

@@ -189,23 +189,27 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
 // ---------------------------------------------------------------------------
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
+// VV = 32: the radix-16 stages with 32 values per thread (two butterflies
+// of each radix-16 stage per thread, half the threads per transform), for
+// column passes that cover more columns per workgroup.
 template <int N, int VV = 16>
 struct Plan {
-  static_assert(VV == 16 || VV == 8, "radix-16 or radix-8 schedule");
+  static_assert(VV == 16 || VV == 8 || VV == 32, "radix-16 or radix-8 schedule");
+  static constexpr int RV = VV == 32 ? 16 : VV;         // stage radix
   static constexpr int LOG = ilog2(N);
-  static constexpr int LV = ilog2(VV);
+  static constexpr int LV = ilog2(RV);
   static constexpr int V = N >= VV ? VV : N;            // values per thread
   static constexpr int T = N / V;                        // threads per transform
-  static constexpr int NF = LOG / LV;                    // full-radix (VV) stages
+  static constexpr int NF = LOG / LV;                    // full-radix (RV) stages
   static constexpr int REM = 1 << (LOG % LV);            // leftover radix (1: none)
   static constexpr int S = NF + (REM > 1 ? 1 : 0);       // number of stages
   // radix of stage s, forward order
   static constexpr int fwd_rad(int s) {
-    return N < VV ? N : (s < NF ? VV : REM);
+    return N < RV ? N : (s < NF ? RV : REM);
   }
   // radix of stage s, inverse order
   static constexpr int inv_rad(int s) {
-    return N < VV ? N : (REM > 1 ? (s == 0 ? REM : VV) : VV);
+    return N < RV ? N : (REM > 1 ? (s == 0 ? REM : RV) : RV);
   }
   // sub-transform length before stage s
   static constexpr int ns_fwd(int s) {

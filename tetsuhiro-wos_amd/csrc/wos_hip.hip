@@ -54,6 +54,10 @@ int launch_cols8_f64(int N, int mode, int W, const ColArgs& a, hipStream_t s);
 int launch_cols8_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s);
 bool col_w_valid8_f64(int N, int W);
 bool col_w_valid8_f32(int N, int W);
+int launch_cols32_f64(int N, int mode, int W, const ColArgs& a, hipStream_t s);
+int launch_cols32_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s);
+bool col_w_valid32_f64(int N, int W);
+bool col_w_valid32_f32(int N, int W);
 int launch_rows_mixed_f64(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_rows_mixed_f32(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_cols_mixed_f64(int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s);
@@ -686,6 +690,7 @@ bool rows_pipe_lds_valid(int prec, int N, int rows) {
 }
 bool col_w_valid(int prec, int N, int W, int radix) {
   if (radix == 8) return prec == WOS_FP64 ? col_w_valid8_f64(N, W) : col_w_valid8_f32(N, W);
+  if (radix == 32) return prec == WOS_FP64 ? col_w_valid32_f64(N, W) : col_w_valid32_f32(N, W);
   return prec == WOS_FP64 ? col_w_valid_f64(N, W) : col_w_valid_f32(N, W);
 }
 int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hipStream_t s) {
@@ -697,6 +702,7 @@ int launch_cols(int prec, int N, int mode, int dir, int W, const ColArgs& a, hip
     }
     return prec == WOS_FP64 ? launch_cols8_f64(N, mode, W, a, s) : launch_cols8_f32(N, mode, W, a, s);
   }
+  if (a.radix == 32) return prec == WOS_FP64 ? launch_cols32_f64(N, mode, W, a, s) : launch_cols32_f32(N, mode, W, a, s);
   return prec == WOS_FP64 ? launch_cols_f64(N, mode, dir, W, a, s) : launch_cols_f32(N, mode, dir, W, a, s);
 }
 int default_col_w(int prec, int N, int cols) {
@@ -1207,7 +1213,7 @@ struct Tuning {
   int col_w = 0;   // columns per column-pass workgroup, dividing tw (0: tw)
   int split = 0;   // one-group row kernels exchange real and imaginary parts separately
   int radix = 0;      // stage schedule of the row passes: 8 radix-8, else radix-16
-  int col_radix = 0;  // stage schedule of the column pass: 8 radix-8, else radix-16
+  int col_radix = 0;  // stage schedule of the column pass: 8 radix-8, 32 radix-16 with 32 values per thread, else radix-16
   int col_split = 0;  // radix-16 asm column pass: exchanges through half the LDS
   // the inverse row pass's rows per workgroup, row groups and split
   // exchanges (it reads the tiled layout the forward pass writes, and the
@@ -1230,13 +1236,15 @@ static Tuning inv_from_fwd(Tuning t) {
 
 // The radix-8 schedule is built for the asm transfer kernels from 256 points.
 static bool radix8_ok(int M) { return pow2(M) && M >= 256 && M <= 8192; }
+// The 32-value column schedule (radix-16 stages, split asm column pass).
+static bool col32_ok(int M) { return M == 1024 || M == 2048 || M == 4096; }
 
 // Fill unset fields: the tile width and rows per workgroup that make one
 // tile chunk of ROWS x TW elements a whole 128-byte line.
 static Tuning resolve_tuning(int prec, int M, Tuning t) {
   const int S = (int)csize(prec);
   t.radix = t.radix == 8 && radix8_ok(M) ? 8 : 16;
-  t.col_radix = t.col_radix == 8 && radix8_ok(M) ? 8 : 16;
+  t.col_radix = t.col_radix == 8 && radix8_ok(M) ? 8 : (t.col_radix == 32 && col32_ok(M) ? 32 : 16);
   if (t.tw <= 0) {
     int cand[] = {prec == WOS_FP32 ? 4 : 2, 4, 8, 16, 2, 1};
     for (int w : cand)
@@ -1333,7 +1341,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   } else {
     if ((rc = get_twiddles(Mk, prec, &tw, 0, tu.radix))) return rc;
     if ((rc = get_twiddles(Mk, WOS_FP64, &tw64, 0, tu.radix))) return rc;
-    if ((rc = get_twiddles(Mk, prec, &twc, 0, tu.col_radix))) return rc;
+    if ((rc = get_twiddles(Mk, prec, &twc, 0, tu.col_radix == 32 ? 16 : tu.col_radix))) return rc;
   }
   L.prec = prec;
   L.M = M;
@@ -1389,7 +1397,7 @@ static int build_transfer(TransferLaunch& L, int prec, void* d, void* inter, int
   ca.tf = d_tf;
   ca.tw = twc;
   ca.radix = tu.col_radix;
-  ca.split = !bs && !mr && L.mode == 1 && tu.col_radix == 16 && M >= 256 ? tu.col_split : 0;
+  ca.split = !bs && !mr && L.mode == 1 && tu.col_radix != 8 && M >= 256 ? tu.col_split : 0;
   RowArgs& rb = L.inv;
   rb = row_args_init();
   rb.in = inter;
@@ -3213,8 +3221,9 @@ static int time_transfer_step(const TransferLaunch& L, int iters, hipStream_t s,
 //     shared by both, and for the forward and the inverse pass each its
 //     fastest rows per workgroup, row groups of the pipelined kernels and
 //     split exchanges (radix-16);
-//   column pass: stage schedule, columns per workgroup (tw, tw/2 or tw/4),
-//     split exchanges (radix-16).
+//   column pass: stage schedule (radix-16, radix-8, or radix-16 stages with
+//     32 values per thread), columns per workgroup (tw, tw/2 or tw/4), split
+//     exchanges (radix-16).
 // The tw with the smallest sum of its best row and column times wins. A
 // pass timed alone runs on caches and a memory system the other passes do
 // not leave it in the step, so the three fastest forward and the three
@@ -3298,11 +3307,12 @@ static int tune_transfer(int prec, int M, int n, int iters, hipStream_t s, const
       if (brow >= 1e300) continue;
       double bcol = 1e300;
       Tuning tc = tr;
-      for (int radix : {16, 8}) {
-        if (radix == 8 && !radix8_ok(M)) continue;
+      for (int radix : {16, 8, 32}) {
+        if ((radix == 8 && !radix8_ok(M)) || (radix == 32 && !col32_ok(M))) continue;
         for (int cw = tw; cw >= 1 && cw * 4 >= tw; cw /= 2) {
           if (!col_w_valid(prec, M, cw, radix)) continue;
-          for (int cs = 0; cs <= (radix == 16 && M >= 256 ? 1 : 0); ++cs) {
+          // split exchanges: radix-16; the 32-value schedule builds them only
+          for (int cs = radix == 32 ? 1 : 0; cs <= (radix != 8 && M >= 256 ? 1 : 0); ++cs) {
             Tuning t = tr;
             t.col_w = cw;
             t.col_radix = radix;
@@ -3366,7 +3376,7 @@ static void tuning_out(const Tuning& t, int* shape) {
   shape[3] = t.groups;
   shape[4] = t.split;
   shape[5] = t.radix == 8 ? 8 : 16;
-  shape[6] = t.col_radix == 8 ? 8 : 16;
+  shape[6] = t.col_radix == 8 ? 8 : t.col_radix == 32 ? 32 : 16;
   shape[7] = t.col_split;
   const Tuning ti = inv_from_fwd(t);
   shape[8] = ti.inv_rows;
@@ -3449,12 +3459,17 @@ static int shape_to_tuning(int prec, int M, const int* sh, Tuning* out, const ch
     *out = t;
     return WOS_OK;
   }
-  if (col_split && col_radix != 16) {
+  if (col_split && col_radix == 8) {
     set_error(std::string(what) + ": split column exchanges are radix-16 column passes only");
     return WOS_ERR_INVALID;
   }
+  if (col_radix == 32 && (!col_split || !col32_ok(M))) {
+    set_error(std::string(what) + ": the 32-value column schedule is the split radix-16 column pass of 1024, 2048 "
+              "and 4096 points");
+    return WOS_ERR_INVALID;
+  }
   if (!(radix == 16 || (radix == 8 && radix8_ok(M) && !split)) ||
-      !(col_radix == 16 || (col_radix == 8 && radix8_ok(M)))) {
+      !(col_radix == 16 || col_radix == 32 || (col_radix == 8 && radix8_ok(M)))) {
     set_error(std::string(what) + ": row radix " + std::to_string(radix) + (split ? " with split exchanges" : "") +
               " or column radix " + std::to_string(col_radix) + " not built for a grid of " + std::to_string(M));
     return WOS_ERR_INVALID;

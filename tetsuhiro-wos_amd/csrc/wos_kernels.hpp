@@ -1145,7 +1145,7 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int blk, int nblk, c
 }
 
 template <typename R, int N, int W, int MODE, int DIR, int VV = 16, bool CS = false>
-__global__ __launch_bounds__((W * Plan<N, VV>::T), ((CS && sizeof(R) == 8) ? 4 : 1))
+__global__ __launch_bounds__((W * Plan<N, VV>::T), (((CS && sizeof(R) == 8) || VV == 32) ? 4 : 1))
 void k_cols(const ColArgs a) {
   __shared__ cpx<R> lds[cols_lds_elems<R, N, W, CS>()];
   cols_body<R, N, W, MODE, DIR, VV, CS>(a, blockIdx.x, gridDim.x, lds);

@@ -10,4 +10,8 @@ int launch_cols8_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s) {
   return launch_cols8_T<float>(N, mode, W, a, s);
 }
 bool col_w_valid8_f32(int N, int W) { return col_w_valid8_T<float>(N, W); }
+int launch_cols32_f32(int N, int mode, int W, const ColArgs& a, hipStream_t s) {
+  return launch_cols32_T<float>(N, mode, W, a, s);
+}
+bool col_w_valid32_f32(int N, int W) { return col_w_valid32_T<float>(N, W); }
 }  // namespace wos

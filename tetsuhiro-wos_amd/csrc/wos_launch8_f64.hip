@@ -10,4 +10,8 @@ int launch_cols8_f64(int N, int mode, int W, const ColArgs& a, hipStream_t s) {
   return launch_cols8_T<double>(N, mode, W, a, s);
 }
 bool col_w_valid8_f64(int N, int W) { return col_w_valid8_T<double>(N, W); }
+int launch_cols32_f64(int N, int mode, int W, const ColArgs& a, hipStream_t s) {
+  return launch_cols32_T<double>(N, mode, W, a, s);
+}
+bool col_w_valid32_f64(int N, int W) { return col_w_valid32_T<double>(N, W); }
 }  // namespace wos
