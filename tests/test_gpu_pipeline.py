@@ -282,3 +282,39 @@ def test_pipeline_autotuned_mixed_stages_match_oracle(wos, oracle, precision):
             print(f"pipeline mixed n={n} {method} unit {u} ({precision}) rel-L2 {e:.2e}", flush=True)
             assert e <= tol
     p.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_shared_source_stage_32_value_columns(wos, oracle, precision):
+    """A shared-source stage (one source, 3 wavelengths at 1024^2: the forward
+    spectrum computed once) at the 32-value column schedule on 8-column tiles
+    (the MODE 4 column pass on radix-16 stages with 32 values per thread, and
+    the source's own column FFT on 16 values per thread in 4-column
+    workgroups), with the pipelined row passes: each unit matches the
+    oracle's Propagate, and graph replays equal the eager run."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    tol = FP64_TOL if prec == wos.FP64 else FP32_TOL
+    ctx = wos.ctx_for(550e-9)
+    n, dx = 1024, 2e-6
+    wls, zs = np.array([450e-9, 550e-9, 650e-9]), np.array([4e-3, 6e-3, 8e-3])
+    p = wos.Pipeline(prec)
+    s = p.add(n, dx, wls, zs, ctx, "asm", src_units=1)
+    src = random_field(n, 77)
+    p.upload(s, 0, src)
+    p.set_tuning(s, dict(tile_w=8, col_w=8, rows=2, row_groups=2, split_lds=0, radix=16, col_radix=32,
+                         col_split_lds=1, inv_rows=2, inv_row_groups=2, inv_split_lds=1))
+    assert p.get_tuning(s)["col_radix"] == 32
+    p.run(1)
+    p.sync()
+    eager = [p.download(s, u) for u in range(3)]
+    for u in range(3):
+        ref, _, _, _ = oracle.propagate(src, None, dx, zs[u], "asm", oracle.ctx_for(wls[u]))
+        e = rel_l2(eager[u], ref)
+        print(f"shared-source 32-value columns {precision} unit {u}: rel-L2 vs oracle {e:.2e}", flush=True)
+        assert e <= tol, (u, e)
+    p.capture()
+    for _ in range(3):
+        p.run(1)
+        p.sync()
+        assert all(np.array_equal(p.download(s, u), eager[u]) for u in range(3))
+    p.close()
