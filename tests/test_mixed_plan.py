@@ -182,19 +182,10 @@ def _dft10(v, d):
     return np.concatenate([a + b, a - b])
 
 
-def _dft20(v, d):
-    """4 x 5: DFT-5 over n1 of x[4 n1 + n2], twiddle W20^(n2 k1), DFT-4 over n2; X[k1 + 5 k2]."""
-    a = np.array([_dft5(v[n2::4], d) for n2 in range(4)])
-    a *= np.exp(d * 2j * np.pi * np.outer(np.arange(4), np.arange(5)) / 20)
-    return np.concatenate([sum(a[n2] * np.exp(d * 2j * np.pi * n2 * k2 / 4) for n2 in range(4)) for k2 in range(4)])
-
-
 def _dft_any(v, d):
     n = len(v)
     if n == 10:
         return _dft10(v, d)
-    if n == 20:
-        return _dft20(v, d)
     if n == 5:
         return _dft5(v, d)
     return np.array([np.sum(v * np.exp(d * 2j * np.pi * np.arange(n) * k / n)) for k in range(n)])
@@ -250,12 +241,11 @@ def _reg_run(regs, n, fwd, v, d, tab, offs):
     return regs
 
 
-@pytest.mark.parametrize("n,fwd,v", [(2000, [10, 10, 10, 2], 10), (2000, [20, 20, 5], 20),
+@pytest.mark.parametrize("n,fwd,v", [(2000, [10, 10, 10, 2], 10),
                                     (1536, [8, 8, 8, 3], 24), (1000, [10, 10, 10], 10)])
 def test_register_resident_plans(n, fwd, v):
     """The register-resident mixed-radix designs (MRDef in wos_mixreg.hpp):
-    2000 points as [10, 10, 10, 2] on 200 threads of 10 values (and the
-    [20, 20, 5] variant with the 4 x 5 radix-20 butterfly), 1536 as
+    2000 points as [10, 10, 10, 2] on 200 threads of 10 values, 1536 as
     [8, 8, 8, 3] on 64 threads of 24, 1000 as [10, 10, 10] on 100 threads of
     10; the inverse plan reversed so the forward output map is the inverse
     input map (forward -> transfer function -> inverse stays in registers),

@@ -78,11 +78,39 @@ __device__ __forceinline__ void dft7(cpx<R>* v) {
   v[4] = t3 - u3;
 }
 
+// 10 points as 2 x 5: n = 2 n1 + n2; DFT-5 over n1 for each n2, twiddle
+// W10^(n2 k1), DFT-2 over n2; X[k1 + 5 k2].
+template <int DIR, typename R>
+__device__ __forceinline__ void dft10(cpx<R>* v) {
+  constexpr double C36 = 0.80901699437494742410, S36 = 0.58778525229247312917;  // cos, sin 36 degrees
+  constexpr double C72 = 0.30901699437494742410, S72 = 0.95105651629515357212;  // cos, sin 72 degrees
+  constexpr double cs[5] = {1, C36, C72, -C72, -C36}, sn[5] = {0, S36, S72, S72, S36};
+  cpx<R> a[5], b[5];
+#pragma unroll
+  for (int n1 = 0; n1 < 5; ++n1) {
+    a[n1] = v[2 * n1];
+    b[n1] = v[2 * n1 + 1];
+  }
+  dft5<DIR>(a);
+  dft5<DIR>(b);
+#pragma unroll
+  for (int k = 1; k < 5; ++k) {
+    const R c = (R)cs[k], s = (R)(DIR < 0 ? -sn[k] : sn[k]);
+    b[k] = {c * b[k].x - s * b[k].y, s * b[k].x + c * b[k].y};
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 5; ++k1) {
+    v[k1] = a[k1] + b[k1];
+    v[k1 + 5] = a[k1] - b[k1];
+  }
+}
+
 template <int RAD, int DIR, typename R>
 __device__ __forceinline__ void dft_mixed(cpx<R>* v) {
   if constexpr (RAD == 3) dft3<DIR>(v);
   else if constexpr (RAD == 5) dft5<DIR>(v);
   else if constexpr (RAD == 7) dft7<DIR>(v);
+  else if constexpr (RAD == 10) dft10<DIR>(v);
   else dft<RAD, DIR, R>(v);
 }
 

@@ -23,82 +23,6 @@
 
 namespace wos {
 
-// exp(2 pi i m / 20), m = 0..12: cosines and sines of multiples of 18 degrees
-struct W20 {
-  static constexpr double C18 = 0.95105651629515357212, S18 = 0.30901699437494742410;
-  static constexpr double C36 = 0.80901699437494742410, S36 = 0.58778525229247312917;
-};
-template <int DIR, int M, typename R>
-__device__ __forceinline__ cpx<R> w20(cpx<R> a) {
-  constexpr double cs[13] = {1, W20::C18, W20::C36, W20::S36, W20::S18, 0, -W20::S18, -W20::S36, -W20::C36,
-                             -W20::C18, -1, -W20::C18, -W20::C36};
-  constexpr double sn[13] = {0, W20::S18, W20::S36, W20::C36, W20::C18, 1, W20::C18, W20::C36, W20::S36,
-                             W20::S18, 0, -W20::S18, -W20::S36};
-  const R c = (R)cs[M], s = (R)(DIR < 0 ? -sn[M] : sn[M]);
-  return {c * a.x - s * a.y, s * a.x + c * a.y};
-}
-
-// 20 points as 4 x 5: n = 4 n1 + n2; DFT-5 over n1 for each n2, twiddle
-// W20^(n2 k1), DFT-4 over n2 for each k1; X[k1 + 5 k2].
-template <int DIR, typename R>
-__device__ __forceinline__ void dft20(cpx<R>* v) {
-  cpx<R> a[4][5];
-#pragma unroll
-  for (int n2 = 0; n2 < 4; ++n2) {
-#pragma unroll
-    for (int n1 = 0; n1 < 5; ++n1) a[n2][n1] = v[4 * n1 + n2];
-    dft5<DIR>(a[n2]);
-  }
-  a[1][1] = w20<DIR, 1>(a[1][1]);
-  a[1][2] = w20<DIR, 2>(a[1][2]);
-  a[1][3] = w20<DIR, 3>(a[1][3]);
-  a[1][4] = w20<DIR, 4>(a[1][4]);
-  a[2][1] = w20<DIR, 2>(a[2][1]);
-  a[2][2] = w20<DIR, 4>(a[2][2]);
-  a[2][3] = w20<DIR, 6>(a[2][3]);
-  a[2][4] = w20<DIR, 8>(a[2][4]);
-  a[3][1] = w20<DIR, 3>(a[3][1]);
-  a[3][2] = w20<DIR, 6>(a[3][2]);
-  a[3][3] = w20<DIR, 9>(a[3][3]);
-  a[3][4] = w20<DIR, 12>(a[3][4]);
-#pragma unroll
-  for (int k1 = 0; k1 < 5; ++k1) {
-    dft4<DIR>(a[0][k1], a[1][k1], a[2][k1], a[3][k1]);
-#pragma unroll
-    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 5 * k2] = a[k2][k1];
-  }
-}
-
-// 10 points as 2 x 5: n = 2 n1 + n2; DFT-5 over n1 for each n2, twiddle
-// W10^(n2 k1) = W20^(2 k1), DFT-2 over n2; X[k1 + 5 k2].
-template <int DIR, typename R>
-__device__ __forceinline__ void dft10(cpx<R>* v) {
-  cpx<R> a[5], b[5];
-#pragma unroll
-  for (int n1 = 0; n1 < 5; ++n1) {
-    a[n1] = v[2 * n1];
-    b[n1] = v[2 * n1 + 1];
-  }
-  dft5<DIR>(a);
-  dft5<DIR>(b);
-  b[1] = w20<DIR, 2>(b[1]);
-  b[2] = w20<DIR, 4>(b[2]);
-  b[3] = w20<DIR, 6>(b[3]);
-  b[4] = w20<DIR, 8>(b[4]);
-#pragma unroll
-  for (int k1 = 0; k1 < 5; ++k1) {
-    v[k1] = a[k1] + b[k1];
-    v[k1 + 5] = a[k1] - b[k1];
-  }
-}
-
-template <int RAD, int DIR, typename R>
-__device__ __forceinline__ void dft_mr(cpx<R>* v) {
-  if constexpr (RAD == 20) dft20<DIR>(v);
-  else if constexpr (RAD == 10) dft10<DIR>(v);
-  else dft_mixed<RAD, DIR>(v);
-}
-
 // Stage radices of the register-resident plans (forward order).
 template <int N>
 struct MRDef {
@@ -170,7 +94,7 @@ __device__ __forceinline__ void mr_compute(cpx<R>* v, int t, const cpx<R>* __res
 #pragma unroll
       for (int r = 1; r < RAD; ++r) v[b * RAD + r] = DIR < 0 ? cmul(v[b * RAD + r], tk[r - 1]) : cmulc(v[b * RAD + r], tk[r - 1]);
     }
-    dft_mr<RAD, DIR>(v + b * RAD);
+    dft_mixed<RAD, DIR>(v + b * RAD);
   }
 }
 
