@@ -263,6 +263,29 @@ __device__ __forceinline__ cpx<float> asm_tf_fp32_full(const TFUnit& u, const TF
   return h;
 }
 
+// Propagating asmTF of the FP32 path from ay = wl*fy directly (the caller
+// has folded the row and formed ay): x <= 0 is clamped to a tiny positive
+// value instead of selecting s = 0 afterwards, so such rows get a phase of
+// 0 turns, whose FP32 remainder rounds to 0: h = (1, 0) exactly, for the
+// evanescent fix-up to scale. *evan is raised for x < 0.
+__device__ __forceinline__ cpx<float> asm_tf_fp32_prop_ay(const TF32Col& c, double ay, bool* evan) {
+  const double x = fma(-ay, ay, c.one_m_ax2);
+  *evan |= x < 0;
+  const double xc = fmax(x, 1e-300);
+  const double r = __builtin_amdgcn_rsq(xc);
+  const double g = xc * r, h = 0.5 * r;
+  float sn, cs;
+  sincos_turns_f32(c.kq * fma(fma(-g, g, xc), h, g), &sn, &cs);
+  return {cs, sn};
+}
+
+// Evanescent factor of the FP32 path at ay = wl*fy (rho > 1), else 1.
+__device__ __forceinline__ float asm_tf_fp32_evan_ay(const TFUnit& u, const TF32Col& c, double ay) {
+  const double x = fma(-ay, ay, c.one_m_ax2);
+  if (x >= 0) return 1.f;
+  return (float)asm_evanescent_inl(u, u.k * u.az * sqrt(-x));
+}
+
 // FP32-path asmTF at any (fx, fy); INL keeps the evanescent branch inline
 template <bool INL>
 __device__ __forceinline__ cpx<float> asm_tf_fp32_generic_t(const TFUnit& u, double fx, double fy) {
@@ -984,15 +1007,48 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int blk, int nblk, c
         // values a thread passes on fill (N/2) W complex slots, which the
         // split layout's half-size array holds too.
         __syncthreads();  // all waves are done reading the forward transform's last exchange
+        if constexpr (sizeof(R) == 4) {
+          // The rows t + T m (m <= V/2) are at most N/2, the non-negative
+          // half: no fold, and wl*fy = ay0 + m (T wl/N) from one FMA with
+          // the constant m. Evanescent rows (rho > 1, rare) come out as
+          // h = (1, 0) and are scaled and rewritten after the loop.
+          const double ay0 = fma((double)t, cc.cwl, cc.coff), dT = (double)T * cc.cwl;
+          bool evan = false;
 #pragma unroll
-        for (int m = 0; m < V2; ++m) {
-          const cpx<R> h = tf_at(t + T * m);
-          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
-          hx[(m * T + t) * W] = h;
-        }
-        if (t == 0) {  // row N/2 (Nyquist) is its own mirror
-          constexpr int m = V2;
-          v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
+          for (int m = 0; m < V2; ++m) {
+            const cpx<R> h = asm_tf_fp32_prop_ay(cc, fma((double)m, dT, ay0), &evan);
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
+            hx[(m * T + t) * W] = h;
+          }
+          if (t == 0) {  // row N/2 (Nyquist) is its own mirror
+            constexpr int m = V2;
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB],
+                                             asm_tf_fp32_prop_ay(cc, fma((double)m, dT, ay0), &evan));
+          }
+          if (evan) {
+            // ay again from an opaque copy of ay0: reusing the main loop's
+            // values would keep them live across it (spilled)
+            double ay0e = ay0;
+            asm volatile("" : "+v"(ay0e));
+#pragma unroll
+            for (int m = 0; m <= V2; ++m) {
+              if (m == V2 && t != 0) break;
+              const float e = asm_tf_fp32_evan_ay(u, cc, fma((double)m, dT, ay0e));
+              v[(m % NB) * RL + m / NB] = cscale(v[(m % NB) * RL + m / NB], e);
+              if (m < V2) hx[(m * T + t) * W] = {e, 0.f};
+            }
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < V2; ++m) {
+            const cpx<R> h = tf_at(t + T * m);
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
+            hx[(m * T + t) * W] = h;
+          }
+          if (t == 0) {  // row N/2 (Nyquist) is its own mirror
+            constexpr int m = V2;
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], tf_at(T * m));
+          }
         }
         __syncthreads();
 #pragma unroll
