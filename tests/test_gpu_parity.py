@@ -160,6 +160,35 @@ def test_evanescent_policies_parity(wos, oracle, z, reg):
     assert wg.as_dict().keys() == wo.as_dict().keys()
 
 
+@pytest.mark.parametrize("n,shape", [(256, None), (2048, None), (2048, (8, 8, 2, 2, 0, 16, 32, 1))])
+@pytest.mark.parametrize("z,reg", [(0.6e-6, False), (-0.6e-6, False), (-0.6e-6, True)])
+def test_batch_fp32_evanescent_parity(wos, oracle, n, shape, z, reg):
+    """The FP32 batch column pass on grids with evanescent bins (dx = 0.3 um,
+    wavelengths 0.9 and 1.0 um: rho reaches 2.4 in the corners): the
+    propagating transfer function returns 1 on those rows and a fix-up loop
+    applies the decay, backward zeroing or Tikhonov factor
+    (optics/propagate.go:168-180). Every unit matches the oracle's
+    Propagate at its wavelength and distance to the FP32 tolerance; 2048
+    points also under the 32-value column schedule."""
+    dx, wls, zs = 0.3e-6, np.array([0.9e-6, 1.0e-6]), np.array([z, 1.5 * z])
+    rng = np.random.default_rng(n)
+    srcs = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in range(2)]
+    b = wos.Batch(n, dx, wls, zs, wos.ctx_for(1e-6, backward_regularize=reg, alpha=1e-3), "asm", wos.FP32)
+    if shape is not None:
+        b.set_tuning(*shape)
+    for u, f in enumerate(srcs):
+        b.upload(u, f)
+    b.run(1)
+    b.sync()
+    for u in range(2):
+        want, _, _, _ = oracle.propagate(srcs[u], None, dx, zs[u], "asm",
+                                         oracle.ctx_for(wls[u], backward_regularize=reg, alpha=1e-3))
+        e = rel_l2(b.download(u), want)
+        print(f"evanescent FP32 n={n} shape={shape} z={zs[u]:.2e} reg={reg} unit {u} rel-L2 {e:.3e}", flush=True)
+        assert e <= FP32_TOL, (n, shape, z, reg, u, e)
+    b.close()
+
+
 def test_split_step_fp64_parity(wos, oracle):
     """PropagateSplitStep on a graded-index fibre (config-3 medium, smaller grid)."""
     n, dx, wl = 128, 0.5e-6, 1.55e-6

@@ -1009,33 +1009,42 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int blk, int nblk, c
         __syncthreads();  // all waves are done reading the forward transform's last exchange
         if constexpr (sizeof(R) == 4) {
           // The rows t + T m (m <= V/2) are at most N/2, the non-negative
-          // half: no fold, and wl*fy = ay0 + m (T wl/N) from one FMA with
-          // the constant m. Evanescent rows (rho > 1, rare) come out as
-          // h = (1, 0) and are scaled and rewritten after the loop.
+          // half: no fold, and wl*fy advances by T wl/N from row to row (one
+          // add; the sum of up to V/2 steps stays within a few ulp of ay,
+          // a phase error below 1e-11 turns). Evanescent rows (rho > 1,
+          // rare) come out as h = (1, 0) and are scaled and rewritten after
+          // the loop, by every thread whose extreme rows reach past the
+          // cut-off: |wl*fy| peaks at row 0 or at the end of the walk (row
+          // N/2 in thread 0, with the same rounding as in the loop; beyond
+          // N/2 in the others).
           const double ay0 = fma((double)t, cc.cwl, cc.coff), dT = (double)T * cc.cwl;
-          bool evan = false;
+          bool unused = false;
+          double ay = ay0;
 #pragma unroll
           for (int m = 0; m < V2; ++m) {
-            const cpx<R> h = asm_tf_fp32_prop_ay(cc, fma((double)m, dT, ay0), &evan);
+            const cpx<R> h = asm_tf_fp32_prop_ay(cc, ay, &unused);
             v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], h);
             hx[(m * T + t) * W] = h;
+            ay += dT;
           }
           if (t == 0) {  // row N/2 (Nyquist) is its own mirror
             constexpr int m = V2;
-            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB],
-                                             asm_tf_fp32_prop_ay(cc, fma((double)m, dT, ay0), &evan));
+            v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], asm_tf_fp32_prop_ay(cc, ay, &unused));
           }
-          if (evan) {
+          if (fmin(fma(-ay, ay, cc.one_m_ax2), fma(-cc.coff, cc.coff, cc.one_m_ax2)) < 0) {
             // ay again from an opaque copy of ay0: reusing the main loop's
             // values would keep them live across it (spilled)
-            double ay0e = ay0;
-            asm volatile("" : "+v"(ay0e));
+            double aye = ay0;
+            asm volatile("" : "+v"(aye));
 #pragma unroll
             for (int m = 0; m <= V2; ++m) {
               if (m == V2 && t != 0) break;
-              const float e = asm_tf_fp32_evan_ay(u, cc, fma((double)m, dT, ay0e));
+              const float e = asm_tf_fp32_evan_ay(u, cc, aye);
+              aye += dT;
               v[(m % NB) * RL + m / NB] = cscale(v[(m % NB) * RL + m / NB], e);
-              if (m < V2) hx[(m * T + t) * W] = {e, 0.f};
+              // the partner's copy: only evanescent rows change (a factor
+              // of exactly 1 leaves their (1, 0) as it is)
+              if (m < V2 && e != 1.f) hx[(m * T + t) * W] = {e, 0.f};
             }
           }
         } else {
