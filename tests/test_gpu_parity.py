@@ -160,20 +160,36 @@ def test_evanescent_policies_parity(wos, oracle, z, reg):
     assert wg.as_dict().keys() == wo.as_dict().keys()
 
 
-@pytest.mark.parametrize("n,shape", [(256, None), (2048, None), (2048, (8, 8, 2, 2, 0, 16, 32, 1))])
+EVAN_CASES = [(p, n, None, m) for p in ("fp32", "fp64") for n, m in ((256, "asm"), (2048, "asm"), (1000, "asm"),
+                                                                    (1000, "asm_pad"))]
+EVAN_CASES += [("fp32", 2048, (8, 8, 2, 2, 0, 16, 32, 1), "asm")]
+
+
+@pytest.mark.parametrize("precision,n,shape,method", EVAN_CASES)
 @pytest.mark.parametrize("z,reg", [(0.6e-6, False), (-0.6e-6, False), (-0.6e-6, True)])
-def test_batch_fp32_evanescent_parity(wos, oracle, n, shape, z, reg):
-    """The FP32 batch column pass on grids with evanescent bins (dx = 0.3 um,
-    wavelengths 0.9 and 1.0 um: rho reaches 2.4 in the corners): the
-    propagating transfer function returns 1 on those rows and a fix-up loop
-    applies the decay, backward zeroing or Tikhonov factor
-    (optics/propagate.go:168-180). Every unit matches the oracle's
-    Propagate at its wavelength and distance to the FP32 tolerance; 2048
-    points also under the 32-value column schedule."""
-    dx, wls, zs = 0.3e-6, np.array([0.9e-6, 1.0e-6]), np.array([z, 1.5 * z])
+def test_batch_evanescent_parity(wos, oracle, precision, n, shape, method, z, reg):
+    """Batch column passes on grids with evanescent bins (dx = 0.3 um:
+    rho reaches 2 and more in the corners), where the transfer function
+    takes the decay, backward zeroing or Tikhonov factor
+    (optics/propagate.go:168-180) and the mirrored half of each column
+    takes its partner's values: the FP32 power-of-two passes evaluate the
+    propagating factor everywhere and fix evanescent rows up afterwards.
+    The second unit's wavelength is M dx / K with a Pythagorean K (100 at
+    256 points, 300 at 1000, 600 for the 2000-point asm_pad transforms, 500
+    at 2048), so bins such as (3K/5, 4K/5) lie on rho = 1, where the
+    reference's own test picks the branch (zeroing: the whole value).
+    Power-of-two and mixed-radix sides (1000 asm_pad: register-resident in
+    FP64), 2048 points also under the 32-value column schedule; every unit
+    matches the oracle's Propagate at its wavelength and distance to the
+    precision's tolerance."""
+    prec = wos.FP64 if precision == "fp64" else wos.FP32
+    M = 2 * n if method == "asm_pad" else n
+    K = {256: 100, 1000: 300, 2000: 600, 2048: 500}[M]
+    dx = 0.3e-6
+    wls, zs = np.array([0.9e-6, M * dx / K]), np.array([z, 1.5 * z])
     rng = np.random.default_rng(n)
     srcs = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in range(2)]
-    b = wos.Batch(n, dx, wls, zs, wos.ctx_for(1e-6, backward_regularize=reg, alpha=1e-3), "asm", wos.FP32)
+    b = wos.Batch(n, dx, wls, zs, wos.ctx_for(1e-6, backward_regularize=reg, alpha=1e-3), method, prec)
     if shape is not None:
         b.set_tuning(*shape)
     for u, f in enumerate(srcs):
@@ -181,11 +197,12 @@ def test_batch_fp32_evanescent_parity(wos, oracle, n, shape, z, reg):
     b.run(1)
     b.sync()
     for u in range(2):
-        want, _, _, _ = oracle.propagate(srcs[u], None, dx, zs[u], "asm",
+        want, _, _, _ = oracle.propagate(srcs[u], None, dx, zs[u], method,
                                          oracle.ctx_for(wls[u], backward_regularize=reg, alpha=1e-3))
         e = rel_l2(b.download(u), want)
-        print(f"evanescent FP32 n={n} shape={shape} z={zs[u]:.2e} reg={reg} unit {u} rel-L2 {e:.3e}", flush=True)
-        assert e <= FP32_TOL, (n, shape, z, reg, u, e)
+        print(f"evanescent {precision} {method} n={n} shape={shape} z={zs[u]:.2e} reg={reg} unit {u} "
+              f"rel-L2 {e:.3e}", flush=True)
+        assert e <= (FP64_TOL if prec == wos.FP64 else FP32_TOL), (n, method, shape, z, reg, u, e)
     b.close()
 
 

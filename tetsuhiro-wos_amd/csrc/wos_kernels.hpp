@@ -206,6 +206,7 @@ struct TF32Col {
   double one_m_ax2;  // 1 - (wl*fx)^2
   double cwl, coff;  // wl*fy = row*cwl + coff
   double kq;         // k*z in turns: (k*z)/(2 pi)
+  int col;           // transform column (for rows at the cut-off)
 };
 
 __device__ __forceinline__ TF32Col tf32_col(const TFUnit& u, int col) {
@@ -218,48 +219,71 @@ __device__ __forceinline__ TF32Col tf32_col(const TFUnit& u, int col) {
   // conjugate transfer function (adjoint): the propagating phase changes
   // sign, the real evanescent factor does not
   c.kq = (u.kind == TF_CONJ_ASM ? -u.kz : u.kz) * 0.15915494309189533577;  // 1/(2 pi)
+  c.col = col;
   return c;
 }
 
+// The FP32 path's x = 1 - rho^2 (formed from reciprocal-multiplied
+// frequencies with FMAs) differs from the reference's 1 - rho^2 (Go's
+// hypot of divided frequencies, optics/propagate.go:162-166) by a few
+// 1e-16. Rows with |x| below this band take the reference's own test, so a
+// bin lying exactly on rho = 1 (e.g. i^2 + j^2 = 300^2 on a 1000-point grid
+// with wl = 0.3 n dx) gets the reference's branch: under evanescent zeroing
+// the two branches differ by the whole value.
+constexpr double kTfEdge = 1e-14;
+
+// Factor of the FP32 path for a row with x = 1 - rho^2 < kTfEdge, which
+// multiplies the propagating value formed from x (exactly (1, 0) where
+// x <= 0): 1 where the reference propagates, else its evanescent factor
+// (optics/propagate.go:168-180).
+template <bool INL>
+__device__ __forceinline__ float asm_tf_fp32_edge(const TFUnit& u, int col, int iy, double x) {
+  if (x >= kTfEdge) return 1.f;
+  if (x > -kTfEdge) {
+    const double rho = go_hypot(u.wl * (freq_of(col, u.M, u.den) + u.fx_off),
+                                u.wl * (freq_of(iy, u.M, u.den) + u.fy_off));
+    if (rho <= 1) return 1.f;
+    x = -((rho - 1) * (rho + 1));
+  }
+  const double decay = u.k * u.az * sqrt(-x);
+  return (float)(INL ? asm_evanescent_inl(u, decay) : asm_evanescent(u, decay));
+}
+
 // Propagating asmTF of the FP32 path at transform row iy: exp(i kz
-// sqrt(1 - rho^2)) with the phase in FP64; rows with rho > 1 return 1 and
-// raise *evan for the evanescent fix-up (not bit-exact with the reference,
-// which only the FP64 path requires).
+// sqrt(1 - rho^2)) with the phase in FP64; rows with rho > 1 return 1, and
+// rows with x < kTfEdge raise *evan for the evanescent fix-up (not
+// bit-exact with the reference, which only the FP64 path requires).
 __device__ __forceinline__ cpx<float> asm_tf_fp32_prop(const TF32Col& c, int iy, int M, bool* evan) {
   const double ay = fma((double)(iy <= M / 2 ? iy : iy - M), c.cwl, c.coff);
   const double x = fma(-ay, ay, c.one_m_ax2);
-  *evan |= x < 0;
+  *evan |= x < kTfEdge;
   float sn, cs;
   sincos_turns_f32(c.kq * sqrt_fast(x), &sn, &cs);
   return {cs, sn};
 }
 
-// Evanescent factor of the FP32 path at row iy (rho > 1), else 1.
+// Fix-up factor of the FP32 path at row iy (asm_tf_fp32_edge), out of line.
 __device__ __noinline__ float asm_tf_fp32_evan(const TFUnit& u, TF32Col c, int iy) {
   const double ay = fma((double)(iy <= u.M / 2 ? iy : iy - u.M), c.cwl, c.coff);
-  const double x = fma(-ay, ay, c.one_m_ax2);
-  if (x >= 0) return 1.f;
-  return (float)asm_evanescent(u, u.k * u.az * sqrt(-x));
+  return asm_tf_fp32_edge<false>(u, c.col, iy, fma(-ay, ay, c.one_m_ax2));
 }
 
-// Evanescent factor of the FP32 path at row iy, inline (see asm_tf_fp64_inl
+// Fix-up factor of the FP32 path at row iy, inline (see asm_tf_fp64_inl
 // for why the column passes make no calls).
 __device__ __forceinline__ float asm_tf_fp32_evan_inl(const TFUnit& u, const TF32Col& c, int iy) {
   const double ay = fma((double)(iy <= u.M / 2 ? iy : iy - u.M), c.cwl, c.coff);
-  const double x = fma(-ay, ay, c.one_m_ax2);
-  if (x >= 0) return 1.f;
-  return (float)asm_evanescent_inl(u, u.k * u.az * sqrt(-x));
+  return asm_tf_fp32_edge<true>(u, c.col, iy, fma(-ay, ay, c.one_m_ax2));
 }
 
-// The FP32-path asmTF at row iy, evanescent rows included (their factor is
-// real: the propagating part returns 1 there). INL: the evanescent branch
-// inline, for the register-resident column passes; the LDS-resident
-// mixed-radix passes keep it out of line (a smaller loop body there).
+// The FP32-path asmTF at row iy, evanescent rows included. INL: the
+// fix-up inline, for the register-resident column passes; the
+// LDS-resident mixed-radix passes keep it out of line (a smaller loop body
+// there).
 template <bool INL = false>
 __device__ __forceinline__ cpx<float> asm_tf_fp32_full(const TFUnit& u, const TF32Col& c, int iy) {
   bool evan = false;
   cpx<float> h = asm_tf_fp32_prop(c, iy, u.M, &evan);
-  if (evan) h = {INL ? asm_tf_fp32_evan_inl(u, c, iy) : asm_tf_fp32_evan(u, c, iy), 0.f};
+  if (evan) h = cscale(h, INL ? asm_tf_fp32_evan_inl(u, c, iy) : asm_tf_fp32_evan(u, c, iy));
   return h;
 }
 
@@ -267,10 +291,10 @@ __device__ __forceinline__ cpx<float> asm_tf_fp32_full(const TFUnit& u, const TF
 // has folded the row and formed ay): x <= 0 is clamped to a tiny positive
 // value instead of selecting s = 0 afterwards, so such rows get a phase of
 // 0 turns, whose FP32 remainder rounds to 0: h = (1, 0) exactly, for the
-// evanescent fix-up to scale. *evan is raised for x < 0.
+// evanescent fix-up to scale. *evan is raised for x < kTfEdge.
 __device__ __forceinline__ cpx<float> asm_tf_fp32_prop_ay(const TF32Col& c, double ay, bool* evan) {
   const double x = fma(-ay, ay, c.one_m_ax2);
-  *evan |= x < 0;
+  *evan |= x < kTfEdge;
   const double xc = fmax(x, 1e-300);
   const double r = __builtin_amdgcn_rsq(xc);
   const double g = xc * r, h = 0.5 * r;
@@ -279,24 +303,26 @@ __device__ __forceinline__ cpx<float> asm_tf_fp32_prop_ay(const TF32Col& c, doub
   return {cs, sn};
 }
 
-// Evanescent factor of the FP32 path at ay = wl*fy (rho > 1), else 1.
-__device__ __forceinline__ float asm_tf_fp32_evan_ay(const TFUnit& u, const TF32Col& c, double ay) {
-  const double x = fma(-ay, ay, c.one_m_ax2);
-  if (x >= 0) return 1.f;
-  return (float)asm_evanescent_inl(u, u.k * u.az * sqrt(-x));
+// Fix-up factor of the FP32 path at row iy with ay = wl*fy given.
+__device__ __forceinline__ float asm_tf_fp32_evan_ay(const TFUnit& u, const TF32Col& c, double ay, int iy) {
+  return asm_tf_fp32_edge<true>(u, c.col, iy, fma(-ay, ay, c.one_m_ax2));
 }
 
 // FP32-path asmTF at any (fx, fy); INL keeps the evanescent branch inline
+// (fx, fy: the reference's frequencies; within kTfEdge of the cut-off the
+// reference's test on them picks the branch)
 template <bool INL>
 __device__ __forceinline__ cpx<float> asm_tf_fp32_generic_t(const TFUnit& u, double fx, double fy) {
   double ax = u.wl * fx, ay = u.wl * fy;
   double rho2 = ax * ax + ay * ay;
-  if (rho2 <= 1) {
+  bool prop = rho2 <= 1;
+  if (fabs(rho2 - 1) < kTfEdge) prop = go_hypot(ax, ay) <= 1;
+  if (prop) {
     float s, c;
-    sincos_phase_f32(u.k * u.z * sqrt(1 - rho2), &s, &c);
+    sincos_phase_f32(u.k * u.z * sqrt(fmax(1 - rho2, 0.0)), &s, &c);
     return {c, s};
   }
-  const double decay = u.k * u.az * sqrt(rho2 - 1);
+  const double decay = u.k * u.az * sqrt(fmax(rho2 - 1, 0.0));
   return {(float)(INL ? asm_evanescent_inl(u, decay) : asm_evanescent(u, decay)), 0.f};
 }
 __device__ __noinline__ cpx<float> asm_tf_fp32_generic(const TFUnit& u, double fx, double fy) {
@@ -1031,7 +1057,7 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int blk, int nblk, c
             constexpr int m = V2;
             v[(m % NB) * RL + m / NB] = cmul(v[(m % NB) * RL + m / NB], asm_tf_fp32_prop_ay(cc, ay, &unused));
           }
-          if (fmin(fma(-ay, ay, cc.one_m_ax2), fma(-cc.coff, cc.coff, cc.one_m_ax2)) < 0) {
+          if (fmin(fma(-ay, ay, cc.one_m_ax2), fma(-cc.coff, cc.coff, cc.one_m_ax2)) < kTfEdge) {
             // ay again from an opaque copy of ay0: reusing the main loop's
             // values would keep them live across it (spilled)
             double aye = ay0;
@@ -1039,12 +1065,14 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int blk, int nblk, c
 #pragma unroll
             for (int m = 0; m <= V2; ++m) {
               if (m == V2 && t != 0) break;
-              const float e = asm_tf_fp32_evan_ay(u, cc, aye);
-              aye += dT;
+              const float e = asm_tf_fp32_evan_ay(u, cc, aye, t + T * m);
               v[(m % NB) * RL + m / NB] = cscale(v[(m % NB) * RL + m / NB], e);
-              // the partner's copy: only evanescent rows change (a factor
-              // of exactly 1 leaves their (1, 0) as it is)
-              if (m < V2 && e != 1.f) hx[(m * T + t) * W] = {e, 0.f};
+              // the partner's copy: only rows whose factor is not 1 change
+              if (m < V2 && e != 1.f) {
+                bool unused2 = false;
+                hx[(m * T + t) * W] = cscale(asm_tf_fp32_prop_ay(cc, aye, &unused2), e);
+              }
+              aye += dT;
             }
           }
         } else {
