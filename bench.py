@@ -36,13 +36,13 @@ VALU_WAVE_INSTR_PER_S = 614.4e9
 # PMC_SHAPE (tile_w, col_w, rows, row_groups, split_lds, radix, col_radix, col_split_lds, inv_rows,
 # inv_row_groups, inv_split_lds); a run whose autotuned shape differs reports no traffic figure
 PMC_HBM_BYTES = {
-    "fp32": {"row_fft_fwd": (1048874 * 2 + 2097202) * 1024, "col_fft_tf_ifft": (1049099 * 2 + 2097749) * 1024,
-             "row_fft_inv": (1048771 * 2 + 2097155) * 1024},
-    "fp64": {"row_fft_fwd": (2097380 * 2 + 4195045) * 1024, "col_fft_tf_ifft": (2100096 * 2 + 4277530) * 1024,
-             "row_fft_inv": (2097780 * 2 + 4195070) * 1024},
+    "fp32": {"row_fft_fwd": (1048872 * 2 + 2097201) * 1024, "col_fft_tf_ifft": (1048974 * 2 + 2097747) * 1024,
+             "row_fft_inv": (1048772 * 2 + 2097158) * 1024},
+    "fp64": {"row_fft_fwd": (2097380 * 2 + 4195109) * 1024, "col_fft_tf_ifft": (2100070 * 2 + 4275576) * 1024,
+             "row_fft_inv": (2097794 * 2 + 4195266) * 1024},
 }
 # VALU wave-instructions per full-batch launch (SQ_INSTS_VALU), from profiles/asm_2048x64_sq_counters.md
-PMC_VALU = {"fp32": {"row_fft_fwd": 1608 * 131072, "col_fft_tf_ifft": 3010 * 131072, "row_fft_inv": 1783 * 131072}}
+PMC_VALU = {"fp32": {"row_fft_fwd": 1608 * 131072, "col_fft_tf_ifft": 2831 * 131072, "row_fft_inv": 1783 * 131072}}
 PMC_SHAPE = {"fp32": (8, 8, 2, 2, 0, 16, 32, 1, 2, 2, 1), "fp64": (4, 4, 2, 0, 1, 16, 16, 1, 2, 2, 1)}
 METRIC = "2048² ASM props/s: FP32 vs FP64 at rel-L2≤1e-5; FFT HBM GB/s % of peak"
 
@@ -335,13 +335,14 @@ def bench_config12(wos, tune=True, retune=False):
             _, _, source = tune_batch(b, tune, retune, 3)
             for u, f in enumerate(comps):
                 b.upload(u, f)
+            # the checked fields first: timing propagates the batch in place
+            b.run(1)
+            b.sync()
+            fields[(name, pname)] = [b.download(u) for u in range(len(comps))]
             # stream launches back to back (a 512^2 propagation takes less GPU
             # time than one graph launch takes to submit), then graph replays
             ms = sorted(b.time(3, 30)[0] for _ in range(3))[1]   # median of three timings
             b.capture()
-            b.run(1)
-            b.sync()
-            fields[(name, pname)] = [b.download(u) for u in range(len(comps))]
             ms_graph = sorted(b.time(3, 30)[0] for _ in range(3))[1]
             b.close()
             S = 16 if prec == wos.FP64 else 8
