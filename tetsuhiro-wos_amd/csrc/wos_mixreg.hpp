@@ -31,32 +31,40 @@ struct MRDef {
 template <>
 struct MRDef<2000> {
   static constexpr int S = 4, V = 10;
-  static constexpr bool F32 = false, F64 = true;  // FP32 keeps the LDS-resident kernels
+  static constexpr bool F32 = false, F64 = true, F32C = false;  // FP32: LDS-resident
   static constexpr int rad(int s) { return s == 3 ? 2 : 10; }
 };
 template <>
 struct MRDef<1536> {
   static constexpr int S = 4, V = 24;
-  static constexpr bool F32 = true, F64 = false;
+  static constexpr bool F32 = true, F64 = false, F32C = true;
   static constexpr int rad(int s) { return s == 3 ? 3 : 8; }
 };
 template <>
 struct MRDef<1000> {
   static constexpr int S = 3, V = 10;
-  static constexpr bool F32 = true, F64 = true;
+  static constexpr bool F32 = true, F64 = true, F32C = true;
   static constexpr int rad(int s) { return 10; }
 };
 
 template <int N>
 constexpr bool mr_built() { return MRDef<N>::S > 0; }
-// The register-resident kernels run the FP64 transforms: a 1000^2 asm_pad
-// Jones propagation (two components) takes 0.133 ms against 0.178 ms on the
-// LDS-resident kernels, at one column per column workgroup. In FP32 they
-// are no faster (0.085 against 0.081-0.084 ms at the best tiles; the
-// four-row row pass alone 23 against 17 us), so FP32 keeps those.
+// The register-resident kernels run the 2000-point FP64 transforms: a
+// 1000^2 asm_pad Jones propagation (two components) takes 0.133 ms against
+// 0.178 ms on the LDS-resident kernels, at one column per column workgroup.
+// In FP32 both passes are slower register-resident (rows 23 against 17 us
+// in four-row workgroups, the column pass 52 against 48 us), so FP32 keeps
+// the LDS-resident kernels there. Rows (F32, F64) and columns (F32C, F64)
+// are chosen apart: each takes natural order in and out, so either pass
+// may use either engine.
 template <typename R, int N>
-constexpr bool mr_use() {
+constexpr bool mr_use_rows() {
   if constexpr (mr_built<N>()) return sizeof(R) == 8 ? MRDef<N>::F64 : MRDef<N>::F32;
+  else return false;
+}
+template <typename R, int N>
+constexpr bool mr_use_cols() {
+  if constexpr (mr_built<N>()) return sizeof(R) == 8 ? MRDef<N>::F64 : MRDef<N>::F32C;
   else return false;
 }
 
