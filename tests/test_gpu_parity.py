@@ -206,6 +206,33 @@ def test_batch_evanescent_parity(wos, oracle, precision, n, shape, method, z, re
     b.close()
 
 
+@pytest.mark.parametrize("n,method", [(256, "asm"), (256, "asm_shift"), (1000, "asm"), (1000, "asm_pad"),
+                                      (1000, "asm_shift")])
+@pytest.mark.parametrize("z,reg", [(0.6e-6, False), (-0.6e-6, False), (-0.6e-6, True)])
+def test_propagate_fp32_evanescent_parity(wos, oracle, n, method, z, reg):
+    """The FP32 drop-in Propagate with evanescent bins, at a wavelength
+    M dx / K with a Pythagorean K (bins on rho = 1: optics/propagate.go:
+    162-180 decides their branch), including the shifted-spectrum
+    transfer (asm_shift: a carrier tilt moves the spectrum, the generic
+    FP32 transfer function takes the reference's test within 1e-14 of the
+    cut-off), against the oracle to the FP32 tolerance."""
+    M = 2 * n if method == "asm_pad" else n
+    dx = 0.3e-6
+    wl = M * dx / {256: 100, 1000: 300, 2000: 600}[M]
+    rng = np.random.default_rng(n + 1)
+    x = (np.arange(n) - n / 2) * dx
+    tilt = np.exp(2j * np.pi * 0.2 / wl * x)[None, :]   # a carrier for asm_shift's centroid
+    src = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * tilt
+    ctx_o = oracle.ctx_for(wl, backward_regularize=reg, alpha=1e-3)
+    ctx_g = wos.ctx_for(wl, backward_regularize=reg, alpha=1e-3)
+    want, _, _, so = oracle.propagate(src, None, dx, z, method, ctx_o)
+    got, _, _, sg = wos.propagate(src, None, dx, z, method, ctx_g, precision=wos.FP32)
+    e = rel_l2(got, want)
+    print(f"drop-in FP32 {method} n={n} z={z:.1e} reg={reg} rel-L2 {e:.3e}", flush=True)
+    assert sg.method_used == so.method_used
+    assert e <= FP32_TOL, (n, method, z, reg, e)
+
+
 def test_split_step_fp64_parity(wos, oracle):
     """PropagateSplitStep on a graded-index fibre (config-3 medium, smaller grid)."""
     n, dx, wl = 128, 0.5e-6, 1.55e-6
